@@ -1,0 +1,364 @@
+"""Python binding of libgdict (the MI355X dictionaries) over its C ABI.
+
+The classes mirror the reference's host interfaces so that tests read like
+the reference's own drivers:
+
+* ``GPULSMOpt`` mirrors ``class GPULSMOpt`` (GPULSMOpt.cuh:5599-5872) with
+  the batch structs of gpu_dictionary_adapter.cuh flattened into arguments;
+* ``FlixIndex`` mirrors the FliX updatable index
+  (FliX/impl_cg_rtx_index_updates.cuh:1711-2693, non-OptiX members).
+
+Arguments are device buffers given as torch tensors (any 4- or 8-byte
+integer dtype; the bits are passed through) or as raw integer addresses.
+Streams are torch streams, raw ``hipStream_t`` integers, or None for the
+current torch stream.  Error codes of the C ABI become the Python
+counterparts of the reference's C++ exceptions.
+
+The product path is the HIP library only: if ``libgdict.so`` is missing the
+import of this module fails loudly; nothing falls back to a CPU path.
+"""
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgdict.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+GD_OK = 0
+GD_INVALID_ARGUMENT = 1
+GD_OUT_OF_MEMORY = 2
+GD_LENGTH = 3
+GD_PUBLICATION_FAILED = 4
+GD_HIP_ERROR = 5
+GD_CAPACITY = 6
+
+PUBLICATION_OUTPUT_OVERFLOW = 1 << 3
+NOT_FOUND = 0xFFFFFFFF
+
+
+class GdictError(RuntimeError):
+    """std::runtime_error (HIP errors, failed publication)."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class InvalidArgument(GdictError, ValueError):
+    """std::invalid_argument."""
+
+
+class OutOfMemory(GdictError, MemoryError):
+    """std::bad_alloc."""
+
+
+class LengthError(GdictError, OverflowError):
+    """std::length_error."""
+
+
+class PublicationFailed(GdictError):
+    """std::runtime_error raised by updates after a failed publication."""
+
+
+class CapacityExhausted(GdictError):
+    """Allocation region or memory budget exhausted."""
+
+
+_ERRORS = {
+    GD_INVALID_ARGUMENT: InvalidArgument,
+    GD_OUT_OF_MEMORY: OutOfMemory,
+    GD_LENGTH: LengthError,
+    GD_PUBLICATION_FAILED: PublicationFailed,
+    GD_HIP_ERROR: GdictError,
+    GD_CAPACITY: CapacityExhausted,
+}
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("max_elements", ctypes.c_uint64),
+        ("batch_capacity", ctypes.c_uint64),
+        ("level_zero_capacity", ctypes.c_uint64),
+        ("delete_order", ctypes.c_uint32),
+        ("base_delete_values", ctypes.c_uint32),
+    ]
+
+
+class _FlixConfig(ctypes.Structure):
+    _fields_ = [
+        ("key_bytes", ctypes.c_uint32),
+        ("value_bytes", ctypes.c_uint32),
+        ("node_slots", ctypes.c_uint32),
+        ("fill_percent", ctypes.c_uint32),
+    ]
+
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_I = ctypes.c_int
+
+# symbol -> argtypes (restype is gd_status = int unless noted)
+SIGNATURES = {
+    "gd_lsm_create": [ctypes.POINTER(_Config), ctypes.POINTER(_P)],
+    "gd_lsm_destroy": [_P],
+    "gd_lsm_clear": [_P, _P],
+    "gd_lsm_bulk_build": [_P, _P, _P, _U64, _P],
+    "gd_lsm_insert": [_P, _P, _P, _U64, _P],
+    "gd_lsm_erase": [_P, _P, _U64, _P],
+    "gd_lsm_apply_mixed": [_P, _P, _P, _P, _U64, _P],
+    "gd_lsm_lookup": [_P, _P, _U64, _P, _P, _I, _P],
+    "gd_lsm_range_sum": [_P, _P, _P, _U64, _P, _P],
+    "gd_lsm_range_count": [_P, _P, _P, _U64, _P, _P],
+    "gd_lsm_successor": [_P, _P, _U64, _P, _P],
+    "gd_lsm_dump_sorted": [_P, _P, _P, _U64, ctypes.POINTER(_U64), _P],
+    "gd_lsm_compact": [_P, _P],
+    "gd_lsm_carry_status": [_P, ctypes.POINTER(ctypes.c_uint32)],
+    "gd_lsm_resident_bytes": [_P, ctypes.POINTER(_U64)],
+    "gd_lsm_shape": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64)],
+    "gd_flix_create": [ctypes.POINTER(_FlixConfig), ctypes.POINTER(_P)],
+    "gd_flix_destroy": [_P],
+    "gd_flix_build": [_P, _P, _U64, _U64, _P],
+    "gd_flix_insert": [_P, _P, _P, _U64, _P],
+    "gd_flix_remove": [_P, _P, _U64, _P],
+    "gd_flix_lookup_sorted": [_P, _P, _U64, _P, _P],
+    "gd_flix_lookup": [_P, _P, _U64, _P, _P],
+    "gd_flix_successor_sorted": [_P, _P, _U64, _P, _P],
+    "gd_flix_rebuild": [_P, _P],
+    "gd_flix_range": [_P, _P, _P, _U64, _P, _P, _P, _P],
+    "gd_flix_total_keys": [_P, ctypes.POINTER(_U64), _P],
+    "gd_flix_resident_bytes": [_P, ctypes.POINTER(_U64)],
+}
+
+_lib = None
+
+
+def build(verbose=False):
+    """Compiles libgdict.so in-tree for gfx950 (hipcc, no GPU needed)."""
+    cmd = ["make", "-s", "-j8", "-C", _HERE]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("building libgdict.so failed:\n" + res.stdout +
+                           res.stderr)
+    if verbose:
+        print(res.stdout)
+
+
+def lib():
+    """Loads libgdict.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libgdict.so is missing at %s; run build() first" % LIB_PATH)
+    handle = ctypes.CDLL(LIB_PATH)
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(handle, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+    handle.gd_last_error_message.argtypes = [_P]
+    handle.gd_last_error_message.restype = ctypes.c_char_p
+    _lib = handle
+    return _lib
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(s):
+    if s is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(s, int):
+        return s
+    return s.cuda_stream
+
+
+def _count(x, count):
+    if count is not None:
+        return int(count)
+    return int(x.numel())
+
+
+def _check(status, handle):
+    if status == GD_OK:
+        return
+    msg = lib().gd_last_error_message(handle)
+    msg = msg.decode() if msg else ""
+    raise _ERRORS.get(status, GdictError)(status, msg)
+
+
+class GPULSMOpt:
+    """Mirror of GPULSMOpt (GPULSMOpt.cuh:5599) over gd_lsm_*."""
+
+    def __init__(self, max_elements=0, batch_capacity=0,
+                 level_zero_capacity=0):
+        cfg = _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0)
+        h = _P()
+        _check(lib().gd_lsm_create(ctypes.byref(cfg), ctypes.byref(h)), None)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gd_lsm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        _check(getattr(lib(), name)(self._h, *args), self._h)
+
+    def clear(self, stream=None):
+        self._call("gd_lsm_clear", _stream(stream))
+
+    def bulk_build(self, keys, values, count=None, stream=None):
+        self._call("gd_lsm_bulk_build", _ptr(keys), _ptr(values),
+                   _count(keys, count), _stream(stream))
+
+    def insert(self, keys, values, count=None, stream=None):
+        self._call("gd_lsm_insert", _ptr(keys), _ptr(values),
+                   _count(keys, count), _stream(stream))
+
+    def erase(self, keys, count=None, stream=None):
+        self._call("gd_lsm_erase", _ptr(keys), _count(keys, count),
+                   _stream(stream))
+
+    def apply_mixed(self, keys, values, is_erase, count=None, stream=None):
+        self._call("gd_lsm_apply_mixed", _ptr(keys), _ptr(values),
+                   _ptr(is_erase), _count(keys, count), _stream(stream))
+
+    def lookup(self, queries, out_values, out_found=None, count=None,
+               stream=None, quotients_grouped=False):
+        self._call("gd_lsm_lookup", _ptr(queries), _count(queries, count),
+                   _ptr(out_values), _ptr(out_found),
+                   1 if quotients_grouped else 0, _stream(stream))
+
+    def range(self, lo, hi, out_sums, count=None, stream=None):
+        self._call("gd_lsm_range_sum", _ptr(lo), _ptr(hi), _count(lo, count),
+                   _ptr(out_sums), _stream(stream))
+
+    def range_count(self, lo, hi, out_counts, count=None, stream=None):
+        self._call("gd_lsm_range_count", _ptr(lo), _ptr(hi),
+                   _count(lo, count), _ptr(out_counts), _stream(stream))
+
+    def successor(self, queries, out_keys, count=None, stream=None):
+        self._call("gd_lsm_successor", _ptr(queries), _count(queries, count),
+                   _ptr(out_keys), _stream(stream))
+
+    def dump_sorted(self, out_keys=None, out_values=None, capacity=0,
+                    stream=None):
+        """Returns the live count; fills out_keys/out_values when given."""
+        n = _U64(0)
+        st = lib().gd_lsm_dump_sorted(self._h, _ptr(out_keys),
+                                      _ptr(out_values), int(capacity),
+                                      ctypes.byref(n), _stream(stream))
+        if st not in (GD_OK, GD_LENGTH):
+            _check(st, self._h)
+        return int(n.value)
+
+    def compact(self, stream=None):
+        self._call("gd_lsm_compact", _stream(stream))
+
+    def canonical_carry_status(self):
+        v = ctypes.c_uint32(0)
+        self._call("gd_lsm_carry_status", ctypes.byref(v))
+        return int(v.value)
+
+    def gpu_resident_bytes(self):
+        v = _U64(0)
+        self._call("gd_lsm_resident_bytes", ctypes.byref(v))
+        return int(v.value)
+
+    def shape(self):
+        runs = ctypes.c_uint32(0)
+        rows = _U64(0)
+        self._call("gd_lsm_shape", ctypes.byref(runs), ctypes.byref(rows))
+        return int(runs.value), int(rows.value)
+
+
+class FlixIndex:
+    """Mirror of the FliX updatable index over gd_flix_*."""
+
+    def __init__(self, key_bytes=4, value_bytes=4, node_slots=0,
+                 fill_percent=0):
+        cfg = _FlixConfig(key_bytes, value_bytes, node_slots, fill_percent)
+        h = _P()
+        _check(lib().gd_flix_create(ctypes.byref(cfg), ctypes.byref(h)), None)
+        self._h = h
+        self.key_bytes = key_bytes
+        self.value_bytes = value_bytes
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gd_flix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        _check(getattr(lib(), name)(self._h, *args), self._h)
+
+    def build_bucket_layer_only(self, keys, max_size=0, count=None,
+                                stream=None):
+        self._call("gd_flix_build", _ptr(keys), _count(keys, count),
+                   int(max_size), _stream(stream))
+
+    def insert(self, keys, values, count=None, stream=None):
+        self._call("gd_flix_insert", _ptr(keys), _ptr(values),
+                   _count(keys, count), _stream(stream))
+
+    def remove(self, keys, count=None, stream=None):
+        self._call("gd_flix_remove", _ptr(keys), _count(keys, count),
+                   _stream(stream))
+
+    def lookups_ordered(self, sorted_keys, out_values, count=None,
+                        stream=None):
+        self._call("gd_flix_lookup_sorted", _ptr(sorted_keys),
+                   _count(sorted_keys, count), _ptr(out_values),
+                   _stream(stream))
+
+    def lookup(self, keys, out_values, count=None, stream=None):
+        self._call("gd_flix_lookup", _ptr(keys), _count(keys, count),
+                   _ptr(out_values), _stream(stream))
+
+    def lookups_successor(self, sorted_keys, out_keys, count=None,
+                          stream=None):
+        self._call("gd_flix_successor_sorted", _ptr(sorted_keys),
+                   _count(sorted_keys, count), _ptr(out_keys),
+                   _stream(stream))
+
+    def rebuild_bucket_layer_only(self, stream=None):
+        self._call("gd_flix_rebuild", _stream(stream))
+
+    def range(self, lo, hi, out_offsets, out_keys=None, out_values=None,
+              count=None, stream=None):
+        self._call("gd_flix_range", _ptr(lo), _ptr(hi), _count(lo, count),
+                   _ptr(out_offsets), _ptr(out_keys), _ptr(out_values),
+                   _stream(stream))
+
+    def compute_total_size(self, stream=None):
+        v = _U64(0)
+        self._call("gd_flix_total_keys", ctypes.byref(v), _stream(stream))
+        return int(v.value)
+
+    def gpu_resident_bytes(self):
+        v = _U64(0)
+        self._call("gd_flix_resident_bytes", ctypes.byref(v))
+        return int(v.value)

@@ -1,0 +1,212 @@
+// Host-side plumbing shared by the LSM and FliX runtimes: error transport
+// across the C ABI, a caching device pool, and per-handle operation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gdict.h"
+
+namespace gdict {
+
+// Internal exception carrying a gd_status; converted at the ABI boundary.
+struct Error : std::runtime_error {
+  gd_status status;
+  Error(gd_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+inline void hip_check(hipError_t e, const char *what, const char *file,
+                      int line) {
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // clear a sticky launch error
+    throw Error(GD_HIP_ERROR, std::string(hipGetErrorString(e)) + " in " +
+                                  what + " at " + file + ":" +
+                                  std::to_string(line));
+  }
+}
+#define GD_HIP(call) ::gdict::hip_check((call), #call, __FILE__, __LINE__)
+#define GD_LAUNCH_CHECK() GD_HIP(hipGetLastError())
+
+// Caching device allocator.  Buffers are recycled by size class and only
+// returned to HIP on destruction.  All work of a handle is totally ordered
+// on the device (event chaining), so a buffer released by host bookkeeping
+// may be handed to work enqueued later without further synchronisation.
+class DevicePool {
+ public:
+  DevicePool() = default;
+  DevicePool(const DevicePool &) = delete;
+  DevicePool &operator=(const DevicePool &) = delete;
+  ~DevicePool() { release_all(); }
+
+  static std::size_t size_class(std::size_t bytes) {
+    if (bytes <= 4096) return 4096;
+    // quarter-power-of-two classes keep waste below 25%
+    std::size_t p = 4096;
+    while (p < bytes) p <<= 1;
+    const std::size_t q = p >> 3;  // p/8 steps between p/2 and p
+    std::size_t c = p >> 1;
+    while (c < bytes) c += q;
+    return c;
+  }
+
+  void *get(std::size_t bytes) {
+    const std::size_t cls = size_class(bytes);
+    auto it = free_.find(cls);
+    if (it != free_.end()) {
+      void *p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      // give cached blocks back and retry once
+      trim();
+      e = hipMalloc(&p, cls);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(GD_OUT_OF_MEMORY,
+                    "device allocation of " + std::to_string(cls) +
+                        " bytes failed (" + hipGetErrorString(e) + ")");
+      }
+    }
+    size_of_[p] = cls;
+    held_ += cls;
+    return p;
+  }
+
+  template <class T>
+  T *get_n(std::size_t n) {
+    return static_cast<T *>(get(n * sizeof(T) ? n * sizeof(T) : 1));
+  }
+
+  void put(void *p) {
+    if (!p) return;
+    auto it = size_of_.find(p);
+    if (it == size_of_.end()) return;
+    free_.emplace(it->second, p);
+  }
+
+  // Frees cached (unused) blocks.  Callers must have synchronised the
+  // device work that last used them.
+  void trim() {
+    for (auto &kv : free_) {
+      (void)hipFree(kv.second);
+      held_ -= kv.first;
+      size_of_.erase(kv.second);
+    }
+    free_.clear();
+  }
+
+  void release_all() {
+    for (auto &kv : size_of_) (void)hipFree(kv.first);
+    size_of_.clear();
+    free_.clear();
+    held_ = 0;
+  }
+
+  std::size_t held_bytes() const { return held_; }
+
+ private:
+  std::multimap<std::size_t, void *> free_;
+  std::unordered_map<void *, std::size_t> size_of_;
+  std::size_t held_ = 0;
+};
+
+// RAII lease of a pool buffer.
+template <class T>
+class Lease {
+ public:
+  Lease() = default;
+  Lease(DevicePool &pool, std::size_t n) : pool_(&pool), p_(pool.get_n<T>(n)) {}
+  Lease(const Lease &) = delete;
+  Lease &operator=(const Lease &) = delete;
+  Lease(Lease &&o) noexcept : pool_(o.pool_), p_(o.p_) { o.p_ = nullptr; }
+  Lease &operator=(Lease &&o) noexcept {
+    reset();
+    pool_ = o.pool_;
+    p_ = o.p_;
+    o.p_ = nullptr;
+    return *this;
+  }
+  ~Lease() { reset(); }
+  void reset() {
+    if (p_ && pool_) pool_->put(p_);
+    p_ = nullptr;
+  }
+  T *get() const { return p_; }
+  T *release() {
+    T *p = p_;
+    p_ = nullptr;
+    return p;
+  }
+
+ private:
+  DevicePool *pool_ = nullptr;
+  T *p_ = nullptr;
+};
+
+// Event chaining that totally orders the work of one handle across streams.
+class OperationOrder {
+ public:
+  OperationOrder() {
+    GD_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+  }
+  ~OperationOrder() {
+    if (done_) {
+      (void)hipEventSynchronize(done_);
+      (void)hipEventDestroy(done_);
+    }
+  }
+  void begin(hipStream_t s) { GD_HIP(hipStreamWaitEvent(s, done_, 0)); }
+  void end(hipStream_t s) { GD_HIP(hipEventRecord(done_, s)); }
+  void wait_host() { GD_HIP(hipEventSynchronize(done_)); }
+
+ private:
+  hipEvent_t done_ = nullptr;
+};
+
+// Base of every handle: last error message.
+struct HandleBase {
+  std::string last_error;
+  std::mutex mutex;
+};
+
+void set_create_error(const std::string &m);
+const char *create_error();
+
+// Runs `body` and converts exceptions into a gd_status + message.
+template <class F>
+gd_status guarded(HandleBase *h, F &&body) {
+  try {
+    body();
+    return GD_OK;
+  } catch (const Error &e) {
+    if (h) h->last_error = e.what(); else set_create_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc &e) {
+    if (h) h->last_error = "bad_alloc"; else set_create_error("bad_alloc");
+    return GD_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    if (h) h->last_error = e.what(); else set_create_error(e.what());
+    return GD_HIP_ERROR;
+  }
+}
+
+inline unsigned grid_for(std::uint64_t items, unsigned threads,
+                         unsigned cap = 1u << 20) {
+  std::uint64_t g = (items + threads - 1) / threads;
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace gdict

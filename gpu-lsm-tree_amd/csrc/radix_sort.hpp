@@ -1,0 +1,245 @@
+// Stable LSD radix sort of (u32 key, u32 payload) pairs for gfx950.
+//
+// One global-histogram launch computes the four 8-bit digit histograms in a
+// single read of the keys; every digit pass is then ONE launch ("onesweep"):
+// a 4096-key tile is ranked in registers with wave64 ballots (a multisplit:
+// eight ballots give each lane the mask of lanes holding the same digit),
+// tiles chain their per-digit offsets through a decoupled look-back, and the
+// tile is re-ordered in LDS so the global stores leave as contiguous runs per
+// digit.  The first pass writes payload = input position, which is what the
+// callers need for "newest wins" (a stable sort keeps equal keys in input
+// order, so the last of a run of equal keys is the newest update).
+#pragma once
+
+#include "device_utils.hpp"
+
+namespace gdict {
+namespace rs {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;  // 4096
+constexpr int kRadix = 256;
+constexpr int kPasses = 4;
+constexpr int kWaves = kThreads / kWave;
+
+// Global histogram of all four digits.  hist is [kPasses][kRadix], zeroed.
+__global__ __launch_bounds__(kThreads) void histogram_kernel(
+    const u32 *__restrict__ keys, u32 n, u32 *__restrict__ hist) {
+  __shared__ u32 s_hist[kPasses][kRadix];
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads)
+    (&s_hist[0][0])[i] = 0;
+  __syncthreads();
+  const u64 stride = u64(gridDim.x) * kThreads;
+  for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+    const u32 k = keys[i];
+#pragma unroll
+    for (int p = 0; p < kPasses; ++p)
+      atomicAdd(&s_hist[p][(k >> (8 * p)) & 0xffu], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads) {
+    const u32 c = (&s_hist[0][0])[i];
+    if (c) atomicAdd(&hist[i], c);
+  }
+}
+
+// Exclusive scan of each pass's histogram -> global digit offsets.
+__global__ __launch_bounds__(kRadix) void scan_kernel(
+    const u32 *__restrict__ hist, u32 *__restrict__ offsets) {
+  __shared__ u32 scratch[kRadix / kWave + 1];
+  for (int p = 0; p < kPasses; ++p) {
+    const u32 v = hist[p * kRadix + threadIdx.x];
+    const u32 ex = block_exclusive_scan<kRadix>(v, scratch);
+    offsets[p * kRadix + threadIdx.x] = ex;
+    __syncthreads();
+  }
+}
+
+// One digit pass.  status has tiles*kRadix words and must be zero;
+// *ticket must be zero.  kFirst: the payload is generated (= position).
+template <bool kFirst>
+__global__ __launch_bounds__(kThreads) void onesweep_kernel(
+    const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in,
+    u32 *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
+    u32 shift, const u32 *__restrict__ digit_offsets, u64 *status,
+    u32 *ticket) {
+  __shared__ u32 s_tile;
+  __shared__ u32 s_wave_hist[kWaves][kRadix];
+  __shared__ u32 s_block_start[kRadix];
+  __shared__ u32 s_global_base[kRadix];
+  __shared__ u32 s_scan[kWaves + 1];
+  __shared__ u32 s_keys[kTile];
+  __shared__ u32 s_vals[kTile];
+
+  const u32 tid = threadIdx.x;
+  const u32 lane = lane_id();
+  const u32 wave = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  for (int i = tid; i < kWaves * kRadix; i += kThreads)
+    (&s_wave_hist[0][0])[i] = 0;
+  __syncthreads();
+  const u32 tile = s_tile;
+  const u64 base = u64(tile) * kTile;
+
+  u32 key[kItems], val[kItems], rank[kItems];
+  const u64 wave_base = base + u64(wave) * (kWave * kItems);
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const u64 idx = wave_base + u64(j) * kWave + lane;
+    const bool valid = idx < n;
+    key[j] = valid ? keys_in[idx] : 0xffffffffu;
+    if constexpr (kFirst) {
+      val[j] = static_cast<u32>(idx);
+    } else {
+      val[j] = valid ? vals_in[idx] : 0u;
+    }
+  }
+
+  // Wave-local stable ranking (item-major, then lane order == input order).
+  volatile u32 *my_hist = s_wave_hist[wave];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const u64 idx = wave_base + u64(j) * kWave + lane;
+    const bool valid = idx < n;
+    const u32 d = (key[j] >> shift) & 0xffu;
+    u64 match = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const u64 vote = ballot(bit);
+      match &= bit ? vote : ~vote;
+    }
+    u32 prior = 0;
+    if (valid) prior = my_hist[d];
+    rank[j] = prior + popc64(match & lanemask_lt());
+    // the highest lane of each digit group advances the counter
+    if (valid && (match >> lane) == 1ull)
+      my_hist[d] = prior + popc64(match);
+  }
+  __syncthreads();
+
+  // Per digit: wave prefixes, tile total, global base via look-back.
+  const u32 d = tid;  // kThreads == kRadix
+  u32 total = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const u32 c = s_wave_hist[w][d];
+    s_wave_hist[w][d] = total;
+    total += c;
+  }
+  // look-back: every thread owns one digit column of the status array
+  u64 exclusive = 0;
+  {
+    u64 *column = status;  // status[t * kRadix + d]
+    if (tile == 0) {
+      store_relaxed(&column[d], kFlagPrefix | total);
+    } else {
+      store_relaxed(&column[u64(tile) * kRadix + d], kFlagAggregate | total);
+      long long t = static_cast<long long>(tile) - 1;
+      while (true) {
+        u64 word;
+        do {
+          word = load_relaxed(&column[u64(t) * kRadix + d]);
+        } while ((word >> kFlagShift) == 0);
+        exclusive += word & kValueMask;
+        if (word & kFlagPrefix) break;
+        --t;
+      }
+      store_relaxed(&column[u64(tile) * kRadix + d],
+                    kFlagPrefix | (exclusive + total));
+    }
+  }
+  const u32 block_start = block_exclusive_scan<kThreads>(total, s_scan);
+  s_block_start[d] = block_start;
+  s_global_base[d] =
+      digit_offsets[d] + static_cast<u32>(exclusive) - block_start;
+  __syncthreads();
+
+  // Scatter into LDS in digit order.
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const u64 idx = wave_base + u64(j) * kWave + lane;
+    if (idx < n) {
+      const u32 dj = (key[j] >> shift) & 0xffu;
+      const u32 lp = s_block_start[dj] + s_wave_hist[wave][dj] + rank[j];
+      s_keys[lp] = key[j];
+      s_vals[lp] = val[j];
+    }
+  }
+  __syncthreads();
+
+  const u32 count =
+      base >= n ? 0u : static_cast<u32>(n - base < u64(kTile) ? n - base
+                                                              : u64(kTile));
+  for (u32 i = tid; i < count; i += kThreads) {
+    const u32 k = s_keys[i];
+    const u32 pos = s_global_base[(k >> shift) & 0xffu] + i;
+    keys_out[pos] = k;
+    vals_out[pos] = s_vals[i];
+  }
+}
+
+__global__ void zero_u32_kernel(u32 *p, u64 n) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = 0;
+}
+
+inline u32 tiles_for(u64 n) { return static_cast<u32>((n + kTile - 1) / kTile); }
+
+// Workspace words (u32) needed by sort_pairs for n keys, not counting the
+// two ping-pong buffers.
+inline u64 control_words(u64 n) {
+  // hist[4*256] + offsets[4*256] + tickets[4] + pad + status[4][tiles*256] u64
+  return 2048 + 64 + 2ull * kPasses * u64(tiles_for(n)) * kRadix;
+}
+
+// Sorts n (key, position) pairs.  keys_in is read only; the sorted keys and
+// positions end in (keys_a, vals_a).  keys_b/vals_b are scratch.  `control`
+// holds control_words(n) u32 words.  All launches go to `stream`.
+inline hipError_t sort_pairs(const u32 *keys_in, u32 n, u32 *keys_a,
+                             u32 *vals_a, u32 *keys_b, u32 *vals_b,
+                             u32 *control, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const u32 tiles = tiles_for(n);
+  u32 *hist = control;
+  u32 *offsets = control + 1024;
+  u32 *tickets = control + 2048;
+  u64 *status = reinterpret_cast<u64 *>(control + 2048 + 64);
+  const u64 zero_words = control_words(n);
+  zero_u32_kernel<<<static_cast<u32>(std::min<u64>((zero_words + 255) / 256,
+                                                   4096)),
+                    256, 0, stream>>>(control, zero_words);
+  const u32 hist_blocks = static_cast<u32>(
+      std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 1024));
+  histogram_kernel<<<hist_blocks, kThreads, 0, stream>>>(keys_in, n, hist);
+  scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets);
+  const u32 *src_k = keys_in;
+  const u32 *src_v = nullptr;
+  u32 *dst_k = keys_b, *dst_v = vals_b;
+  for (int p = 0; p < kPasses; ++p) {
+    u64 *pass_status = status + u64(p) * tiles * kRadix;
+    if (p == 0) {
+      onesweep_kernel<true><<<tiles, kThreads, 0, stream>>>(
+          src_k, src_v, dst_k, dst_v, n, 0, offsets, pass_status, tickets);
+    } else {
+      onesweep_kernel<false><<<tiles, kThreads, 0, stream>>>(
+          src_k, src_v, dst_k, dst_v, n, 8u * p, offsets + p * kRadix,
+          pass_status, tickets + p);
+    }
+    src_k = dst_k;
+    src_v = dst_v;
+    if (dst_k == keys_b) {
+      dst_k = keys_a;
+      dst_v = vals_a;
+    } else {
+      dst_k = keys_b;
+      dst_v = vals_b;
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rs
+}  // namespace gdict

@@ -1,0 +1,164 @@
+"""Deterministic synthetic workloads (counter-based, numpy).
+
+Streams are u64(i) = splitmix64(seed + i).  Uniform 31-bit keys are
+u64 >> 33, values are MurmurHash3_x86_32(key, seed 0), widths are
+log-uniform.  The paper-sweep key bijection follows
+FliX/paper_lsmu_sweep.cu:64-67.
+"""
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(x):
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + _GOLDEN
+        x = (x ^ (x >> np.uint64(30))) * _M1
+        x = (x ^ (x >> np.uint64(27))) * _M2
+    return x ^ (x >> np.uint64(31))
+
+
+def stream(seed, n, offset=0):
+    idx = np.arange(offset, offset + n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        return splitmix64(idx + np.uint64(seed))
+
+
+def _rotl(x, r):
+    return (x << np.uint32(r)) | (x >> np.uint32(32 - r))
+
+
+def murmur3_u32(keys, seed=0):
+    k = np.asarray(keys, dtype=np.uint32)
+    with np.errstate(over="ignore"):
+        k = k * np.uint32(0xCC9E2D51)
+        k = _rotl(k, 15)
+        k = k * np.uint32(0x1B873593)
+        h = np.full(k.shape, seed, dtype=np.uint32) ^ k
+        h = _rotl(h, 13)
+        h = h * np.uint32(5) + np.uint32(0xE6546B64)
+        h = h ^ np.uint32(4)
+        h = h ^ (h >> np.uint32(16))
+        h = h * np.uint32(0x85EBCA6B)
+        h = h ^ (h >> np.uint32(13))
+        h = h * np.uint32(0xC2B2AE35)
+        h = h ^ (h >> np.uint32(16))
+    return h
+
+
+def log_uniform(x, a, b):
+    u = (np.asarray(x, dtype=np.uint64) >> np.uint64(11)).astype(np.float64) \
+        * 2.0 ** -53
+    la, lb = np.log2(a), np.log2(b)
+    v = np.floor(np.exp2(la + u * (lb - la)))
+    return np.clip(v, a, b).astype(np.uint64)
+
+
+def paper_key_for_index(index):
+    i = np.asarray(index, dtype=np.uint32)
+    with np.errstate(over="ignore"):
+        return (i * np.uint32(747796405) + np.uint32(289133645)) & \
+            np.uint32((1 << 30) - 1)
+
+
+# ---------------------------------------------------------------- configs
+SEEDS = {1: 0xC0F10001, 2: 0xC0F10002, 3: 0xC0F10003, 4: 0xC0F10004,
+         5: 0xC0F10005}
+
+
+def uniform31(seed, n, offset=0):
+    return (stream(seed, n, offset) >> np.uint64(33)).astype(np.uint32)
+
+
+class LsmWorkload:
+    """Plain-LSM workload: insert/erase batches, then lookups and ranges.
+
+    keys in [0, 2^31); values murmur3(key).  In every batch after the
+    first, entry i is a tombstone with probability `erase_fraction` (decided
+    by its own stream), and a tombstone erases a key drawn uniformly from the
+    keys inserted by earlier batches.  Lookups draw `hit_fraction` of their
+    keys from the inserted keys and the rest uniformly from [0, 2^31).
+    Range lower bounds are uniform, widths log-uniform in [wmin, wmax],
+    hi = min(lo + w - 1, 2^31 - 1).
+    """
+
+    def __init__(self, seed, n_keys, batch, n_lookups, n_ranges, wmin, wmax,
+                 erase_fraction=0.0, hit_fraction=0.5):
+        self.seed = seed
+        self.n_keys = n_keys
+        self.batch = batch
+        self.n_batches = (n_keys + batch - 1) // batch
+        self.n_lookups = n_lookups
+        self.n_ranges = n_ranges
+        self.wmin, self.wmax = wmin, wmax
+        self.erase_fraction = erase_fraction
+        self.hit_fraction = hit_fraction
+
+    def batches(self):
+        """Yields (keys, values, is_erase) per batch."""
+        pool = np.empty(self.n_keys, dtype=np.uint32)
+        pooled = 0
+        for t in range(self.n_batches):
+            n = min(self.batch, self.n_keys - t * self.batch)
+            base = t * self.batch
+            keys = uniform31(self.seed, n, base)
+            is_erase = np.zeros(n, dtype=np.uint8)
+            if t > 0 and self.erase_fraction > 0 and pooled:
+                r = stream(self.seed ^ 0xE5E5, n, base)
+                thresh = np.uint64(int(self.erase_fraction * 2 ** 32))
+                sel = (r >> np.uint64(32)) < thresh
+                pick = (r & np.uint64(0xFFFFFFFF)) % np.uint64(pooled)
+                keys[sel] = pool[pick[sel].astype(np.int64)]
+                is_erase[sel] = 1
+            vals = murmur3_u32(keys)
+            vals[is_erase == 1] = 0
+            fresh = keys[is_erase == 0]
+            pool[pooled:pooled + len(fresh)] = fresh
+            pooled += len(fresh)
+            yield keys, vals, is_erase
+
+    def inserted_keys(self):
+        """All keys inserted (by insert entries), in order."""
+        parts = []
+        for keys, _, e in self.batches():
+            parts.append(keys[e == 0])
+        return np.concatenate(parts) if parts else np.empty(0, np.uint32)
+
+    def lookups(self, inserted):
+        n = self.n_lookups
+        r = stream(self.seed ^ 0x100C, n)
+        hit = (r >> np.uint64(32)) < np.uint64(int(self.hit_fraction * 2 ** 32))
+        pick = (r & np.uint64(0xFFFFFFFF)) % np.uint64(max(len(inserted), 1))
+        q = uniform31(self.seed ^ 0x2002, n)
+        if len(inserted):
+            q[hit] = inserted[pick[hit].astype(np.int64)]
+        return q
+
+    def ranges(self):
+        n = self.n_ranges
+        lo = uniform31(self.seed ^ 0x3003, n)
+        w = log_uniform(stream(self.seed ^ 0x4004, n), self.wmin, self.wmax)
+        hi = np.minimum(lo.astype(np.uint64) + w - np.uint64(1),
+                        np.uint64(2 ** 31 - 1)).astype(np.uint32)
+        return lo, hi
+
+
+def config1(scale_log=0):
+    """2^20 keys in 16 batches of 2^16; 2^20 lookups (50% hit); 2^14 range
+    counts with widths in [2^10, 2^16].  scale_log shrinks every size."""
+    s = scale_log
+    return LsmWorkload(SEEDS[1], 1 << (20 - s), 1 << (16 - s), 1 << (20 - s),
+                       1 << (14 - s), 1 << 10, 1 << 16, 0.0, 0.5)
+
+
+def config2(scale_log=0):
+    """2^26 keys in batches of 2^20, 10% tombstones of earlier keys;
+    2^26 lookups (70% drawn from inserted keys); 2^20 range counts with
+    widths in [2^4, 2^20]."""
+    s = scale_log
+    return LsmWorkload(SEEDS[2], 1 << (26 - s), 1 << (20 - s), 1 << (26 - s),
+                       1 << (20 - s), 1 << 4, 1 << 20, 0.1, 0.7)

@@ -1,0 +1,200 @@
+/*
+ * gdict: MI355X-native GPU dictionaries behind a plain C ABI.
+ *
+ * Two structures, both device resident, both driven from the host with
+ * device pointers supplied by the caller (the caller owns every input and
+ * output buffer; a handle owns its own device memory):
+ *
+ *   gd_lsm_*   an LSM dictionary over uint32 keys and uint32 values that
+ *              replaces the GPULSMOpt class of the reference
+ *              (GPULSMOpt.cuh:5599-7805) and the batch structs of
+ *              gpu_dictionary_adapter.cuh:16-49;
+ *   gd_flix_*  a coarse-granular updatable sorted index (bucket layer of
+ *              fixed-size nodes chained into an allocation region) that
+ *              replaces the FliX cg_rtx_index_updates class
+ *              (FliX/impl_cg_rtx_index_updates.cuh:1422-2693, non-OptiX
+ *              members), with 32- or 64-bit keys and values.
+ *
+ * Errors never cross the ABI as C++ exceptions: every call returns a
+ * gd_status and stores a detail string retrievable with
+ * gd_last_error_message().  All work of one handle is totally ordered: each
+ * call makes its stream wait for the previous call's work and records a new
+ * completion event (the reference's begin/end_operation,
+ * GPULSMOpt.cuh:6852-6858), and a per-handle mutex serialises host calls
+ * (GPULSMOpt.cuh:7685).  Calls are asynchronous with respect to the host
+ * except where a function says otherwise.
+ */
+#ifndef GDICT_H
+#define GDICT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gd_status {
+  GD_OK = 0,
+  GD_INVALID_ARGUMENT = 1,   /* std::invalid_argument in the reference   */
+  GD_OUT_OF_MEMORY = 2,      /* std::bad_alloc                           */
+  GD_LENGTH = 3,             /* std::length_error                        */
+  GD_PUBLICATION_FAILED = 4, /* std::runtime_error after a failed epoch   */
+  GD_HIP_ERROR = 5,          /* std::runtime_error from CUDA_CHECK        */
+  GD_CAPACITY = 6            /* allocation region / budget exhausted     */
+} gd_status;
+
+/* Publication status bits (GPULSMOpt.cuh:535-541). */
+#define GD_PUBLICATION_SUCCESS 0u
+#define GD_PUBLICATION_JOB_OVERFLOW (1u << 0)
+#define GD_PUBLICATION_OUTPUT_OVERFLOW (1u << 3)
+#define GD_PUBLICATION_JOB_TOO_LARGE (1u << 4)
+#define GD_PUBLICATION_LEVEL_OVERFLOW (1u << 5)
+
+/* Replaces DictionaryConfig (gpu_dictionary_adapter.cuh:16-23).
+ * delete_order / base_delete_values are accepted for source compatibility;
+ * the reference declares them and never reads them. */
+typedef struct gd_config {
+  uint64_t max_elements;
+  uint64_t batch_capacity;
+  uint64_t level_zero_capacity;
+  uint32_t delete_order;
+  uint32_t base_delete_values;
+} gd_config;
+
+typedef struct gd_lsm gd_lsm;
+
+/* Detail string of the last failing call on `handle` (gd_lsm* or gd_flix*);
+ * for a NULL handle, the last failing create call of this thread. */
+const char *gd_last_error_message(const void *handle);
+
+/* ------------------------------------------------------------------ LSM */
+
+/* explicit GPULSMOpt(const DictionaryConfig&)  GPULSMOpt.cuh:5606 */
+gd_status gd_lsm_create(const gd_config *config, gd_lsm **out);
+/* ~GPULSMOpt()  GPULSMOpt.cuh:5723 (waits for outstanding work) */
+gd_status gd_lsm_destroy(gd_lsm *h);
+/* clear(stream)  GPULSMOpt.cuh:5731 */
+gd_status gd_lsm_clear(gd_lsm *h, hipStream_t stream);
+/* bulk_build(keys, values, count, stream)  GPULSMOpt.cuh:5738.  The last
+ * occurrence of a key wins.  Synchronises the stream (as the reference does
+ * for the unique count, GPULSMOpt.cuh:5776-5779). */
+gd_status gd_lsm_bulk_build(gd_lsm *h, const uint32_t *keys,
+                            const uint32_t *values, uint64_t count,
+                            hipStream_t stream);
+/* insert(const DeviceKeyValueBatch&, stream)  GPULSMOpt.cuh:5836 */
+gd_status gd_lsm_insert(gd_lsm *h, const uint32_t *keys,
+                        const uint32_t *values, uint64_t count,
+                        hipStream_t stream);
+/* erase(const DeviceKeyBatch&, stream)  GPULSMOpt.cuh:5843 */
+gd_status gd_lsm_erase(gd_lsm *h, const uint32_t *keys, uint64_t count,
+                       hipStream_t stream);
+/* No reference counterpart: one batch mixing inserts and erases
+ * (is_erase[i] != 0 makes entry i a tombstone); the age of an entry is its
+ * position, exactly as if insert/erase had been one call. */
+gd_status gd_lsm_apply_mixed(gd_lsm *h, const uint32_t *keys,
+                             const uint32_t *values, const uint8_t *is_erase,
+                             uint64_t count, hipStream_t stream);
+/* lookup(const DeviceLookupBatch&, stream, quotients_grouped)
+ * GPULSMOpt.cuh:5850.  out_found may be NULL: a miss then writes
+ * 0xFFFFFFFF, otherwise value 0 and found 0.  quotients_grouped is a hint. */
+gd_status gd_lsm_lookup(gd_lsm *h, const uint32_t *queries, uint64_t count,
+                        uint32_t *out_values, uint8_t *out_found,
+                        int quotients_grouped, hipStream_t stream);
+/* range(const DeviceRangeOutputBatch&, stream)  GPULSMOpt.cuh:5858:
+ * out_sums[i] = sum of live values with lo[i] <= key <= hi[i], accumulated in
+ * 64 bits and truncated to 32; lo > hi gives 0. */
+gd_status gd_lsm_range_sum(gd_lsm *h, const uint32_t *lo, const uint32_t *hi,
+                           uint64_t count, uint32_t *out_sums,
+                           hipStream_t stream);
+/* No GPULSMOpt counterpart (LSMu's count, FliX/impl_lsm_tree.cuh:1208):
+ * number of live keys in [lo, hi], truncated to 32 bits. */
+gd_status gd_lsm_range_count(gd_lsm *h, const uint32_t *lo,
+                             const uint32_t *hi, uint64_t count,
+                             uint32_t *out_counts, hipStream_t stream);
+/* successor(const DeviceSuccessorBatch&, stream)  GPULSMOpt.cuh:5865:
+ * smallest live key >= query, 0xFFFFFFFF when none. */
+gd_status gd_lsm_successor(gd_lsm *h, const uint32_t *queries, uint64_t count,
+                           uint32_t *out_keys, hipStream_t stream);
+/* No reference counterpart: writes the live contents in ascending key order
+ * (device buffers of `capacity` entries) and the number of live keys to
+ * *out_count (host).  Synchronises the stream.  GD_LENGTH when capacity is
+ * too small (out_count still receives the size). */
+gd_status gd_lsm_dump_sorted(gd_lsm *h, uint32_t *out_keys,
+                             uint32_t *out_values, uint64_t capacity,
+                             uint64_t *out_count, hipStream_t stream);
+/* No reference counterpart (LSMu's cleanup, FliX/impl_lsm_tree.cuh:1256):
+ * merges every run into one, dropping tombstones and shadowed versions. */
+gd_status gd_lsm_compact(gd_lsm *h, hipStream_t stream);
+/* canonical_carry_status()  GPULSMOpt.cuh:5872 (synchronous). */
+gd_status gd_lsm_carry_status(gd_lsm *h, uint32_t *out_status);
+/* gpu_resident_bytes()  GPULSMOpt.cuh:6341 */
+gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes);
+/* No reference counterpart: number of sorted runs currently searched by
+ * reads, and an upper bound on the rows they hold (host bookkeeping). */
+gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub);
+
+/* ----------------------------------------------------------------- FliX */
+
+typedef struct gd_flix gd_flix;
+
+typedef struct gd_flix_config {
+  uint32_t key_bytes;        /* 4 or 8 (key32 / key64)                     */
+  uint32_t value_bytes;      /* 4 (smallsize offsets, as the reference) or 8 */
+  uint32_t node_slots;       /* pairs per node; 0 -> 32 (NODESIZE=5)       */
+  uint32_t fill_percent;     /* initial fill; 0 -> 50                      */
+} gd_flix_config;
+
+/* Reserved keys: 0 (empty slot) and 1 (tombstone) (FliX/definitions.cuh:83-103,
+ * FliX/definitions_updates.cuh:259).  Usable keys are [2, 2^bits - 4]. */
+
+/* constructor + build_bucket_layer_only(keys, size, max_size, ...)
+ * FliX/impl_cg_rtx_index_updates.cuh:1711.  Values are the input positions
+ * (init_offsets); keys need not be sorted or unique (first occurrence wins).
+ * max_size bounds the number of keys the index may hold; 0 -> 2*size. */
+gd_status gd_flix_create(const gd_flix_config *config, gd_flix **out);
+gd_status gd_flix_destroy(gd_flix *h);
+gd_status gd_flix_build(gd_flix *h, const void *keys, uint64_t count,
+                        uint64_t max_size, hipStream_t stream);
+/* insert(sorted_keys, offsets, n, stream)  impl:2195.  Insert-if-absent;
+ * input need not be sorted. */
+gd_status gd_flix_insert(gd_flix *h, const void *keys, const void *values,
+                         uint64_t count, hipStream_t stream);
+/* remove(sorted_keys, n, stream)  impl:2253 */
+gd_status gd_flix_remove(gd_flix *h, const void *keys, uint64_t count,
+                         hipStream_t stream);
+/* lookups_ordered(sorted_keys, results, n, stream)  impl:2485: queries must
+ * be ascending; results are in query order; miss -> all-ones value. */
+gd_status gd_flix_lookup_sorted(gd_flix *h, const void *sorted_queries,
+                                uint64_t count, void *out_values,
+                                hipStream_t stream);
+/* No reference counterpart: lookups in any order (sorted internally). */
+gd_status gd_flix_lookup(gd_flix *h, const void *queries, uint64_t count,
+                         void *out_values, hipStream_t stream);
+/* lookups_successor(sorted_keys, out_keys, n, stream)  impl:2647: ceiling
+ * key, key_type(0xFFFFFFFF) when none.  Queries must be ascending. */
+gd_status gd_flix_successor_sorted(gd_flix *h, const void *sorted_queries,
+                                   uint64_t count, void *out_keys,
+                                   hipStream_t stream);
+/* rebuild_bucket_layer_only(&ms, &bytes)  impl:1821 */
+gd_status gd_flix_rebuild(gd_flix *h, hipStream_t stream);
+/* No reference counterpart (range_lookup_sum is a TODO, impl:2640-2644):
+ * pass 1 (out_keys == NULL) writes per-query counts to out_offsets[0..n)
+ * and out_offsets[n] = total after an exclusive scan; pass 2 fills pairs of
+ * query i at out_keys/out_values[out_offsets[i] ...]. */
+gd_status gd_flix_range(gd_flix *h, const void *lo, const void *hi,
+                        uint64_t count, uint64_t *out_offsets,
+                        void *out_keys, void *out_values, hipStream_t stream);
+/* compute_total_size(stream)  impl:1657 (synchronous) */
+gd_status gd_flix_total_keys(gd_flix *h, uint64_t *out_keys,
+                             hipStream_t stream);
+/* gpu_resident_bytes()  impl:1600 */
+gd_status gd_flix_resident_bytes(gd_flix *h, uint64_t *out_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDICT_H */

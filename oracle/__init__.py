@@ -1,0 +1,273 @@
+"""ctypes wrapper of the oracle (oracle.cpp): TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py may use
+this module, and only as the checker or the CPU baseline.  The product
+(libgdict.so and its binding) never imports it.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_lib = None
+
+_SIGS = {
+    "orc_splitmix64": ([_U64], _U64),
+    "orc_murmur3_u32": ([_U32, _U32], _U32),
+    "orc_murmur3_bytes": ([_P, _U64, _U32], _U32),
+    "orc_log_uniform": ([_U64, _U64, _U64], _U64),
+    "orc_fill_splitmix64": ([_U64, _U64, _P], None),
+    "orc_fill_murmur3": ([_P, _U64, _U32, _P], None),
+    "orc_fill_log_uniform": ([_U64, _U64, _U64, _U64, _P], None),
+    "orc_zipf_fill": ([ctypes.c_double, _U64, _U64, _U64, _P], None),
+    "orc_paper_key_for_index": ([_U32], _U32),
+    "orc_paper_mix32": ([_U32], _U32),
+    "orc_range_digest": ([_P, _U64, _U64, _P, _P], None),
+    "orc_lsm_create": ([_U64, _U64, _U64], _P),
+    "orc_lsm_destroy": ([_P], None),
+    "orc_lsm_clear": ([_P], None),
+    "orc_lsm_bulk_build": ([_P, _P, _P, _U64], ctypes.c_int),
+    "orc_lsm_insert": ([_P, _P, _P, _U64], ctypes.c_int),
+    "orc_lsm_erase": ([_P, _P, _U64], ctypes.c_int),
+    "orc_lsm_apply_mixed": ([_P, _P, _P, _P, _U64], ctypes.c_int),
+    "orc_lsm_lookup": ([_P, _P, _U64, _P, _P], None),
+    "orc_lsm_range_sum": ([_P, _P, _P, _U64, _P], None),
+    "orc_lsm_range_count": ([_P, _P, _P, _U64, _P], None),
+    "orc_lsm_successor": ([_P, _P, _U64, _P], None),
+    "orc_lsm_size": ([_P], _U64),
+    "orc_lsm_dump": ([_P, _P, _P], None),
+    "orc_lsm_status": ([_P], _U32),
+    "orc_lsm_capacity": ([_P], _U64),
+    "orc_lsm_message": ([_P], ctypes.c_char_p),
+    "orc_flix_create": ([ctypes.c_int, ctypes.c_int], _P),
+    "orc_flix_destroy": ([_P], None),
+    "orc_flix_build": ([_P, _P, _U64], None),
+    "orc_flix_insert": ([_P, _P, _P, _U64], None),
+    "orc_flix_remove": ([_P, _P, _U64], None),
+    "orc_flix_lookup": ([_P, _P, _U64, _P], None),
+    "orc_flix_successor": ([_P, _P, _U64, _P], None),
+    "orc_flix_range_counts": ([_P, _P, _P, _U64, _P], None),
+    "orc_flix_range_enumerate": ([_P, _P, _P, _U64, _P, _P, _P], None),
+    "orc_flix_size": ([_P], _U64),
+    "orc_flix_dump": ([_P, _P, _P], None),
+}
+
+
+def build():
+    res = subprocess.run(["make", "-s", "-C", _HERE], capture_output=True,
+                         text=True)
+    if res.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + res.stdout + res.stderr)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = h
+    return _lib
+
+
+def _a(x):
+    return None if x is None else x.ctypes.data
+
+
+def u32(x):
+    return np.ascontiguousarray(x, dtype=np.uint32)
+
+
+# --------------------------------------------------------------- generators
+def splitmix64_stream(seed, n):
+    out = np.empty(n, dtype=np.uint64)
+    lib().orc_fill_splitmix64(seed, n, _a(out))
+    return out
+
+
+def murmur3(keys, seed=0):
+    keys = u32(keys)
+    out = np.empty(len(keys), dtype=np.uint32)
+    lib().orc_fill_murmur3(_a(keys), len(keys), seed, _a(out))
+    return out
+
+
+def log_uniform_stream(seed, n, a, b):
+    out = np.empty(n, dtype=np.uint64)
+    lib().orc_fill_log_uniform(seed, n, a, b, _a(out))
+    return out
+
+
+def zipf_ranks(s, universe, seed, n):
+    out = np.empty(n, dtype=np.uint64)
+    lib().orc_zipf_fill(s, universe, seed, n, _a(out))
+    return out
+
+
+def range_digest(results, query_offset=0):
+    r = u32(results)
+    s = _U64(0)
+    x = _U64(0)
+    lib().orc_range_digest(_a(r), len(r), query_offset, ctypes.byref(s),
+                           ctypes.byref(x))
+    return int(s.value), int(x.value)
+
+
+# ---------------------------------------------------------------- LSM oracle
+class LsmOracle:
+    """Replays GPULSMOpt's observable semantics (see oracle.cpp)."""
+
+    def __init__(self, max_elements=0, batch_capacity=0,
+                 level_zero_capacity=0):
+        self._h = lib().orc_lsm_create(max_elements, batch_capacity,
+                                       level_zero_capacity)
+        if not self._h:
+            raise ValueError("GPULSMOpt capacity exceeds 32-bit key space")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_lsm_destroy(self._h)
+            self._h = None
+
+    def clear(self):
+        lib().orc_lsm_clear(self._h)
+
+    def bulk_build(self, keys, values):
+        k, v = u32(keys), u32(values)
+        return lib().orc_lsm_bulk_build(self._h, _a(k), _a(v), len(k))
+
+    def insert(self, keys, values):
+        k, v = u32(keys), u32(values)
+        return lib().orc_lsm_insert(self._h, _a(k), _a(v), len(k))
+
+    def erase(self, keys):
+        k = u32(keys)
+        return lib().orc_lsm_erase(self._h, _a(k), len(k))
+
+    def apply_mixed(self, keys, values, is_erase):
+        k, v = u32(keys), u32(values)
+        e = np.ascontiguousarray(is_erase, dtype=np.uint8)
+        return lib().orc_lsm_apply_mixed(self._h, _a(k), _a(v), _a(e), len(k))
+
+    def lookup(self, queries, with_found=False):
+        q = u32(queries)
+        out = np.empty(len(q), dtype=np.uint32)
+        found = np.empty(len(q), dtype=np.uint8) if with_found else None
+        lib().orc_lsm_lookup(self._h, _a(q), len(q), _a(out), _a(found))
+        return (out, found) if with_found else out
+
+    def range_sum(self, lo, hi):
+        lo, hi = u32(lo), u32(hi)
+        out = np.empty(len(lo), dtype=np.uint32)
+        lib().orc_lsm_range_sum(self._h, _a(lo), _a(hi), len(lo), _a(out))
+        return out
+
+    def range_count(self, lo, hi):
+        lo, hi = u32(lo), u32(hi)
+        out = np.empty(len(lo), dtype=np.uint32)
+        lib().orc_lsm_range_count(self._h, _a(lo), _a(hi), len(lo), _a(out))
+        return out
+
+    def successor(self, queries):
+        q = u32(queries)
+        out = np.empty(len(q), dtype=np.uint32)
+        lib().orc_lsm_successor(self._h, _a(q), len(q), _a(out))
+        return out
+
+    def size(self):
+        return int(lib().orc_lsm_size(self._h))
+
+    def dump(self):
+        n = self.size()
+        k = np.empty(n, dtype=np.uint32)
+        v = np.empty(n, dtype=np.uint32)
+        if n:
+            lib().orc_lsm_dump(self._h, _a(k), _a(v))
+        return k, v
+
+    def status(self):
+        return int(lib().orc_lsm_status(self._h))
+
+    def capacity(self):
+        return int(lib().orc_lsm_capacity(self._h))
+
+    def message(self):
+        return lib().orc_lsm_message(self._h).decode()
+
+
+# --------------------------------------------------------------- FliX oracle
+class FlixOracle:
+    def __init__(self, key_bytes=4, value_bytes=4):
+        self.kt = np.uint32 if key_bytes == 4 else np.uint64
+        self.vt = np.uint32 if value_bytes == 4 else np.uint64
+        self._h = lib().orc_flix_create(key_bytes, value_bytes)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_flix_destroy(self._h)
+            self._h = None
+
+    def _k(self, x):
+        return np.ascontiguousarray(x, dtype=self.kt)
+
+    def build(self, keys):
+        k = self._k(keys)
+        lib().orc_flix_build(self._h, _a(k), len(k))
+
+    def insert(self, keys, values):
+        k = self._k(keys)
+        v = np.ascontiguousarray(values, dtype=self.vt)
+        lib().orc_flix_insert(self._h, _a(k), _a(v), len(k))
+
+    def remove(self, keys):
+        k = self._k(keys)
+        lib().orc_flix_remove(self._h, _a(k), len(k))
+
+    def lookup(self, queries):
+        q = self._k(queries)
+        out = np.empty(len(q), dtype=self.vt)
+        lib().orc_flix_lookup(self._h, _a(q), len(q), _a(out))
+        return out
+
+    def successor(self, queries):
+        q = self._k(queries)
+        out = np.empty(len(q), dtype=self.kt)
+        lib().orc_flix_successor(self._h, _a(q), len(q), _a(out))
+        return out
+
+    def range(self, lo, hi):
+        lo, hi = self._k(lo), self._k(hi)
+        counts = np.empty(len(lo), dtype=np.uint64)
+        lib().orc_flix_range_counts(self._h, _a(lo), _a(hi), len(lo),
+                                    _a(counts))
+        offsets = np.zeros(len(lo) + 1, dtype=np.uint64)
+        np.cumsum(counts, out=offsets[1:])
+        total = int(offsets[-1])
+        ok = np.empty(total, dtype=self.kt)
+        ov = np.empty(total, dtype=self.vt)
+        lib().orc_flix_range_enumerate(self._h, _a(lo), _a(hi), len(lo),
+                                       _a(offsets), _a(ok), _a(ov))
+        return offsets, ok, ov
+
+    def size(self):
+        return int(lib().orc_flix_size(self._h))
+
+    def dump(self):
+        n = self.size()
+        k = np.empty(n, dtype=self.kt)
+        v = np.empty(n, dtype=self.vt)
+        if n:
+            lib().orc_flix_dump(self._h, _a(k), _a(v))
+        return k, v

@@ -1,0 +1,384 @@
+"""GPU parity of the LSM dictionary (libgdict.so through its C ABI) against
+the oracle, at sizes the oracle finishes in seconds, plus closed-form pins
+taken from the reference's own validation (FliX/paper_lsmu_sweep.cu)."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NOT_FOUND = 0xFFFFFFFF
+
+
+class Pair:
+    """GPU dictionary + oracle driven in lock step."""
+
+    def __init__(self, gd, orc, io, **cfg):
+        self.d = gd.GPULSMOpt(**cfg)
+        self.o = orc.LsmOracle(**cfg)
+        self.io = io
+        self.gd = gd
+
+    def insert(self, keys, vals):
+        self.d.insert(self.io.up(keys), self.io.up(vals), count=len(keys))
+        assert self.o.insert(keys, vals) == 0
+
+    def erase(self, keys):
+        self.d.erase(self.io.up(keys), count=len(keys))
+        assert self.o.erase(keys) == 0
+
+    def mixed(self, keys, vals, erase):
+        self.d.apply_mixed(self.io.up(keys), self.io.up(vals),
+                           self.io.up(erase, np.uint8), count=len(keys))
+        assert self.o.apply_mixed(keys, vals, erase) == 0
+
+    def check_lookup(self, q, with_found):
+        io = self.io
+        out = io.empty(len(q))
+        found = io.empty(len(q), 1) if with_found else None
+        self.d.lookup(io.up(q), out, found, count=len(q))
+        if with_found:
+            ev, ef = self.o.lookup(q, with_found=True)
+            assert np.array_equal(io.down(out, len(q)), ev)
+            assert np.array_equal(io.down(found, len(q), np.uint8), ef)
+        else:
+            assert np.array_equal(io.down(out, len(q)), self.o.lookup(q))
+
+    def check_ranges(self, lo, hi):
+        io = self.io
+        s = io.empty(len(lo))
+        c = io.empty(len(lo))
+        self.d.range(io.up(lo), io.up(hi), s, count=len(lo))
+        self.d.range_count(io.up(lo), io.up(hi), c, count=len(lo))
+        assert np.array_equal(io.down(s, len(lo)), self.o.range_sum(lo, hi))
+        assert np.array_equal(io.down(c, len(lo)), self.o.range_count(lo, hi))
+
+    def check_successor(self, q):
+        io = self.io
+        out = io.empty(len(q))
+        self.d.successor(io.up(q), out, count=len(q))
+        assert np.array_equal(io.down(out, len(q)), self.o.successor(q))
+
+    def check_contents(self):
+        io = self.io
+        n = self.d.dump_sorted()
+        ek, ev = self.o.dump()
+        assert n == len(ek)
+        k = io.empty(n)
+        v = io.empty(n)
+        assert self.d.dump_sorted(k, v, n) == n
+        assert np.array_equal(io.down(k, n), ek)
+        assert np.array_equal(io.down(v, n), ev)
+
+
+def _queries(rng, keys, n, span):
+    q = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+    if len(keys):
+        take = rng.integers(0, len(keys), n // 2)
+        q[: n // 2] = keys[take]
+    return q
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_mixed_stream_every_batch(gd, orc, io, seed):
+    """Inserts, erases and mixed batches with duplicates; full check of the
+    sorted contents, lookups (with and without found), ranges and successors
+    after every batch."""
+    rng = np.random.default_rng(seed)
+    span = [1 << 12, 1 << 20, 1 << 32][seed - 1]
+    p = Pair(gd, orc, io, max_elements=1 << 18, batch_capacity=1 << 11)
+    live = np.empty(0, np.uint32)
+    for b in range(40):
+        n = int(rng.integers(1, 5000))
+        keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+        vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        kind = b % 5
+        if kind in (0, 1, 2):
+            p.insert(keys, vals)
+        elif kind == 3 and len(live):
+            p.erase(live[rng.integers(0, len(live), n)])
+        else:
+            erase = (rng.random(n) < 0.3).astype(np.uint8)
+            p.mixed(keys, vals, erase)
+        live = p.o.dump()[0]
+        if b % 4 == 3 or b == 39:
+            p.check_contents()
+            q = _queries(rng, live, 3000, span)
+            p.check_lookup(q, with_found=bool(b % 8 == 3))
+            lo = rng.integers(0, span, 500, dtype=np.uint64)
+            w = rng.integers(0, max(span // 64, 2), 500, dtype=np.uint64)
+            hi = np.minimum(lo + w, 0xFFFFFFFF)
+            lo[:20], hi[:20] = hi[:20], lo[:20]  # some lo > hi
+            p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
+            p.check_successor(q)
+
+
+def test_edge_keys_and_empty(gd, orc, io):
+    p = Pair(gd, orc, io, max_elements=1 << 12, batch_capacity=64)
+    q = np.array([0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFE, 0xFFFFFFFF],
+                 np.uint32)
+    # empty dictionary
+    p.check_lookup(q, True)
+    p.check_lookup(q, False)
+    p.check_successor(q)
+    p.check_ranges(np.array([0, 5], np.uint32),
+                   np.array([0xFFFFFFFF, 4], np.uint32))
+    p.check_contents()
+    # zero-count calls are no-ops
+    p.d.insert(io.up(q), io.up(q), count=0)
+    p.d.erase(io.up(q), count=0)
+    # extreme keys, both halves of the key space, and a value of 0xFFFFFFFF
+    keys = np.array([0, 0xFFFFFFFF, 0x80000000, 0x7FFFFFFF, 5, 5],
+                    np.uint32)
+    vals = np.array([10, 0xFFFFFFFF, 12, 13, 14, 15], np.uint32)
+    p.insert(keys, vals)
+    p.check_contents()
+    p.check_lookup(q, True)
+    p.check_successor(np.array([0, 6, 0x7FFFFFFF, 0x80000001, 0xFFFFFFFF],
+                               np.uint32))
+    p.check_ranges(np.array([0, 0x7FFFFFFF, 0, 0xFFFFFFFF], np.uint32),
+                   np.array([0xFFFFFFFF, 0x80000000, 0, 0xFFFFFFFF],
+                            np.uint32))
+    p.erase(np.array([0x80000000, 0], np.uint32))
+    p.check_contents()
+    p.check_lookup(q, True)
+    p.check_successor(q)
+
+
+def test_invalid_arguments(gd, io):
+    d = gd.GPULSMOpt(max_elements=1024, batch_capacity=16)
+    with pytest.raises(gd.InvalidArgument):
+        d.insert(None, None, count=4)
+    with pytest.raises(gd.InvalidArgument):
+        d.lookup(None, None, count=4)
+    with pytest.raises(ValueError):
+        gd.GPULSMOpt(max_elements=1 << 33)
+
+
+def test_bulk_build_last_occurrence_wins(gd, orc, io):
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, 1 << 16, 200000, dtype=np.uint32)
+    vals = np.arange(200000, dtype=np.uint32)
+    p = Pair(gd, orc, io, max_elements=1 << 20, batch_capacity=1 << 14)
+    p.d.bulk_build(io.up(keys), io.up(vals), count=len(keys))
+    assert p.o.bulk_build(keys, vals) == 0
+    p.check_contents()
+    # closed form: the value of key k is the last index i with keys[i] == k
+    uniq, first_rev = np.unique(keys[::-1], return_index=True)
+    expect = (len(keys) - 1 - first_rev).astype(np.uint32)
+    p.check_lookup(uniq, False)
+    assert np.array_equal(p.o.lookup(uniq), expect)
+    # updates after a bulk build
+    p.insert(keys[:5000], vals[:5000] + 7)
+    p.erase(keys[5000:9000])
+    p.check_contents()
+    p.check_successor(_queries(rng, keys, 4000, 1 << 17))
+
+
+def _sparse_validation_key(quotient):
+    # FliX/paper_lsmu_sweep.cu:434-439
+    q = np.asarray(quotient, dtype=np.uint32)
+    with np.errstate(over="ignore"):
+        suffix = (q * np.uint32(40503) + np.uint32(17)) & np.uint32(0xFFFF)
+    return (q << np.uint32(16)) | suffix
+
+
+def test_forced_unified_scenarios(gd, orc, io):
+    """FliX/paper_lsmu_sweep.cu:441-589: the scenario data and expectations
+    are restated from the reference driver (closed forms, no oracle)."""
+    d = gd.GPULSMOpt(max_elements=1 << 20, batch_capacity=1 << 16,
+                     level_zero_capacity=(1 << 16) * 16)
+    expected = {}
+    removed = []
+
+    def validate():
+        qk = np.array(list(expected.keys()) + removed, np.uint32)
+        qv = np.array(list(expected.values()) + [NOT_FOUND] * len(removed),
+                      np.uint32)
+        extra = []
+        s = 0
+        while len(extra) < 4096:
+            k = (25000 << 16) | (s & 0xFFFF)
+            if k not in expected:
+                extra.append(k)
+            s += 1
+        qk = np.concatenate([qk, np.array(extra, np.uint32)])
+        qv = np.concatenate([qv, np.full(len(extra), NOT_FOUND, np.uint32)])
+        out = io.empty(len(qk))
+        d.lookup(io.up(qk), out, count=len(qk))
+        got = io.down(out, len(qk))
+        assert int((got != qv).sum()) == 0
+
+    for b in range(16):  # sparse_unique
+        q = np.arange(b * 2048, (b + 1) * 2048, dtype=np.uint32)
+        k = _sparse_validation_key(q)
+        v = (0x01000000 + q).astype(np.uint32)
+        expected.update(zip(k.tolist(), v.tolist()))
+        d.insert(io.up(k), io.up(v), count=len(k))
+    validate()
+    for b in range(16):  # dense_unique
+        sfx = np.arange(b * 4096, (b + 1) * 4096, dtype=np.uint32)
+        k = (np.uint32(30000) << np.uint32(16)) | sfx
+        v = (0x02000000 + sfx).astype(np.uint32)
+        expected.update(zip(k.tolist(), v.tolist()))
+        d.insert(io.up(k), io.up(v), count=len(k))
+    validate()
+    for b in range(16):  # duplicate_newest_wins
+        i = np.arange(4096, dtype=np.uint32)
+        k = _sparse_validation_key(i % 1024)
+        v = (0x10000000 + b * 4096 + i).astype(np.uint32)
+        for kk, vv in zip(k.tolist(), v.tolist()):
+            expected[kk] = vv
+        d.insert(io.up(k), io.up(v), count=len(k))
+    validate()
+    k = _sparse_validation_key(np.arange(1024, dtype=np.uint32))
+    for kk in k.tolist():  # tombstone_visibility
+        expected.pop(kk, None)
+        removed.append(kk)
+    kd = io.up(k)
+    for _ in range(16):
+        d.erase(kd, count=len(k))
+    validate()
+
+
+def test_multilevel_rollover(gd, io):
+    """FliX/paper_lsmu_sweep.cu:591-641: max=512, batch=4, L0=64, 512
+    batches of keys ordinal % 256.  Expected value of key k is
+    0x28000000 + (last ordinal with ordinal % 256 == k) = 0x28000000+1792+k."""
+    d = gd.GPULSMOpt(max_elements=512, batch_capacity=4,
+                     level_zero_capacity=64)
+    ordinals = np.arange(512 * 4, dtype=np.uint32)
+    keys = (ordinals % 256).astype(np.uint32)
+    vals = (0x28000000 + ordinals).astype(np.uint32)
+    kd, vd = io.up(keys), io.up(vals)
+    for b in range(512):
+        d.insert(kd[4 * b:4 * b + 4], vd[4 * b:4 * b + 4], count=4)
+    q = np.arange(256, dtype=np.uint32)
+    out = io.empty(256)
+    d.lookup(io.up(q), out, count=256)
+    expect = (0x28000000 + 1792 + q).astype(np.uint32)
+    assert np.array_equal(io.down(out, 256), expect)
+    assert d.canonical_carry_status() == 0
+
+
+def test_capacity_exhaustion(gd, orc, io):
+    """FliX/paper_lsmu_sweep.cu:643-695: 17 epochs of 16 batches of 4 unique
+    keys into a 1024-key dictionary fail with kPublicationOutputOverflow,
+    and the next update throws an error that mentions the capacity."""
+    d = gd.GPULSMOpt(max_elements=1024, batch_capacity=4,
+                     level_zero_capacity=1024)
+    o = orc.LsmOracle(max_elements=1024, batch_capacity=4,
+                      level_zero_capacity=1024)
+    keys = np.arange(17 * 16 * 4, dtype=np.uint32)
+    vals = (0x30000000 + keys).astype(np.uint32)
+    kd, vd = io.up(keys), io.up(vals)
+    for b in range(17 * 16):
+        d.insert(kd[4 * b:4 * b + 4], vd[4 * b:4 * b + 4], count=4)
+        assert o.insert(keys[4 * b:4 * b + 4], vals[4 * b:4 * b + 4]) == 0
+    status = d.canonical_carry_status()
+    assert status & gd.PUBLICATION_OUTPUT_OVERFLOW
+    assert status == o.status()
+    with pytest.raises(gd.PublicationFailed) as err:
+        d.insert(kd[:4], vd[:4], count=4)
+    assert "capacity" in str(err.value)
+    assert o.insert(keys[:4], vals[:4]) != 0
+    # accepted updates stay readable
+    out = io.empty(len(keys))
+    d.lookup(kd, out, count=len(keys))
+    assert np.array_equal(io.down(out, len(keys)), vals)
+    # clear() recovers
+    d.clear()
+    d.insert(kd[:4], vd[:4], count=4)
+
+
+def test_tiled_update_reports_failure(gd, orc, io):
+    """A multi-tile update that fails mid-way throws after keeping the
+    accepted tiles (GPULSMOpt.cuh:6913-6922)."""
+    cfg = dict(max_elements=64, batch_capacity=4)
+    d = gd.GPULSMOpt(**cfg)
+    o = orc.LsmOracle(**cfg)
+    keys = np.arange(200, dtype=np.uint32)
+    vals = keys + 1
+    with pytest.raises(gd.PublicationFailed):
+        d.insert(io.up(keys), io.up(vals), count=200)
+    assert o.insert(keys, vals) != 0
+    q = np.arange(256, dtype=np.uint32)
+    out = io.empty(256)
+    d.lookup(io.up(q), out, count=256)
+    assert np.array_equal(io.down(out, 256), o.lookup(q))
+
+
+def test_paper_sweep_closed_form(gd, io):
+    """FliX/paper_lsmu_sweep.cu:64-121,1137-1153: keys key_for_index(i) are
+    unique (odd multiplier, bijection on 2^30); values are 1, so hits return
+    1, misses (indices >= 2^27) return not_found, and range sums equal the
+    number of keys in range."""
+    import importlib
+    wl = importlib.import_module("gdict_amd.workload")
+    n = 1 << 18
+    keys = wl.paper_key_for_index(np.arange(n, dtype=np.uint32))
+    ones = np.ones(n, np.uint32)
+    d = gd.GPULSMOpt(max_elements=n, batch_capacity=1 << 14)
+    kd, vd = io.up(keys), io.up(ones)
+    for b in range(n // (1 << 14)):
+        s = slice(b << 14, (b + 1) << 14)
+        d.insert(kd[s], vd[s], count=1 << 14)
+    rng = np.random.default_rng(5)
+    hits = keys[rng.integers(0, n, 1 << 16)]
+    misses = wl.paper_key_for_index(
+        rng.integers(1 << 27, 1 << 30, 1 << 16, dtype=np.uint64)
+        .astype(np.uint32))
+    out = io.empty(1 << 16)
+    d.lookup(io.up(hits), out, count=1 << 16)
+    assert np.all(io.down(out, 1 << 16) == 1)
+    d.lookup(io.up(misses), out, count=1 << 16)
+    assert np.all(io.down(out, 1 << 16) == NOT_FOUND)
+    lo = rng.integers(0, 1 << 30, 4096, dtype=np.uint64).astype(np.uint32)
+    hi = np.minimum(lo.astype(np.uint64) + (1 << 14), (1 << 30) - 1) \
+        .astype(np.uint32)
+    s = io.empty(4096)
+    d.range(io.up(lo), io.up(hi), s, count=4096)
+    sk = np.sort(keys)
+    expect = (np.searchsorted(sk, hi, "right") -
+              np.searchsorted(sk, lo, "left")).astype(np.uint32)
+    assert np.array_equal(io.down(s, 4096), expect)
+
+
+def test_config1_plain_lsm(gd, orc, io):
+    """Config 1 at full size: 2^20 keys in 16 batches of 2^16, 2^20 lookups
+    (50% present), 2^14 range counts with widths in [2^10, 2^16]."""
+    import importlib
+    wl = importlib.import_module("gdict_amd.workload")
+    w = wl.config1()
+    p = Pair(gd, orc, io, max_elements=w.n_keys, batch_capacity=w.batch)
+    inserted = []
+    for keys, vals, _ in w.batches():
+        p.insert(keys, vals)
+        inserted.append(keys)
+    p.check_contents()
+    q = w.lookups(np.concatenate(inserted))
+    p.check_lookup(q, False)
+    p.check_lookup(q[:1 << 16], True)
+    lo, hi = w.ranges()
+    p.check_ranges(lo, hi)
+    p.check_successor(q[:1 << 16])
+
+
+def test_config2_scaled(gd, orc, io):
+    """Config 2 shape (mixed batches with 10% tombstones of earlier keys,
+    70% hit lookups, widths in [2^4, 2^20]) at 1/64 of its size."""
+    import importlib
+    wl = importlib.import_module("gdict_amd.workload")
+    w = wl.config2(scale_log=6)
+    p = Pair(gd, orc, io, max_elements=w.n_keys, batch_capacity=w.batch)
+    inserted = []
+    for t, (keys, vals, erase) in enumerate(w.batches()):
+        p.mixed(keys, vals, erase)
+        inserted.append(keys[erase == 0])
+        if t % 16 == 15:
+            p.check_contents()
+    q = w.lookups(np.concatenate(inserted))
+    p.check_lookup(q, True)
+    lo, hi = w.ranges()
+    p.check_ranges(lo, hi)
+    p.check_successor(q[:1 << 14])
