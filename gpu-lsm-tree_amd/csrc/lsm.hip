@@ -241,10 +241,17 @@ class Lsm : public HandleBase {
 
   u32 carry_status() const { return status_; }
   u64 resident_bytes() const { return pool_.held_bytes(); }
-  void shape(u32 *runs, u64 *rows) const {
+  // Synchronous: exact rows per run (refreshes the host upper bounds).
+  void shape(u32 *runs, u64 *rows) {
+    order_.wait_host();
     *runs = static_cast<u32>(runs_.size());
     u64 t = 0;
-    for (const auto &r : runs_) t += r.rows_ub;
+    for (auto &r : runs_) {
+      RunMeta m{};
+      GD_HIP(hipMemcpy(&m, r.meta(), sizeof(m), hipMemcpyDeviceToHost));
+      r.rows_ub = m.count;
+      t += m.count;
+    }
     *rows = t;
   }
 

@@ -1,6 +1,7 @@
 """Deterministic synthetic workloads (counter-based, numpy).
 
-Streams are u64(i) = splitmix64(seed + i).  Uniform 31-bit keys are
+Streams are u64(i) = splitmix64(seed + i); independent sub-streams of one
+seed start at hashed counters (sub_seed).  Uniform 31-bit keys are
 u64 >> 33, values are MurmurHash3_x86_32(key, seed 0), widths are
 log-uniform.  The paper-sweep key bijection follows
 FliX/paper_lsmu_sweep.cu:64-67.
@@ -70,6 +71,12 @@ SEEDS = {1: 0xC0F10001, 2: 0xC0F10002, 3: 0xC0F10003, 4: 0xC0F10004,
          5: 0xC0F10005}
 
 
+def sub_seed(seed, tag):
+    """Start counter of an independent sub-stream: a hashed 64-bit value, so
+    sub-streams of one seed never overlap in practice."""
+    return int(splitmix64(np.uint64((int(seed) << 16) ^ int(tag))))
+
+
 def uniform31(seed, n, offset=0):
     return (stream(seed, n, offset) >> np.uint64(33)).astype(np.uint32)
 
@@ -108,7 +115,7 @@ class LsmWorkload:
             keys = uniform31(self.seed, n, base)
             is_erase = np.zeros(n, dtype=np.uint8)
             if t > 0 and self.erase_fraction > 0 and pooled:
-                r = stream(self.seed ^ 0xE5E5, n, base)
+                r = stream(sub_seed(self.seed, 1), n, base)
                 thresh = np.uint64(int(self.erase_fraction * 2 ** 32))
                 sel = (r >> np.uint64(32)) < thresh
                 pick = (r & np.uint64(0xFFFFFFFF)) % np.uint64(pooled)
@@ -130,18 +137,19 @@ class LsmWorkload:
 
     def lookups(self, inserted):
         n = self.n_lookups
-        r = stream(self.seed ^ 0x100C, n)
+        r = stream(sub_seed(self.seed, 2), n)
         hit = (r >> np.uint64(32)) < np.uint64(int(self.hit_fraction * 2 ** 32))
         pick = (r & np.uint64(0xFFFFFFFF)) % np.uint64(max(len(inserted), 1))
-        q = uniform31(self.seed ^ 0x2002, n)
+        q = uniform31(sub_seed(self.seed, 3), n)
         if len(inserted):
             q[hit] = inserted[pick[hit].astype(np.int64)]
         return q
 
     def ranges(self):
         n = self.n_ranges
-        lo = uniform31(self.seed ^ 0x3003, n)
-        w = log_uniform(stream(self.seed ^ 0x4004, n), self.wmin, self.wmax)
+        lo = uniform31(sub_seed(self.seed, 4), n)
+        w = log_uniform(stream(sub_seed(self.seed, 5), n), self.wmin,
+                        self.wmax)
         hi = np.minimum(lo.astype(np.uint64) + w - np.uint64(1),
                         np.uint64(2 ** 31 - 1)).astype(np.uint32)
         return lo, hi

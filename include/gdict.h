@@ -133,7 +133,8 @@ gd_status gd_lsm_carry_status(gd_lsm *h, uint32_t *out_status);
 /* gpu_resident_bytes()  GPULSMOpt.cuh:6341 */
 gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes);
 /* No reference counterpart: number of sorted runs currently searched by
- * reads, and an upper bound on the rows they hold (host bookkeeping). */
+ * reads and the rows they hold (tombstones and shadowed versions included).
+ * Synchronous. */
 gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub);
 
 /* ----------------------------------------------------------------- FliX */

@@ -58,6 +58,14 @@ _SIGS = {
     "orc_flix_range_enumerate": ([_P, _P, _P, _U64, _P, _P, _P], None),
     "orc_flix_size": ([_P], _U64),
     "orc_flix_dump": ([_P, _P, _P], None),
+    "cpu_lsm_create": ([ctypes.c_uint], _P),
+    "cpu_lsm_destroy": ([_P], None),
+    "cpu_lsm_clear": ([_P], None),
+    "cpu_lsm_apply": ([_P, _P, _P, _P, _U64, ctypes.c_int], None),
+    "cpu_lsm_lookup": ([_P, _P, _U64, _P, _P], None),
+    "cpu_lsm_range": ([_P, _P, _P, _U64, _P, ctypes.c_int], None),
+    "cpu_lsm_rows": ([_P], _U64),
+    "cpu_lsm_runs": ([_P], ctypes.c_uint),
 }
 
 
@@ -271,3 +279,46 @@ class FlixOracle:
         if n:
             lib().orc_flix_dump(self._h, _a(k), _a(v))
         return k, v
+
+
+# ------------------------------------------------------- CPU LSM baseline
+class CpuLsm:
+    """Multithreaded sorted-array LSM (cpu_lsm.cpp), the CPU baseline."""
+
+    def __init__(self, threads):
+        self.threads = threads
+        self._h = lib().cpu_lsm_create(threads)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cpu_lsm_destroy(self._h)
+            self._h = None
+
+    def clear(self):
+        lib().cpu_lsm_clear(self._h)
+
+    def apply(self, keys, values=None, is_erase=None, mode=None):
+        k = u32(keys)
+        v = u32(values) if values is not None else None
+        e = (np.ascontiguousarray(is_erase, dtype=np.uint8)
+             if is_erase is not None else None)
+        if mode is None:
+            mode = 2 if e is not None else (0 if v is not None else 1)
+        lib().cpu_lsm_apply(self._h, _a(k), _a(v), _a(e), len(k), mode)
+
+    def lookup(self, queries, with_found=False):
+        q = u32(queries)
+        out = np.empty(len(q), dtype=np.uint32)
+        found = np.empty(len(q), dtype=np.uint8) if with_found else None
+        lib().cpu_lsm_lookup(self._h, _a(q), len(q), _a(out), _a(found))
+        return (out, found) if with_found else out
+
+    def range(self, lo, hi, count_only=False):
+        lo, hi = u32(lo), u32(hi)
+        out = np.empty(len(lo), dtype=np.uint32)
+        lib().cpu_lsm_range(self._h, _a(lo), _a(hi), len(lo), _a(out),
+                            1 if count_only else 0)
+        return out
+
+    def rows(self):
+        return int(lib().cpu_lsm_rows(self._h))
