@@ -1,0 +1,558 @@
+// Host runtime of the FliX updatable index and its C ABI (gd_flix_*).
+//
+// Replaces the non-OptiX members of cg_rtx_index_updates
+// (FliX/impl_cg_rtx_index_updates.cuh:1422-2693): build_bucket_layer_only
+// (:1711), insert (:2195), remove (:2253), lookups_ordered (:2485),
+// lookups_successor (:2647), rebuild_bucket_layer_only (:1821),
+// compute_total_size (:1657), gpu_resident_bytes (:1600).  Batches are
+// sorted internally (onesweep radix sort over 32- or 64-bit keys), so unsorted
+// input is accepted; sorted input behaves as in the reference.
+#include <algorithm>
+#include <memory>
+#include <string>
+
+#include "flix_kernels.hpp"
+#include "host_common.hpp"
+#include "radix_sort.hpp"
+
+namespace gdict {
+namespace flix {
+
+struct RingCounters {
+  unsigned long long alloc_cursor;
+  unsigned long long free_count;
+  unsigned long long first_fail;
+  u32 fail_flag;
+  u32 pad;
+};
+
+__global__ void ring_counters_init_kernel(RingCounters *c, u64 free_count) {
+  c->alloc_cursor = 0;
+  c->free_count = free_count;
+  c->first_fail = ~0ull;
+  c->fail_flag = 0;
+}
+
+__global__ void ring_repair_kernel(RingCounters *c) {
+  if (c->fail_flag) {
+    if (c->alloc_cursor > c->first_fail) c->alloc_cursor = c->first_fail;
+    c->first_fail = ~0ull;
+    c->fail_flag = 0;
+  }
+}
+
+__global__ void widen_kernel(const u32 *in, u64 n, u64 *out) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = in[i];
+}
+
+class FlixBase {
+ public:
+  virtual ~FlixBase() = default;
+  virtual void build(const void *keys, u64 n, u64 max_size, hipStream_t s) = 0;
+  virtual void insert(const void *keys, const void *vals, u64 n,
+                      hipStream_t s) = 0;
+  virtual void remove(const void *keys, u64 n, hipStream_t s) = 0;
+  virtual void lookup(const void *q, u64 n, void *out, hipStream_t s) = 0;
+  virtual void successor(const void *q, u64 n, void *out, hipStream_t s) = 0;
+  virtual void range(const void *lo, const void *hi, u64 n, u64 *offsets,
+                     void *ok, void *ov, hipStream_t s) = 0;
+  virtual void rebuild(hipStream_t s) = 0;
+  virtual u64 total_keys(hipStream_t s) = 0;
+  virtual u64 resident_bytes() const = 0;
+};
+
+template <class K, class V>
+class FlixImpl : public FlixBase {
+ public:
+  FlixImpl(u32 slots, u32 fill_percent) : S_(slots), fill_percent_(fill_percent) {}
+
+  ~FlixImpl() override {
+    try {
+      order_.wait_host();
+    } catch (...) {
+    }
+    release_structure();
+  }
+
+  // ----------------------------------------------------------- build
+  void build(const void *keys_v, u64 n, u64 max_size, hipStream_t s) override {
+    const K *keys = static_cast<const K *>(keys_v);
+    if (n && !keys) throw Error(GD_INVALID_ARGUMENT, "null build keys");
+    if (n > 0xffffffffull)
+      throw Error(GD_INVALID_ARGUMENT, "build size exceeds 2^32-1");
+    order_.begin(s);
+    try {
+      max_size_ = std::max<u64>(max_size ? max_size : 2 * n, n);
+      Lease<K> uk(pool_, std::max<u64>(n, 1));
+      Lease<V> uv(pool_, std::max<u64>(n, 1));
+      u64 m = 0;
+      if (n) m = sort_unique(keys, nullptr, n, /*positions=*/true, nullptr,
+                             uk.get(), uv.get(), s);
+      layout(uk.get(), uv.get(), m, s);
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+  }
+
+  // ---------------------------------------------------------- updates
+  void insert(const void *keys_v, const void *vals_v, u64 n,
+              hipStream_t s) override {
+    require_built();
+    if (!n) return;
+    const K *keys = static_cast<const K *>(keys_v);
+    const V *vals = static_cast<const V *>(vals_v);
+    if (!keys || !vals) throw Error(GD_INVALID_ARGUMENT, "null insert batch");
+    if (n > 0xffffffffull)
+      throw Error(GD_INVALID_ARGUMENT, "batch size exceeds 2^32-1");
+    order_.begin(s);
+    bool failed = false;
+    try {
+      Lease<K> bk(pool_, n);
+      Lease<V> bv(pool_, n);
+      const Pool<K, V> existing = pool_view();
+      const u64 m = sort_unique(keys, vals, n, false, &existing, bk.get(),
+                                bv.get(), s);
+      if (m) {
+        Lease<u64> begin(pool_, u64(P_) + 1);
+        Lease<u32> work(pool_, std::min<u64>(P_, m) + 1);
+        const u32 work_items = slices(bk.get(), m, begin.get(), work.get(), s);
+        if (work_items) {
+          insert_kernel<K, V><<<(work_items + 3) / 4, 256, 0, s>>>(
+              pool_view(), ring_view(), bk.get(), bv.get(), begin.get(),
+              work.get(), work_items);
+          GD_LAUNCH_CHECK();
+        }
+        RingCounters rc{};
+        GD_HIP(hipMemcpyAsync(&rc, counters_, sizeof(rc),
+                              hipMemcpyDeviceToHost, s));
+        GD_HIP(hipStreamSynchronize(s));
+        if (rc.fail_flag) {
+          ring_repair_kernel<<<1, 1, 0, s>>>(counters_);
+          GD_LAUNCH_CHECK();
+          failed = true;
+        }
+      }
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+    if (failed)
+      throw Error(GD_CAPACITY,
+                  "FliX allocation region exhausted (" +
+                      std::to_string(ring_cap_) +
+                      " nodes); some keys were not inserted; rebuild with a "
+                      "larger max_size");
+  }
+
+  void remove(const void *keys_v, u64 n, hipStream_t s) override {
+    require_built();
+    if (!n) return;
+    const K *keys = static_cast<const K *>(keys_v);
+    if (!keys) throw Error(GD_INVALID_ARGUMENT, "null delete batch");
+    if (n > 0xffffffffull)
+      throw Error(GD_INVALID_ARGUMENT, "batch size exceeds 2^32-1");
+    order_.begin(s);
+    try {
+      const u32 nn = static_cast<u32>(n);
+      Lease<K> ka(pool_, n), kb(pool_, n);
+      Lease<u32> va(pool_, n), vb(pool_, n);
+      Lease<u32> ctrl(pool_, rs::control_words(n, rs::passes_for<K>()));
+      GD_HIP(rs::sort_pairs<K>(keys, nn, ka.get(), va.get(), kb.get(),
+                               vb.get(), ctrl.get(), s));
+      Lease<u64> begin(pool_, u64(P_) + 1);
+      Lease<u32> work(pool_, std::min<u64>(P_, n) + 1);
+      const u32 work_items = slices(ka.get(), n, begin.get(), work.get(), s);
+      if (work_items) {
+        delete_kernel<K, V><<<(work_items + 3) / 4, 256, 0, s>>>(
+            pool_view(), ring_view(), ka.get(), begin.get(), work.get(),
+            work_items);
+        GD_LAUNCH_CHECK();
+      }
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+  }
+
+  // ------------------------------------------------------------ reads
+  void lookup(const void *q, u64 n, void *out, hipStream_t s) override {
+    require_built();
+    if (!n) return;
+    if (!q || !out) throw Error(GD_INVALID_ARGUMENT, "null lookup buffers");
+    order_.begin(s);
+    lookup_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+        pool_view(), static_cast<const K *>(q), n, static_cast<V *>(out));
+    GD_LAUNCH_CHECK();
+    order_.end(s);
+  }
+
+  void successor(const void *q, u64 n, void *out, hipStream_t s) override {
+    require_built();
+    if (!n) return;
+    if (!q || !out) throw Error(GD_INVALID_ARGUMENT, "null successor buffers");
+    order_.begin(s);
+    successor_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+        pool_view(), static_cast<const K *>(q), n, static_cast<K *>(out));
+    GD_LAUNCH_CHECK();
+    order_.end(s);
+  }
+
+  void range(const void *lo, const void *hi, u64 n, u64 *offsets, void *ok,
+             void *ov, hipStream_t s) override {
+    require_built();
+    if (!n) return;
+    if (!lo || !hi || !offsets || (ok && !ov))
+      throw Error(GD_INVALID_ARGUMENT, "null range buffers");
+    order_.begin(s);
+    try {
+      const Pool<K, V> pv = pool_view();
+      if (!ok) {
+        Lease<u64> counts(pool_, n + 1);
+        range_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            pv, static_cast<const K *>(lo), static_cast<const K *>(hi), n,
+            counts.get(), nullptr, nullptr);
+        GD_LAUNCH_CHECK();
+        exclusive_scan(counts.get(), n, offsets, s);
+      } else {
+        range_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            pv, static_cast<const K *>(lo), static_cast<const K *>(hi), n,
+            offsets, static_cast<K *>(ok), static_cast<V *>(ov));
+        GD_LAUNCH_CHECK();
+      }
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+  }
+
+  // ---------------------------------------------------------- rebuild
+  void rebuild(hipStream_t s) override {
+    require_built();
+    order_.begin(s);
+    try {
+      Lease<u64> sizes(pool_, u64(P_) + 1);
+      Lease<u64> offs(pool_, u64(P_) + 1);
+      chain_sizes_kernel<K, V><<<grid_for(P_, 256), 256, 0, s>>>(
+          pool_view(), sizes.get());
+      GD_LAUNCH_CHECK();
+      exclusive_scan(sizes.get(), P_, offs.get(), s);
+      u64 total = 0;
+      GD_HIP(hipMemcpyAsync(&total, offs.get() + P_, sizeof(u64),
+                            hipMemcpyDeviceToHost, s));
+      GD_HIP(hipStreamSynchronize(s));
+      Lease<K> uk(pool_, std::max<u64>(total, 1));
+      Lease<V> uv(pool_, std::max<u64>(total, 1));
+      flatten_kernel<K, V><<<(P_ + 3) / 4, 256, 0, s>>>(
+          pool_view(), offs.get(), uk.get(), uv.get());
+      GD_LAUNCH_CHECK();
+      max_size_ = std::max(max_size_, total);
+      layout(uk.get(), uv.get(), total, s);
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+  }
+
+  u64 total_keys(hipStream_t s) override {
+    require_built();
+    order_.begin(s);
+    u64 total = 0;
+    try {
+      Lease<unsigned long long> t(pool_, 1);
+      rs::zero_u32_kernel<<<1, 64, 0, s>>>(reinterpret_cast<u32 *>(t.get()), 2);
+      node_total_kernel<K, V><<<grid_for(P_, 256, 4096), 256, 0, s>>>(
+          pool_view(), t.get());
+      GD_LAUNCH_CHECK();
+      GD_HIP(hipMemcpyAsync(&total, t.get(), sizeof(u64),
+                            hipMemcpyDeviceToHost, s));
+      GD_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+    return total;
+  }
+
+  u64 resident_bytes() const override { return pool_.held_bytes(); }
+
+ private:
+  void require_built() const {
+    if (!node_keys_)
+      throw Error(GD_INVALID_ARGUMENT, "FliX index used before build");
+  }
+
+  Pool<K, V> pool_view() const {
+    return Pool<K, V>{node_keys_, node_vals_, node_count_, node_next_,
+                      maxv_, S_, P_, total_nodes_};
+  }
+
+  Ring ring_view() const {
+    return Ring{ring_ids_, ring_cap_, &counters_->alloc_cursor,
+                &counters_->free_count, &counters_->first_fail,
+                &counters_->fail_flag};
+  }
+
+  void release_structure() {
+    pool_.put(node_keys_);
+    pool_.put(node_vals_);
+    pool_.put(node_count_);
+    pool_.put(node_next_);
+    pool_.put(maxv_);
+    pool_.put(ring_ids_);
+    pool_.put(counters_);
+    node_keys_ = nullptr;
+    node_vals_ = nullptr;
+    node_count_ = nullptr;
+    node_next_ = nullptr;
+    maxv_ = nullptr;
+    ring_ids_ = nullptr;
+    counters_ = nullptr;
+  }
+
+  // out[0..n) = exclusive scan of in[0..n), out[n] = total.
+  void exclusive_scan(const u64 *in, u64 n, u64 *out, hipStream_t s) {
+    const u64 tiles = (n + 1 + 255) / 256;
+    Lease<u32> ctrl(pool_, 64 + 2 * tiles);
+    rs::zero_u32_kernel<<<grid_for(64 + 2 * tiles, 256, 1024), 256, 0, s>>>(
+        ctrl.get(), 64 + 2 * tiles);
+    scan_u64_kernel<<<static_cast<u32>(tiles), 256, 0, s>>>(
+        in, n, out, reinterpret_cast<u64 *>(ctrl.get() + 64), ctrl.get());
+    GD_LAUNCH_CHECK();
+  }
+
+  // Sorts (key, position), keeps the first occurrence of each key (and, when
+  // `existing` is given, only keys not yet stored), compacts into (ok, ov)
+  // with values gathered from `vals` (or = input positions).  Returns the
+  // number of rows (synchronises).
+  u64 sort_unique(const K *keys, const V *vals, u64 n, bool positions,
+                  const Pool<K, V> *existing, K *ok, V *ov, hipStream_t s) {
+    const u32 nn = static_cast<u32>(n);
+    Lease<K> ka(pool_, n), kb(pool_, n);
+    Lease<u32> va(pool_, n), vb(pool_, n);
+    Lease<u32> ctrl(pool_, rs::control_words(n, rs::passes_for<K>()));
+    GD_HIP(rs::sort_pairs<K>(keys, nn, ka.get(), va.get(), kb.get(), vb.get(),
+                             ctrl.get(), s));
+    Lease<u32> keep(pool_, n);
+    if (existing) {
+      mark_new_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+          *existing, ka.get(), n, keep.get());
+    } else {
+      mark_first_kernel<K><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+          ka.get(), n, keep.get());
+    }
+    GD_LAUNCH_CHECK();
+    Lease<u64> keep64(pool_, n + 1);
+    widen(keep.get(), n, keep64.get(), s);
+    Lease<u64> offs(pool_, n + 1);
+    exclusive_scan(keep64.get(), n, offs.get(), s);
+    compact_batch_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+        ka.get(), va.get(), n, keep.get(), offs.get(), vals,
+        positions ? 1 : 0, ok, ov);
+    GD_LAUNCH_CHECK();
+    u64 m = 0;
+    GD_HIP(hipMemcpyAsync(&m, offs.get() + n, sizeof(u64),
+                          hipMemcpyDeviceToHost, s));
+    GD_HIP(hipStreamSynchronize(s));
+    return m;
+  }
+
+  void widen(const u32 *in, u64 n, u64 *out, hipStream_t s) {
+    widen_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(in, n, out);
+    GD_LAUNCH_CHECK();
+  }
+
+  // Bucket slices of a sorted batch + compact list of touched buckets.
+  u32 slices(const K *batch, u64 m, u64 *begin, u32 *work, hipStream_t s) {
+    bucket_slices_kernel<K><<<grid_for(u64(P_) + 1, 256), 256, 0, s>>>(
+        maxv_, P_, batch, m, begin);
+    GD_LAUNCH_CHECK();
+    u32 *count = work + std::min<u64>(P_, m);
+    rs::zero_u32_kernel<<<1, 64, 0, s>>>(count, 1);
+    work_list_kernel<<<grid_for(P_, 256), 256, 0, s>>>(begin, P_, work,
+                                                        count);
+    GD_LAUNCH_CHECK();
+    u32 items = 0;
+    GD_HIP(hipMemcpyAsync(&items, count, sizeof(u32), hipMemcpyDeviceToHost,
+                          s));
+    GD_HIP(hipStreamSynchronize(s));
+    return items;
+  }
+
+  // Lays sorted unique (key, value) rows into a fresh bucket layer.
+  void layout(const K *uk, const V *uv, u64 m, hipStream_t s) {
+    const u32 fill = std::max<u32>(1, std::min<u32>(
+        S_, static_cast<u32>((u64(S_) * fill_percent_ + 99) / 100)));
+    const u64 P = std::max<u64>(1, (m + fill - 1) / fill);
+    const u64 need = (2 * std::max(max_size_, m) + S_ - 1) / S_;
+    const u64 region = std::max<u64>(need > P ? need - P : 0, P / 4) + 1024;
+    const u64 total = P + region;
+    if (total >= 0xffffffffull)
+      throw Error(GD_INVALID_ARGUMENT, "FliX node pool exceeds 2^32-1 nodes");
+    // new structure first, then release the old one (rebuild reads it)
+    K *keys = pool_.get_n<K>(total * S_);
+    V *vals = pool_.get_n<V>(total * S_);
+    u32 *count = pool_.get_n<u32>(total);
+    u32 *next = pool_.get_n<u32>(total);
+    K *maxv = pool_.get_n<K>(P);
+    u32 *ring = pool_.get_n<u32>(region);
+    RingCounters *ctr = pool_.get_n<RingCounters>(1);
+    release_structure();
+    node_keys_ = keys;
+    node_vals_ = vals;
+    node_count_ = count;
+    node_next_ = next;
+    maxv_ = maxv;
+    ring_ids_ = ring;
+    counters_ = ctr;
+    P_ = static_cast<u32>(P);
+    total_nodes_ = total;
+    ring_cap_ = region;
+    // allocation-region headers start empty
+    rs::zero_u32_kernel<<<grid_for(total, 256, 1u << 16), 256, 0, s>>>(
+        node_count_, total);
+    rs::zero_u32_kernel<<<grid_for(total, 256, 1u << 16), 256, 0, s>>>(
+        node_next_, total);
+    Pool<K, V> pv = pool_view();
+    layout_kernel<K, V><<<grid_for(P * S_, 256, 1u << 16), 256, 0, s>>>(
+        pv, maxv_, uk, uv, m, fill);
+    ring_init_kernel<<<grid_for(region, 256, 1u << 16), 256, 0, s>>>(
+        ring_ids_, P, region);
+    ring_counters_init_kernel<<<1, 1, 0, s>>>(counters_, region);
+    GD_LAUNCH_CHECK();
+  }
+
+  DevicePool pool_;
+  OperationOrder order_;
+  u32 S_;
+  u32 fill_percent_;
+  u64 max_size_ = 0;
+  u32 P_ = 0;
+  u64 total_nodes_ = 0;
+  u64 ring_cap_ = 0;
+  K *node_keys_ = nullptr;
+  V *node_vals_ = nullptr;
+  u32 *node_count_ = nullptr;
+  u32 *node_next_ = nullptr;
+  K *maxv_ = nullptr;
+  u32 *ring_ids_ = nullptr;
+  RingCounters *counters_ = nullptr;
+};
+
+}  // namespace flix
+}  // namespace gdict
+
+using gdict::guarded;
+using gdict::flix::FlixBase;
+
+// The error header (HandleBase) sits at offset 0 of both handle types, which
+// is what gd_last_error_message relies on.
+struct gd_flix : gdict::HandleBase {
+  std::unique_ptr<FlixBase> impl;
+};
+
+extern "C" {
+
+gd_status gd_flix_create(const gd_flix_config *config, gd_flix **out) {
+  if (!config || !out) {
+    gdict::set_create_error("null config or output handle");
+    return GD_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    const uint32_t kb = config->key_bytes, vb = config->value_bytes;
+    const uint32_t slots = config->node_slots ? config->node_slots : 32;
+    const uint32_t fill = config->fill_percent ? config->fill_percent : 50;
+    if ((kb != 4 && kb != 8) || (vb != 4 && vb != 8))
+      throw gdict::Error(GD_INVALID_ARGUMENT,
+                         "key_bytes and value_bytes must be 4 or 8");
+    if (slots < 8 || slots > 64 || fill > 100)
+      throw gdict::Error(GD_INVALID_ARGUMENT,
+                         "node_slots must be in [8, 64], fill_percent <= 100");
+    auto h = std::make_unique<gd_flix>();
+    using gdict::u32;
+    using gdict::u64;
+    using gdict::flix::FlixImpl;
+    if (kb == 4 && vb == 4) h->impl.reset(new FlixImpl<u32, u32>(slots, fill));
+    else if (kb == 4) h->impl.reset(new FlixImpl<u32, u64>(slots, fill));
+    else if (vb == 4) h->impl.reset(new FlixImpl<u64, u32>(slots, fill));
+    else h->impl.reset(new FlixImpl<u64, u64>(slots, fill));
+    *out = h.release();
+  });
+}
+
+gd_status gd_flix_destroy(gd_flix *h) {
+  delete h;
+  return GD_OK;
+}
+
+#define GD_FLIX_CALL(h, body)                                   \
+  do {                                                          \
+    if (!(h) || !(h)->impl) return GD_INVALID_ARGUMENT;         \
+    FlixBase *f = (h)->impl.get();                              \
+    std::lock_guard<std::mutex> lock__((h)->mutex);             \
+    return guarded((h), [&] { body; });                         \
+  } while (0)
+
+gd_status gd_flix_build(gd_flix *h, const void *keys, uint64_t count,
+                        uint64_t max_size, hipStream_t s) {
+  GD_FLIX_CALL(h, f->build(keys, count, max_size, s));
+}
+
+gd_status gd_flix_insert(gd_flix *h, const void *keys, const void *values,
+                         uint64_t count, hipStream_t s) {
+  GD_FLIX_CALL(h, f->insert(keys, values, count, s));
+}
+
+gd_status gd_flix_remove(gd_flix *h, const void *keys, uint64_t count,
+                         hipStream_t s) {
+  GD_FLIX_CALL(h, f->remove(keys, count, s));
+}
+
+gd_status gd_flix_lookup_sorted(gd_flix *h, const void *sorted_queries,
+                                uint64_t count, void *out_values,
+                                hipStream_t s) {
+  GD_FLIX_CALL(h, f->lookup(sorted_queries, count, out_values, s));
+}
+
+gd_status gd_flix_lookup(gd_flix *h, const void *queries, uint64_t count,
+                         void *out_values, hipStream_t s) {
+  GD_FLIX_CALL(h, f->lookup(queries, count, out_values, s));
+}
+
+gd_status gd_flix_successor_sorted(gd_flix *h, const void *sorted_queries,
+                                   uint64_t count, void *out_keys,
+                                   hipStream_t s) {
+  GD_FLIX_CALL(h, f->successor(sorted_queries, count, out_keys, s));
+}
+
+gd_status gd_flix_rebuild(gd_flix *h, hipStream_t s) {
+  GD_FLIX_CALL(h, f->rebuild(s));
+}
+
+gd_status gd_flix_range(gd_flix *h, const void *lo, const void *hi,
+                        uint64_t count, uint64_t *out_offsets, void *out_keys,
+                        void *out_values, hipStream_t s) {
+  GD_FLIX_CALL(h, f->range(lo, hi, count, out_offsets, out_keys, out_values,
+                           s));
+}
+
+gd_status gd_flix_total_keys(gd_flix *h, uint64_t *out_keys, hipStream_t s) {
+  if (!out_keys) return GD_INVALID_ARGUMENT;
+  GD_FLIX_CALL(h, *out_keys = f->total_keys(s));
+}
+
+gd_status gd_flix_resident_bytes(gd_flix *h, uint64_t *out_bytes) {
+  if (!out_bytes) return GD_INVALID_ARGUMENT;
+  GD_FLIX_CALL(h, *out_bytes = f->resident_bytes());
+}
+
+}  // extern "C"

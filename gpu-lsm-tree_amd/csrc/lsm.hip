@@ -307,7 +307,7 @@ class Lsm : public HandleBase {
     zero_two_kernel<<<grid_for(fin_words + 4, 256, 1024), 256, 0, s>>>(
         reinterpret_cast<u32 *>(r.meta()), 4, fin_ctrl, fin_words);
     GD_LAUNCH_CHECK();
-    GD_HIP(rs::sort_pairs(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
+    GD_HIP(rs::sort_pairs<u32>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
                           ctrl.get(), s));
     finalize_batch_kernel<<<fin_tiles, kFinThreads, 0, s>>>(
         ka.get(), va.get(), n, values, flags, mode, r.keys, r.vals, r.meta(),

@@ -115,7 +115,7 @@ enum FinalizeMode : int {
 // Turns a stably sorted batch (keys sk, input positions sp) into a run:
 // keep the last occurrence of each key (the newest), gather its value and
 // tombstone flag, encode, and compact with a single-pass chained scan.
-__global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
+static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
     const u32 *__restrict__ sk, const u32 *__restrict__ sp, u32 n,
     const u32 *__restrict__ values, const u8 *__restrict__ erase_flags,
     int mode, u32 *__restrict__ out_keys, u32 *__restrict__ out_vals,
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
 
 // Builds the radix directory of a run from its final count/split.  Launch
 // over max(count_ub, 2^max_bits + 1) threads (grid-stride).
-__global__ void build_directory_kernel(const u32 *__restrict__ keys,
+static __global__ void build_directory_kernel(const u32 *__restrict__ keys,
                                        RunMeta *meta, u32 *__restrict__ dir,
                                        u32 max_bits) {
   const u32 count = meta->count;
@@ -254,7 +254,7 @@ __device__ __forceinline__ u32 merge_path(const MergeSide &a, u32 ca, u32 sa,
   return static_cast<u32>(lo);
 }
 
-__global__ void merge_partition_kernel(MergeSide a, MergeSide b, u32 tiles,
+static __global__ void merge_partition_kernel(MergeSide a, MergeSide b, u32 tiles,
                                        u32 *__restrict__ parts) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > tiles) return;
@@ -268,7 +268,7 @@ __global__ void merge_partition_kernel(MergeSide a, MergeSide b, u32 tiles,
 // Merges run A (newer) with run B (older) into `out`: on equal keys the A
 // row survives, and tombstones are dropped when `drop_tombstones` (the
 // output becomes the oldest run, GPULSMOpt.cuh:3175-3179).
-__global__ __launch_bounds__(kMergeThreads) void merge_kernel(
+static __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     MergeSide a, MergeSide b, const u32 *__restrict__ parts,
     u32 *__restrict__ out_keys, u32 *__restrict__ out_vals, RunMeta *out_meta,
     int drop_tombstones, u64 *status, u32 *ticket) {
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
 }
 
 // --------------------------------------------------------------- queries
-__global__ void lookup_kernel(const u32 *__restrict__ queries, u64 n,
+static __global__ void lookup_kernel(const u32 *__restrict__ queries, u64 n,
                               u32 *__restrict__ out_values,
                               u8 *__restrict__ out_found, RunSet runs) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
@@ -449,7 +449,7 @@ __global__ void lookup_kernel(const u32 *__restrict__ queries, u64 n,
 constexpr int kRangeThreads = 256;
 constexpr int kRangeWaves = kRangeThreads / kWave;
 
-__global__ __launch_bounds__(kRangeThreads) void range_kernel(
+static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
     const u32 *__restrict__ lo, const u32 *__restrict__ hi, u64 n,
     u32 *__restrict__ out, int count_only, RunSet runs) {
   __shared__ u32 s_lo[kRangeWaves][kMaxRuns];
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(
   }
 }
 
-__global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
+static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
                                  u32 *__restrict__ out, RunSet runs) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -558,7 +558,7 @@ __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
 }
 
 // Writes a tombstone-free run as plain (key, value) arrays.
-__global__ void decode_run_kernel(const u32 *__restrict__ keys,
+static __global__ void decode_run_kernel(const u32 *__restrict__ keys,
                                   const u32 *__restrict__ vals,
                                   const RunMeta *meta, u32 *__restrict__ ok,
                                   u32 *__restrict__ ov) {
@@ -572,7 +572,7 @@ __global__ void decode_run_kernel(const u32 *__restrict__ keys,
   }
 }
 
-__global__ void zero_words_kernel(u32 *p, u64 n) {
+static __global__ void zero_words_kernel(u32 *p, u64 n) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
     p[i] = 0;

@@ -20,22 +20,29 @@ constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;  // 4096
 constexpr int kRadix = 256;
-constexpr int kPasses = 4;
+constexpr int kMaxPasses = 8;
 constexpr int kWaves = kThreads / kWave;
 
-// Global histogram of all four digits.  hist is [kPasses][kRadix], zeroed.
-__global__ __launch_bounds__(kThreads) void histogram_kernel(
-    const u32 *__restrict__ keys, u32 n, u32 *__restrict__ hist) {
+template <class K>
+constexpr int passes_for() {
+  return static_cast<int>(sizeof(K));
+}
+
+// Global histogram of every digit.  hist is [passes][kRadix], zeroed.
+template <class K>
+static __global__ __launch_bounds__(kThreads) void histogram_kernel(
+    const K *__restrict__ keys, u32 n, u32 *__restrict__ hist) {
+  constexpr int kPasses = passes_for<K>();
   __shared__ u32 s_hist[kPasses][kRadix];
   for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads)
     (&s_hist[0][0])[i] = 0;
   __syncthreads();
   const u64 stride = u64(gridDim.x) * kThreads;
   for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
-    const u32 k = keys[i];
+    const K k = keys[i];
 #pragma unroll
     for (int p = 0; p < kPasses; ++p)
-      atomicAdd(&s_hist[p][(k >> (8 * p)) & 0xffu], 1u);
+      atomicAdd(&s_hist[p][static_cast<u32>(k >> (8 * p)) & 0xffu], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads) {
@@ -45,10 +52,10 @@ __global__ __launch_bounds__(kThreads) void histogram_kernel(
 }
 
 // Exclusive scan of each pass's histogram -> global digit offsets.
-__global__ __launch_bounds__(kRadix) void scan_kernel(
-    const u32 *__restrict__ hist, u32 *__restrict__ offsets) {
+static __global__ __launch_bounds__(kRadix) void scan_kernel(
+    const u32 *__restrict__ hist, u32 *__restrict__ offsets, int passes) {
   __shared__ u32 scratch[kRadix / kWave + 1];
-  for (int p = 0; p < kPasses; ++p) {
+  for (int p = 0; p < passes; ++p) {
     const u32 v = hist[p * kRadix + threadIdx.x];
     const u32 ex = block_exclusive_scan<kRadix>(v, scratch);
     offsets[p * kRadix + threadIdx.x] = ex;
@@ -58,10 +65,10 @@ __global__ __launch_bounds__(kRadix) void scan_kernel(
 
 // One digit pass.  status has tiles*kRadix words and must be zero;
 // *ticket must be zero.  kFirst: the payload is generated (= position).
-template <bool kFirst>
-__global__ __launch_bounds__(kThreads) void onesweep_kernel(
-    const u32 *__restrict__ keys_in, const u32 *__restrict__ vals_in,
-    u32 *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
+template <class K, bool kFirst>
+static __global__ __launch_bounds__(kThreads) void onesweep_kernel(
+    const K *__restrict__ keys_in, const u32 *__restrict__ vals_in,
+    K *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
     u32 shift, const u32 *__restrict__ digit_offsets, u64 *status,
     u32 *ticket) {
   __shared__ u32 s_tile;
@@ -69,7 +76,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   __shared__ u32 s_block_start[kRadix];
   __shared__ u32 s_global_base[kRadix];
   __shared__ u32 s_scan[kWaves + 1];
-  __shared__ u32 s_keys[kTile];
+  __shared__ K s_keys[kTile];
   __shared__ u32 s_vals[kTile];
 
   const u32 tid = threadIdx.x;
@@ -82,13 +89,14 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   const u32 tile = s_tile;
   const u64 base = u64(tile) * kTile;
 
-  u32 key[kItems], val[kItems], rank[kItems];
+  K key[kItems];
+  u32 val[kItems], rank[kItems];
   const u64 wave_base = base + u64(wave) * (kWave * kItems);
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const u64 idx = wave_base + u64(j) * kWave + lane;
     const bool valid = idx < n;
-    key[j] = valid ? keys_in[idx] : 0xffffffffu;
+    key[j] = valid ? keys_in[idx] : ~K(0);
     if constexpr (kFirst) {
       val[j] = static_cast<u32>(idx);
     } else {
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   for (int j = 0; j < kItems; ++j) {
     const u64 idx = wave_base + u64(j) * kWave + lane;
     const bool valid = idx < n;
-    const u32 d = (key[j] >> shift) & 0xffu;
+    const u32 d = static_cast<u32>(key[j] >> shift) & 0xffu;
     u64 match = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -161,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   for (int j = 0; j < kItems; ++j) {
     const u64 idx = wave_base + u64(j) * kWave + lane;
     if (idx < n) {
-      const u32 dj = (key[j] >> shift) & 0xffu;
+      const u32 dj = static_cast<u32>(key[j] >> shift) & 0xffu;
       const u32 lp = s_block_start[dj] + s_wave_hist[wave][dj] + rank[j];
       s_keys[lp] = key[j];
       s_vals[lp] = val[j];
@@ -173,14 +181,14 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(
       base >= n ? 0u : static_cast<u32>(n - base < u64(kTile) ? n - base
                                                               : u64(kTile));
   for (u32 i = tid; i < count; i += kThreads) {
-    const u32 k = s_keys[i];
-    const u32 pos = s_global_base[(k >> shift) & 0xffu] + i;
+    const K k = s_keys[i];
+    const u32 pos = s_global_base[static_cast<u32>(k >> shift) & 0xffu] + i;
     keys_out[pos] = k;
     vals_out[pos] = s_vals[i];
   }
 }
 
-__global__ void zero_u32_kernel(u32 *p, u64 n) {
+static __global__ void zero_u32_kernel(u32 *p, u64 n) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
     p[i] = 0;
@@ -190,41 +198,45 @@ inline u32 tiles_for(u64 n) { return static_cast<u32>((n + kTile - 1) / kTile); 
 
 // Workspace words (u32) needed by sort_pairs for n keys, not counting the
 // two ping-pong buffers.
-inline u64 control_words(u64 n) {
-  // hist[4*256] + offsets[4*256] + tickets[4] + pad + status[4][tiles*256] u64
-  return 2048 + 64 + 2ull * kPasses * u64(tiles_for(n)) * kRadix;
+inline u64 control_words(u64 n, int passes = 4) {
+  // hist[8*256] + offsets[8*256] + tickets[64] + status[passes][tiles*256]
+  return 4096 + 64 + 2ull * passes * u64(tiles_for(n)) * kRadix;
 }
 
-// Sorts n (key, position) pairs.  keys_in is read only; the sorted keys and
-// positions end in (keys_a, vals_a).  keys_b/vals_b are scratch.  `control`
-// holds control_words(n) u32 words.  All launches go to `stream`.
-inline hipError_t sort_pairs(const u32 *keys_in, u32 n, u32 *keys_a,
-                             u32 *vals_a, u32 *keys_b, u32 *vals_b,
-                             u32 *control, hipStream_t stream) {
+// Sorts n (key, position) pairs stably.  keys_in is read only; the sorted
+// keys and positions end in (keys_a, vals_a) (an even number of passes).
+// keys_b/vals_b are scratch.  `control` holds control_words(n, passes) u32
+// words.  All launches go to `stream`.
+template <class K>
+inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
+                             K *keys_b, u32 *vals_b, u32 *control,
+                             hipStream_t stream) {
+  constexpr int kPasses = passes_for<K>();
   if (n == 0) return hipSuccess;
   const u32 tiles = tiles_for(n);
   u32 *hist = control;
-  u32 *offsets = control + 1024;
-  u32 *tickets = control + 2048;
-  u64 *status = reinterpret_cast<u64 *>(control + 2048 + 64);
-  const u64 zero_words = control_words(n);
+  u32 *offsets = control + 2048;
+  u32 *tickets = control + 4096;
+  u64 *status = reinterpret_cast<u64 *>(control + 4096 + 64);
+  const u64 zero_words = control_words(n, kPasses);
   zero_u32_kernel<<<static_cast<u32>(std::min<u64>((zero_words + 255) / 256,
                                                    4096)),
                     256, 0, stream>>>(control, zero_words);
   const u32 hist_blocks = static_cast<u32>(
       std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 1024));
-  histogram_kernel<<<hist_blocks, kThreads, 0, stream>>>(keys_in, n, hist);
-  scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets);
-  const u32 *src_k = keys_in;
+  histogram_kernel<K><<<hist_blocks, kThreads, 0, stream>>>(keys_in, n, hist);
+  scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets, kPasses);
+  const K *src_k = keys_in;
   const u32 *src_v = nullptr;
-  u32 *dst_k = keys_b, *dst_v = vals_b;
+  K *dst_k = keys_b;
+  u32 *dst_v = vals_b;
   for (int p = 0; p < kPasses; ++p) {
     u64 *pass_status = status + u64(p) * tiles * kRadix;
     if (p == 0) {
-      onesweep_kernel<true><<<tiles, kThreads, 0, stream>>>(
+      onesweep_kernel<K, true><<<tiles, kThreads, 0, stream>>>(
           src_k, src_v, dst_k, dst_v, n, 0, offsets, pass_status, tickets);
     } else {
-      onesweep_kernel<false><<<tiles, kThreads, 0, stream>>>(
+      onesweep_kernel<K, false><<<tiles, kThreads, 0, stream>>>(
           src_k, src_v, dst_k, dst_v, n, 8u * p, offsets + p * kRadix,
           pass_status, tickets + p);
     }

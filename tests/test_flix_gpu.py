@@ -1,0 +1,204 @@
+"""GPU parity of the FliX updatable index (libgdict.so) against the oracle,
+plus the known answer of the reference's interface test (FliX/main.cu:98-156)."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KT = {4: np.uint32, 8: np.uint64}
+
+
+class FlixPair:
+    def __init__(self, gd, orc, io, kb, vb, **cfg):
+        self.f = gd.FlixIndex(key_bytes=kb, value_bytes=vb, **cfg)
+        self.o = orc.FlixOracle(kb, vb)
+        self.io = io
+        self.kb, self.vb = kb, vb
+        self.kt, self.vt = KT[kb], KT[vb]
+
+    def up_k(self, a):
+        return self.io.up(np.asarray(a, self.kt), self.kt)
+
+    def up_v(self, a):
+        return self.io.up(np.asarray(a, self.vt), self.vt)
+
+    def build(self, keys, max_size=0):
+        self.f.build_bucket_layer_only(self.up_k(keys), max_size=max_size,
+                                       count=len(keys))
+        self.o.build(keys)
+
+    def insert(self, keys, vals):
+        self.f.insert(self.up_k(keys), self.up_v(vals), count=len(keys))
+        self.o.insert(keys, vals)
+
+    def remove(self, keys):
+        self.f.remove(self.up_k(keys), count=len(keys))
+        self.o.remove(keys)
+
+    def lookup(self, q, sorted_api=False):
+        out = self.io.empty(len(q), self.vb)
+        if sorted_api:
+            self.f.lookups_ordered(self.up_k(q), out, count=len(q))
+        else:
+            self.f.lookup(self.up_k(q), out, count=len(q))
+        return self.io.down(out, len(q), self.vt)
+
+    def successor(self, q):
+        out = self.io.empty(len(q), self.kb)
+        self.f.lookups_successor(self.up_k(q), out, count=len(q))
+        return self.io.down(out, len(q), self.kt)
+
+    def range(self, lo, hi):
+        n = len(lo)
+        import torch
+        offs = torch.zeros(n + 1, dtype=torch.int64, device=self.io.device)
+        dlo, dhi = self.up_k(lo), self.up_k(hi)
+        self.f.range(dlo, dhi, offs, count=n)
+        o = self.io.down(offs, n + 1, np.uint64)
+        total = int(o[-1])
+        ok = self.io.empty(total, self.kb)
+        ov = self.io.empty(total, self.vb)
+        self.f.range(dlo, dhi, offs, ok, ov, count=n)
+        return (o, self.io.down(ok, total, self.kt),
+                self.io.down(ov, total, self.vt))
+
+    def check_all(self, rng, span_lo, span_hi, nq=4000):
+        ek, ev = self.o.dump()
+        # full enumeration through the range API = sorted contents
+        o, k, v = self.range(np.array([0], self.kt),
+                             np.array([np.iinfo(self.kt).max], self.kt))
+        assert np.array_equal(k, ek)
+        assert np.array_equal(v, ev)
+        assert self.f.compute_total_size() == len(ek)
+        q = rng.integers(span_lo, span_hi, nq, dtype=np.uint64).astype(self.kt)
+        if len(ek):
+            q[: nq // 2] = ek[rng.integers(0, len(ek), nq // 2)]
+        assert np.array_equal(self.lookup(q), self.o.lookup(q))
+        qs = np.sort(q)
+        assert np.array_equal(self.lookup(qs, True), self.o.lookup(qs))
+        assert np.array_equal(self.successor(qs), self.o.successor(qs))
+        lo = np.sort(rng.integers(span_lo, span_hi, 300, dtype=np.uint64)
+                     .astype(self.kt))
+        w = rng.integers(0, max((span_hi - span_lo) // 200, 2), 300,
+                         dtype=np.uint64)
+        hi = np.minimum(lo.astype(np.uint64) + w,
+                        np.uint64(np.iinfo(self.kt).max)).astype(self.kt)
+        lo[:10], hi[:10] = hi[:10], lo[:10]
+        go, gk, gv = self.range(lo, hi)
+        eo, ekk, evv = self.o.range(lo, hi)
+        assert np.array_equal(go, eo)
+        assert np.array_equal(gk, ekk)
+        assert np.array_equal(gv, evv)
+
+
+def test_interface_known_answer(gd, io):
+    """FliX/main.cu:98-156, expectations derived by hand: build values are
+    input positions, inserts skip nothing new, deletes remove six keys."""
+    f = gd.FlixIndex(key_bytes=4, value_bytes=4)
+    build = np.array([100, 90, 80, 70, 60, 50, 40, 30, 20, 10], np.uint32)
+    ins = np.arange(5, 106, 10, dtype=np.uint32)
+    offs = np.arange(10, 21, dtype=np.uint32)
+    dels = np.array([30, 35, 40, 45, 75, 80], np.uint32)
+    probes = np.arange(5, 106, 5, dtype=np.uint32)
+    NF = 0xFFFFFFFF
+
+    def show():
+        out = io.empty(len(probes))
+        f.lookups_ordered(io.up(probes), out, count=len(probes))
+        return io.down(out, len(probes))
+
+    f.build_bucket_layer_only(io.up(build), max_size=21, count=10)
+    exp = np.array([NF if p % 10 else (100 - p) // 10 for p in probes],
+                   np.uint32)
+    assert np.array_equal(show(), exp)
+    f.insert(io.up(ins), io.up(offs), count=len(ins))
+    exp = np.array([(100 - p) // 10 if p % 10 == 0 else 10 + (p - 5) // 10
+                    for p in probes], np.uint32)
+    assert np.array_equal(show(), exp)
+    f.remove(io.up(dels), count=len(dels))
+    exp[np.isin(probes, dels)] = NF
+    assert np.array_equal(show(), exp)
+    assert f.compute_total_size() == 21 - 6
+
+
+@pytest.mark.parametrize("kb,vb", [(4, 4), (8, 8), (4, 8), (8, 4)])
+def test_random_updates_parity(gd, orc, io, kb, vb):
+    rng = np.random.default_rng(kb * 10 + vb)
+    kt, vt = KT[kb], KT[vb]
+    hi_key = (1 << 32) - 4 if kb == 4 else (1 << 62)
+    p = FlixPair(gd, orc, io, kb, vb)
+    keys = rng.integers(2, hi_key, 20000, dtype=np.uint64).astype(kt)
+    keys[:500] = keys[500:1000]  # duplicates in the build input
+    p.build(keys, max_size=200000)
+    p.check_all(rng, 2, hi_key)
+    for step in range(8):
+        n = int(rng.integers(1, 6000))
+        ins = rng.integers(2, hi_key, n, dtype=np.uint64).astype(kt)
+        live = p.o.dump()[0]
+        if len(live):
+            ins[: n // 4] = live[rng.integers(0, len(live), n // 4)]  # present
+        ins[n // 4: n // 4 + n // 8] = ins[:n // 8]  # in-batch duplicates
+        vals = rng.integers(0, 1 << 31, n, dtype=np.uint64).astype(vt)
+        p.insert(ins, vals)
+        live = p.o.dump()[0]
+        m = int(rng.integers(1, 5000))
+        dels = rng.integers(2, hi_key, m, dtype=np.uint64).astype(kt)
+        dels[: m * 3 // 4] = live[rng.integers(0, len(live), m * 3 // 4)]
+        p.remove(dels)
+        p.check_all(rng, 2, hi_key)
+    p.f.rebuild_bucket_layer_only()
+    p.check_all(rng, 2, hi_key)
+    p.insert(rng.integers(2, hi_key, 3000, dtype=np.uint64).astype(kt),
+             np.arange(3000, dtype=vt))
+    p.check_all(rng, 2, hi_key)
+
+
+def test_clustered_runs_and_skewed_inserts(gd, orc, io):
+    """Config 3 shape, scaled: clustered runs of consecutive uint64 keys at
+    random offsets; inserts concentrated into one bucket (node splits over
+    many pieces); deletes that empty whole buckets."""
+    rng = np.random.default_rng(3)
+    p = FlixPair(gd, orc, io, 8, 8, node_slots=32, fill_percent=50)
+    starts = rng.integers(2, 1 << 62, 64, dtype=np.uint64)
+    keys = (starts[:, None] + np.arange(256, dtype=np.uint64)).ravel()
+    p.build(keys, max_size=1 << 16)
+    p.check_all(rng, 2, 1 << 62)
+    # 10000 new keys inside one run's gap: all land in a single bucket
+    base = int(starts[0]) + 300
+    dense = (np.uint64(base) + np.arange(0, 20000, 2, dtype=np.uint64))
+    p.insert(dense, dense * np.uint64(3))
+    p.check_all(rng, 2, 1 << 62)
+    # delete two whole runs and half of the dense block
+    p.remove(np.concatenate([keys[:512], dense[::2]]))
+    p.check_all(rng, 2, 1 << 62)
+    p.f.rebuild_bucket_layer_only()
+    p.check_all(rng, 2, 1 << 62)
+
+
+def test_allocation_region_exhaustion(gd, orc, io):
+    """A too-small max_size makes an insert fail with CapacityExhausted;
+    the index stays a consistent sorted set of old and new keys."""
+    rng = np.random.default_rng(9)
+    f = gd.FlixIndex(key_bytes=4, value_bytes=4, node_slots=16)
+    keys = np.arange(2, 4002, dtype=np.uint32) * 16
+    f.build_bucket_layer_only(io.up(keys), max_size=4000, count=len(keys))
+    ins = rng.integers(2, 70000, 60000, dtype=np.uint64).astype(np.uint32)
+    with pytest.raises(gd.CapacityExhausted):
+        f.insert(io.up(ins), io.up(ins), count=len(ins))
+    import torch
+    offs = torch.zeros(2, dtype=torch.int64, device=io.device)
+    lo, hi = io.up(np.array([0], np.uint32)), io.up(np.array([0xFFFFFFFF],
+                                                             np.uint32))
+    f.range(lo, hi, offs, count=1)
+    total = int(io.down(offs, 2, np.uint64)[1])
+    ok, ov = io.empty(total), io.empty(total)
+    f.range(lo, hi, offs, ok, ov, count=1)
+    k = io.down(ok, total)
+    assert np.all(np.diff(k.astype(np.int64)) > 0)
+    assert set(keys.tolist()) <= set(k.tolist())
+    assert set(k.tolist()) <= set(keys.tolist()) | set(ins.tolist())
+    assert f.compute_total_size() == total
+    # after a rebuild with room, inserts succeed again
+    f.rebuild_bucket_layer_only()
+    f.insert(io.up(ins[:100]), io.up(ins[:100]), count=100)
