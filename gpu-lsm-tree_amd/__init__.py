@@ -95,6 +95,31 @@ class _FlixConfig(ctypes.Structure):
     ]
 
 
+class _ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("lsm", _Config),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("max_step_keys", ctypes.c_uint64),
+        ("max_inflight_steps", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class _ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("requests", ctypes.c_uint64),
+        ("steps", ctypes.c_uint64),
+        ("coalesced_updates", ctypes.c_uint64),
+        ("coalesced_queries", ctypes.c_uint64),
+        ("latency_p50_us", ctypes.c_double),
+        ("latency_p99_us", ctypes.c_double),
+        ("latency_p999_us", ctypes.c_double),
+        ("latency_max_us", ctypes.c_double),
+        ("peak_staged_bytes", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64),
+    ]
+
+
 _P = ctypes.c_void_p
 _U64 = ctypes.c_uint64
 _I = ctypes.c_int
@@ -117,6 +142,13 @@ SIGNATURES = {
     "gd_lsm_carry_status": [_P, ctypes.POINTER(ctypes.c_uint32)],
     "gd_lsm_resident_bytes": [_P, ctypes.POINTER(_U64)],
     "gd_lsm_shape": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64)],
+    "gd_serve_create": [ctypes.POINTER(_ServeConfig), ctypes.POINTER(_P)],
+    "gd_serve_destroy": [_P],
+    "gd_serve_submit": [_P, ctypes.c_int, _P, _P, _U64, _P, _P,
+                        ctypes.POINTER(_U64)],
+    "gd_serve_wait": [_P, _U64, ctypes.POINTER(_U64)],
+    "gd_serve_drain": [_P],
+    "gd_serve_get_stats": [_P, ctypes.POINTER(_ServeStats), _I],
     "gd_flix_create": [ctypes.POINTER(_FlixConfig), ctypes.POINTER(_P)],
     "gd_flix_destroy": [_P],
     "gd_flix_build": [_P, _P, _U64, _U64, _P],
@@ -135,8 +167,9 @@ _lib = None
 
 
 def build(verbose=False):
-    """Compiles libgdict.so in-tree for gfx950 (hipcc, no GPU needed)."""
-    cmd = ["make", "-s", "-j8", "-C", _HERE]
+    """Compiles libgdict.so in-tree for gfx950 (hipcc, no GPU needed).
+    Always recompiles (make -B), so a call is a real build check."""
+    cmd = ["make", "-B", "-s", "-j8", "-C", _HERE]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("building libgdict.so failed:\n" + res.stdout +
@@ -287,6 +320,66 @@ class GPULSMOpt:
         rows = _U64(0)
         self._call("gd_lsm_shape", ctypes.byref(runs), ctypes.byref(rows))
         return int(runs.value), int(rows.value)
+
+
+class Server:
+    """Continuous-batching engine over one LSM dictionary (gd_serve_*).
+
+    submit() returns a ticket; the request's device buffers must stay alive
+    until wait(ticket) returns (this wrapper keeps references until then).
+    """
+
+    INSERT, ERASE, LOOKUP, RANGE_COUNT, RANGE_SUM = range(5)
+
+    def __init__(self, max_elements=0, batch_capacity=0,
+                 level_zero_capacity=0, hbm_budget_bytes=0, max_step_keys=0,
+                 max_inflight_steps=0):
+        cfg = _ServeConfig(
+            _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0),
+            hbm_budget_bytes, max_step_keys, max_inflight_steps, 0)
+        h = _P()
+        _check(lib().gd_serve_create(ctypes.byref(cfg), ctypes.byref(h)),
+               None)
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gd_serve_destroy(self._h)
+            self._h = None
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, op, a, b=None, out=None, out_found=None, count=None):
+        t = _U64(0)
+        _check(lib().gd_serve_submit(self._h, int(op), _ptr(a), _ptr(b),
+                                     _count(a, count), _ptr(out),
+                                     _ptr(out_found), ctypes.byref(t)),
+               self._h)
+        self._keep[t.value] = (a, b, out, out_found)
+        return int(t.value)
+
+    def wait(self, ticket):
+        """Returns the step that served the request."""
+        step = _U64(0)
+        st = lib().gd_serve_wait(self._h, int(ticket), ctypes.byref(step))
+        self._keep.pop(int(ticket), None)
+        _check(st, self._h)
+        return int(step.value)
+
+    def drain(self):
+        _check(lib().gd_serve_drain(self._h), self._h)
+
+    def stats(self, reset=False):
+        s = _ServeStats()
+        _check(lib().gd_serve_get_stats(self._h, ctypes.byref(s),
+                                        1 if reset else 0), self._h)
+        return {f: getattr(s, f) for f, _ in _ServeStats._fields_}
 
 
 class FlixIndex:

@@ -1,25 +1,7 @@
-// Host runtime of the LSM dictionary and its C ABI (gd_lsm_*).
-//
-// Replaces class GPULSMOpt (GPULSMOpt.cuh:5599-7805).  The observable
-// contract is the reference's; the structure is this library's own:
-//   * every admitted tile (<= batch_capacity keys) is sorted on the GPU by a
-//     onesweep LDS radix sort and becomes an immutable sorted run, so reads
-//     never scan unsorted pending rows;
-//   * runs merge pairwise (merge-path partitioned, streamed through LDS)
-//     whenever the older neighbour is no larger than the newer one, which is
-//     a binary counter for equal batches: log2(N/b) merges per row and at
-//     most log2(N/b) + 1 runs to search;
-//   * the reference's epoch of 16 batches survives as the capacity check
-//     point (GPULSMOpt.cuh:6971-6972, :7350-7363).
-#include <algorithm>
-#include <cstring>
-#include <memory>
+// C ABI of the LSM dictionary (gd_lsm_*); the runtime is lsm_runtime.hpp.
 #include <string>
-#include <vector>
 
-#include "host_common.hpp"
-#include "lsm_kernels.hpp"
-#include "radix_sort.hpp"
+#include "lsm_runtime.hpp"
 
 namespace gdict {
 
@@ -29,418 +11,6 @@ thread_local std::string g_create_error;
 void set_create_error(const std::string &m) { g_create_error = m; }
 const char *create_error() { return g_create_error.c_str(); }
 
-namespace lsm {
-
-constexpr u64 kMaximumOperationTile = u64(1) << 20;  // GPULSMOpt.cuh:56
-constexpr u32 kBatchesPerEpoch = 16;                  // GPULSMOpt.cuh:42
-constexpr u64 kMaximumRows = 0xffffffffull;           // GPULSMOpt.cuh:59-60
-
-struct Run {
-  u32 *keys = nullptr;
-  u32 *vals = nullptr;
-  u32 *dir = nullptr;  // RunMeta header, then the directory entries
-  u64 rows_ub = 0;     // upper bound on the rows (exact after a refresh)
-  u32 max_bits = 1;
-
-  RunMeta *meta() const { return reinterpret_cast<RunMeta *>(dir); }
-  u32 *entries() const { return dir + kMetaBytes / sizeof(u32); }
-  RunView view() const { return {keys, vals, entries(), meta()}; }
-};
-
-__global__ void zero_two_kernel(u32 *a, u64 na, u32 *b, u64 nb) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb;
-       i += stride) {
-    if (i < na) a[i] = 0;
-    else b[i - na] = 0;
-  }
-}
-
-class Lsm : public HandleBase {
- public:
-  // GPULSMOpt.cuh:5606-5616 and :255-270
-  static u64 batch_capacity_for(const gd_config &c) {
-    return std::min<u64>(kMaximumOperationTile,
-                         std::max<u64>(1, c.batch_capacity));
-  }
-  static u64 capacity_for(const gd_config &c) {
-    const u64 cap =
-        std::max<u64>(c.max_elements, batch_capacity_for(c) * kBatchesPerEpoch);
-    if (cap > kMaximumRows)
-      throw Error(GD_INVALID_ARGUMENT,
-                  "GPULSMOpt capacity exceeds 32-bit key space");
-    return cap;
-  }
-
-  explicit Lsm(const gd_config &c)
-      : batch_capacity_(batch_capacity_for(c)), capacity_(capacity_for(c)) {}
-
-  ~Lsm() {
-    try {
-      order_.wait_host();
-    } catch (...) {
-    }
-    for (auto &r : runs_) free_run(r);
-    runs_.clear();
-  }
-
-  // ------------------------------------------------------------- updates
-  void clear(hipStream_t s) {
-    order_.begin(s);
-    drop_all_runs();
-    reset_state();
-    order_.end(s);
-  }
-
-  void bulk_build(const u32 *keys, const u32 *values, u64 n, hipStream_t s) {
-    if ((n && (!keys || !values)) || n > kMaximumRows)
-      throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt initial input");
-    order_.begin(s);
-    drop_all_runs();
-    reset_state();
-    if (n) {
-      Run r = sorted_run(keys, values, nullptr, static_cast<u32>(n),
-                         kModeInsert, s);
-      runs_.insert(runs_.begin(), r);
-      const u32 unique = read_count(runs_.front(), s);
-      runs_.front().rows_ub = unique;
-      if (unique > capacity_) {
-        drop_all_runs();
-        order_.end(s);
-        throw Error(GD_OUT_OF_MEMORY,
-                    "bad_alloc: bulk_build of " + std::to_string(unique) +
-                        " unique keys exceeds capacity " +
-                        std::to_string(capacity_));
-      }
-    }
-    order_.end(s);
-  }
-
-  void admit(const u32 *keys, const u32 *values, const u8 *flags, u64 n,
-             int mode, hipStream_t s) {
-    if (failed_) {
-      throw Error(GD_PUBLICATION_FAILED,
-                  "GPULSMOpt canonical carry capacity exhausted with status " +
-                      std::to_string(status_) + "; capacity=" +
-                      std::to_string(capacity_) +
-                      "; pending updates were preserved");
-    }
-    if (!n) return;
-    if (!keys || (mode != kModeErase && !values) ||
-        (mode == kModeMixed && !flags))
-      throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt update batch");
-    order_.begin(s);
-    u64 consumed = 0;
-    bool incomplete = false;
-    try {
-      while (consumed < n) {
-        const u32 tile =
-            static_cast<u32>(std::min<u64>(n - consumed, batch_capacity_));
-        Run r = sorted_run(keys + consumed,
-                           values ? values + consumed : nullptr,
-                           flags ? flags + consumed : nullptr, tile, mode, s);
-        runs_.insert(runs_.begin(), r);
-        consumed += tile;
-        rebalance(s);
-        if (++pending_batches_ == kBatchesPerEpoch) {
-          pending_batches_ = 0;
-          epoch_capacity_check(s);
-        }
-        if (consumed < n && failed_) {
-          incomplete = true;
-          break;
-        }
-      }
-    } catch (...) {
-      order_.end(s);
-      throw;
-    }
-    order_.end(s);
-    if (incomplete)
-      throw Error(GD_PUBLICATION_FAILED,
-                  "GPULSMOpt publication failed while tiling an update "
-                  "(capacity=" + std::to_string(capacity_) +
-                      "); accepted pending records were preserved");
-  }
-
-  // -------------------------------------------------------------- reads
-  void lookup(const u32 *q, u64 n, u32 *out_values, u8 *out_found,
-              hipStream_t s) {
-    if (!n) return;
-    if (!q || !out_values)
-      throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt lookup");
-    order_.begin(s);
-    const RunSet set = run_set();
-    lookup_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
-        q, n, out_values, out_found, set);
-    GD_LAUNCH_CHECK();
-    order_.end(s);
-  }
-
-  void range(const u32 *lo, const u32 *hi, u64 n, u32 *out, bool count_only,
-             hipStream_t s) {
-    if (!n) return;
-    if (!lo || !hi || !out)
-      throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt range");
-    order_.begin(s);
-    const RunSet set = run_set();
-    range_kernel<<<grid_for(n, kRangeWaves, 1u << 16), kRangeThreads, 0, s>>>(
-        lo, hi, n, out, count_only ? 1 : 0, set);
-    GD_LAUNCH_CHECK();
-    order_.end(s);
-  }
-
-  void successor(const u32 *q, u64 n, u32 *out, hipStream_t s) {
-    if (!n) return;
-    if (!q || !out)
-      throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt successor");
-    order_.begin(s);
-    const RunSet set = run_set();
-    successor_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(q, n, out,
-                                                                 set);
-    GD_LAUNCH_CHECK();
-    order_.end(s);
-  }
-
-  u64 dump(u32 *out_keys, u32 *out_vals, u64 capacity, hipStream_t s) {
-    order_.begin(s);
-    u64 count = 0;
-    try {
-      if (!runs_.empty()) {
-        Run merged = merge_all(runs_, /*consume=*/false, s);
-        count = read_count(merged, s);
-        if (count <= capacity && count) {
-          if (!out_keys || !out_vals) {
-            free_run(merged);
-            throw Error(GD_INVALID_ARGUMENT, "null dump output");
-          }
-          decode_run_kernel<<<grid_for(count, 256, 1u << 16), 256, 0, s>>>(
-              merged.keys, merged.vals, merged.meta(), out_keys, out_vals);
-          GD_LAUNCH_CHECK();
-        }
-        free_run(merged);
-        GD_HIP(hipStreamSynchronize(s));
-      }
-    } catch (...) {
-      order_.end(s);
-      throw;
-    }
-    order_.end(s);
-    return count;
-  }
-
-  void compact(hipStream_t s) {
-    order_.begin(s);
-    if (!runs_.empty()) {
-      Run merged = merge_all(runs_, /*consume=*/true, s);
-      runs_.clear();
-      runs_.push_back(merged);
-    }
-    order_.end(s);
-  }
-
-  u32 carry_status() const { return status_; }
-  u64 resident_bytes() const { return pool_.held_bytes(); }
-  // Synchronous: exact rows per run (refreshes the host upper bounds).
-  void shape(u32 *runs, u64 *rows) {
-    order_.wait_host();
-    *runs = static_cast<u32>(runs_.size());
-    u64 t = 0;
-    for (auto &r : runs_) {
-      RunMeta m{};
-      GD_HIP(hipMemcpy(&m, r.meta(), sizeof(m), hipMemcpyDeviceToHost));
-      r.rows_ub = m.count;
-      t += m.count;
-    }
-    *rows = t;
-  }
-
- private:
-  // ----------------------------------------------------------- helpers
-  void reset_state() {
-    pending_batches_ = 0;
-    failed_ = false;
-    status_ = 0;
-  }
-
-  Run alloc_run(u64 rows_ub) {
-    Run r;
-    r.rows_ub = rows_ub;
-    r.max_bits = dir_bits_for(rows_ub, kMaxDirBits);
-    const u64 rows = std::max<u64>(rows_ub, 1);
-    r.keys = pool_.get_n<u32>(rows);
-    r.vals = pool_.get_n<u32>(rows);
-    r.dir = static_cast<u32 *>(
-        pool_.get(kMetaBytes + ((u64(1) << r.max_bits) + 1) * sizeof(u32)));
-    return r;
-  }
-
-  void free_run(Run &r) {
-    pool_.put(r.keys);
-    pool_.put(r.vals);
-    pool_.put(r.dir);
-    r.keys = r.vals = r.dir = nullptr;
-  }
-
-  void drop_all_runs() {
-    for (auto &r : runs_) free_run(r);
-    runs_.clear();
-  }
-
-  void launch_directory(const Run &r, hipStream_t s) {
-    const u64 items = std::max<u64>(r.rows_ub, (u64(1) << r.max_bits) + 1);
-    build_directory_kernel<<<grid_for(items, 256, 8192), 256, 0, s>>>(
-        r.keys, r.meta(), r.entries(), r.max_bits);
-    GD_LAUNCH_CHECK();
-  }
-
-  // Sorts one tile of updates into a new run.
-  Run sorted_run(const u32 *keys, const u32 *values, const u8 *flags, u32 n,
-                 int mode, hipStream_t s) {
-    Run r = alloc_run(n);
-    const u32 fin_tiles = (n + kFinTile - 1) / kFinTile;
-    const u64 sort_words = rs::control_words(n);
-    const u64 fin_words = 64 + 2ull * fin_tiles;  // ticket pad + status u64
-    Lease<u32> ka(pool_, n), va(pool_, n), kb(pool_, n), vb(pool_, n);
-    Lease<u32> ctrl(pool_, sort_words + fin_words);
-    u32 *fin_ctrl = ctrl.get() + sort_words;
-    zero_two_kernel<<<grid_for(fin_words + 4, 256, 1024), 256, 0, s>>>(
-        reinterpret_cast<u32 *>(r.meta()), 4, fin_ctrl, fin_words);
-    GD_LAUNCH_CHECK();
-    GD_HIP(rs::sort_pairs<u32>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
-                          ctrl.get(), s));
-    finalize_batch_kernel<<<fin_tiles, kFinThreads, 0, s>>>(
-        ka.get(), va.get(), n, values, flags, mode, r.keys, r.vals, r.meta(),
-        reinterpret_cast<u64 *>(fin_ctrl + 64), fin_ctrl);
-    GD_LAUNCH_CHECK();
-    launch_directory(r, s);
-    return r;
-  }
-
-  // Merges a (newer) with b (older) into a new run; inputs stay owned by
-  // the caller.
-  Run merge_two(const Run &a, const Run &b, bool drop_tombstones,
-                hipStream_t s) {
-    const u64 ub = a.rows_ub + b.rows_ub;
-    Run out = alloc_run(ub);
-    const u32 tiles =
-        static_cast<u32>(std::max<u64>(1, (ub + kMergeTile - 1) / kMergeTile));
-    const u64 ctrl_words = 64 + 2ull * tiles;
-    Lease<u32> parts(pool_, u64(tiles) + 1);
-    Lease<u32> ctrl(pool_, ctrl_words);
-    zero_two_kernel<<<grid_for(ctrl_words + 4, 256, 1024), 256, 0, s>>>(
-        reinterpret_cast<u32 *>(out.meta()), 4, ctrl.get(), ctrl_words);
-    GD_LAUNCH_CHECK();
-    const MergeSide sa{a.keys, a.vals, a.meta()};
-    const MergeSide sb{b.keys, b.vals, b.meta()};
-    merge_partition_kernel<<<grid_for(u64(tiles) + 1, 256), 256, 0, s>>>(
-        sa, sb, tiles, parts.get());
-    GD_LAUNCH_CHECK();
-    merge_kernel<<<tiles, kMergeThreads, 0, s>>>(
-        sa, sb, parts.get(), out.keys, out.vals, out.meta(),
-        drop_tombstones ? 1 : 0, reinterpret_cast<u64 *>(ctrl.get() + 64),
-        ctrl.get());
-    GD_LAUNCH_CHECK();
-    launch_directory(out, s);
-    return out;
-  }
-
-  Run empty_run(hipStream_t s) {
-    Run r = alloc_run(0);
-    zero_two_kernel<<<1, 256, 0, s>>>(reinterpret_cast<u32 *>(r.meta()), 4,
-                                      r.entries(), 3);
-    GD_LAUNCH_CHECK();
-    launch_directory(r, s);
-    return r;
-  }
-
-  // Merges every run (newest first) into one tombstone-free run.  With
-  // consume=true the inputs are released.
-  Run merge_all(std::vector<Run> &runs, bool consume, hipStream_t s) {
-    Run acc;
-    bool acc_owned = false;
-    if (runs.size() == 1) {
-      Run e = empty_run(s);
-      acc = merge_two(runs[0], e, true, s);
-      free_run(e);
-      if (consume) free_run(runs[0]);
-      return acc;
-    }
-    for (std::size_t i = 1; i < runs.size(); ++i) {
-      const Run &a = i == 1 ? runs[0] : acc;
-      Run next = merge_two(a, runs[i], i + 1 == runs.size(), s);
-      if (acc_owned) free_run(acc);
-      acc = next;
-      acc_owned = true;
-    }
-    if (consume)
-      for (auto &r : runs) free_run(r);
-    return acc;
-  }
-
-  // Binary-counter policy: merge the two newest runs while the older one is
-  // no larger.  Tombstones drop when the output becomes the oldest run.
-  void rebalance(hipStream_t s) {
-    while (runs_.size() >= 2 &&
-           (runs_[1].rows_ub <= runs_[0].rows_ub ||
-            runs_.size() >= static_cast<std::size_t>(kMaxRuns))) {
-      const bool oldest = runs_.size() == 2;
-      Run out = merge_two(runs_[0], runs_[1], oldest, s);
-      free_run(runs_[0]);
-      free_run(runs_[1]);
-      runs_.erase(runs_.begin(), runs_.begin() + 2);
-      runs_.insert(runs_.begin(), out);
-    }
-  }
-
-  u32 read_count(const Run &r, hipStream_t s) {
-    RunMeta m{};
-    GD_HIP(hipMemcpyAsync(&m, r.meta(), sizeof(m), hipMemcpyDeviceToHost, s));
-    GD_HIP(hipStreamSynchronize(s));
-    return m.count;
-  }
-
-  // Epoch boundary: survivors must fit the capacity (GPULSMOpt.cuh:7350).
-  void epoch_capacity_check(hipStream_t s) {
-    u64 ub = 0;
-    for (const auto &r : runs_) ub += r.rows_ub;
-    if (ub <= capacity_) return;
-    // exact row counts
-    u64 exact = 0;
-    for (auto &r : runs_) {
-      r.rows_ub = read_count(r, s);
-      exact += r.rows_ub;
-    }
-    if (exact <= capacity_) return;
-    // only a full merge tells the live count
-    Run merged = merge_all(runs_, /*consume=*/true, s);
-    runs_.clear();
-    merged.rows_ub = read_count(merged, s);
-    runs_.push_back(merged);
-    if (merged.rows_ub > capacity_) {
-      failed_ = true;
-      status_ = GD_PUBLICATION_OUTPUT_OVERFLOW;
-    }
-  }
-
-  RunSet run_set() const {
-    RunSet set{};
-    set.count = static_cast<u32>(runs_.size());
-    for (std::size_t i = 0; i < runs_.size(); ++i) set.run[i] = runs_[i].view();
-    return set;
-  }
-
-  u64 batch_capacity_ = 1;
-  u64 capacity_ = 0;
-  DevicePool pool_;
-  OperationOrder order_;
-  std::vector<Run> runs_;  // newest first
-  u32 pending_batches_ = 0;
-  bool failed_ = false;
-  u32 status_ = 0;
-};
-
-}  // namespace lsm
 }  // namespace gdict
 
 using gdict::guarded;

@@ -1,17 +1,18 @@
 // Device side of the LSM dictionary (uint32 keys, uint32 values).
 //
 // Data model in HBM.  The dictionary is a stack of immutable sorted runs,
-// newest first.  A run of c rows is three allocations' worth of streams:
-//   keys[c]  u32: low 31 bits of the key | tombstone flag in bit 31
-//   vals[c]  u32: value (0 for a tombstone)
+// newest first.  A run of c rows is two allocations:
+//   rows[c]  8-byte rows, .x = low 31 bits of the key | tombstone flag in
+//            bit 31, .y = value (0 for a tombstone)
 //   dir      RunMeta header (count, split, directory bits, tombstones)
 //            followed by a radix directory of 2^b + 1 u32 row offsets:
 //            dir[j] = first row whose key >= j << (32 - b).
 // Bit 31 of a key is implied by the row position (rows >= split have it
 // set), which frees it for the tombstone flag, so a row is exactly 8 bytes
-// like the reference's Row (GPULSMOpt.cuh:343-377).  Every search enters
-// through the directory (b >= 1), and inside one directory bucket all keys
-// share bit 31, so comparisons use the low 31 bits only.
+// like the reference's Row (GPULSMOpt.cuh:343-377), and key and value share
+// one cache line, so a point hit costs one line of rows.  Every search enters
+// through the directory (b >= 1); inside one directory bucket all keys share
+// bit 31, so comparisons use the low 31 bits only.
 #pragma once
 
 #include "device_utils.hpp"
@@ -33,9 +34,10 @@ struct RunMeta {
 };
 constexpr u64 kMetaBytes = 256;  // header slot in front of the directory
 
+using Row = uint2;  // .x key word, .y value
+
 struct RunView {
-  const u32 *keys;
-  const u32 *vals;
+  const Row *rows;
   const u32 *dir;
   const RunMeta *meta;
 };
@@ -72,15 +74,15 @@ __device__ __forceinline__ u32 run_lower_bound(const RunView &r, u32 b,
   const u32 target = key & kLowMask;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
-    if ((r.keys[mid] & kLowMask) < target) lo = mid + 1;
+    if ((r.rows[mid].x & kLowMask) < target) lo = mid + 1;
     else hi = mid;
   }
   return lo;
 }
 
-// Point search; returns the row position or kInvalid.
+// Point search; returns the row position or kInvalid (row in `row`).
 __device__ __forceinline__ u32 run_find(const RunView &r, u32 b, u32 key,
-                                        u32 &word) {
+                                        Row &row) {
   const u32 bucket = key >> (32u - b);
   u32 lo = r.dir[bucket];
   const u32 end = r.dir[bucket + 1];
@@ -88,13 +90,13 @@ __device__ __forceinline__ u32 run_find(const RunView &r, u32 b, u32 key,
   const u32 target = key & kLowMask;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
-    if ((r.keys[mid] & kLowMask) < target) lo = mid + 1;
+    if ((r.rows[mid].x & kLowMask) < target) lo = mid + 1;
     else hi = mid;
   }
   if (lo < end) {
-    const u32 w = r.keys[lo];
-    if ((w & kLowMask) == target) {
-      word = w;
+    const Row w = r.rows[lo];
+    if ((w.x & kLowMask) == target) {
+      row = w;
       return lo;
     }
   }
@@ -102,9 +104,10 @@ __device__ __forceinline__ u32 run_find(const RunView &r, u32 b, u32 key,
 }
 
 // ------------------------------------------------------------ batch runs
-constexpr int kFinThreads = 256;
+constexpr int kFinThreads = 512;
 constexpr int kFinItems = 8;
-constexpr int kFinTile = kFinThreads * kFinItems;
+constexpr int kFinTile = kFinThreads * kFinItems;  // 4096
+constexpr int kFinWaves = kFinThreads / kWave;
 
 enum FinalizeMode : int {
   kModeInsert = 0,
@@ -115,15 +118,21 @@ enum FinalizeMode : int {
 // Turns a stably sorted batch (keys sk, input positions sp) into a run:
 // keep the last occurrence of each key (the newest), gather its value and
 // tombstone flag, encode, and compact with a single-pass chained scan.
+// Items are striped (item j of a thread is row base + j*256 + tid), so every
+// load and store of a row j is one coalesced access of the block; ballots
+// give the survivors' ranks.
 static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
     const u32 *__restrict__ sk, const u32 *__restrict__ sp, u32 n,
     const u32 *__restrict__ values, const u8 *__restrict__ erase_flags,
-    int mode, u32 *__restrict__ out_keys, u32 *__restrict__ out_vals,
-    RunMeta *meta, u64 *status, u32 *ticket) {
+    int mode, Row *__restrict__ out, RunMeta *meta, u64 *status,
+    u32 *ticket) {
   __shared__ u32 s_tile;
-  __shared__ u32 s_scan[kFinThreads / kWave + 1];
+  __shared__ u32 s_off[kFinItems][kFinWaves];
+  __shared__ u32 s_total;
   __shared__ u64 s_base;
   __shared__ u32 s_low, s_tomb;
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) {
     s_tile = atomicAdd(ticket, 1u);
     s_low = 0;
@@ -131,52 +140,59 @@ static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
   }
   __syncthreads();
   const u32 tile = s_tile;
-  const u64 first = u64(tile) * kFinTile + u64(threadIdx.x) * kFinItems;
-
+  const u64 base = u64(tile) * kFinTile;
   u32 key[kFinItems];
-  u32 keep_mask = 0;
-  u32 kept = 0;
+  u64 mask[kFinItems];
 #pragma unroll
   for (int j = 0; j < kFinItems; ++j) {
-    const u64 i = first + j;
+    const u64 i = base + u64(j) * kFinThreads + threadIdx.x;
+    bool keep = false;
     key[j] = 0;
     if (i < n) {
       key[j] = sk[i];
-      const bool last = (i + 1 == n) || sk[i + 1] != key[j];
-      if (last) {
-        keep_mask |= 1u << j;
-        ++kept;
-      }
+      keep = (i + 1 == n) || sk[i + 1] != key[j];
     }
+    mask[j] = ballot(keep);
+    if (lane == 0) s_off[j][wave] = popc64(mask[j]);
   }
-  const u32 local = block_exclusive_scan<kFinThreads>(kept, s_scan);
-  const u32 tile_total = s_scan[kFinThreads / kWave];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 run = 0;
+    for (int j = 0; j < kFinItems; ++j)
+      for (int w = 0; w < kFinWaves; ++w) {
+        const u32 c = s_off[j][w];
+        s_off[j][w] = run;
+        run += c;
+      }
+    s_total = run;
+  }
+  __syncthreads();
+  const u32 tile_total = s_total;
   if (threadIdx.x < kWave) {
     const u64 ex = wave_lookback(status, tile, tile_total);
     if (threadIdx.x == 0) s_base = ex;
   }
   __syncthreads();
-  u64 o = s_base + local;
+  const u64 out_base = s_base;
   u32 low = 0, tombs = 0;
 #pragma unroll
   for (int j = 0; j < kFinItems; ++j) {
-    if (!(keep_mask >> j & 1u)) continue;
-    const u32 p = sp[first + j];
+    if (!((mask[j] >> lane) & 1ull)) continue;
+    const u64 i = base + u64(j) * kFinThreads + threadIdx.x;
+    const u64 o = out_base + s_off[j][wave] + popc64(mask[j] & lanemask_lt());
+    const u32 p = sp[i];
     bool tomb;
     if (mode == kModeErase) tomb = true;
     else if (mode == kModeMixed) tomb = erase_flags[p] != 0;
     else tomb = false;
     const u32 v = tomb ? 0u : values[p];
-    out_keys[o] = (key[j] & kLowMask) | (tomb ? kTombBit : 0u);
-    out_vals[o] = v;
+    out[o] = make_uint2((key[j] & kLowMask) | (tomb ? kTombBit : 0u), v);
     low += key[j] < kTombBit;
     tombs += tomb;
-    ++o;
   }
-  // order-independent totals
   low = wave_sum_u32(low);
   tombs = wave_sum_u32(tombs);
-  if (lane_id() == 0) {
+  if (lane == 0) {
     if (low) atomicAdd(&s_low, low);
     if (tombs) atomicAdd(&s_tomb, tombs);
   }
@@ -188,11 +204,195 @@ static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
   }
 }
 
+// ------------------------------------------------ small batches, one launch
+// A batch of at most kSmallTile keys becomes a run in ONE workgroup: an LDS
+// LSD radix sort (the same wave64 ballot multisplit as the onesweep passes,
+// with the whole batch as one tile), newest-wins dedupe, value gather,
+// encode, and the radix directory.  Replaces nine launches for the small
+// arrivals of a serving stream.
+constexpr int kSmallThreads = 512;
+constexpr int kSmallItems = 8;
+constexpr int kSmallTile = kSmallThreads * kSmallItems;  // 4096
+constexpr int kSmallWaves = kSmallThreads / kWave;
+
+static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
+    const u32 *__restrict__ keys, u32 n, const u32 *__restrict__ values,
+    const u8 *__restrict__ erase_flags, int mode, Row *__restrict__ out,
+    RunMeta *meta, u32 *__restrict__ dir, u32 max_bits) {
+  // 57 KB of LDS: keys (2 x 16 KB), 16-bit positions (2 x 8 KB), per-wave
+  // digit counters (8 KB)
+  __shared__ u32 s_k0[kSmallTile];
+  __shared__ u32 s_k1[kSmallTile];
+  __shared__ unsigned short s_p0[kSmallTile];
+  __shared__ unsigned short s_p1[kSmallTile];
+  __shared__ u32 s_hist[kSmallWaves * 256];
+  __shared__ u32 s_start[256];
+  __shared__ u32 s_scan[32];
+  __shared__ u32 s_misc[8];
+  const u32 tid = threadIdx.x;
+  const u32 lane = lane_id();
+  const u32 wave = tid / kWave;
+
+  for (u32 i = tid; i < n; i += kSmallThreads) {
+    s_k0[i] = keys[i];
+    s_p0[i] = static_cast<unsigned short>(i);
+  }
+  __syncthreads();
+  u32 *src_k = s_k0, *dst_k = s_k1;
+  unsigned short *src_p = s_p0, *dst_p = s_p1;
+  for (int pass = 0; pass < 4; ++pass) {
+    const u32 shift = 8u * pass;
+    for (u32 i = tid; i < kSmallWaves * 256; i += kSmallThreads) s_hist[i] = 0;
+    __syncthreads();
+    u32 key[kSmallItems], pos[kSmallItems], rank[kSmallItems];
+    volatile u32 *my_hist = s_hist + wave * 256;
+#pragma unroll
+    for (int j = 0; j < kSmallItems; ++j) {
+      const u32 idx = wave * (kWave * kSmallItems) + j * kWave + lane;
+      const bool valid = idx < n;
+      key[j] = valid ? src_k[idx] : 0xffffffffu;
+      pos[j] = valid ? src_p[idx] : 0u;
+      const u32 d = (key[j] >> shift) & 0xffu;
+      u64 match = ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const u64 vote = ballot(bit);
+        match &= bit ? vote : ~vote;
+      }
+      u32 prior = 0;
+      if (valid) prior = my_hist[d];
+      rank[j] = prior + popc64(match & lanemask_lt());
+      if (valid && (match >> lane) == 1ull) my_hist[d] = prior + popc64(match);
+    }
+    __syncthreads();
+    // wave prefixes per digit (threads 0..255 own one digit each)
+    u32 total = 0;
+    if (tid < 256) {
+      for (int w = 0; w < kSmallWaves; ++w) {
+        const u32 c = s_hist[w * 256 + tid];
+        s_hist[w * 256 + tid] = total;
+        total += c;
+      }
+    }
+    // exclusive scan of the digit totals over 256 digits
+    {
+      const u32 v = tid < 256 ? total : 0u;
+      const u32 inc = wave_inclusive_scan(v);
+      if (lane == kWave - 1) s_scan[wave] = inc;
+      __syncthreads();
+      if (tid == 0) {
+        u32 run = 0;
+        for (int w = 0; w < kSmallWaves; ++w) {
+          const u32 t = s_scan[w];
+          s_scan[w] = run;
+          run += t;
+        }
+      }
+      __syncthreads();
+      if (tid < 256) s_start[tid] = s_scan[wave] + inc - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSmallItems; ++j) {
+      const u32 idx = wave * (kWave * kSmallItems) + j * kWave + lane;
+      if (idx < n) {
+        const u32 d = (key[j] >> shift) & 0xffu;
+        const u32 lp = s_start[d] + s_hist[wave * 256 + d] + rank[j];
+        dst_k[lp] = key[j];
+        dst_p[lp] = static_cast<unsigned short>(pos[j]);
+      }
+    }
+    __syncthreads();
+    u32 *tk = src_k;
+    unsigned short *tp = src_p;
+    src_k = dst_k;
+    src_p = dst_p;
+    dst_k = tk;
+    dst_p = tp;
+  }
+  // sorted (key, position) in src_*; keep the last occurrence of each key.
+  // Survivor ranks: ballots per (item row j, wave), scanned by thread 0.
+  u32 *s_off = s_hist;  // reuse: kSmallItems * kSmallWaves words
+  u64 mask[kSmallItems];
+#pragma unroll
+  for (int j = 0; j < kSmallItems; ++j) {
+    const u32 i = j * kSmallThreads + tid;
+    const bool keep = i < n && (i + 1 == n || src_k[i + 1] != src_k[i]);
+    mask[j] = ballot(keep);
+    if (lane == 0) s_off[j * kSmallWaves + wave] = popc64(mask[j]);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    u32 run = 0;
+    for (int k = 0; k < kSmallItems * kSmallWaves; ++k) {
+      const u32 c = s_off[k];
+      s_off[k] = run;
+      run += c;
+    }
+    s_misc[0] = run;  // count
+    s_misc[1] = 0;    // split
+    s_misc[2] = 0;    // tombs
+  }
+  __syncthreads();
+  const u32 count = s_misc[0];
+  u32 low = 0, tombs = 0;
+  // decoded output keys go to dst_k for the directory pass
+#pragma unroll
+  for (int j = 0; j < kSmallItems; ++j) {
+    if (!((mask[j] >> lane) & 1ull)) continue;
+    const u32 i = j * kSmallThreads + tid;
+    const u32 o = s_off[j * kSmallWaves + wave] +
+                  popc64(mask[j] & lanemask_lt());
+    const u32 k = src_k[i];
+    const u32 p = src_p[i];
+    bool tomb;
+    if (mode == kModeErase) tomb = true;
+    else if (mode == kModeMixed) tomb = erase_flags[p] != 0;
+    else tomb = false;
+    out[o] = make_uint2((k & kLowMask) | (tomb ? kTombBit : 0u),
+                        tomb ? 0u : values[p]);
+    dst_k[o] = k;
+    low += k < kTombBit;
+    tombs += tomb;
+  }
+  low = wave_sum_u32(low);
+  tombs = wave_sum_u32(tombs);
+  if (lane == 0) {
+    if (low) atomicAdd(&s_misc[1], low);
+    if (tombs) atomicAdd(&s_misc[2], tombs);
+  }
+  __syncthreads();
+  const u32 b = dir_bits_for(count, max_bits);
+  const u32 shift = 32u - b;
+  const u32 nb = 1u << b;
+  if (tid == 0) {
+    meta->count = count;
+    meta->split = s_misc[1];
+    meta->tombs = s_misc[2];
+    meta->dir_bits = b;
+  }
+  const long long last_bucket =
+      count ? static_cast<long long>(dst_k[count - 1] >> shift) : -1;
+  const u32 items = count > nb + 1 ? count : nb + 1;
+  for (u32 t = tid; t < items; t += kSmallThreads) {
+    if (t < count) {
+      const u32 bucket = dst_k[t] >> shift;
+      const long long prev =
+          t ? static_cast<long long>(dst_k[t - 1] >> shift) : -1;
+      for (long long j = prev + 1; j <= static_cast<long long>(bucket); ++j)
+        dir[j] = t;
+    }
+    if (t <= nb && static_cast<long long>(t) > last_bucket) dir[t] = count;
+  }
+}
+
 // Builds the radix directory of a run from its final count/split.  Launch
 // over max(count_ub, 2^max_bits + 1) threads (grid-stride).
-static __global__ void build_directory_kernel(const u32 *__restrict__ keys,
-                                       RunMeta *meta, u32 *__restrict__ dir,
-                                       u32 max_bits) {
+static __global__ void build_directory_kernel(const Row *__restrict__ rows,
+                                              RunMeta *meta,
+                                              u32 *__restrict__ dir,
+                                              u32 max_bits) {
   const u32 count = meta->count;
   const u32 split = meta->split;
   const u32 b = dir_bits_for(count, max_bits);
@@ -201,7 +401,7 @@ static __global__ void build_directory_kernel(const u32 *__restrict__ keys,
   if (blockIdx.x == 0 && threadIdx.x == 0) meta->dir_bits = b;
   const long long last_bucket =
       count ? static_cast<long long>(
-                  decode_key(keys[count - 1], count - 1, split) >> shift)
+                  decode_key(rows[count - 1].x, count - 1, split) >> shift)
             : -1;
   const u64 items = count > u64(nb) + 1 ? u64(count) : u64(nb) + 1;
   const u64 stride = u64(gridDim.x) * blockDim.x;
@@ -209,10 +409,10 @@ static __global__ void build_directory_kernel(const u32 *__restrict__ keys,
        t += stride) {
     if (t < count) {
       const u32 pos = static_cast<u32>(t);
-      const u32 bucket = decode_key(keys[pos], pos, split) >> shift;
+      const u32 bucket = decode_key(rows[pos].x, pos, split) >> shift;
       const long long prev =
           pos ? static_cast<long long>(
-                    decode_key(keys[pos - 1], pos - 1, split) >> shift)
+                    decode_key(rows[pos - 1].x, pos - 1, split) >> shift)
               : -1;
       for (long long j = prev + 1; j <= static_cast<long long>(bucket); ++j)
         dir[j] = pos;
@@ -223,18 +423,18 @@ static __global__ void build_directory_kernel(const u32 *__restrict__ keys,
 
 // --------------------------------------------------------------- merging
 constexpr int kMergeThreads = 256;
-constexpr int kMergeItems = 8;
-constexpr int kMergeTile = kMergeThreads * kMergeItems;  // 2048
+constexpr int kMergeItems = 16;
+constexpr int kMergeTile = kMergeThreads * kMergeItems;  // 4096
+constexpr u32 kStageHalf = (kMergeTile + 2) / 2;  // rows staged in s_key
 
 struct MergeSide {
-  const u32 *keys;
-  const u32 *vals;
+  const Row *rows;
   const RunMeta *meta;
 };
 
 __device__ __forceinline__ u32 side_key(const MergeSide &s, u32 pos,
                                         u32 split) {
-  return decode_key(s.keys[pos], pos, split);
+  return decode_key(s.rows[pos].x, pos, split);
 }
 
 // Number of A elements among the first `diag` merged elements, where A (the
@@ -254,10 +454,23 @@ __device__ __forceinline__ u32 merge_path(const MergeSide &a, u32 ca, u32 sa,
   return static_cast<u32>(lo);
 }
 
-static __global__ void merge_partition_kernel(MergeSide a, MergeSide b, u32 tiles,
-                                       u32 *__restrict__ parts) {
+// Tile split points of the merge; also clears the merge's chained-scan
+// state and the output run's header (one launch instead of three).
+static __global__ void merge_partition_kernel(MergeSide a, MergeSide b,
+                                              u32 tiles,
+                                              u32 *__restrict__ parts,
+                                              u64 *status, u32 *ticket,
+                                              RunMeta *out_meta) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > tiles) return;
+  if (t < tiles) status[t] = 0;
+  if (t == 0) {
+    *ticket = 0;
+    out_meta->count = 0;
+    out_meta->split = 0;
+    out_meta->dir_bits = 0;
+    out_meta->tombs = 0;
+  }
   const u32 ca = a.meta->count, cb = b.meta->count;
   const u64 total = u64(ca) + cb;
   u64 diag = u64(t) * kMergeTile;
@@ -270,11 +483,13 @@ static __global__ void merge_partition_kernel(MergeSide a, MergeSide b, u32 tile
 // output becomes the oldest run, GPULSMOpt.cuh:3175-3179).
 static __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     MergeSide a, MergeSide b, const u32 *__restrict__ parts,
-    u32 *__restrict__ out_keys, u32 *__restrict__ out_vals, RunMeta *out_meta,
-    int drop_tombstones, u64 *status, u32 *ticket) {
+    Row *__restrict__ out, RunMeta *out_meta, int drop_tombstones,
+    u64 *status, u32 *ticket) {
   __shared__ u32 s_tile;
-  __shared__ u32 s_key[kMergeTile + 2];
-  __shared__ u32 s_val[kMergeTile + 2];
+  // inputs: decoded key, value, tombstone flag; s_key/s_val are reused as
+  // the 8-byte staging of the survivors
+  __shared__ __align__(8) u32 s_key[kMergeTile + 2];
+  __shared__ __align__(8) u32 s_val[kMergeTile + 2];
   __shared__ u8 s_tomb[kMergeTile + 2];
   __shared__ u32 s_scan[kMergeThreads / kWave + 1];
   __shared__ u64 s_base;
@@ -306,17 +521,17 @@ static __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
       continue;
     }
     const u32 pos = i0 + t - 1;
-    const u32 w = a.keys[pos];
-    s_key[t] = decode_key(w, pos, sa);
-    s_tomb[t] = static_cast<u8>(w >> 31);
-    if (t) s_val[t] = a.vals[pos];
+    const Row r = a.rows[pos];
+    s_key[t] = decode_key(r.x, pos, sa);
+    s_tomb[t] = static_cast<u8>(r.x >> 31);
+    s_val[t] = r.y;
   }
   for (u32 t = threadIdx.x; t < nb; t += kMergeThreads) {
     const u32 pos = j0 + t;
-    const u32 w = b.keys[pos];
-    s_key[na + 1 + t] = decode_key(w, pos, sb);
-    s_tomb[na + 1 + t] = static_cast<u8>(w >> 31);
-    s_val[na + 1 + t] = b.vals[pos];
+    const Row r = b.rows[pos];
+    s_key[na + 1 + t] = decode_key(r.x, pos, sb);
+    s_tomb[na + 1 + t] = static_cast<u8>(r.x >> 31);
+    s_val[na + 1 + t] = r.y;
   }
   __syncthreads();
 
@@ -380,24 +595,27 @@ static __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     if (threadIdx.x == 0) s_base = ex;
   }
   __syncthreads();  // also: every LDS input read is complete
-  // stage survivors in LDS, then write them out contiguously
+  // stage survivors as 8-byte rows over the (now free) key/value arrays:
+  // rows [0, kStageHalf) in s_key, the rest in s_val
+  Row *stage_lo = reinterpret_cast<Row *>(s_key);
+  Row *stage_hi = reinterpret_cast<Row *>(s_val);
   u32 low = 0, tombs = 0, at = local;
 #pragma unroll
   for (int s = 0; s < kMergeItems; ++s) {
     if (okeep[s]) {
-      s_key[at] = (ok[s] & kLowMask) | (ot[s] ? kTombBit : 0u);
-      s_val[at] = ov[s];
+      const Row r = make_uint2((ok[s] & kLowMask) | (ot[s] ? kTombBit : 0u),
+                               ov[s]);
+      if (at < kStageHalf) stage_lo[at] = r;
+      else stage_hi[at - kStageHalf] = r;
       ++at;
       low += ok[s] < kTombBit;
       tombs += ot[s];
     }
   }
   __syncthreads();
-  const u64 base = s_base;
-  for (u32 t = threadIdx.x; t < tile_total; t += kMergeThreads) {
-    out_keys[base + t] = s_key[t];
-    out_vals[base + t] = s_val[t];
-  }
+  const u64 obase = s_base;
+  for (u32 t = threadIdx.x; t < tile_total; t += kMergeThreads)
+    out[obase + t] = t < kStageHalf ? stage_lo[t] : stage_hi[t - kStageHalf];
   low = wave_sum_u32(low);
   tombs = wave_sum_u32(tombs);
   if (lane_id() == 0) {
@@ -414,27 +632,65 @@ static __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
 
 // --------------------------------------------------------------- queries
 static __global__ void lookup_kernel(const u32 *__restrict__ queries, u64 n,
-                              u32 *__restrict__ out_values,
-                              u8 *__restrict__ out_found, RunSet runs) {
+                                     u32 *__restrict__ out_values,
+                                     u8 *__restrict__ out_found, RunSet runs) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride) {
     const u32 key = queries[i];
-    bool found = false, live = false;
+    bool live = false;
     u32 value = 0;
     for (u32 r = 0; r < runs.count; ++r) {
       const RunView &rv = runs.run[r];
       const u32 b = rv.meta->dir_bits;
-      u32 w;
-      const u32 pos = run_find(rv, b, key, w);
-      if (pos != kInvalid) {
-        found = true;
-        live = !(w & kTombBit);
-        if (live) value = rv.vals[pos];
+      Row row;
+      if (run_find(rv, b, key, row) != kInvalid) {
+        live = !(row.x & kTombBit);
+        value = row.y;
         break;
       }
     }
-    (void)found;
+    if (out_found) {
+      out_values[i] = live ? value : 0u;
+      out_found[i] = live ? 1 : 0;
+    } else {
+      out_values[i] = live ? value : kInvalid;
+    }
+  }
+}
+
+// Lookups over queries partitioned by their top key byte (qk = grouped
+// keys, qp = their original positions), each result written back to its
+// original position.  Workgroups are dealt round-robin to the 8 XCDs, so
+// block b serves the contiguous chunk (b % 8) * (G / 8) + b / 8: every XCD
+// walks its own eighth of the grouped queries, i.e. its own eighth of the
+// key space, and the run rows it needs stay in that XCD's L2.
+static __global__ void lookup_grouped_kernel(const u32 *__restrict__ qk,
+                                             const u32 *__restrict__ qp,
+                                             u64 n,
+                                             u32 *__restrict__ out_values,
+                                             u8 *__restrict__ out_found,
+                                             RunSet runs) {
+  const u32 G = gridDim.x;  // a multiple of 8
+  const u32 logical = (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
+  const u64 chunk = (n + G - 1) / G;
+  const u64 begin = u64(logical) * chunk;
+  const u64 end = begin + chunk < n ? begin + chunk : n;
+  for (u64 p = begin + threadIdx.x; p < end; p += blockDim.x) {
+    const u32 key = qk[p];
+    const u32 i = qp[p];
+    bool live = false;
+    u32 value = 0;
+    for (u32 r = 0; r < runs.count; ++r) {
+      const RunView &rv = runs.run[r];
+      const u32 b = rv.meta->dir_bits;
+      Row row;
+      if (run_find(rv, b, key, row) != kInvalid) {
+        live = !(row.x & kTombBit);
+        value = row.y;
+        break;
+      }
+    }
     if (out_found) {
       out_values[i] = live ? value : 0u;
       out_found[i] = live ? 1 : 0;
@@ -487,11 +743,11 @@ static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
       }
       const u32 split = rv.meta->split;
       for (u32 idx = L + lane; idx < H; idx += kWave) {
-        const u32 w = rv.keys[idx];
-        if (w & kTombBit) continue;
+        const Row w = rv.rows[idx];
+        if (w.x & kTombBit) continue;
         bool shadowed = false;
         if (newer_rows) {
-          const u32 key = decode_key(w, idx, split);
+          const u32 key = decode_key(w.x, idx, split);
           for (u32 s = 0; s < r && !shadowed; ++s) {
             u32 l2 = s_lo[wave][s], h2 = s_hi[wave][s];
             if (l2 >= h2) continue;
@@ -499,15 +755,15 @@ static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
             const u32 ns = nv.meta->split;
             while (l2 < h2) {
               const u32 mid = (l2 + h2) >> 1;
-              if (decode_key(nv.keys[mid], mid, ns) < key) l2 = mid + 1;
+              if (decode_key(nv.rows[mid].x, mid, ns) < key) l2 = mid + 1;
               else h2 = mid;
             }
             if (l2 < s_hi[wave][s] &&
-                decode_key(nv.keys[l2], l2, ns) == key)
+                decode_key(nv.rows[l2].x, l2, ns) == key)
               shadowed = true;
           }
         }
-        if (!shadowed) acc += count_only ? 1u : rv.vals[idx];
+        if (!shadowed) acc += count_only ? 1u : w.y;
       }
       newer_rows = true;
     }
@@ -518,7 +774,7 @@ static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
 }
 
 static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
-                                 u32 *__restrict__ out, RunSet runs) {
+                                        u32 *__restrict__ out, RunSet runs) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride) {
@@ -531,7 +787,7 @@ static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
         const u32 b = rv.meta->dir_bits, c = rv.meta->count;
         const u32 p = run_lower_bound(rv, b, c, lower);
         if (p < c) {
-          const u64 k = decode_key(rv.keys[p], p, rv.meta->split);
+          const u64 k = decode_key(rv.rows[p].x, p, rv.meta->split);
           if (k < best) best = k;
         }
       }
@@ -540,10 +796,10 @@ static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
       bool live = false;
       for (u32 r = 0; r < runs.count; ++r) {
         const RunView &rv = runs.run[r];
-        u32 w;
-        if (run_find(rv, rv.meta->dir_bits, static_cast<u32>(best), w) !=
+        Row row;
+        if (run_find(rv, rv.meta->dir_bits, static_cast<u32>(best), row) !=
             kInvalid) {
-          live = !(w & kTombBit);
+          live = !(row.x & kTombBit);
           break;
         }
       }
@@ -558,24 +814,19 @@ static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
 }
 
 // Writes a tombstone-free run as plain (key, value) arrays.
-static __global__ void decode_run_kernel(const u32 *__restrict__ keys,
-                                  const u32 *__restrict__ vals,
-                                  const RunMeta *meta, u32 *__restrict__ ok,
-                                  u32 *__restrict__ ov) {
+static __global__ void decode_run_kernel(const Row *__restrict__ rows,
+                                         const RunMeta *meta,
+                                         u32 *__restrict__ ok,
+                                         u32 *__restrict__ ov) {
   const u32 count = meta->count, split = meta->split;
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
        i += stride) {
     const u32 pos = static_cast<u32>(i);
-    ok[i] = decode_key(keys[pos], pos, split);
-    ov[i] = vals[pos];
+    const Row r = rows[pos];
+    ok[i] = decode_key(r.x, pos, split);
+    ov[i] = r.y;
   }
-}
-
-static __global__ void zero_words_kernel(u32 *p, u64 n) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    p[i] = 0;
 }
 
 }  // namespace lsm

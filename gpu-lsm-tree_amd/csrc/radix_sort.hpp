@@ -17,8 +17,6 @@ namespace gdict {
 namespace rs {
 
 constexpr int kThreads = 256;
-constexpr int kItems = 16;
-constexpr int kTile = kThreads * kItems;  // 4096
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 8;
 constexpr int kWaves = kThreads / kWave;
@@ -51,6 +49,21 @@ static __global__ __launch_bounds__(kThreads) void histogram_kernel(
   }
 }
 
+// Histogram of one digit (key >> shift) & 0xff into hist[0..256), zeroed.
+template <class K>
+__global__ __launch_bounds__(kThreads) void digit_histogram_kernel(
+    const K *__restrict__ keys, u32 n, u32 shift, u32 *__restrict__ hist) {
+  __shared__ u32 s_hist[kRadix];
+  s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const u64 stride = u64(gridDim.x) * kThreads;
+  for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride)
+    atomicAdd(&s_hist[static_cast<u32>(keys[i] >> shift) & 0xffu], 1u);
+  __syncthreads();
+  const u32 c = s_hist[threadIdx.x];
+  if (c) atomicAdd(&hist[threadIdx.x], c);
+}
+
 // Exclusive scan of each pass's histogram -> global digit offsets.
 static __global__ __launch_bounds__(kRadix) void scan_kernel(
     const u32 *__restrict__ hist, u32 *__restrict__ offsets, int passes) {
@@ -65,17 +78,21 @@ static __global__ __launch_bounds__(kRadix) void scan_kernel(
 
 // One digit pass.  status has tiles*kRadix words and must be zero;
 // *ticket must be zero.  kFirst: the payload is generated (= position).
-template <class K, bool kFirst>
-static __global__ __launch_bounds__(kThreads) void onesweep_kernel(
+// kBlock threads (256 or 512) x kItems keys per tile; threads below 256
+// own one digit each for the per-digit bookkeeping.
+template <class K, bool kFirst, int kItems, int kBlock>
+__global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K *__restrict__ keys_in, const u32 *__restrict__ vals_in,
     K *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
     u32 shift, const u32 *__restrict__ digit_offsets, u64 *status,
     u32 *ticket) {
+  constexpr int kTile = kBlock * kItems;
+  constexpr int kBlockWaves = kBlock / kWave;
   __shared__ u32 s_tile;
-  __shared__ u32 s_wave_hist[kWaves][kRadix];
+  __shared__ u32 s_wave_hist[kBlockWaves][kRadix];
   __shared__ u32 s_block_start[kRadix];
   __shared__ u32 s_global_base[kRadix];
-  __shared__ u32 s_scan[kWaves + 1];
+  __shared__ u32 s_scan[kBlockWaves + 1];
   __shared__ K s_keys[kTile];
   __shared__ u32 s_vals[kTile];
 
@@ -83,7 +100,7 @@ static __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   const u32 lane = lane_id();
   const u32 wave = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  for (int i = tid; i < kWaves * kRadix; i += kThreads)
+  for (int i = tid; i < kBlockWaves * kRadix; i += kBlock)
     (&s_wave_hist[0][0])[i] = 0;
   __syncthreads();
   const u32 tile = s_tile;
@@ -128,17 +145,17 @@ static __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   __syncthreads();
 
   // Per digit: wave prefixes, tile total, global base via look-back.
-  const u32 d = tid;  // kThreads == kRadix
+  const u32 d = tid;
   u32 total = 0;
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    const u32 c = s_wave_hist[w][d];
-    s_wave_hist[w][d] = total;
-    total += c;
-  }
-  // look-back: every thread owns one digit column of the status array
   u64 exclusive = 0;
-  {
+  if (tid < kRadix) {
+#pragma unroll
+    for (int w = 0; w < kBlockWaves; ++w) {
+      const u32 c = s_wave_hist[w][d];
+      s_wave_hist[w][d] = total;
+      total += c;
+    }
+    // look-back: every digit owns one column of the status array
     u64 *column = status;  // status[t * kRadix + d]
     if (tile == 0) {
       store_relaxed(&column[d], kFlagPrefix | total);
@@ -158,10 +175,12 @@ static __global__ __launch_bounds__(kThreads) void onesweep_kernel(
                     kFlagPrefix | (exclusive + total));
     }
   }
-  const u32 block_start = block_exclusive_scan<kThreads>(total, s_scan);
-  s_block_start[d] = block_start;
-  s_global_base[d] =
-      digit_offsets[d] + static_cast<u32>(exclusive) - block_start;
+  const u32 block_start = block_exclusive_scan<kBlock>(total, s_scan);
+  if (tid < kRadix) {
+    s_block_start[d] = block_start;
+    s_global_base[d] =
+        digit_offsets[d] + static_cast<u32>(exclusive) - block_start;
+  }
   __syncthreads();
 
   // Scatter into LDS in digit order.
@@ -180,7 +199,7 @@ static __global__ __launch_bounds__(kThreads) void onesweep_kernel(
   const u32 count =
       base >= n ? 0u : static_cast<u32>(n - base < u64(kTile) ? n - base
                                                               : u64(kTile));
-  for (u32 i = tid; i < count; i += kThreads) {
+  for (u32 i = tid; i < count; i += kBlock) {
     const K k = s_keys[i];
     const u32 pos = s_global_base[static_cast<u32>(k >> shift) & 0xffu] + i;
     keys_out[pos] = k;
@@ -194,13 +213,42 @@ static __global__ void zero_u32_kernel(u32 *p, u64 n) {
     p[i] = 0;
 }
 
-inline u32 tiles_for(u64 n) { return static_cast<u32>((n + kTile - 1) / kTile); }
+// Items per thread (4096-key tiles).  2048-key tiles were measured slower
+// for 2^20-key batches on MI355X (more look-back steps per digit).
+inline int items_for(u64 n) {
+  (void)n;
+  return 16;
+}
+
+// Tile = 4096 keys: 512 threads x 8 keys below 2^22 keys (twice the waves
+// per CU for the 256-tile sorts of 2^20-key batches), 256 x 16 above.
+inline int block_for(u64 n) { return n < (u64(1) << 22) ? 512 : 256; }
+
+inline u32 tiles_for(u64 n) {
+  const u64 tile = 4096;
+  (void)items_for(n);
+  return static_cast<u32>((n + tile - 1) / tile);
+}
 
 // Workspace words (u32) needed by sort_pairs for n keys, not counting the
 // two ping-pong buffers.
 inline u64 control_words(u64 n, int passes = 4) {
   // hist[8*256] + offsets[8*256] + tickets[64] + status[passes][tiles*256]
   return 4096 + 64 + 2ull * passes * u64(tiles_for(n)) * kRadix;
+}
+
+template <class K, bool kFirst>
+inline void launch_pass(int items, u32 tiles, const K *src_k,
+                        const u32 *src_v, K *dst_k, u32 *dst_v, u32 n,
+                        u32 shift, const u32 *offsets, u64 *status,
+                        u32 *ticket, hipStream_t stream) {
+  (void)items;
+  if (block_for(n) == 512)
+    onesweep_kernel<K, kFirst, 8, 512><<<tiles, 512, 0, stream>>>(
+        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket);
+  else
+    onesweep_kernel<K, kFirst, 16, 256><<<tiles, 256, 0, stream>>>(
+        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket);
 }
 
 // Sorts n (key, position) pairs stably.  keys_in is read only; the sorted
@@ -213,6 +261,7 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
                              hipStream_t stream) {
   constexpr int kPasses = passes_for<K>();
   if (n == 0) return hipSuccess;
+  const int items = items_for(n);
   const u32 tiles = tiles_for(n);
   u32 *hist = control;
   u32 *offsets = control + 2048;
@@ -223,7 +272,7 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
                                                    4096)),
                     256, 0, stream>>>(control, zero_words);
   const u32 hist_blocks = static_cast<u32>(
-      std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 1024));
+      std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 2048));
   histogram_kernel<K><<<hist_blocks, kThreads, 0, stream>>>(keys_in, n, hist);
   scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets, kPasses);
   const K *src_k = keys_in;
@@ -233,12 +282,12 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   for (int p = 0; p < kPasses; ++p) {
     u64 *pass_status = status + u64(p) * tiles * kRadix;
     if (p == 0) {
-      onesweep_kernel<K, true><<<tiles, kThreads, 0, stream>>>(
-          src_k, src_v, dst_k, dst_v, n, 0, offsets, pass_status, tickets);
+      launch_pass<K, true>(items, tiles, src_k, src_v, dst_k, dst_v, n, 0,
+                           offsets, pass_status, tickets, stream);
     } else {
-      onesweep_kernel<K, false><<<tiles, kThreads, 0, stream>>>(
-          src_k, src_v, dst_k, dst_v, n, 8u * p, offsets + p * kRadix,
-          pass_status, tickets + p);
+      launch_pass<K, false>(items, tiles, src_k, src_v, dst_k, dst_v, n,
+                            8u * p, offsets + p * kRadix, pass_status,
+                            tickets + p, stream);
     }
     src_k = dst_k;
     src_v = dst_v;
@@ -250,6 +299,35 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
       dst_v = vals_b;
     }
   }
+  return hipGetLastError();
+}
+
+// Stable partition of n (key, position) pairs by one 8-bit digit
+// (key >> shift) & 0xff: one histogram, one scan and one onesweep pass.
+// Results land in (keys_out, pos_out).  `control` holds
+// control_words(n, 1) words.
+template <class K>
+inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
+                                  K *keys_out, u32 *pos_out, u32 *control,
+                                  hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const int items = items_for(n);
+  const u32 tiles = tiles_for(n);
+  u32 *hist = control;
+  u32 *offsets = control + 2048;
+  u32 *tickets = control + 4096;
+  u64 *status = reinterpret_cast<u64 *>(control + 4096 + 64);
+  const u64 zero_words = control_words(n, 1);
+  zero_u32_kernel<<<static_cast<u32>(std::min<u64>((zero_words + 255) / 256,
+                                                   4096)),
+                    256, 0, stream>>>(control, zero_words);
+  const u32 hist_blocks = static_cast<u32>(
+      std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 2048));
+  digit_histogram_kernel<K><<<hist_blocks, kThreads, 0, stream>>>(
+      keys_in, n, shift, hist);
+  scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets, 1);
+  launch_pass<K, true>(items, tiles, keys_in, nullptr, keys_out, pos_out, n,
+                       shift, offsets, status, tickets, stream);
   return hipGetLastError();
 }
 

@@ -137,6 +137,64 @@ gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes);
  * Synchronous. */
 gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub);
 
+/* -------------------------------------------------------------- serving */
+
+/* No reference counterpart: a continuous-batching engine over one LSM
+ * dictionary.  Requests of any size are submitted from any thread and
+ * admitted into steps (arrival order, HBM budget); a step applies all its
+ * updates as one batch, then answers all its lookups and ranges with one
+ * launch each, while the next step's updates overlap on a second stream.
+ * Results equal a sequential replay in step order (updates before queries
+ * within a step).  Request buffers are device memory owned by the caller:
+ * inputs must be fully written before gd_serve_submit (the engine's streams
+ * do not wait for the caller's streams), and every buffer must stay valid
+ * until the request completes; outputs are valid once gd_serve_wait returns. */
+typedef struct gd_server gd_server;
+
+typedef enum gd_op {
+  GD_OP_INSERT = 0,      /* a = keys, b = values                         */
+  GD_OP_ERASE = 1,       /* a = keys                                     */
+  GD_OP_LOOKUP = 2,      /* a = queries, out = values, out_found optional */
+  GD_OP_RANGE_COUNT = 3, /* a = lo, b = hi, out = counts                 */
+  GD_OP_RANGE_SUM = 4    /* a = lo, b = hi, out = sums                   */
+} gd_op;
+
+typedef struct gd_serve_config {
+  gd_config lsm;               /* the dictionary                           */
+  uint64_t hbm_budget_bytes;   /* dictionary + in-flight staging cap; 0 = none */
+  uint64_t max_step_keys;      /* keys coalesced per step; 0 = 2^22 / 2^24 */
+  uint32_t max_inflight_steps; /* pipelined steps; 0 = 2                   */
+  uint32_t reserved;
+} gd_serve_config;
+
+typedef struct gd_serve_stats {
+  uint64_t requests;           /* completed since the last reset           */
+  uint64_t steps;
+  uint64_t coalesced_updates;  /* update requests merged into steps        */
+  uint64_t coalesced_queries;
+  double latency_p50_us;       /* completion - submission                  */
+  double latency_p99_us;
+  double latency_p999_us;
+  double latency_max_us;
+  uint64_t peak_staged_bytes;
+  uint64_t resident_bytes;
+} gd_serve_stats;
+
+gd_status gd_serve_create(const gd_serve_config *config, gd_server **out);
+/* Drains every submitted request, then releases the engine. */
+gd_status gd_serve_destroy(gd_server *h);
+/* GD_CAPACITY when one request alone exceeds the HBM budget. */
+gd_status gd_serve_submit(gd_server *h, gd_op op, const void *a,
+                          const void *b, uint64_t count, void *out,
+                          uint8_t *out_found, uint64_t *out_ticket);
+/* Blocks until the request completes; returns its status (an update in a
+ * step whose publication failed reports GD_PUBLICATION_FAILED); out_step
+ * (nullable) receives the step that served it. */
+gd_status gd_serve_wait(gd_server *h, uint64_t ticket, uint64_t *out_step);
+/* Blocks until every submitted request has completed. */
+gd_status gd_serve_drain(gd_server *h);
+gd_status gd_serve_get_stats(gd_server *h, gd_serve_stats *out, int reset);
+
 /* ----------------------------------------------------------------- FliX */
 
 typedef struct gd_flix gd_flix;
