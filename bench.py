@@ -43,34 +43,349 @@ def parse():
     ap.add_argument("--cpu-threads", type=int,
                     default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=2000,
+                    help="requests in the serving comparison (0 = skip)")
+    ap.add_argument("--flix-batches", type=int, default=4,
+                    help="config-3 FliX batches to run (0 = skip)")
+    ap.add_argument("--serve-rate-x", type=float, default=2.0,
+                    help="serving arrival rate as a multiple of the plain "
+                         "path's request rate")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="first run an int32 negation over BYTES bytes (a "
+                         "plain copy of known size that calibrates "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--pmc-report", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU: summarise two rocprofv3 --pmc runs "
+                         "(FETCH_SIZE, then WRITE_SIZE) of this script into "
+                         "profiles/<kernel>_pmc.txt and exit")
+    ap.add_argument("--pmc-command", default="",
+                    help="the rocprofv3 command lines, quoted in the report")
     return ap.parse_args()
+
+
+PMC_KERNELS = ("merge_kernel", "lookup_grouped_kernel", "onesweep_kernel",
+               "finalize_batch_kernel", "build_directory_kernel")
+COPY_PROBE_KERNEL = "neg_kernel"
+
+
+def progress(msg):
+    """One progress line per finished phase, on stderr."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def pmc_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each divided by
-    the copy-kernel calibration factor stated in the file)."""
+    summary: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each divided by
+    the factor the same counter shows on a plain copy of known size."""
     path = os.path.join(ROOT, "profiles", kernel + "_pmc.txt")
     if not os.path.exists(path):
         return None
     text = open(path).read()
     vals = {}
     for key in ("FETCH_KIB_PER_DISPATCH", "WRITE_KIB_PER_DISPATCH",
-                "CALIBRATION_FACTOR"):
+                "FETCH_CALIBRATION", "WRITE_CALIBRATION"):
         m = re.search(key + r"\s*[:=]\s*([0-9.eE+]+)", text)
         if not m:
             return None
         vals[key] = float(m.group(1))
-    kib = vals["FETCH_KIB_PER_DISPATCH"] + vals["WRITE_KIB_PER_DISPATCH"]
-    return kib * 1024.0 / vals["CALIBRATION_FACTOR"]
+    return int(1024.0 * (vals["FETCH_KIB_PER_DISPATCH"] /
+                         vals["FETCH_CALIBRATION"] +
+                         vals["WRITE_KIB_PER_DISPATCH"] /
+                         vals["WRITE_CALIBRATION"]))
+
+
+def _pmc_per_kernel(directory, counter):
+    """{kernel short name: [counter value per dispatch]} from the
+    counter_collection CSV that rocprofv3 --pmc --output-format csv wrote."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(directory, "**",
+                                   "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under " + directory)
+    per = {}
+    for path in files:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                for k in PMC_KERNELS + (COPY_PROBE_KERNEL,):
+                    if k in name:
+                        key = (k, row.get("Dispatch_Id"))
+                        per[key] = per.get(key, 0.0) + \
+                            float(row["Counter_Value"])
+    out = {}
+    for (k, _), v in per.items():
+        out.setdefault(k, []).append(v)
+    return out
+
+
+def pmc_report(args):
+    fetch = _pmc_per_kernel(args.pmc_report[0], "FETCH_SIZE")
+    write = _pmc_per_kernel(args.pmc_report[1], "WRITE_SIZE")
+    if COPY_PROBE_KERNEL not in fetch or COPY_PROBE_KERNEL not in write:
+        raise SystemExit("the copy probe (--copy-probe) is missing")
+    probe_kib = args.copy_probe / 1024.0
+    # the probe reads and writes probe_kib each; take the largest dispatch
+    # (the probe's own, not a smaller negation elsewhere)
+    f_cal = max(fetch[COPY_PROBE_KERNEL]) / probe_kib
+    w_cal = max(write[COPY_PROBE_KERNEL]) / probe_kib
+    for k in PMC_KERNELS:
+        if k not in fetch or k not in write:
+            continue
+        fk = float(np.mean(fetch[k]))
+        wk = float(np.mean(write[k]))
+        text = (
+            "# rocprofv3 --pmc HBM traffic of %s (config-2 bench step)\n\n"
+            "Commands (FETCH_SIZE pass, then WRITE_SIZE pass; summary by "
+            "bench.py --pmc-report):\n\n%s\n\n"
+            "Calibration: an int32 negation over %d bytes (reads and writes "
+            "that many bytes) shows FETCH_SIZE %.1f KiB and WRITE_SIZE %.1f "
+            "KiB, so each counter is divided by its factor below.\n\n"
+            "DISPATCHES = %d\n"
+            "FETCH_KIB_PER_DISPATCH = %.3f\n"
+            "WRITE_KIB_PER_DISPATCH = %.3f\n"
+            "FETCH_CALIBRATION = %.5f\n"
+            "WRITE_CALIBRATION = %.5f\n"
+            "HBM_BYTES_PER_DISPATCH = %d\n" % (
+                k, args.pmc_command or "(not given)", args.copy_probe,
+                max(fetch[COPY_PROBE_KERNEL]), max(write[COPY_PROBE_KERNEL]),
+                len(fetch[k]), fk, wk, f_cal, w_cal,
+                int(1024 * (fk / f_cal + wk / w_cal))))
+        path = os.path.join(ROOT, "profiles", k + "_pmc.txt")
+        with open(path, "w") as f:
+            f.write(text)
+        print(path)
+        print(text)
+
+
+def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
+    """Plain bulk vs serving on a mixed request stream over the 2^26-key
+    index.  Plain: every request is one synchronous call, back to back
+    (latency = call time, no queueing).  Serving: the same requests arrive
+    as an open-loop Poisson stream at serve_rate_x times the plain request
+    rate (latency = completion - submission, queueing included), then all
+    at once (saturated throughput)."""
+    import threading
+    rs = wl.RequestStream(wl.SEEDS[4], args.requests).requests(inserted)
+    ops = int(sum(len(r[1]) for r in rs))
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)) \
+            .to(dev)
+
+    staged = []
+    for kind, a, b in rs:
+        da = up(a)
+        db = up(b) if b is not None else None
+        out = torch.empty(len(a), dtype=torch.int32, device=dev) \
+            if kind in ("lookup", "range") else None
+        staged.append((kind, da, db, out, len(a)))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    # every engine below starts from the dictionary's current contents
+    live = d.dump_sorted()
+    live_k = torch.empty(max(live, 1), dtype=torch.int32, device=dev)
+    live_v = torch.empty(max(live, 1), dtype=torch.int32, device=dev)
+    d.dump_sorted(live_k, live_v, live)
+
+    # ---- plain bulk: one synchronous call per request, in arrival order,
+    # on a dictionary holding the live pairs (room for the requests' inserts)
+    cap = w.n_keys * 2
+    d.close()
+    d = pkg.GPULSMOpt(max_elements=cap, batch_capacity=w.batch)
+    d.insert(live_k, live_v, count=live, stream=stream)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for kind, da, db, out, n in staged:
+        h0 = time.perf_counter()
+        if kind == "insert":
+            d.insert(da, db, count=n, stream=stream)
+        elif kind == "erase":
+            d.erase(da, count=n, stream=stream)
+        elif kind == "lookup":
+            d.lookup(da, out, count=n, stream=stream)
+        else:
+            d.range_count(da, db, out, count=n, stream=stream)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - h0) * 1e6)
+    plain_s = time.perf_counter() - t0
+    lat = np.array(lat)
+    plain = {"mops": round(ops / plain_s / 1e6, 3),
+             "requests_per_s": round(len(rs) / plain_s, 1),
+             "p50_us": round(float(np.percentile(lat, 50)), 1),
+             "p99_us": round(float(np.percentile(lat, 99)), 1),
+             "seconds": round(plain_s, 4)}
+
+    # ---- serving engine holding the same contents (the live pairs of the
+    # plain dictionary before its requests, loaded as one insert request)
+    def loaded_server():
+        srv = pkg.Server(max_elements=cap, batch_capacity=w.batch,
+                         max_inflight_steps=3)
+        srv.wait(srv.submit(pkg.Server.INSERT, live_k, live_v, count=live))
+        srv.drain()
+        srv.stats(reset=True)
+        return srv
+
+    def submit_one(srv, req):
+        kind, da, db, out, n = req
+        op = {"insert": pkg.Server.INSERT, "erase": pkg.Server.ERASE,
+              "lookup": pkg.Server.LOOKUP,
+              "range": pkg.Server.RANGE_COUNT}[kind]
+        return srv.submit(op, da, db, out, None, count=n)
+
+    # open loop at serve_rate_x times the plain request rate
+    srv = loaded_server()
+    rate = args.serve_rate_x * len(rs) / plain_s
+    gaps = -np.log(np.maximum((wl.stream(wl.SEEDS[5], len(rs)) >>
+                                np.uint64(11)).astype(np.float64) *
+                               2.0 ** -53, 1e-300)) / rate
+    arrivals = np.cumsum(gaps)
+    lag = 0.0
+    t0 = time.perf_counter()
+    tickets = []
+    for i, req in enumerate(staged):
+        target = t0 + arrivals[i]
+        while True:
+            now = time.perf_counter()
+            if now >= target:
+                break
+        lag = max(lag, now - target)
+        tickets.append(submit_one(srv, req))
+    srv.drain()
+    open_s = time.perf_counter() - t0
+    st = srv.stats()
+    serving = {"offered_requests_per_s": round(rate, 1),
+               "mops": round(ops / open_s / 1e6, 3),
+               "p50_us": round(st["latency_p50_us"], 1),
+               "p99_us": round(st["latency_p99_us"], 1),
+               "p999_us": round(st["latency_p999_us"], 1),
+               "steps": int(st["steps"]),
+               "max_submit_lag_us": round(lag * 1e6, 1),
+               "seconds": round(open_s, 4)}
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+
+    # saturated: every request available at once
+    srv = loaded_server()
+    t0 = time.perf_counter()
+    tickets = [submit_one(srv, req) for req in staged]
+    srv.drain()
+    sat_s = time.perf_counter() - t0
+    st = srv.stats()
+    saturated = {"mops": round(ops / sat_s / 1e6, 3),
+                 "steps": int(st["steps"]),
+                 "p99_us": round(st["latency_p99_us"], 1),
+                 "seconds": round(sat_s, 4)}
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    d.close()
+    return {
+        "workload": "%d requests, sizes log-uniform [32, 2^18], 40%% insert "
+                    "/ 10%% erase / 40%% lookup / 10%% range count, over the "
+                    "config-2 dictionary (2^26 keys); %d ops" % (len(rs), ops),
+        "plain": plain, "serving_open_loop": serving,
+        "serving_saturated": saturated,
+        "throughput_ratio_open_loop": round(serving["mops"] / plain["mops"],
+                                            3),
+        "throughput_ratio_saturated": round(saturated["mops"] / plain["mops"],
+                                            3),
+        "p99_ratio_open_loop": round(serving["p99_us"] / plain["p99_us"], 3),
+    }
+
+
+def flix_measurement(args, pkg, wl, torch, dev):
+    """Config 3 (FliX, uint64 keys and values): bulk build of 2^27 clustered
+    keys, then `flix_batches` batches of 2^22 mixed ops, each followed by
+    2^24 lookups and 2^18 range enumerations.  Timed with HIP events."""
+    w = wl.config3(args.scale_log)
+    built = w.build_keys()
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)) \
+            .to(dev)
+
+    f = pkg.FlixIndex(key_bytes=8, value_bytes=8, tuning="auto")
+    dk = up(built)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        return e
+
+    e0 = ev()
+    f.build_bucket_layer_only(dk, max_size=len(built) + 2 * args.flix_batches
+                              * w.batch)
+    e1 = ev()
+    e1.synchronize()
+    build_ms = e0.elapsed_time(e1)
+    upd_ms = look_ms = rng_ms = 0.0
+    pairs = 0
+    for t in range(args.flix_batches):
+        ins, vals, dels = w.batch_ops(t, built)
+        q, lo, hi = w.queries(t, built)
+        di, dv, dd = up(ins), up(vals), up(dels)
+        dq, dlo, dhi = up(q), up(lo), up(hi)
+        out = torch.empty(len(q), dtype=torch.int64, device=dev)
+        offs = torch.empty(len(lo) + 1, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        a = ev()
+        f.insert(di, dv)
+        f.remove(dd)
+        b = ev()
+        f.lookup(dq, out)
+        c = ev()
+        f.range(dlo, dhi, offs)
+        total = int(offs[-1].item())
+        ok = torch.empty(max(total, 1), dtype=torch.int64, device=dev)
+        ov = torch.empty(max(total, 1), dtype=torch.int64, device=dev)
+        f.range(dlo, dhi, offs, ok, ov)
+        e = ev()
+        e.synchronize()
+        upd_ms += a.elapsed_time(b)
+        look_ms += b.elapsed_time(c)
+        rng_ms += c.elapsed_time(e)
+        pairs += total
+    nb = max(args.flix_batches, 1)
+    ops = args.flix_batches * (w.batch + w.lookups + w.ranges)
+    total_ms = upd_ms + look_ms + rng_ms
+    return {
+        "workload": "config3: FliX uint64 keys/values, build %d clustered "
+                    "keys, %d batches of %d mixed ops (75%% ins / 25%% del) "
+                    "+ %d lookups + %d range enumerations each"
+                    % (len(built), args.flix_batches, w.batch, w.lookups,
+                       w.ranges),
+        "build_mkeys_per_s": round(len(built) / build_ms / 1e3, 1),
+        "mixed_mops": round(ops / total_ms / 1e3, 1) if total_ms else None,
+        "update_mops": round(args.flix_batches * w.batch / upd_ms / 1e3, 1)
+        if upd_ms else None,
+        "lookup_mops": round(args.flix_batches * w.lookups / look_ms / 1e3, 1)
+        if look_ms else None,
+        "range_mpairs_per_s": round(pairs / rng_ms / 1e3, 1)
+        if rng_ms else None,
+        "ms_per_batch": {"update": round(upd_ms / nb, 3),
+                         "lookup": round(look_ms / nb, 3),
+                         "range": round(rng_ms / nb, 3)},
+        "total_keys": f.compute_total_size(),
+    }
 
 
 def main():
     args = parse()
+    if args.pmc_report:
+        pmc_report(args)
+        return
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: every box here has one GPU, "
-              "and the dictionary is single-GPU (see the sharding note in "
-              "DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the dictionary lives on a single "
+              "device and the multi-GPU (sharded) layout is not built. Use "
+              "--gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -80,6 +395,13 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe // 4, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---------------------------------------------------------- workload
     t0 = time.time()
@@ -92,6 +414,7 @@ def main():
     queries = w.lookups(inserted)
     lo, hi = w.ranges()
     gen_s = time.time() - t0
+    progress("workload generated in %.1f s" % gen_s)
 
     def up(a, dt=np.int32):
         return torch.from_numpy(np.ascontiguousarray(a).view(dt)).to(dev)
@@ -167,31 +490,51 @@ def main():
     value = n_ops / (ms_per_step * 1e-3) / 1e6
     lat_arr = np.array(lats)
 
+    progress("timed steps done")
     # ---------------------------------------------------------- roofline
+    # One more (untimed) step with the library's kernel profile on: HIP
+    # events on the launch stream around every merge and lookup kernel.
+    d.set_profiling(True)
+    step(False)
+    torch.cuda.synchronize()
+    prof = d.read_profile(reset=True)
+    d.set_profiling(False)
     runs, rows = shape
     found = d_found.cpu().numpy()
     counts = d_cnt.cpu().numpy().view(np.uint32).astype(np.uint64)
     Q = len(queries)
-    lookup_ms = ph_tot["lookup"] / args.steps
-    lookup_bytes = Q * (4 + 4 + 1) + min(rows * 8, Q * runs * 64)
     R = len(lo)
-    range_ms = ph_tot["range"] / args.steps
-    range_bytes = R * (2 * 4 + 4) + min(rows * 8, int(counts.sum()) * 8 +
-                                        R * runs * 64)
-    kernels = {
-        "lookup_kernel": (lookup_ms, lookup_bytes),
-        "range_kernel": (range_ms, range_bytes),
-    }
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    dom_ms, dom_bytes = kernels[dom]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom,
+
+    def roof(name, ms, nbytes, launches, terms):
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": name,
                 "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": pmc_traffic(dom),
-                "algorithmic_bytes": int(dom_bytes),
-                "avg_launch_ms": round(dom_ms, 4),
-                "terms": {"Q": Q, "R_rows": int(rows), "L_runs": int(runs)}}
+                "traffic": pmc_traffic(name),
+                "algorithmic_bytes_per_launch": int(nbytes / launches),
+                "avg_launch_ms": round(ms / launches, 4),
+                "launches": int(launches), "terms": terms}
+
+    kernels = []
+    if prof["merge_launches"]:
+        kernels.append(roof(
+            "merge_kernel", prof["merge_ms"],
+            (prof["merge_rows_in"] + prof["merge_rows_out"]) * 8,
+            prof["merge_launches"],
+            {"rows_in_S": int(prof["merge_rows_in"]),
+             "rows_out_M": int(prof["merge_rows_out"]),
+             "bytes_per_row": 8}))
+    if prof["lookup_launches"]:
+        grouped = Q >= (1 << d.get_tuning()["grouped_lookup_min_log"])
+        kernels.append(roof(
+            "lookup_grouped_kernel" if grouped else "lookup_kernel",
+            prof["lookup_ms"],
+            Q * (4 + 4 + 1) + min(rows * 8, Q * runs * 64),
+            prof["lookup_launches"],
+            {"Q": Q, "R_rows": int(rows), "L_runs": int(runs),
+             "formula": "Q*(K+V+1) + min(R*(K+V), Q*L*64)"}))
+    kernels.sort(key=lambda k: -k["avg_launch_ms"] * k["launches"])
+    roofline = dict(kernels[0]) if kernels else None
 
     result = {
         "metric": METRIC,
@@ -220,9 +563,11 @@ def main():
         "phase_ms": {k: round(v / args.steps, 4) for k, v in ph_tot.items()},
         "lookup_hit_rate": round(float(found.mean()), 4),
         "roofline": roofline,
+        "roofline_kernels": kernels,
         "workload_gen_s": round(gen_s, 2),
     }
 
+    progress("kernel profile done")
     # ------------------------------------------------------ CPU baseline
     if not args.no_cpu:
         import oracle as orc
@@ -254,6 +599,14 @@ def main():
             "seconds_measured": round(c3 - c0, 3),
             "seconds_full_step_est": round(cpu_s, 3)}
         result["bit_exact_vs_cpu"] = bool(exact)
+    if args.requests > 0:
+        progress("cpu baseline done")
+        result["serving"] = serving_comparison(args, pkg, wl, torch, dev, d,
+                                               w, inserted)
+        progress("serving comparison done")
+    if args.flix_batches > 0:
+        d.close()
+        result["flix"] = flix_measurement(args, pkg, wl, torch, dev)
     print(json.dumps(result))
 
 

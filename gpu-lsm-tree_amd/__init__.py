@@ -95,6 +95,41 @@ class _FlixConfig(ctypes.Structure):
     ]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [
+        ("dir_rows_log", ctypes.c_uint32),
+        ("merge_tile_log", ctypes.c_uint32),
+        ("sort_block", ctypes.c_uint32),
+        ("grouped_lookup_min_log", ctypes.c_uint32),
+        ("small_batch_max", ctypes.c_uint32),
+        ("max_runs", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
+    ]
+
+
+TUNING_FIELDS = ("dir_rows_log", "merge_tile_log", "sort_block",
+                 "grouped_lookup_min_log", "small_batch_max", "max_runs")
+
+
+def _tuning(d):
+    t = _Tuning()
+    for k, v in (d or {}).items():
+        if k not in TUNING_FIELDS:
+            raise InvalidArgument(GD_INVALID_ARGUMENT, "unknown knob " + k)
+        setattr(t, k, int(v))
+    return t
+
+
+def tuned_config(section="lsm"):
+    """The autotuner's choices (tuned.json next to this file), or {}."""
+    import json
+    path = os.path.join(_HERE, "tuned.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get(section, {})
+
+
 class _ServeConfig(ctypes.Structure):
     _fields_ = [
         ("lsm", _Config),
@@ -102,6 +137,7 @@ class _ServeConfig(ctypes.Structure):
         ("max_step_keys", ctypes.c_uint64),
         ("max_inflight_steps", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+        ("tuning", ctypes.POINTER(_Tuning)),
     ]
 
 
@@ -117,6 +153,18 @@ class _ServeStats(ctypes.Structure):
         ("latency_max_us", ctypes.c_double),
         ("peak_staged_bytes", ctypes.c_uint64),
         ("resident_bytes", ctypes.c_uint64),
+    ]
+
+
+class _Profile(ctypes.Structure):
+    _fields_ = [
+        ("merge_launches", ctypes.c_uint64),
+        ("merge_ms", ctypes.c_double),
+        ("merge_rows_in", ctypes.c_uint64),
+        ("merge_rows_out", ctypes.c_uint64),
+        ("lookup_launches", ctypes.c_uint64),
+        ("lookup_ms", ctypes.c_double),
+        ("lookup_queries", ctypes.c_uint64),
     ]
 
 
@@ -142,6 +190,10 @@ SIGNATURES = {
     "gd_lsm_carry_status": [_P, ctypes.POINTER(ctypes.c_uint32)],
     "gd_lsm_resident_bytes": [_P, ctypes.POINTER(_U64)],
     "gd_lsm_shape": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64)],
+    "gd_lsm_set_tuning": [_P, ctypes.POINTER(_Tuning)],
+    "gd_lsm_set_profiling": [_P, _I],
+    "gd_lsm_read_profile": [_P, ctypes.POINTER(_Profile), _I],
+    "gd_lsm_get_tuning": [_P, ctypes.POINTER(_Tuning)],
     "gd_serve_create": [ctypes.POINTER(_ServeConfig), ctypes.POINTER(_P)],
     "gd_serve_destroy": [_P],
     "gd_serve_submit": [_P, ctypes.c_int, _P, _P, _U64, _P, _P,
@@ -234,11 +286,33 @@ class GPULSMOpt:
     """Mirror of GPULSMOpt (GPULSMOpt.cuh:5599) over gd_lsm_*."""
 
     def __init__(self, max_elements=0, batch_capacity=0,
-                 level_zero_capacity=0):
+                 level_zero_capacity=0, tuning=None):
+        """tuning: dict of gd_lsm_tuning knobs, or "auto" for tuned.json."""
         cfg = _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0)
         h = _P()
         _check(lib().gd_lsm_create(ctypes.byref(cfg), ctypes.byref(h)), None)
         self._h = h
+        if tuning == "auto":
+            tuning = tuned_config("lsm")
+        if tuning:
+            self.set_tuning(tuning)
+
+    def set_tuning(self, knobs):
+        t = _tuning(knobs)
+        self._call("gd_lsm_set_tuning", ctypes.byref(t))
+
+    def set_profiling(self, enable=True):
+        self._call("gd_lsm_set_profiling", 1 if enable else 0)
+
+    def read_profile(self, reset=True):
+        p = _Profile()
+        self._call("gd_lsm_read_profile", ctypes.byref(p), 1 if reset else 0)
+        return {f: getattr(p, f) for f, _ in _Profile._fields_}
+
+    def get_tuning(self):
+        t = _Tuning()
+        self._call("gd_lsm_get_tuning", ctypes.byref(t))
+        return {k: int(getattr(t, k)) for k in TUNING_FIELDS}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -333,10 +407,14 @@ class Server:
 
     def __init__(self, max_elements=0, batch_capacity=0,
                  level_zero_capacity=0, hbm_budget_bytes=0, max_step_keys=0,
-                 max_inflight_steps=0):
+                 max_inflight_steps=0, tuning=None):
+        if tuning == "auto":
+            tuning = tuned_config("lsm")
+        self._tuning = _tuning(tuning) if tuning else None
         cfg = _ServeConfig(
             _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0),
-            hbm_budget_bytes, max_step_keys, max_inflight_steps, 0)
+            hbm_budget_bytes, max_step_keys, max_inflight_steps, 0,
+            ctypes.pointer(self._tuning) if self._tuning else None)
         h = _P()
         _check(lib().gd_serve_create(ctypes.byref(cfg), ctypes.byref(h)),
                None)
@@ -386,7 +464,12 @@ class FlixIndex:
     """Mirror of the FliX updatable index over gd_flix_*."""
 
     def __init__(self, key_bytes=4, value_bytes=4, node_slots=0,
-                 fill_percent=0):
+                 fill_percent=0, tuning=None):
+        """tuning="auto": node_slots / fill_percent from tuned.json."""
+        if tuning == "auto":
+            t = tuned_config("flix")
+            node_slots = node_slots or t.get("node_slots", 0)
+            fill_percent = fill_percent or t.get("fill_percent", 0)
         cfg = _FlixConfig(key_bytes, value_bytes, node_slots, fill_percent)
         h = _P()
         _check(lib().gd_flix_create(ctypes.byref(cfg), ctypes.byref(h)), None)

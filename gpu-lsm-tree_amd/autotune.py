@@ -1,0 +1,220 @@
+"""Autotuner for the launch-shape knobs (run on the target GPU).
+
+It measures, on the GPU at hand, the knobs that change speed and never a
+result:
+
+* LSM (gd_lsm_tuning): directory density, merge tile, onesweep block shape,
+  the batch size from which lookups are grouped, and the fused small-batch
+  limit;
+* serving: keys coalesced per step and steps in flight;
+* FliX: slots per node and initial fill.
+
+It writes gpu-lsm-tree_amd/tuned.json, which GPULSMOpt(tuning="auto") and
+Server(tuning="auto") load.  Search is coordinate descent from the defaults
+over a config-2-shaped workload scaled by 2^-scale_log (a few minutes).
+
+    python gpu-lsm-tree_amd/autotune.py [--scale-log 2] [--out PATH]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(_HERE, "tuned.json")
+
+LSM_SPACE = {
+    "dir_rows_log": [2, 3, 4],
+    "merge_tile_log": [11, 12],
+    "sort_block": [256, 512],
+    "grouped_lookup_min_log": [16, 18, 20, 63],
+    "small_batch_max": [1024, 4096],
+    "max_runs": [6, 10, 16],
+}
+LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 512,
+               "grouped_lookup_min_log": 18, "small_batch_max": 4096,
+               "max_runs": 10}
+SERVE_SPACE = {"max_step_keys": [1 << 20, 1 << 22, 1 << 24],
+               "max_inflight_steps": [1, 2, 3]}
+FLIX_SPACE = {"node_slots": [16, 32, 64], "fill_percent": [50, 75]}
+
+
+def _pkg():
+    if "gdict_amd" in sys.modules:
+        return sys.modules["gdict_amd"]
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "gdict_amd", os.path.join(_HERE, "__init__.py"),
+        submodule_search_locations=[_HERE])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gdict_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class LsmBench:
+    """A config-2-shaped step: mixed batches, grouped lookups, ranges."""
+
+    def __init__(self, pkg, torch, dev, scale_log):
+        import gdict_amd.workload as wl
+        self.pkg, self.torch, self.dev = pkg, torch, dev
+        w = wl.config2(scale_log)
+        b = list(w.batches())
+        keys = np.concatenate([x[0] for x in b])
+        vals = np.concatenate([x[1] for x in b])
+        erase = np.concatenate([x[2] for x in b])
+        q = w.lookups(keys[erase == 0])
+        lo, hi = w.ranges()
+        up = lambda a, dt=np.int32: torch.from_numpy(  # noqa: E731
+            np.ascontiguousarray(a).view(dt)).to(dev)
+        self.w = w
+        self.k, self.v, self.e = up(keys), up(vals), up(erase, np.uint8)
+        self.q, self.lo, self.hi = up(q), up(lo), up(hi)
+        self.out = torch.empty(len(q), dtype=torch.int32, device=dev)
+        self.found = torch.empty(len(q), dtype=torch.uint8, device=dev)
+        self.cnt = torch.empty(len(lo), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+
+    def step(self, d):
+        w = self.w
+        d.clear()
+        for t in range(w.n_batches):
+            a = t * w.batch
+            n = min(w.batch, w.n_keys - a)
+            d.apply_mixed(self.k[a:a + n], self.v[a:a + n],
+                          self.e[a:a + n], count=n)
+        d.lookup(self.q, self.out, self.found)
+        d.range_count(self.lo, self.hi, self.cnt)
+
+    def time_ms(self, knobs, reps=3):
+        d = self.pkg.GPULSMOpt(max_elements=self.w.n_keys,
+                               batch_capacity=self.w.batch, tuning=knobs)
+        self.step(d)
+        self.torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.step(d)
+        self.torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        d.close()
+        return ms
+
+
+def coordinate_descent(space, start, measure, log):
+    best = dict(start)
+    best_ms = measure(best)
+    log.append({"knobs": dict(best), "ms": best_ms})
+    for knob, values in space.items():
+        for v in values:
+            if v == best[knob]:
+                continue
+            trial = dict(best)
+            trial[knob] = v
+            ms = measure(trial)
+            log.append({"knobs": dict(trial), "ms": ms})
+            if ms < best_ms * 0.99:  # ignore noise-level differences
+                best, best_ms = trial, ms
+    return best, best_ms
+
+
+def tune_serve(pkg, torch, dev, lsm_knobs, scale_log, log):
+    import gdict_amd.workload as wl
+    n_base = 1 << (24 - scale_log)
+    base = wl.uniform31(0xBA5E, n_base)
+    reqs = wl.RequestStream(wl.SEEDS[4], 400, 32, 1 << (18 - scale_log)) \
+        .requests(base)
+    up = lambda a: torch.from_numpy(  # noqa: E731
+        np.ascontiguousarray(a).view(np.int32)).to(dev)
+    staged = []
+    for kind, a, b in reqs:
+        out = torch.empty(len(a), dtype=torch.int32, device=dev) \
+            if kind in ("lookup", "range") else None
+        staged.append((kind, up(a), up(b) if b is not None else None, out,
+                       len(a)))
+    bk, bv = up(base), up(wl.murmur3_u32(base))
+    torch.cuda.synchronize()
+    S = pkg.Server
+    opmap = {"insert": S.INSERT, "erase": S.ERASE, "lookup": S.LOOKUP,
+             "range": S.RANGE_COUNT}
+
+    def measure(knobs):
+        srv = S(max_elements=4 * n_base, batch_capacity=1 << 20,
+                tuning=lsm_knobs, **knobs)
+        srv.wait(srv.submit(S.INSERT, bk, bv))
+        srv.drain()
+        t0 = time.perf_counter()
+        for kind, a, b, out, n in staged:
+            srv.submit(opmap[kind], a, b, out, None, count=n)
+        srv.drain()
+        ms = (time.perf_counter() - t0) * 1e3
+        srv.close()
+        return ms
+
+    return coordinate_descent(SERVE_SPACE, {"max_step_keys": 1 << 22,
+                                            "max_inflight_steps": 2},
+                              measure, log)
+
+
+def tune_flix(pkg, torch, dev, scale_log, log):
+    rng = np.random.default_rng(3)
+    n = 1 << (24 - scale_log)
+    runs = n >> 12
+    starts = rng.integers(2, 1 << 62, runs, dtype=np.uint64)
+    keys = (starts[:, None] + np.arange(4096, dtype=np.uint64)).ravel()
+    ins = rng.integers(2, 1 << 62, n // 4, dtype=np.uint64)
+    q = keys[rng.integers(0, len(keys), n)]
+    up = lambda a: torch.from_numpy(  # noqa: E731
+        np.ascontiguousarray(a).view(np.int64)).to(dev)
+    dk, di, dq = up(keys), up(ins), up(q)
+    out = torch.empty(len(q), dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+
+    def measure(knobs):
+        f = pkg.FlixIndex(key_bytes=8, value_bytes=8, **knobs)
+        f.build_bucket_layer_only(dk, max_size=2 * n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f.insert(di, di)
+        f.lookup(dq, out)
+        f.remove(di[: len(ins) // 2])
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        f.close()
+        return ms
+
+    return coordinate_descent(FLIX_SPACE, {"node_slots": 32,
+                                           "fill_percent": 50}, measure, log)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale-log", type=int, default=2)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    args = ap.parse_args(argv)
+    import torch
+    pkg = _pkg()
+    dev = torch.device("cuda:0")
+    log = {"lsm": [], "serve": [], "flix": []}
+    bench = LsmBench(pkg, torch, dev, args.scale_log)
+    lsm, lsm_ms = coordinate_descent(LSM_SPACE, LSM_DEFAULT, bench.time_ms,
+                                     log["lsm"])
+    print("lsm", lsm, round(lsm_ms, 3), flush=True)
+    serve, serve_ms = tune_serve(pkg, torch, dev, lsm, args.scale_log,
+                                 log["serve"])
+    print("serve", serve, round(serve_ms, 3), flush=True)
+    flix, flix_ms = tune_flix(pkg, torch, dev, args.scale_log, log["flix"])
+    print("flix", flix, round(flix_ms, 3), flush=True)
+    result = {"device": torch.cuda.get_device_name(0),
+              "scale_log": args.scale_log, "lsm": lsm, "serve": serve,
+              "flix": flix, "measurements_ms": log}
+    with open(args.out, "w") as f:
+        json.dump(result, f, indent=1)
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

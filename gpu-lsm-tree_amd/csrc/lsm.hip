@@ -132,6 +132,25 @@ gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes) {
   GD_LSM_CALL(h, *out_bytes = h->resident_bytes());
 }
 
+gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning) {
+  if (!tuning) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, h->set_tuning(*tuning));
+}
+
+gd_status gd_lsm_get_tuning(gd_lsm *h, gd_lsm_tuning *out) {
+  if (!out) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, *out = h->tuning());
+}
+
+gd_status gd_lsm_set_profiling(gd_lsm *h, int enable) {
+  GD_LSM_CALL(h, h->set_profiling(enable != 0));
+}
+
+gd_status gd_lsm_read_profile(gd_lsm *h, gd_lsm_profile *out, int reset) {
+  if (!out) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, h->read_profile(out, reset != 0));
+}
+
 gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub) {
   if (!out_runs || !out_rows_ub) return GD_INVALID_ARGUMENT;
   GD_LSM_CALL(h, h->shape(out_runs, out_rows_ub));

@@ -52,12 +52,12 @@ __device__ __forceinline__ u32 decode_key(u32 w, u32 pos, u32 split) {
   return (w & kLowMask) | (pos >= split ? kTombBit : 0u);
 }
 
-__host__ __device__ __forceinline__ u32 dir_bits_for(u64 count,
-                                                     u32 max_bits) {
-  // about 8 rows per bucket
+// Directory bits for `count` rows: about 2^rows_log rows per bucket.
+__host__ __device__ __forceinline__ u32 dir_bits_for(u64 count, u32 max_bits,
+                                                     u32 rows_log = 3) {
   u32 lg = 0;
   while (lg < 63 && (u64(1) << (lg + 1)) <= count) ++lg;
-  u32 b = lg > 3 ? lg - 3 : 1;
+  u32 b = lg > rows_log ? lg - rows_log : 1;
   if (b < 1) b = 1;
   if (b > max_bits) b = max_bits;
   return b;
@@ -218,7 +218,7 @@ constexpr int kSmallWaves = kSmallThreads / kWave;
 static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
     const u32 *__restrict__ keys, u32 n, const u32 *__restrict__ values,
     const u8 *__restrict__ erase_flags, int mode, Row *__restrict__ out,
-    RunMeta *meta, u32 *__restrict__ dir, u32 max_bits) {
+    RunMeta *meta, u32 *__restrict__ dir, u32 max_bits, u32 rows_log) {
   // 57 KB of LDS: keys (2 x 16 KB), 16-bit positions (2 x 8 KB), per-wave
   // digit counters (8 KB)
   __shared__ u32 s_k0[kSmallTile];
@@ -363,7 +363,7 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
     if (tombs) atomicAdd(&s_misc[2], tombs);
   }
   __syncthreads();
-  const u32 b = dir_bits_for(count, max_bits);
+  const u32 b = dir_bits_for(count, max_bits, rows_log);
   const u32 shift = 32u - b;
   const u32 nb = 1u << b;
   if (tid == 0) {
@@ -392,10 +392,10 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
 static __global__ void build_directory_kernel(const Row *__restrict__ rows,
                                               RunMeta *meta,
                                               u32 *__restrict__ dir,
-                                              u32 max_bits) {
+                                              u32 max_bits, u32 rows_log) {
   const u32 count = meta->count;
   const u32 split = meta->split;
-  const u32 b = dir_bits_for(count, max_bits);
+  const u32 b = dir_bits_for(count, max_bits, rows_log);
   const u32 shift = 32u - b;
   const u32 nb = 1u << b;
   if (blockIdx.x == 0 && threadIdx.x == 0) meta->dir_bits = b;
@@ -423,9 +423,6 @@ static __global__ void build_directory_kernel(const Row *__restrict__ rows,
 
 // --------------------------------------------------------------- merging
 constexpr int kMergeThreads = 256;
-constexpr int kMergeItems = 16;
-constexpr int kMergeTile = kMergeThreads * kMergeItems;  // 4096
-constexpr u32 kStageHalf = (kMergeTile + 2) / 2;  // rows staged in s_key
 
 struct MergeSide {
   const Row *rows;
@@ -457,7 +454,7 @@ __device__ __forceinline__ u32 merge_path(const MergeSide &a, u32 ca, u32 sa,
 // Tile split points of the merge; also clears the merge's chained-scan
 // state and the output run's header (one launch instead of three).
 static __global__ void merge_partition_kernel(MergeSide a, MergeSide b,
-                                              u32 tiles,
+                                              u32 tiles, u32 tile_rows,
                                               u32 *__restrict__ parts,
                                               u64 *status, u32 *ticket,
                                               RunMeta *out_meta) {
@@ -473,18 +470,22 @@ static __global__ void merge_partition_kernel(MergeSide a, MergeSide b,
   }
   const u32 ca = a.meta->count, cb = b.meta->count;
   const u64 total = u64(ca) + cb;
-  u64 diag = u64(t) * kMergeTile;
+  u64 diag = u64(t) * tile_rows;
   if (diag > total) diag = total;
   parts[t] = merge_path(a, ca, a.meta->split, b, cb, b.meta->split, diag);
 }
 
 // Merges run A (newer) with run B (older) into `out`: on equal keys the A
 // row survives, and tombstones are dropped when `drop_tombstones` (the
-// output becomes the oldest run, GPULSMOpt.cuh:3175-3179).
-static __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
+// output becomes the oldest run, GPULSMOpt.cuh:3175-3179).  A tile is
+// 256 * kMergeItems output diagonals.
+template <int kMergeItems>
+__global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     MergeSide a, MergeSide b, const u32 *__restrict__ parts,
     Row *__restrict__ out, RunMeta *out_meta, int drop_tombstones,
     u64 *status, u32 *ticket) {
+  constexpr int kMergeTile = kMergeThreads * kMergeItems;
+  constexpr u32 kStageHalf = (kMergeTile + 2) / 2;  // rows staged in s_key
   __shared__ u32 s_tile;
   // inputs: decoded key, value, tombstone flag; s_key/s_val are reused as
   // the 8-byte staging of the survivors

@@ -31,8 +31,7 @@ namespace lsm {
 constexpr u64 kMaximumOperationTile = u64(1) << 20;  // GPULSMOpt.cuh:56
 constexpr u32 kBatchesPerEpoch = 16;                  // GPULSMOpt.cuh:42
 constexpr u64 kMaximumRows = 0xffffffffull;           // GPULSMOpt.cuh:59-60
-// Lookup batches at least this large are grouped by key before searching.
-constexpr u64 kGroupedLookupMin = u64(1) << 18;
+
 
 struct Run {
   Row *rows = nullptr;
@@ -79,6 +78,8 @@ class Lsm : public HandleBase {
       reclaim(/*wait=*/true);
     } catch (...) {
     }
+    clear_profile();
+    if (prof_meta_) (void)hipHostFree(prof_meta_);
     defer_ = false;
     for (auto &r : runs_) free_run(r);
     runs_.clear();
@@ -176,21 +177,27 @@ class Lsm : public HandleBase {
     order_.begin(s);
     try {
       const RunSet set = run_set();
-      if (n >= kGroupedLookupMin && n <= 0xffffffffull && !runs_.empty()) {
+      if (n >= grouped_lookup_min() && n <= 0xffffffffull && !runs_.empty()) {
         // group the queries by their top key byte first (one radix pass)
         const u32 nn = static_cast<u32>(n);
         Lease<u32> qk(pool_, n), qp(pool_, n);
         Lease<u32> ctrl(pool_, rs::control_words(n, 1));
         GD_HIP(rs::partition_pairs<u32>(q, nn, 24, qk.get(), qp.get(),
-                                        ctrl.get(), s));
+                                        ctrl.get(), s, tuning_.sort_block));
         // a multiple of 8 workgroups (one share per XCD), ~4 queries/thread
         const u32 blocks =
             (std::min<u32>(grid_for(n, 1024, 1u << 16), 1u << 16) + 7u) & ~7u;
+        ProfRec rec{};
+        if (profiling_) rec = prof_begin(1, s, n);
         lookup_grouped_kernel<<<blocks, 256, 0, s>>>(
             qk.get(), qp.get(), n, out_values, out_found, set);
+        if (profiling_) prof_end(rec, s);
       } else {
+        ProfRec rec{};
+        if (profiling_) rec = prof_begin(1, s, n);
         lookup_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
             q, n, out_values, out_found, set);
+        if (profiling_) prof_end(rec, s);
       }
       GD_LAUNCH_CHECK();
     } catch (...) {
@@ -308,6 +315,71 @@ class Lsm : public HandleBase {
 
   bool failed() const { return failed_; }
 
+  void set_tuning(const gd_lsm_tuning &t) {
+    gd_lsm_tuning v = tuning_;
+    if (t.dir_rows_log) v.dir_rows_log = std::min<u32>(t.dir_rows_log, 8);
+    if (t.merge_tile_log) {
+      if (t.merge_tile_log != 11 && t.merge_tile_log != 12)
+        throw Error(GD_INVALID_ARGUMENT, "merge_tile_log must be 11 or 12");
+      v.merge_tile_log = t.merge_tile_log;
+    }
+    if (t.sort_block) {
+      if (t.sort_block != 256 && t.sort_block != 512)
+        throw Error(GD_INVALID_ARGUMENT, "sort_block must be 256 or 512");
+      v.sort_block = t.sort_block;
+    }
+    if (t.grouped_lookup_min_log)
+      v.grouped_lookup_min_log = std::min<u32>(t.grouped_lookup_min_log, 63);
+    if (t.small_batch_max)
+      v.small_batch_max = std::min<u32>(t.small_batch_max, kSmallTile);
+    if (t.max_runs)  // max_runs
+      v.max_runs = std::max<u32>(2, std::min<u32>(t.max_runs,
+                                                     kMaxRuns - 1));
+    tuning_ = v;
+  }
+  const gd_lsm_tuning &tuning() const { return tuning_; }
+  u64 grouped_lookup_min() const {
+    return tuning_.grouped_lookup_min_log >= 63
+               ? ~0ull
+               : (u64(1) << tuning_.grouped_lookup_min_log);
+  }
+
+  // ------------------------------------------------------ kernel profile
+  // Opt-in: HIP events on the launch stream around every merge and lookup
+  // kernel, and the merge's row counts (copied to pinned memory behind the
+  // kernel) for its algorithmic bytes (S + M) * 8.
+  void set_profiling(bool on) {
+    if (on && !prof_meta_) {
+      GD_HIP(hipHostMalloc(reinterpret_cast<void **>(&prof_meta_),
+                           kProfSlots * 3 * sizeof(RunMeta),
+                           hipHostMallocDefault));
+    }
+    profiling_ = on;
+  }
+
+  void read_profile(gd_lsm_profile *out, bool reset) {
+    order_.wait_host();
+    *out = gd_lsm_profile{};
+    for (std::size_t i = 0; i < prof_.size(); ++i) {
+      ProfRec &r = prof_[i];
+      GD_HIP(hipEventSynchronize(r.end));
+      float ms = 0;
+      GD_HIP(hipEventElapsedTime(&ms, r.start, r.end));
+      if (r.kind == 0) {
+        const RunMeta *m = prof_meta_ + 3 * (r.slot % kProfSlots);
+        out->merge_launches += 1;
+        out->merge_ms += ms;
+        out->merge_rows_in += u64(m[0].count) + m[1].count;
+        out->merge_rows_out += m[2].count;
+      } else {
+        out->lookup_launches += 1;
+        out->lookup_ms += ms;
+        out->lookup_queries += r.n;
+      }
+    }
+    if (reset) clear_profile();
+  }
+
   u32 carry_status() const { return status_; }
   u64 resident_bytes() const { return pool_.held_bytes(); }
   // Synchronous: exact rows per run (refreshes the host upper bounds).
@@ -335,7 +407,7 @@ class Lsm : public HandleBase {
   Run alloc_run(u64 rows_ub) {
     Run r;
     r.rows_ub = rows_ub;
-    r.max_bits = dir_bits_for(rows_ub, kMaxDirBits);
+    r.max_bits = dir_bits_for(rows_ub, kMaxDirBits, tuning_.dir_rows_log);
     r.rows = pool_.get_n<Row>(std::max<u64>(rows_ub, 1));
     r.dir = static_cast<u32 *>(
         pool_.get(kMetaBytes + ((u64(1) << r.max_bits) + 1) * sizeof(u32)));
@@ -362,7 +434,7 @@ class Lsm : public HandleBase {
   void launch_directory(const Run &r, hipStream_t s) {
     const u64 items = std::max<u64>(r.rows_ub, (u64(1) << r.max_bits) + 1);
     build_directory_kernel<<<grid_for(items, 256, 8192), 256, 0, s>>>(
-        r.rows, r.meta(), r.entries(), r.max_bits);
+        r.rows, r.meta(), r.entries(), r.max_bits, tuning_.dir_rows_log);
     GD_LAUNCH_CHECK();
   }
 
@@ -370,11 +442,11 @@ class Lsm : public HandleBase {
   Run sorted_run(const u32 *keys, const u32 *values, const u8 *flags, u32 n,
                  int mode, hipStream_t s) {
     Run r = alloc_run(n);
-    if (n <= static_cast<u32>(kSmallTile)) {
+    if (n <= tuning_.small_batch_max) {
       // one workgroup sorts, deduplicates and indexes the whole batch
       small_batch_run_kernel<<<1, kSmallThreads, 0, s>>>(
           keys, n, values, flags, mode, r.rows, r.meta(), r.entries(),
-          r.max_bits);
+          r.max_bits, tuning_.dir_rows_log);
       GD_LAUNCH_CHECK();
       return r;
     }
@@ -388,7 +460,7 @@ class Lsm : public HandleBase {
         reinterpret_cast<u32 *>(r.meta()), 4, fin_ctrl, fin_words);
     GD_LAUNCH_CHECK();
     GD_HIP(rs::sort_pairs<u32>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
-                          ctrl.get(), s));
+                               ctrl.get(), s, tuning_.sort_block));
     finalize_batch_kernel<<<fin_tiles, kFinThreads, 0, s>>>(
         ka.get(), va.get(), n, values, flags, mode, r.rows, r.meta(),
         reinterpret_cast<u64 *>(fin_ctrl + 64), fin_ctrl);
@@ -403,8 +475,9 @@ class Lsm : public HandleBase {
                 hipStream_t s) {
     const u64 ub = a.rows_ub + b.rows_ub;
     Run out = alloc_run(ub);
+    const u32 tile_rows = tuning_.merge_tile_log == 11 ? 2048u : 4096u;
     const u32 tiles =
-        static_cast<u32>(std::max<u64>(1, (ub + kMergeTile - 1) / kMergeTile));
+        static_cast<u32>(std::max<u64>(1, (ub + tile_rows - 1) / tile_rows));
     const u64 ctrl_words = 64 + 2ull * tiles;
     Lease<u32> parts(pool_, u64(tiles) + 1);
     Lease<u32> ctrl(pool_, ctrl_words);
@@ -412,12 +485,30 @@ class Lsm : public HandleBase {
     const MergeSide sa{a.rows, a.meta()};
     const MergeSide sb{b.rows, b.meta()};
     merge_partition_kernel<<<grid_for(u64(tiles) + 1, 256), 256, 0, s>>>(
-        sa, sb, tiles, parts.get(), status, ctrl.get(), out.meta());
+        sa, sb, tiles, tile_rows, parts.get(), status, ctrl.get(),
+        out.meta());
     GD_LAUNCH_CHECK();
-    merge_kernel<<<tiles, kMergeThreads, 0, s>>>(
-        sa, sb, parts.get(), out.rows, out.meta(), drop_tombstones ? 1 : 0,
-        status, ctrl.get());
+    ProfRec rec{};
+    if (profiling_) rec = prof_begin(0, s, 0);
+    if (tile_rows == 2048)
+      merge_kernel<8><<<tiles, kMergeThreads, 0, s>>>(
+          sa, sb, parts.get(), out.rows, out.meta(), drop_tombstones ? 1 : 0,
+          status, ctrl.get());
+    else
+      merge_kernel<16><<<tiles, kMergeThreads, 0, s>>>(
+          sa, sb, parts.get(), out.rows, out.meta(), drop_tombstones ? 1 : 0,
+          status, ctrl.get());
     GD_LAUNCH_CHECK();
+    if (profiling_) {
+      prof_end(rec, s);
+      RunMeta *slot = prof_meta_ + 3 * (rec.slot % kProfSlots);
+      GD_HIP(hipMemcpyAsync(slot, a.meta(), sizeof(RunMeta),
+                            hipMemcpyDeviceToHost, s));
+      GD_HIP(hipMemcpyAsync(slot + 1, b.meta(), sizeof(RunMeta),
+                            hipMemcpyDeviceToHost, s));
+      GD_HIP(hipMemcpyAsync(slot + 2, out.meta(), sizeof(RunMeta),
+                            hipMemcpyDeviceToHost, s));
+    }
     launch_directory(out, s);
     return out;
   }
@@ -456,17 +547,41 @@ class Lsm : public HandleBase {
   }
 
   // Binary-counter policy: merge the two newest runs while the older one is
-  // no larger.  Tombstones drop when the output becomes the oldest run.
+  // no larger (equal batches: log2(N/b) merges per row).  Uneven batches can
+  // stack runs of decreasing size that never trigger it, so the number of
+  // runs is also capped: over the cap, the adjacent pair with the smallest
+  // combined size merges (adjacent runs are consecutive in age, so the
+  // newer-wins rule of the merge still holds).  Tombstones drop when the
+  // output becomes the oldest run.
   void rebalance(hipStream_t s) {
-    while (runs_.size() >= 2 &&
-           (runs_[1].rows_ub <= runs_[0].rows_ub ||
-            runs_.size() >= static_cast<std::size_t>(kMaxRuns))) {
-      const bool oldest = runs_.size() == 2;
-      Run out = merge_two(runs_[0], runs_[1], oldest, s);
-      free_run(runs_[0]);
-      free_run(runs_[1]);
-      runs_.erase(runs_.begin(), runs_.begin() + 2);
-      runs_.insert(runs_.begin(), out);
+    const std::size_t cap =
+        std::max<std::size_t>(2, std::min<std::size_t>(tuning_.max_runs
+                                                           ? tuning_.max_runs
+                                                           : 10,
+                                                       kMaxRuns - 1));
+    while (runs_.size() >= 2) {
+      std::size_t i;
+      if (runs_[1].rows_ub <= runs_[0].rows_ub) {
+        i = 0;
+      } else if (runs_.size() > cap) {
+        i = 0;
+        u64 best = ~0ull;
+        for (std::size_t j = 0; j + 1 < runs_.size(); ++j) {
+          const u64 c = runs_[j].rows_ub + runs_[j + 1].rows_ub;
+          if (c < best) {
+            best = c;
+            i = j;
+          }
+        }
+      } else {
+        break;
+      }
+      const bool oldest = i + 2 == runs_.size();
+      Run out = merge_two(runs_[i], runs_[i + 1], oldest, s);
+      free_run(runs_[i]);
+      free_run(runs_[i + 1]);
+      runs_.erase(runs_.begin() + i, runs_.begin() + i + 2);
+      runs_.insert(runs_.begin() + i, out);
     }
   }
 
@@ -514,6 +629,37 @@ class Lsm : public HandleBase {
 
   u64 batch_capacity_ = 1;
   u64 capacity_ = 0;
+  gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(kSmallTile), 10, {0, 0}};
+
+  struct ProfRec {
+    int kind;  // 0 merge, 1 lookup
+    hipEvent_t start, end;
+    u64 slot, n;
+  };
+  static constexpr u64 kProfSlots = 4096;
+  bool profiling_ = false;
+  std::vector<ProfRec> prof_;
+  RunMeta *prof_meta_ = nullptr;  // pinned: [slot][a, b, out]
+  u64 prof_next_ = 0;
+
+  ProfRec prof_begin(int kind, hipStream_t s, u64 n) {
+    ProfRec r{kind, nullptr, nullptr, prof_next_++, n};
+    GD_HIP(hipEventCreate(&r.start));
+    GD_HIP(hipEventCreate(&r.end));
+    GD_HIP(hipEventRecord(r.start, s));
+    return r;
+  }
+  void prof_end(ProfRec r, hipStream_t s) {
+    GD_HIP(hipEventRecord(r.end, s));
+    prof_.push_back(r);
+  }
+  void clear_profile() {
+    for (auto &r : prof_) {
+      (void)hipEventDestroy(r.start);
+      (void)hipEventDestroy(r.end);
+    }
+    prof_.clear();
+  }
   bool defer_ = false;
   std::vector<void *> pending_free_;
   std::deque<Deferred> deferred_;

@@ -213,20 +213,14 @@ static __global__ void zero_u32_kernel(u32 *p, u64 n) {
     p[i] = 0;
 }
 
-// Items per thread (4096-key tiles).  2048-key tiles were measured slower
-// for 2^20-key batches on MI355X (more look-back steps per digit).
-inline int items_for(u64 n) {
-  (void)n;
-  return 16;
-}
 
 // Tile = 4096 keys: 512 threads x 8 keys below 2^22 keys (twice the waves
 // per CU for the 256-tile sorts of 2^20-key batches), 256 x 16 above.
+// (2048-key tiles were measured slower on MI355X: more look-back steps.)
 inline int block_for(u64 n) { return n < (u64(1) << 22) ? 512 : 256; }
 
 inline u32 tiles_for(u64 n) {
   const u64 tile = 4096;
-  (void)items_for(n);
   return static_cast<u32>((n + tile - 1) / tile);
 }
 
@@ -238,12 +232,11 @@ inline u64 control_words(u64 n, int passes = 4) {
 }
 
 template <class K, bool kFirst>
-inline void launch_pass(int items, u32 tiles, const K *src_k,
+inline void launch_pass(int block, u32 tiles, const K *src_k,
                         const u32 *src_v, K *dst_k, u32 *dst_v, u32 n,
                         u32 shift, const u32 *offsets, u64 *status,
                         u32 *ticket, hipStream_t stream) {
-  (void)items;
-  if (block_for(n) == 512)
+  if (block == 512)
     onesweep_kernel<K, kFirst, 8, 512><<<tiles, 512, 0, stream>>>(
         src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket);
   else
@@ -258,10 +251,10 @@ inline void launch_pass(int items, u32 tiles, const K *src_k,
 template <class K>
 inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
                              K *keys_b, u32 *vals_b, u32 *control,
-                             hipStream_t stream) {
+                             hipStream_t stream, int block = 0) {
   constexpr int kPasses = passes_for<K>();
   if (n == 0) return hipSuccess;
-  const int items = items_for(n);
+  const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
   u32 *hist = control;
   u32 *offsets = control + 2048;
@@ -282,10 +275,10 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   for (int p = 0; p < kPasses; ++p) {
     u64 *pass_status = status + u64(p) * tiles * kRadix;
     if (p == 0) {
-      launch_pass<K, true>(items, tiles, src_k, src_v, dst_k, dst_v, n, 0,
+      launch_pass<K, true>(shape, tiles, src_k, src_v, dst_k, dst_v, n, 0,
                            offsets, pass_status, tickets, stream);
     } else {
-      launch_pass<K, false>(items, tiles, src_k, src_v, dst_k, dst_v, n,
+      launch_pass<K, false>(shape, tiles, src_k, src_v, dst_k, dst_v, n,
                             8u * p, offsets + p * kRadix, pass_status,
                             tickets + p, stream);
     }
@@ -309,9 +302,9 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
 template <class K>
 inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
                                   K *keys_out, u32 *pos_out, u32 *control,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, int block = 0) {
   if (n == 0) return hipSuccess;
-  const int items = items_for(n);
+  const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
   u32 *hist = control;
   u32 *offsets = control + 2048;
@@ -326,7 +319,7 @@ inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
   digit_histogram_kernel<K><<<hist_blocks, kThreads, 0, stream>>>(
       keys_in, n, shift, hist);
   scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets, 1);
-  launch_pass<K, true>(items, tiles, keys_in, nullptr, keys_out, pos_out, n,
+  launch_pass<K, true>(shape, tiles, keys_in, nullptr, keys_out, pos_out, n,
                        shift, offsets, status, tickets, stream);
   return hipGetLastError();
 }

@@ -168,6 +168,7 @@ class Server : public HandleBase {
   explicit Server(const gd_serve_config &c)
       : cfg_(c), lsm_(c.lsm) {
     lsm_.set_deferred_reclaim(true);
+    if (c.tuning) lsm_.set_tuning(*c.tuning);
     GD_HIP(hipStreamCreateWithFlags(&us_, hipStreamNonBlocking));
     GD_HIP(hipStreamCreateWithFlags(&qs_, hipStreamNonBlocking));
     max_inflight_ = c.max_inflight_steps ? c.max_inflight_steps : 2;
@@ -451,12 +452,13 @@ class Server : public HandleBase {
           dev_pieces + look_at, static_cast<u32>(look.size()), n_look, q,
           nullptr, nullptr);
       GD_LAUNCH_CHECK();
-      if (n_look >= lsm::kGroupedLookupMin && !lsm_.empty()) {
+      if (n_look >= lsm_.grouped_lookup_min() && !lsm_.empty()) {
         u32 *qk = scratch<u32>(st, n_look);
         u32 *qp = scratch<u32>(st, n_look);
         u32 *ctrl = scratch<u32>(st, rs::control_words(n_look, 1));
         GD_HIP(rs::partition_pairs<u32>(q, static_cast<u32>(n_look), 24, qk,
-                                        qp, ctrl, qs_));
+                                        qp, ctrl, qs_,
+                                        lsm_.tuning().sort_block));
         const u32 blocks =
             (std::min<u32>(grid_for(n_look, 1024, 1u << 16), 1u << 16) + 7u) &
             ~7u;

@@ -170,3 +170,107 @@ def config2(scale_log=0):
     s = scale_log
     return LsmWorkload(SEEDS[2], 1 << (26 - s), 1 << (20 - s), 1 << (26 - s),
                        1 << (20 - s), 1 << 4, 1 << 20, 0.1, 0.7)
+
+
+class RequestStream:
+    """Mixed request stream for the serving comparison: `n` requests with
+    sizes log-uniform in [smin, smax]; 40% inserts (new uniform 31-bit keys,
+    values murmur3), 10% erases (keys drawn from the base keys and earlier
+    inserts), 40% lookups (half drawn from known keys) and 10% range counts
+    (widths log-uniform in [2^4, 2^16])."""
+
+    KINDS = ("insert", "erase", "lookup", "range")
+
+    def __init__(self, seed, n, smin=32, smax=1 << 18):
+        self.seed, self.n, self.smin, self.smax = seed, n, smin, smax
+
+    def requests(self, base_keys):
+        r = stream(sub_seed(self.seed, 10), self.n)
+        sizes = log_uniform(stream(sub_seed(self.seed, 11), self.n),
+                            self.smin, self.smax).astype(np.int64)
+        u = (r >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        base = np.ascontiguousarray(base_keys, dtype=np.uint32)
+        B = len(base)
+        ins = np.empty(int(sizes.sum()), dtype=np.uint32)  # inserted so far
+        m = 0
+        out = []
+        off = 0
+        for i in range(self.n):
+            n = int(sizes[i])
+            x = stream(sub_seed(self.seed, 12), 3 * n, off)
+            off += 3 * n
+            pick = (x[n:2 * n] % np.uint64(max(B + m, 1))).astype(np.int64)
+
+            def known(p):
+                from_ins = p >= B
+                k = np.empty(len(p), dtype=np.uint32)
+                if B:
+                    k[~from_ins] = base[p[~from_ins]]
+                k[from_ins] = ins[p[from_ins] - B]
+                return k
+
+            if u[i] < 0.4:
+                k = (x[:n] >> np.uint64(33)).astype(np.uint32)
+                out.append(("insert", k, murmur3_u32(k)))
+                ins[m:m + n] = k
+                m += n
+            elif u[i] < 0.5:
+                out.append(("erase", known(pick), None))
+            elif u[i] < 0.9:
+                k = (x[:n] >> np.uint64(33)).astype(np.uint32)
+                half = n // 2
+                k[:half] = known(pick[:half])
+                out.append(("lookup", k, None))
+            else:
+                lo = (x[:n] >> np.uint64(33)).astype(np.uint32)
+                w = log_uniform(x[2 * n:], 16, 1 << 16)
+                hi = np.minimum(lo.astype(np.uint64) + w - np.uint64(1),
+                                np.uint64(2 ** 31 - 1)).astype(np.uint32)
+                out.append(("range", lo, hi))
+        return out
+
+
+class FlixWorkload:
+    """Config 3 (FliX): a bulk build of `n_build` uint64 keys in clustered
+    runs of 2^12 consecutive keys at random 64-bit offsets (kept inside
+    [2, 2^64 - 4]), then batches of `batch` mixed ops (75% inserts of new
+    uniform keys, 25% deletes of built keys), each followed by `lookups`
+    lookups of built keys and `ranges` range queries whose width targets
+    16..1024 pairs inside one run."""
+
+    def __init__(self, seed, n_build, batch, lookups, ranges):
+        self.seed, self.n_build, self.batch = seed, n_build, batch
+        self.lookups, self.ranges = lookups, ranges
+
+    def build_keys(self):
+        runs = max(1, self.n_build >> 12)
+        st = stream(sub_seed(self.seed, 20), runs)
+        starts = (st >> np.uint64(2)) + np.uint64(2)  # < 2^62 + 2
+        keys = (starts[:, None] + np.arange(4096, dtype=np.uint64)).ravel()
+        return keys[: self.n_build]
+
+    def batch_ops(self, t, built):
+        n = self.batch
+        x = stream(sub_seed(self.seed, 21), 2 * n, t * 2 * n)
+        n_ins = (3 * n) // 4
+        ins = (x[:n_ins] >> np.uint64(1)) | np.uint64(1) << np.uint64(62)
+        pick = (x[n_ins:n] % np.uint64(len(built))).astype(np.int64)
+        dels = built[pick]
+        vals = splitmix64(ins)
+        return ins, vals, dels
+
+    def queries(self, t, built):
+        x = stream(sub_seed(self.seed, 22), self.lookups + 2 * self.ranges,
+                   t * (self.lookups + 2 * self.ranges))
+        q = built[(x[: self.lookups] % np.uint64(len(built))).astype(np.int64)]
+        r = x[self.lookups:]
+        lo = built[(r[: self.ranges] % np.uint64(len(built))).astype(np.int64)]
+        w = log_uniform(r[self.ranges:], 16, 1024)
+        hi = lo + w - np.uint64(1)
+        return q, lo, hi
+
+
+def config3(scale_log=0):
+    s = scale_log
+    return FlixWorkload(SEEDS[3], 1 << (27 - s), 1 << (22 - s),
+                        1 << (24 - s), 1 << (18 - s))

@@ -132,6 +132,43 @@ gd_status gd_lsm_compact(gd_lsm *h, hipStream_t stream);
 gd_status gd_lsm_carry_status(gd_lsm *h, uint32_t *out_status);
 /* gpu_resident_bytes()  GPULSMOpt.cuh:6341 */
 gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes);
+/* No reference counterpart: launch-shape knobs picked by the autotuner
+ * (gpu-lsm-tree_amd/autotune.py).  None changes any result: they select
+ * directory density, merge tile, sort block shape and when to group
+ * lookups.  Zero fields keep the defaults. */
+typedef struct gd_lsm_tuning {
+  uint32_t dir_rows_log;           /* ~2^k rows per directory bucket (3)   */
+  uint32_t merge_tile_log;         /* merge tile 2^k rows: 11 or 12 (12)   */
+  uint32_t sort_block;             /* onesweep block: 256 or 512 (by size) */
+  uint32_t grouped_lookup_min_log; /* group lookups from 2^k queries (18);
+                                      63 = never                           */
+  uint32_t small_batch_max;        /* fused one-workgroup admission up to
+                                      this many keys, <= 4096 (4096)       */
+  uint32_t max_runs;               /* runs kept before the smallest adjacent
+                                      pair merges, 2..47 (10)              */
+  uint32_t reserved[2];
+} gd_lsm_tuning;
+
+gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning);
+gd_status gd_lsm_get_tuning(gd_lsm *h, gd_lsm_tuning *out);
+
+/* No reference counterpart: opt-in per-kernel timing (HIP events on the
+ * launch stream) of the merge and lookup kernels, with the merges' row
+ * counts for their algorithmic bytes (rows_in + rows_out) * 8. */
+typedef struct gd_lsm_profile {
+  uint64_t merge_launches;
+  double merge_ms;
+  uint64_t merge_rows_in;   /* source rows of all merges  */
+  uint64_t merge_rows_out;  /* surviving rows             */
+  uint64_t lookup_launches;
+  double lookup_ms;
+  uint64_t lookup_queries;
+} gd_lsm_profile;
+
+gd_status gd_lsm_set_profiling(gd_lsm *h, int enable);
+/* Synchronous. */
+gd_status gd_lsm_read_profile(gd_lsm *h, gd_lsm_profile *out, int reset);
+
 /* No reference counterpart: number of sorted runs currently searched by
  * reads and the rows they hold (tombstones and shadowed versions included).
  * Synchronous. */
@@ -165,6 +202,7 @@ typedef struct gd_serve_config {
   uint64_t max_step_keys;      /* keys coalesced per step; 0 = 2^22 / 2^24 */
   uint32_t max_inflight_steps; /* pipelined steps; 0 = 2                   */
   uint32_t reserved;
+  const gd_lsm_tuning *tuning; /* nullable                                 */
 } gd_serve_config;
 
 typedef struct gd_serve_stats {

@@ -13,8 +13,8 @@ NOT_FOUND = 0xFFFFFFFF
 class Pair:
     """GPU dictionary + oracle driven in lock step."""
 
-    def __init__(self, gd, orc, io, **cfg):
-        self.d = gd.GPULSMOpt(**cfg)
+    def __init__(self, gd, orc, io, tuning=None, **cfg):
+        self.d = gd.GPULSMOpt(tuning=tuning, **cfg)
         self.o = orc.LsmOracle(**cfg)
         self.io = io
         self.gd = gd
@@ -382,3 +382,75 @@ def test_config2_scaled(gd, orc, io):
     lo, hi = w.ranges()
     p.check_ranges(lo, hi)
     p.check_successor(q[:1 << 14])
+
+
+@pytest.mark.parametrize("knobs", [
+    {"dir_rows_log": 1, "merge_tile_log": 11, "sort_block": 256},
+    {"dir_rows_log": 5, "grouped_lookup_min_log": 8, "small_batch_max": 1},
+    {"max_runs": 2, "small_batch_max": 1024, "sort_block": 512},
+    {"max_runs": 40, "grouped_lookup_min_log": 63, "merge_tile_log": 12},
+])
+def test_tuning_knobs_keep_results(gd, orc, io, knobs):
+    """Launch-shape knobs change speed only: a mixed stream with small and
+    large batches gives the oracle's contents and answers under each."""
+    rng = np.random.default_rng(77)
+    p = Pair(gd, orc, io, tuning=knobs, max_elements=1 << 19,
+             batch_capacity=1 << 13)
+    got = p.d.get_tuning()
+    for k, v in knobs.items():
+        assert got[k] == v
+    for b in range(24):
+        n = int(rng.integers(1, 1 << 15)) if b % 3 else int(
+            rng.integers(1, 3000))
+        keys = rng.integers(0, 1 << 22, n, dtype=np.uint64).astype(np.uint32)
+        vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        erase = (rng.random(n) < 0.2).astype(np.uint8)
+        p.mixed(keys, vals, erase)
+    p.check_contents()
+    live = p.o.dump()[0]
+    q = _queries(rng, live, 1 << 14, 1 << 22)
+    p.check_lookup(q, True)
+    lo = rng.integers(0, 1 << 22, 2000, dtype=np.uint64)
+    hi = np.minimum(lo + rng.integers(0, 1 << 16, 2000, dtype=np.uint64),
+                    0xFFFFFFFF)
+    p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
+    p.check_successor(q[:4096])
+
+
+def test_invalid_tuning_rejected(gd):
+    for bad in ({"sort_block": 384}, {"merge_tile_log": 13},
+                {"no_such_knob": 1}):
+        with pytest.raises(gd.InvalidArgument):
+            gd.GPULSMOpt(max_elements=1 << 16, batch_capacity=1 << 10,
+                         tuning=bad)
+
+
+def test_kernel_profile_counts(gd, orc, io):
+    """The opt-in kernel profile: merge rows in/out agree with the run
+    sizes it merged and lookups count every query; reset clears it."""
+    p = Pair(gd, orc, io, max_elements=1 << 18, batch_capacity=1 << 12)
+    p.d.set_profiling(True)
+    rng = np.random.default_rng(5)
+    # distinct keys (odd multiplier: a bijection mod 2^32), so merges keep
+    # every row
+    distinct = ((np.arange(8 * 4096, dtype=np.uint64) * 2654435761) %
+                (1 << 32)).astype(np.uint32)
+    rng.shuffle(distinct)
+    for b in range(8):
+        keys = distinct[b * 4096:(b + 1) * 4096]
+        p.insert(keys, keys)
+    q = _queries(rng, p.o.dump()[0], 5000, 1 << 32)
+    p.check_lookup(q, False)
+    p.check_lookup(q[:3000], True)
+    prof = p.d.read_profile(reset=True)
+    # eight equal batches under the binary counter: 4+2+1 merges
+    assert prof["merge_launches"] == 7
+    assert prof["merge_rows_in"] == 4096 * (2 * 4 + 4 * 2 + 8 * 1)
+    assert prof["merge_rows_out"] == prof["merge_rows_in"]  # distinct keys
+    assert prof["merge_ms"] > 0
+    assert prof["lookup_launches"] == 2
+    assert prof["lookup_queries"] == 8000
+    assert prof["lookup_ms"] > 0
+    again = p.d.read_profile(reset=False)
+    assert again["merge_launches"] == 0 and again["lookup_launches"] == 0
+    p.d.set_profiling(False)
