@@ -58,6 +58,10 @@ def parse():
                     help="no GPU: summarise two rocprofv3 --pmc runs "
                          "(FETCH_SIZE, then WRITE_SIZE) of this script into "
                          "profiles/<kernel>_pmc.txt and exit")
+    ap.add_argument("--trace-report", nargs=2, metavar=("TRACE_DIR", "OUT"),
+                    help="no GPU: write the per-kernel table of a rocprofv3 "
+                         "--kernel-trace --stats --output-format csv run to "
+                         "OUT and exit")
     ap.add_argument("--pmc-command", default="",
                     help="the rocprofv3 command lines, quoted in the report")
     return ap.parse_args()
@@ -119,6 +123,38 @@ def _pmc_per_kernel(directory, counter):
     for (k, _), v in per.items():
         out.setdefault(k, []).append(v)
     return out
+
+
+def trace_report(args):
+    """Per-kernel totals from rocprofv3's kernel_stats.csv."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(args.trace_report[0], "**",
+                                   "*kernel_stats.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no kernel_stats.csv under " + args.trace_report[0])
+    rows = []
+    for path in files:
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                name = r["Name"].split("(")[0]
+                rows.append((float(r["TotalDurationNs"]) / 1e6, name,
+                             int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                             float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3))
+    rows.sort(reverse=True)
+    tot = sum(r[0] for r in rows)
+    out = ["command: " + (args.pmc_command or "(not given)"),
+           "source: rocprofv3 kernel_stats.csv (durations = end - start per "
+           "dispatch); summary by bench.py --trace-report",
+           "%9s %6s %6s %10s %10s %10s  kernel" % (
+               "total_ms", "pct", "calls", "avg_us", "min_us", "max_us")]
+    for ms, name, calls, avg, mn, mx in rows:
+        out.append("%9.3f %5.1f%% %6d %10.2f %10.2f %10.2f  %s" % (
+            ms, 100 * ms / tot, calls, avg, mn, mx, name))
+    out.append("%9.3f total ms of kernel time" % tot)
+    with open(args.trace_report[1], "w") as f:
+        f.write("\n".join(out) + "\n")
+    print("\n".join(out))
 
 
 def pmc_report(args):
@@ -381,6 +417,9 @@ def main():
     args = parse()
     if args.pmc_report:
         pmc_report(args)
+        return
+    if args.trace_report:
+        trace_report(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the dictionary lives on a single "

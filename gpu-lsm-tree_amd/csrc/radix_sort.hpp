@@ -26,54 +26,53 @@ constexpr int passes_for() {
   return static_cast<int>(sizeof(K));
 }
 
-// Global histogram of every digit.  hist is [passes][kRadix], zeroed.
+// Histogram blocks: one partial histogram row per block, written whole (no
+// global atomics: a few hundred blocks adding into the same 1024 counters
+// serialise on those addresses).  16K keys per block, at most 256 rows.
+inline u32 hist_blocks_for(u64 n) {
+  const u64 b = (n + 16383) / 16384;
+  return static_cast<u32>(b < 1 ? 1 : (b > 256 ? 256 : b));
+}
+
+// Per-block digit histograms of `passes` digits starting at digit `first`
+// (digit p = (key >> 8p) & 0xff).  rows[blockIdx][p * kRadix + d].  The
+// blocks also clear the look-back state of the passes that follow:
+// `zero` (u32 words) is cleared grid-stride.
 template <class K>
 static __global__ __launch_bounds__(kThreads) void histogram_kernel(
-    const K *__restrict__ keys, u32 n, u32 *__restrict__ hist) {
-  constexpr int kPasses = passes_for<K>();
-  __shared__ u32 s_hist[kPasses][kRadix];
-  for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads)
+    const K *__restrict__ keys, u32 n, u32 first, u32 passes,
+    u32 *__restrict__ rows, u32 *__restrict__ zero, u64 zero_words) {
+  __shared__ u32 s_hist[kMaxPasses][kRadix];
+  for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kThreads)
     (&s_hist[0][0])[i] = 0;
   __syncthreads();
   const u64 stride = u64(gridDim.x) * kThreads;
   for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
     const K k = keys[i];
-#pragma unroll
-    for (int p = 0; p < kPasses; ++p)
-      atomicAdd(&s_hist[p][static_cast<u32>(k >> (8 * p)) & 0xffu], 1u);
+    for (u32 p = 0; p < passes; ++p)
+      atomicAdd(&s_hist[p][static_cast<u32>(k >> (8 * (first + p))) & 0xffu],
+                1u);
   }
+  for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < zero_words;
+       i += stride)
+    zero[i] = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads) {
-    const u32 c = (&s_hist[0][0])[i];
-    if (c) atomicAdd(&hist[i], c);
-  }
+  u32 *row = rows + u64(blockIdx.x) * passes * kRadix;
+  for (u32 i = threadIdx.x; i < passes * kRadix; i += kThreads)
+    row[i] = (&s_hist[0][0])[i];
 }
 
-// Histogram of one digit (key >> shift) & 0xff into hist[0..256), zeroed.
-template <class K>
-__global__ __launch_bounds__(kThreads) void digit_histogram_kernel(
-    const K *__restrict__ keys, u32 n, u32 shift, u32 *__restrict__ hist) {
-  __shared__ u32 s_hist[kRadix];
-  s_hist[threadIdx.x] = 0;
-  __syncthreads();
-  const u64 stride = u64(gridDim.x) * kThreads;
-  for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride)
-    atomicAdd(&s_hist[static_cast<u32>(keys[i] >> shift) & 0xffu], 1u);
-  __syncthreads();
-  const u32 c = s_hist[threadIdx.x];
-  if (c) atomicAdd(&hist[threadIdx.x], c);
-}
-
-// Exclusive scan of each pass's histogram -> global digit offsets.
+// Sums the per-block rows of pass blockIdx.x and writes the exclusive scan
+// over digits -> global digit offsets.
 static __global__ __launch_bounds__(kRadix) void scan_kernel(
-    const u32 *__restrict__ hist, u32 *__restrict__ offsets, int passes) {
+    const u32 *__restrict__ rows, u32 n_rows, u32 passes,
+    u32 *__restrict__ offsets) {
   __shared__ u32 scratch[kRadix / kWave + 1];
-  for (int p = 0; p < passes; ++p) {
-    const u32 v = hist[p * kRadix + threadIdx.x];
-    const u32 ex = block_exclusive_scan<kRadix>(v, scratch);
-    offsets[p * kRadix + threadIdx.x] = ex;
-    __syncthreads();
-  }
+  const u32 p = blockIdx.x;
+  u32 v = 0;
+  for (u32 r = 0; r < n_rows; ++r)
+    v += rows[(u64(r) * passes + p) * kRadix + threadIdx.x];
+  offsets[p * kRadix + threadIdx.x] = block_exclusive_scan<kRadix>(v, scratch);
 }
 
 // One digit pass.  status has tiles*kRadix words and must be zero;
@@ -224,11 +223,38 @@ inline u32 tiles_for(u64 n) {
   return static_cast<u32>((n + tile - 1) / tile);
 }
 
-// Workspace words (u32) needed by sort_pairs for n keys, not counting the
-// two ping-pong buffers.
+// Workspace (u32 words) of a sort of n keys in `passes` digit passes, not
+// counting the ping-pong buffers:
+//   rows    [hist_blocks][passes][256]   per-block histograms
+//   offsets [passes][256]                global digit offsets
+//   tickets [64]                         tile tickets, one per pass
+//   status  u64 [passes][tiles][256]     look-back words
+struct Control {
+  u32 *rows, *offsets, *tickets;
+  u64 *status;
+  u32 hist_blocks;
+};
+
+inline u64 control_prefix_words(u64 n, int passes) {
+  const u64 w = u64(hist_blocks_for(n)) * passes * kRadix +
+                u64(passes) * kRadix + 64;
+  return (w + 1) & ~u64(1);  // status is 8-byte aligned
+}
+
 inline u64 control_words(u64 n, int passes = 4) {
-  // hist[8*256] + offsets[8*256] + tickets[64] + status[passes][tiles*256]
-  return 4096 + 64 + 2ull * passes * u64(tiles_for(n)) * kRadix;
+  return control_prefix_words(n, passes) +
+         2ull * passes * u64(tiles_for(n)) * kRadix;
+}
+
+inline Control control_layout(u32 *control, u64 n, int passes) {
+  Control c;
+  c.hist_blocks = hist_blocks_for(n);
+  c.rows = control;
+  c.offsets = control + u64(c.hist_blocks) * passes * kRadix;
+  c.tickets = c.offsets + passes * kRadix;
+  c.status = reinterpret_cast<u64 *>(control +
+                                     control_prefix_words(n, passes));
+  return c;
 }
 
 template <class K, bool kFirst>
@@ -256,18 +282,16 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   if (n == 0) return hipSuccess;
   const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
-  u32 *hist = control;
-  u32 *offsets = control + 2048;
-  u32 *tickets = control + 4096;
-  u64 *status = reinterpret_cast<u64 *>(control + 4096 + 64);
-  const u64 zero_words = control_words(n, kPasses);
-  zero_u32_kernel<<<static_cast<u32>(std::min<u64>((zero_words + 255) / 256,
-                                                   4096)),
-                    256, 0, stream>>>(control, zero_words);
-  const u32 hist_blocks = static_cast<u32>(
-      std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 2048));
-  histogram_kernel<K><<<hist_blocks, kThreads, 0, stream>>>(keys_in, n, hist);
-  scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets, kPasses);
+  const Control c = control_layout(control, n, kPasses);
+  // the histogram blocks clear tickets and look-back words for the passes
+  const u64 zero_words = control_words(n, kPasses) - (c.tickets - control);
+  histogram_kernel<K><<<c.hist_blocks, kThreads, 0, stream>>>(
+      keys_in, n, 0, kPasses, c.rows, c.tickets, zero_words);
+  scan_kernel<<<kPasses, kRadix, 0, stream>>>(c.rows, c.hist_blocks, kPasses,
+                                              c.offsets);
+  u32 *tickets = c.tickets;
+  u64 *status = c.status;
+  const u32 *offsets = c.offsets;
   const K *src_k = keys_in;
   const u32 *src_v = nullptr;
   K *dst_k = keys_b;
@@ -306,19 +330,14 @@ inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
   if (n == 0) return hipSuccess;
   const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
-  u32 *hist = control;
-  u32 *offsets = control + 2048;
-  u32 *tickets = control + 4096;
-  u64 *status = reinterpret_cast<u64 *>(control + 4096 + 64);
-  const u64 zero_words = control_words(n, 1);
-  zero_u32_kernel<<<static_cast<u32>(std::min<u64>((zero_words + 255) / 256,
-                                                   4096)),
-                    256, 0, stream>>>(control, zero_words);
-  const u32 hist_blocks = static_cast<u32>(
-      std::min<u64>((u64(n) + kThreads * 16 - 1) / (kThreads * 16), 2048));
-  digit_histogram_kernel<K><<<hist_blocks, kThreads, 0, stream>>>(
-      keys_in, n, shift, hist);
-  scan_kernel<<<1, kRadix, 0, stream>>>(hist, offsets, 1);
+  const Control c = control_layout(control, n, 1);
+  const u64 zero_words = control_words(n, 1) - (c.tickets - control);
+  histogram_kernel<K><<<c.hist_blocks, kThreads, 0, stream>>>(
+      keys_in, n, shift / 8, 1, c.rows, c.tickets, zero_words);
+  scan_kernel<<<1, kRadix, 0, stream>>>(c.rows, c.hist_blocks, 1, c.offsets);
+  const u32 *offsets = c.offsets;
+  u32 *tickets = c.tickets;
+  u64 *status = c.status;
   launch_pass<K, true>(shape, tiles, keys_in, nullptr, keys_out, pos_out, n,
                        shift, offsets, status, tickets, stream);
   return hipGetLastError();
