@@ -136,7 +136,7 @@ class _ServeConfig(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_uint64),
         ("max_step_keys", ctypes.c_uint64),
         ("max_inflight_steps", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("key_bytes", ctypes.c_uint32),
         ("tuning", ctypes.POINTER(_Tuning)),
     ]
 
@@ -194,6 +194,25 @@ SIGNATURES = {
     "gd_lsm_set_profiling": [_P, _I],
     "gd_lsm_read_profile": [_P, ctypes.POINTER(_Profile), _I],
     "gd_lsm_get_tuning": [_P, ctypes.POINTER(_Tuning)],
+    "gd_lsm64_create": [ctypes.POINTER(_Config), ctypes.POINTER(_P)],
+    "gd_lsm64_destroy": [_P],
+    "gd_lsm64_clear": [_P, _P],
+    "gd_lsm64_bulk_build": [_P, _P, _P, _U64, _P],
+    "gd_lsm64_insert": [_P, _P, _P, _U64, _P],
+    "gd_lsm64_erase": [_P, _P, _U64, _P],
+    "gd_lsm64_apply_mixed": [_P, _P, _P, _P, _U64, _P],
+    "gd_lsm64_lookup": [_P, _P, _U64, _P, _P, _P],
+    "gd_lsm64_range_sum": [_P, _P, _P, _U64, _P, _P],
+    "gd_lsm64_range_count": [_P, _P, _P, _U64, _P, _P],
+    "gd_lsm64_successor": [_P, _P, _U64, _P, _P],
+    "gd_lsm64_dump_sorted": [_P, _P, _P, _U64, ctypes.POINTER(_U64), _P],
+    "gd_lsm64_compact": [_P, _P],
+    "gd_lsm64_carry_status": [_P, ctypes.POINTER(ctypes.c_uint32)],
+    "gd_lsm64_resident_bytes": [_P, ctypes.POINTER(_U64)],
+    "gd_lsm64_shape": [_P, ctypes.POINTER(ctypes.c_uint32),
+                       ctypes.POINTER(_U64)],
+    "gd_lsm64_set_tuning": [_P, ctypes.POINTER(_Tuning)],
+    "gd_lsm64_get_tuning": [_P, ctypes.POINTER(_Tuning)],
     "gd_serve_create": [ctypes.POINTER(_ServeConfig), ctypes.POINTER(_P)],
     "gd_serve_destroy": [_P],
     "gd_serve_submit": [_P, ctypes.c_int, _P, _P, _U64, _P, _P,
@@ -283,14 +302,21 @@ def _check(status, handle):
 
 
 class GPULSMOpt:
-    """Mirror of GPULSMOpt (GPULSMOpt.cuh:5599) over gd_lsm_*."""
+    """Mirror of GPULSMOpt (GPULSMOpt.cuh:5599) over gd_lsm_*; key_bytes=8
+    selects the uint64 key/value dictionary (gd_lsm64_*)."""
 
     def __init__(self, max_elements=0, batch_capacity=0,
-                 level_zero_capacity=0, tuning=None):
+                 level_zero_capacity=0, tuning=None, key_bytes=4):
         """tuning: dict of gd_lsm_tuning knobs, or "auto" for tuned.json."""
+        if key_bytes not in (4, 8):
+            raise InvalidArgument(GD_INVALID_ARGUMENT,
+                                  "key_bytes must be 4 or 8")
+        self.key_bytes = key_bytes
+        self._p = "gd_lsm_" if key_bytes == 4 else "gd_lsm64_"
         cfg = _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0)
         h = _P()
-        _check(lib().gd_lsm_create(ctypes.byref(cfg), ctypes.byref(h)), None)
+        _check(getattr(lib(), self._p + "create")(ctypes.byref(cfg),
+                                                   ctypes.byref(h)), None)
         self._h = h
         if tuning == "auto":
             tuning = tuned_config("lsm")
@@ -299,24 +325,24 @@ class GPULSMOpt:
 
     def set_tuning(self, knobs):
         t = _tuning(knobs)
-        self._call("gd_lsm_set_tuning", ctypes.byref(t))
+        self._call("set_tuning", ctypes.byref(t))
 
     def set_profiling(self, enable=True):
-        self._call("gd_lsm_set_profiling", 1 if enable else 0)
+        self._call("set_profiling", 1 if enable else 0)
 
     def read_profile(self, reset=True):
         p = _Profile()
-        self._call("gd_lsm_read_profile", ctypes.byref(p), 1 if reset else 0)
+        self._call("read_profile", ctypes.byref(p), 1 if reset else 0)
         return {f: getattr(p, f) for f, _ in _Profile._fields_}
 
     def get_tuning(self):
         t = _Tuning()
-        self._call("gd_lsm_get_tuning", ctypes.byref(t))
+        self._call("get_tuning", ctypes.byref(t))
         return {k: int(getattr(t, k)) for k in TUNING_FIELDS}
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().gd_lsm_destroy(self._h)
+            getattr(lib(), self._p + "destroy")(self._h)
             self._h = None
 
     def __del__(self):
@@ -326,73 +352,76 @@ class GPULSMOpt:
             pass
 
     def _call(self, name, *args):
-        _check(getattr(lib(), name)(self._h, *args), self._h)
+        _check(getattr(lib(), self._p + name)(self._h, *args), self._h)
 
     def clear(self, stream=None):
-        self._call("gd_lsm_clear", _stream(stream))
+        self._call("clear", _stream(stream))
 
     def bulk_build(self, keys, values, count=None, stream=None):
-        self._call("gd_lsm_bulk_build", _ptr(keys), _ptr(values),
+        self._call("bulk_build", _ptr(keys), _ptr(values),
                    _count(keys, count), _stream(stream))
 
     def insert(self, keys, values, count=None, stream=None):
-        self._call("gd_lsm_insert", _ptr(keys), _ptr(values),
-                   _count(keys, count), _stream(stream))
-
-    def erase(self, keys, count=None, stream=None):
-        self._call("gd_lsm_erase", _ptr(keys), _count(keys, count),
+        self._call("insert", _ptr(keys), _ptr(values), _count(keys, count),
                    _stream(stream))
 
+    def erase(self, keys, count=None, stream=None):
+        self._call("erase", _ptr(keys), _count(keys, count), _stream(stream))
+
     def apply_mixed(self, keys, values, is_erase, count=None, stream=None):
-        self._call("gd_lsm_apply_mixed", _ptr(keys), _ptr(values),
-                   _ptr(is_erase), _count(keys, count), _stream(stream))
+        self._call("apply_mixed", _ptr(keys), _ptr(values), _ptr(is_erase),
+                   _count(keys, count), _stream(stream))
 
     def lookup(self, queries, out_values, out_found=None, count=None,
                stream=None, quotients_grouped=False):
-        self._call("gd_lsm_lookup", _ptr(queries), _count(queries, count),
-                   _ptr(out_values), _ptr(out_found),
-                   1 if quotients_grouped else 0, _stream(stream))
+        if self.key_bytes == 4:
+            self._call("lookup", _ptr(queries), _count(queries, count),
+                       _ptr(out_values), _ptr(out_found),
+                       1 if quotients_grouped else 0, _stream(stream))
+        else:
+            self._call("lookup", _ptr(queries), _count(queries, count),
+                       _ptr(out_values), _ptr(out_found), _stream(stream))
 
     def range(self, lo, hi, out_sums, count=None, stream=None):
-        self._call("gd_lsm_range_sum", _ptr(lo), _ptr(hi), _count(lo, count),
+        self._call("range_sum", _ptr(lo), _ptr(hi), _count(lo, count),
                    _ptr(out_sums), _stream(stream))
 
     def range_count(self, lo, hi, out_counts, count=None, stream=None):
-        self._call("gd_lsm_range_count", _ptr(lo), _ptr(hi),
-                   _count(lo, count), _ptr(out_counts), _stream(stream))
+        self._call("range_count", _ptr(lo), _ptr(hi), _count(lo, count),
+                   _ptr(out_counts), _stream(stream))
 
     def successor(self, queries, out_keys, count=None, stream=None):
-        self._call("gd_lsm_successor", _ptr(queries), _count(queries, count),
+        self._call("successor", _ptr(queries), _count(queries, count),
                    _ptr(out_keys), _stream(stream))
 
     def dump_sorted(self, out_keys=None, out_values=None, capacity=0,
                     stream=None):
         """Returns the live count; fills out_keys/out_values when given."""
         n = _U64(0)
-        st = lib().gd_lsm_dump_sorted(self._h, _ptr(out_keys),
-                                      _ptr(out_values), int(capacity),
-                                      ctypes.byref(n), _stream(stream))
+        st = getattr(lib(), self._p + "dump_sorted")(
+            self._h, _ptr(out_keys), _ptr(out_values), int(capacity),
+            ctypes.byref(n), _stream(stream))
         if st not in (GD_OK, GD_LENGTH):
             _check(st, self._h)
         return int(n.value)
 
     def compact(self, stream=None):
-        self._call("gd_lsm_compact", _stream(stream))
+        self._call("compact", _stream(stream))
 
     def canonical_carry_status(self):
         v = ctypes.c_uint32(0)
-        self._call("gd_lsm_carry_status", ctypes.byref(v))
+        self._call("carry_status", ctypes.byref(v))
         return int(v.value)
 
     def gpu_resident_bytes(self):
         v = _U64(0)
-        self._call("gd_lsm_resident_bytes", ctypes.byref(v))
+        self._call("resident_bytes", ctypes.byref(v))
         return int(v.value)
 
     def shape(self):
         runs = ctypes.c_uint32(0)
         rows = _U64(0)
-        self._call("gd_lsm_shape", ctypes.byref(runs), ctypes.byref(rows))
+        self._call("shape", ctypes.byref(runs), ctypes.byref(rows))
         return int(runs.value), int(rows.value)
 
 
@@ -407,13 +436,14 @@ class Server:
 
     def __init__(self, max_elements=0, batch_capacity=0,
                  level_zero_capacity=0, hbm_budget_bytes=0, max_step_keys=0,
-                 max_inflight_steps=0, tuning=None):
+                 max_inflight_steps=0, tuning=None, key_bytes=4):
         if tuning == "auto":
             tuning = tuned_config("lsm")
         self._tuning = _tuning(tuning) if tuning else None
+        self.key_bytes = key_bytes
         cfg = _ServeConfig(
             _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0),
-            hbm_budget_bytes, max_step_keys, max_inflight_steps, 0,
+            hbm_budget_bytes, max_step_keys, max_inflight_steps, key_bytes,
             ctypes.pointer(self._tuning) if self._tuning else None)
         h = _P()
         _check(lib().gd_serve_create(ctypes.byref(cfg), ctypes.byref(h)),

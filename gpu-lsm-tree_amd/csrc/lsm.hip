@@ -15,10 +15,37 @@ const char *create_error() { return g_create_error.c_str(); }
 
 using gdict::guarded;
 using gdict::lsm::Lsm;
+using gdict::lsm::Lsm64;
 
 struct gd_lsm : Lsm {
   using Lsm::Lsm;
 };
+
+struct gd_lsm64 : Lsm64 {
+  using Lsm64::Lsm64;
+};
+
+namespace {
+
+template <class H>
+gd_status dump_sorted(H *h, typename H::K *out_keys, typename H::V *out_values,
+                      uint64_t capacity, uint64_t *out_count, hipStream_t s) {
+  if (!h || !out_count) return GD_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  uint64_t n = 0;
+  gd_status st =
+      guarded(h, [&] { n = h->dump(out_keys, out_values, capacity, s); });
+  if (st != GD_OK) return st;
+  *out_count = n;
+  if (n > capacity) {
+    h->last_error = "dump capacity " + std::to_string(capacity) +
+                    " is smaller than the " + std::to_string(n) + " live keys";
+    return GD_LENGTH;
+  }
+  return GD_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -106,18 +133,7 @@ gd_status gd_lsm_successor(gd_lsm *h, const uint32_t *queries, uint64_t count,
 gd_status gd_lsm_dump_sorted(gd_lsm *h, uint32_t *out_keys,
                              uint32_t *out_values, uint64_t capacity,
                              uint64_t *out_count, hipStream_t s) {
-  if (!h || !out_count) return GD_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> lock(h->mutex);
-  uint64_t n = 0;
-  gd_status st = guarded(h, [&] { n = h->dump(out_keys, out_values, capacity, s); });
-  if (st != GD_OK) return st;
-  *out_count = n;
-  if (n > capacity) {
-    h->last_error = "dump capacity " + std::to_string(capacity) +
-                    " is smaller than the " + std::to_string(n) + " live keys";
-    return GD_LENGTH;
-  }
-  return GD_OK;
+  return dump_sorted(h, out_keys, out_values, capacity, out_count, s);
 }
 
 gd_status gd_lsm_compact(gd_lsm *h, hipStream_t s) { GD_LSM_CALL(h, h->compact(s)); }
@@ -152,6 +168,115 @@ gd_status gd_lsm_read_profile(gd_lsm *h, gd_lsm_profile *out, int reset) {
 }
 
 gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub) {
+  if (!out_runs || !out_rows_ub) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, h->shape(out_runs, out_rows_ub));
+}
+
+// ------------------------------------------------------------ 64-bit LSM
+gd_status gd_lsm64_create(const gd_config *config, gd_lsm64 **out) {
+  if (!config || !out) {
+    gdict::set_create_error("null config or output handle");
+    return GD_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    (void)Lsm64::capacity_for(*config);
+    *out = new gd_lsm64(*config);
+  });
+}
+
+gd_status gd_lsm64_destroy(gd_lsm64 *h) {
+  delete h;
+  return GD_OK;
+}
+
+gd_status gd_lsm64_clear(gd_lsm64 *h, hipStream_t s) {
+  GD_LSM_CALL(h, h->clear(s));
+}
+
+gd_status gd_lsm64_bulk_build(gd_lsm64 *h, const uint64_t *keys,
+                              const uint64_t *values, uint64_t count,
+                              hipStream_t s) {
+  GD_LSM_CALL(h, h->bulk_build(keys, values, count, s));
+}
+
+gd_status gd_lsm64_insert(gd_lsm64 *h, const uint64_t *keys,
+                          const uint64_t *values, uint64_t count,
+                          hipStream_t s) {
+  GD_LSM_CALL(h, h->admit(keys, values, nullptr, count,
+                          gdict::lsm::kModeInsert, s));
+}
+
+gd_status gd_lsm64_erase(gd_lsm64 *h, const uint64_t *keys, uint64_t count,
+                         hipStream_t s) {
+  GD_LSM_CALL(h, h->admit(keys, nullptr, nullptr, count,
+                          gdict::lsm::kModeErase, s));
+}
+
+gd_status gd_lsm64_apply_mixed(gd_lsm64 *h, const uint64_t *keys,
+                               const uint64_t *values,
+                               const uint8_t *is_erase, uint64_t count,
+                               hipStream_t s) {
+  GD_LSM_CALL(h, h->admit(keys, values, is_erase, count,
+                          gdict::lsm::kModeMixed, s));
+}
+
+gd_status gd_lsm64_lookup(gd_lsm64 *h, const uint64_t *queries,
+                          uint64_t count, uint64_t *out_values,
+                          uint8_t *out_found, hipStream_t s) {
+  GD_LSM_CALL(h, h->lookup(queries, count, out_values, out_found, s));
+}
+
+gd_status gd_lsm64_range_sum(gd_lsm64 *h, const uint64_t *lo,
+                             const uint64_t *hi, uint64_t count,
+                             uint64_t *out_sums, hipStream_t s) {
+  GD_LSM_CALL(h, h->range(lo, hi, count, out_sums, false, s));
+}
+
+gd_status gd_lsm64_range_count(gd_lsm64 *h, const uint64_t *lo,
+                               const uint64_t *hi, uint64_t count,
+                               uint64_t *out_counts, hipStream_t s) {
+  GD_LSM_CALL(h, h->range(lo, hi, count, out_counts, true, s));
+}
+
+gd_status gd_lsm64_successor(gd_lsm64 *h, const uint64_t *queries,
+                             uint64_t count, uint64_t *out_keys,
+                             hipStream_t s) {
+  GD_LSM_CALL(h, h->successor(queries, count, out_keys, s));
+}
+
+gd_status gd_lsm64_dump_sorted(gd_lsm64 *h, uint64_t *out_keys,
+                               uint64_t *out_values, uint64_t capacity,
+                               uint64_t *out_count, hipStream_t s) {
+  return dump_sorted(h, out_keys, out_values, capacity, out_count, s);
+}
+
+gd_status gd_lsm64_compact(gd_lsm64 *h, hipStream_t s) {
+  GD_LSM_CALL(h, h->compact(s));
+}
+
+gd_status gd_lsm64_carry_status(gd_lsm64 *h, uint32_t *out_status) {
+  if (!out_status) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, *out_status = h->carry_status());
+}
+
+gd_status gd_lsm64_resident_bytes(gd_lsm64 *h, uint64_t *out_bytes) {
+  if (!out_bytes) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, *out_bytes = h->resident_bytes());
+}
+
+gd_status gd_lsm64_set_tuning(gd_lsm64 *h, const gd_lsm_tuning *tuning) {
+  if (!tuning) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, h->set_tuning(*tuning));
+}
+
+gd_status gd_lsm64_get_tuning(gd_lsm64 *h, gd_lsm_tuning *out) {
+  if (!out) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, *out = h->tuning());
+}
+
+gd_status gd_lsm64_shape(gd_lsm64 *h, uint32_t *out_runs,
+                         uint64_t *out_rows_ub) {
   if (!out_runs || !out_rows_ub) return GD_INVALID_ARGUMENT;
   GD_LSM_CALL(h, h->shape(out_runs, out_rows_ub));
 }

@@ -1,18 +1,22 @@
-// Device side of the LSM dictionary (uint32 keys, uint32 values).
+// Device side of the LSM dictionary, for (uint32 key, uint32 value) rows and
+// (uint64 key, uint64 value) rows.  Every kernel is a template over the row
+// traits KV<K> below.
 //
 // Data model in HBM.  The dictionary is a stack of immutable sorted runs,
 // newest first.  A run of c rows is two allocations:
-//   rows[c]  8-byte rows, .x = low 31 bits of the key | tombstone flag in
-//            bit 31, .y = value (0 for a tombstone)
+//   rows[c]  rows of (key word, value): the key word holds the low B-1 bits
+//            of the key with the tombstone flag in its top bit, the value
+//            is 0 for a tombstone (B = key bits)
 //   dir      RunMeta header (count, split, directory bits, tombstones)
 //            followed by a radix directory of 2^b + 1 u32 row offsets:
-//            dir[j] = first row whose key >= j << (32 - b).
-// Bit 31 of a key is implied by the row position (rows >= split have it
-// set), which frees it for the tombstone flag, so a row is exactly 8 bytes
-// like the reference's Row (GPULSMOpt.cuh:343-377), and key and value share
-// one cache line, so a point hit costs one line of rows.  Every search enters
-// through the directory (b >= 1); inside one directory bucket all keys share
-// bit 31, so comparisons use the low 31 bits only.
+//            dir[j] = first row whose key >= j << (B - b).
+// The top bit of a key is implied by the row position (rows >= split have
+// it set), which frees it for the tombstone flag, so a row is exactly
+// 2 * sizeof(K) bytes like the reference's 8-byte Row (GPULSMOpt.cuh:
+// 343-377), and key and value share one cache line, so a point hit costs
+// one line of rows.  Every search enters through the directory (b >= 1);
+// inside one directory bucket all keys share the top bit, so comparisons
+// use the low B-1 bits only.
 #pragma once
 
 #include "device_utils.hpp"
@@ -20,36 +24,85 @@
 namespace gdict {
 namespace lsm {
 
-constexpr u32 kTombBit = 0x80000000u;
-constexpr u32 kLowMask = 0x7fffffffu;
-constexpr u32 kInvalid = 0xffffffffu;
 constexpr int kMaxRuns = 48;
 constexpr u32 kMaxDirBits = 24;
 
 struct RunMeta {
   u32 count;
-  u32 split;     // rows with key < 2^31
+  u32 split;     // rows whose key has the top bit clear
   u32 dir_bits;  // b, 1 <= b <= kMaxDirBits
   u32 tombs;     // tombstone rows
 };
 constexpr u64 kMetaBytes = 256;  // header slot in front of the directory
 
-using Row = uint2;  // .x key word, .y value
+struct alignas(16) Row64 {
+  u64 k;  // key word
+  u64 v;  // value
+};
 
+// Row traits.  kSmallTile: keys a fused one-workgroup admission sorts in
+// LDS; kMergeItemsMax: merge items per thread (merge tile = 256 x items),
+// both sized so the LDS staging stays below 64 KB.
+template <class K>
+struct KV;
+
+template <>
+struct KV<u32> {
+  using K = u32;
+  using V = u32;
+  using Row = uint2;
+  static constexpr int kBits = 32;
+  static constexpr K kTomb = 0x80000000u;
+  static constexpr K kLow = 0x7fffffffu;
+  static constexpr K kAllOnes = 0xffffffffu;
+  static constexpr int kSmallTile = 4096;
+  static constexpr int kMergeItemsMax = 16;
+  __device__ __forceinline__ static K kw(const Row &r) { return r.x; }
+  __device__ __forceinline__ static V val(const Row &r) { return r.y; }
+  __device__ __forceinline__ static Row make(K kw, V v) {
+    return make_uint2(kw, v);
+  }
+};
+
+template <>
+struct KV<u64> {
+  using K = u64;
+  using V = u64;
+  using Row = Row64;
+  static constexpr int kBits = 64;
+  static constexpr K kTomb = 0x8000000000000000ull;
+  static constexpr K kLow = 0x7fffffffffffffffull;
+  static constexpr K kAllOnes = ~0ull;
+  static constexpr int kSmallTile = 2048;
+  static constexpr int kMergeItemsMax = 8;
+  __device__ __forceinline__ static K kw(const Row &r) { return r.k; }
+  __device__ __forceinline__ static V val(const Row &r) { return r.v; }
+  __device__ __forceinline__ static Row make(K kw, V v) {
+    Row r;
+    r.k = kw;
+    r.v = v;
+    return r;
+  }
+};
+
+template <class T>
 struct RunView {
-  const Row *rows;
+  const typename T::Row *rows;
   const u32 *dir;
   const RunMeta *meta;
 };
 
+template <class T>
 struct RunSet {
   u32 count;
   u32 pad;
-  RunView run[kMaxRuns];
+  RunView<T> run[kMaxRuns];
 };
 
-__device__ __forceinline__ u32 decode_key(u32 w, u32 pos, u32 split) {
-  return (w & kLowMask) | (pos >= split ? kTombBit : 0u);
+template <class T>
+__device__ __forceinline__ typename T::K decode_key(typename T::K w, u32 pos,
+                                                    u32 split) {
+  return (w & T::kLow) | (pos >= split ? T::kTomb : typename T::K(0));
 }
 
 // Directory bits for `count` rows: about 2^rows_log rows per bucket.
@@ -63,44 +116,52 @@ __host__ __device__ __forceinline__ u32 dir_bits_for(u64 count, u32 max_bits,
   return b;
 }
 
-// lower_bound(x) over the whole run, x in [0, 2^32].
-__device__ __forceinline__ u32 run_lower_bound(const RunView &r, u32 b,
-                                               u32 count, u64 x) {
-  if (x > 0xffffffffull) return count;
-  const u32 key = static_cast<u32>(x);
-  const u32 bucket = key >> (32u - b);
+// First row whose key is >= x.
+template <class T>
+__device__ __forceinline__ u32 run_lower_bound(const RunView<T> &r, u32 b,
+                                               typename T::K x) {
+  const u32 bucket = static_cast<u32>(x >> (T::kBits - b));
   u32 lo = r.dir[bucket];
   u32 hi = r.dir[bucket + 1];
-  const u32 target = key & kLowMask;
+  const typename T::K target = x & T::kLow;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
-    if ((r.rows[mid].x & kLowMask) < target) lo = mid + 1;
+    if ((T::kw(r.rows[mid]) & T::kLow) < target) lo = mid + 1;
     else hi = mid;
   }
   return lo;
 }
 
-// Point search; returns the row position or kInvalid (row in `row`).
-__device__ __forceinline__ u32 run_find(const RunView &r, u32 b, u32 key,
-                                        Row &row) {
-  const u32 bucket = key >> (32u - b);
+// First row whose key is > x.
+template <class T>
+__device__ __forceinline__ u32 run_upper_bound(const RunView<T> &r, u32 b,
+                                               u32 count, typename T::K x) {
+  return x == T::kAllOnes ? count : run_lower_bound(r, b, x + 1);
+}
+
+// Point search; returns the row position or 0xffffffff (row in `row`).
+template <class T>
+__device__ __forceinline__ u32 run_find(const RunView<T> &r, u32 b,
+                                        typename T::K key,
+                                        typename T::Row &row) {
+  const u32 bucket = static_cast<u32>(key >> (T::kBits - b));
   u32 lo = r.dir[bucket];
   const u32 end = r.dir[bucket + 1];
   u32 hi = end;
-  const u32 target = key & kLowMask;
+  const typename T::K target = key & T::kLow;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
-    if ((r.rows[mid].x & kLowMask) < target) lo = mid + 1;
+    if ((T::kw(r.rows[mid]) & T::kLow) < target) lo = mid + 1;
     else hi = mid;
   }
   if (lo < end) {
-    const Row w = r.rows[lo];
-    if ((w.x & kLowMask) == target) {
+    const typename T::Row w = r.rows[lo];
+    if ((T::kw(w) & T::kLow) == target) {
       row = w;
       return lo;
     }
   }
-  return kInvalid;
+  return 0xffffffffu;
 }
 
 // ------------------------------------------------------------ batch runs
@@ -118,14 +179,18 @@ enum FinalizeMode : int {
 // Turns a stably sorted batch (keys sk, input positions sp) into a run:
 // keep the last occurrence of each key (the newest), gather its value and
 // tombstone flag, encode, and compact with a single-pass chained scan.
-// Items are striped (item j of a thread is row base + j*256 + tid), so every
+// Items are striped (item j of a thread is row base + j*512 + tid), so every
 // load and store of a row j is one coalesced access of the block; ballots
 // give the survivors' ranks.
-static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
-    const u32 *__restrict__ sk, const u32 *__restrict__ sp, u32 n,
-    const u32 *__restrict__ values, const u8 *__restrict__ erase_flags,
-    int mode, Row *__restrict__ out, RunMeta *meta, u64 *status,
+template <class T>
+__global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
+    const typename T::K *__restrict__ sk, const u32 *__restrict__ sp, u32 n,
+    const typename T::V *__restrict__ values,
+    const u8 *__restrict__ erase_flags, int mode,
+    typename T::Row *__restrict__ out, RunMeta *meta, u64 *status,
     u32 *ticket) {
+  using K = typename T::K;
+  using V = typename T::V;
   __shared__ u32 s_tile;
   __shared__ u32 s_off[kFinItems][kFinWaves];
   __shared__ u32 s_total;
@@ -141,7 +206,7 @@ static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
   __syncthreads();
   const u32 tile = s_tile;
   const u64 base = u64(tile) * kFinTile;
-  u32 key[kFinItems];
+  K key[kFinItems];
   u64 mask[kFinItems];
 #pragma unroll
   for (int j = 0; j < kFinItems; ++j) {
@@ -185,9 +250,9 @@ static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
     if (mode == kModeErase) tomb = true;
     else if (mode == kModeMixed) tomb = erase_flags[p] != 0;
     else tomb = false;
-    const u32 v = tomb ? 0u : values[p];
-    out[o] = make_uint2((key[j] & kLowMask) | (tomb ? kTombBit : 0u), v);
-    low += key[j] < kTombBit;
+    const V v = tomb ? V(0) : values[p];
+    out[o] = T::make((key[j] & T::kLow) | (tomb ? T::kTomb : K(0)), v);
+    low += key[j] < T::kTomb;
     tombs += tomb;
   }
   low = wave_sum_u32(low);
@@ -205,26 +270,32 @@ static __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
 }
 
 // ------------------------------------------------ small batches, one launch
-// A batch of at most kSmallTile keys becomes a run in ONE workgroup: an LDS
-// LSD radix sort (the same wave64 ballot multisplit as the onesweep passes,
-// with the whole batch as one tile), newest-wins dedupe, value gather,
-// encode, and the radix directory.  Replaces nine launches for the small
-// arrivals of a serving stream.
+// A batch of at most T::kSmallTile keys becomes a run in ONE workgroup: an
+// LDS LSD radix sort (the same wave64 ballot multisplit as the onesweep
+// passes, with the whole batch as one tile), newest-wins dedupe, value
+// gather, encode, and the radix directory.  Replaces nine launches for the
+// small arrivals of a serving stream.
 constexpr int kSmallThreads = 512;
-constexpr int kSmallItems = 8;
-constexpr int kSmallTile = kSmallThreads * kSmallItems;  // 4096
 constexpr int kSmallWaves = kSmallThreads / kWave;
 
-static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
-    const u32 *__restrict__ keys, u32 n, const u32 *__restrict__ values,
-    const u8 *__restrict__ erase_flags, int mode, Row *__restrict__ out,
-    RunMeta *meta, u32 *__restrict__ dir, u32 max_bits, u32 rows_log) {
-  // 57 KB of LDS: keys (2 x 16 KB), 16-bit positions (2 x 8 KB), per-wave
-  // digit counters (8 KB)
-  __shared__ u32 s_k0[kSmallTile];
-  __shared__ u32 s_k1[kSmallTile];
-  __shared__ unsigned short s_p0[kSmallTile];
-  __shared__ unsigned short s_p1[kSmallTile];
+template <class T>
+__global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
+    const typename T::K *__restrict__ keys, u32 n,
+    const typename T::V *__restrict__ values,
+    const u8 *__restrict__ erase_flags, int mode,
+    typename T::Row *__restrict__ out, RunMeta *meta, u32 *__restrict__ dir,
+    u32 max_bits, u32 rows_log) {
+  using K = typename T::K;
+  using V = typename T::V;
+  constexpr int kTile = T::kSmallTile;
+  constexpr int kItems = kTile / kSmallThreads;
+  constexpr int kPasses = T::kBits / 8;
+  // keys (2 x tile), 16-bit positions (2 x tile), per-wave digit counters:
+  // 57 KB for 32-bit keys (4096 keys), 48 KB for 64-bit keys (2048)
+  __shared__ K s_k0[kTile];
+  __shared__ K s_k1[kTile];
+  __shared__ unsigned short s_p0[kTile];
+  __shared__ unsigned short s_p1[kTile];
   __shared__ u32 s_hist[kSmallWaves * 256];
   __shared__ u32 s_start[256];
   __shared__ u32 s_scan[32];
@@ -238,21 +309,22 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
     s_p0[i] = static_cast<unsigned short>(i);
   }
   __syncthreads();
-  u32 *src_k = s_k0, *dst_k = s_k1;
+  K *src_k = s_k0, *dst_k = s_k1;
   unsigned short *src_p = s_p0, *dst_p = s_p1;
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = 0; pass < kPasses; ++pass) {
     const u32 shift = 8u * pass;
     for (u32 i = tid; i < kSmallWaves * 256; i += kSmallThreads) s_hist[i] = 0;
     __syncthreads();
-    u32 key[kSmallItems], pos[kSmallItems], rank[kSmallItems];
+    K key[kItems];
+    u32 pos[kItems], rank[kItems];
     volatile u32 *my_hist = s_hist + wave * 256;
 #pragma unroll
-    for (int j = 0; j < kSmallItems; ++j) {
-      const u32 idx = wave * (kWave * kSmallItems) + j * kWave + lane;
+    for (int j = 0; j < kItems; ++j) {
+      const u32 idx = wave * (kWave * kItems) + j * kWave + lane;
       const bool valid = idx < n;
-      key[j] = valid ? src_k[idx] : 0xffffffffu;
+      key[j] = valid ? src_k[idx] : T::kAllOnes;
       pos[j] = valid ? src_p[idx] : 0u;
-      const u32 d = (key[j] >> shift) & 0xffu;
+      const u32 d = static_cast<u32>(key[j] >> shift) & 0xffu;
       u64 match = ballot(valid);
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
@@ -294,17 +366,17 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSmallItems; ++j) {
-      const u32 idx = wave * (kWave * kSmallItems) + j * kWave + lane;
+    for (int j = 0; j < kItems; ++j) {
+      const u32 idx = wave * (kWave * kItems) + j * kWave + lane;
       if (idx < n) {
-        const u32 d = (key[j] >> shift) & 0xffu;
+        const u32 d = static_cast<u32>(key[j] >> shift) & 0xffu;
         const u32 lp = s_start[d] + s_hist[wave * 256 + d] + rank[j];
         dst_k[lp] = key[j];
         dst_p[lp] = static_cast<unsigned short>(pos[j]);
       }
     }
     __syncthreads();
-    u32 *tk = src_k;
+    K *tk = src_k;
     unsigned short *tp = src_p;
     src_k = dst_k;
     src_p = dst_p;
@@ -313,10 +385,10 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   }
   // sorted (key, position) in src_*; keep the last occurrence of each key.
   // Survivor ranks: ballots per (item row j, wave), scanned by thread 0.
-  u32 *s_off = s_hist;  // reuse: kSmallItems * kSmallWaves words
-  u64 mask[kSmallItems];
+  u32 *s_off = s_hist;  // reuse: kItems * kSmallWaves words
+  u64 mask[kItems];
 #pragma unroll
-  for (int j = 0; j < kSmallItems; ++j) {
+  for (int j = 0; j < kItems; ++j) {
     const u32 i = j * kSmallThreads + tid;
     const bool keep = i < n && (i + 1 == n || src_k[i + 1] != src_k[i]);
     mask[j] = ballot(keep);
@@ -325,7 +397,7 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   __syncthreads();
   if (tid == 0) {
     u32 run = 0;
-    for (int k = 0; k < kSmallItems * kSmallWaves; ++k) {
+    for (int k = 0; k < kItems * kSmallWaves; ++k) {
       const u32 c = s_off[k];
       s_off[k] = run;
       run += c;
@@ -339,21 +411,21 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   u32 low = 0, tombs = 0;
   // decoded output keys go to dst_k for the directory pass
 #pragma unroll
-  for (int j = 0; j < kSmallItems; ++j) {
+  for (int j = 0; j < kItems; ++j) {
     if (!((mask[j] >> lane) & 1ull)) continue;
     const u32 i = j * kSmallThreads + tid;
     const u32 o = s_off[j * kSmallWaves + wave] +
                   popc64(mask[j] & lanemask_lt());
-    const u32 k = src_k[i];
+    const K k = src_k[i];
     const u32 p = src_p[i];
     bool tomb;
     if (mode == kModeErase) tomb = true;
     else if (mode == kModeMixed) tomb = erase_flags[p] != 0;
     else tomb = false;
-    out[o] = make_uint2((k & kLowMask) | (tomb ? kTombBit : 0u),
-                        tomb ? 0u : values[p]);
+    out[o] = T::make((k & T::kLow) | (tomb ? T::kTomb : K(0)),
+                     tomb ? V(0) : values[p]);
     dst_k[o] = k;
-    low += k < kTombBit;
+    low += k < T::kTomb;
     tombs += tomb;
   }
   low = wave_sum_u32(low);
@@ -364,7 +436,7 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   }
   __syncthreads();
   const u32 b = dir_bits_for(count, max_bits, rows_log);
-  const u32 shift = 32u - b;
+  const u32 shift = T::kBits - b;
   const u32 nb = 1u << b;
   if (tid == 0) {
     meta->count = count;
@@ -377,11 +449,10 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   const u32 items = count > nb + 1 ? count : nb + 1;
   for (u32 t = tid; t < items; t += kSmallThreads) {
     if (t < count) {
-      const u32 bucket = dst_k[t] >> shift;
+      const long long bucket = static_cast<long long>(dst_k[t] >> shift);
       const long long prev =
           t ? static_cast<long long>(dst_k[t - 1] >> shift) : -1;
-      for (long long j = prev + 1; j <= static_cast<long long>(bucket); ++j)
-        dir[j] = t;
+      for (long long j = prev + 1; j <= bucket; ++j) dir[j] = t;
     }
     if (t <= nb && static_cast<long long>(t) > last_bucket) dir[t] = count;
   }
@@ -389,19 +460,20 @@ static __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
 
 // Builds the radix directory of a run from its final count/split.  Launch
 // over max(count_ub, 2^max_bits + 1) threads (grid-stride).
-static __global__ void build_directory_kernel(const Row *__restrict__ rows,
-                                              RunMeta *meta,
-                                              u32 *__restrict__ dir,
-                                              u32 max_bits, u32 rows_log) {
+template <class T>
+__global__ void build_directory_kernel(const typename T::Row *__restrict__ rows,
+                                       RunMeta *meta, u32 *__restrict__ dir,
+                                       u32 max_bits, u32 rows_log) {
   const u32 count = meta->count;
   const u32 split = meta->split;
   const u32 b = dir_bits_for(count, max_bits, rows_log);
-  const u32 shift = 32u - b;
+  const u32 shift = T::kBits - b;
   const u32 nb = 1u << b;
   if (blockIdx.x == 0 && threadIdx.x == 0) meta->dir_bits = b;
   const long long last_bucket =
       count ? static_cast<long long>(
-                  decode_key(rows[count - 1].x, count - 1, split) >> shift)
+                  decode_key<T>(T::kw(rows[count - 1]), count - 1, split) >>
+                  shift)
             : -1;
   const u64 items = count > u64(nb) + 1 ? u64(count) : u64(nb) + 1;
   const u64 stride = u64(gridDim.x) * blockDim.x;
@@ -409,13 +481,14 @@ static __global__ void build_directory_kernel(const Row *__restrict__ rows,
        t += stride) {
     if (t < count) {
       const u32 pos = static_cast<u32>(t);
-      const u32 bucket = decode_key(rows[pos].x, pos, split) >> shift;
+      const long long bucket = static_cast<long long>(
+          decode_key<T>(T::kw(rows[pos]), pos, split) >> shift);
       const long long prev =
           pos ? static_cast<long long>(
-                    decode_key(rows[pos - 1].x, pos - 1, split) >> shift)
+                    decode_key<T>(T::kw(rows[pos - 1]), pos - 1, split) >>
+                    shift)
               : -1;
-      for (long long j = prev + 1; j <= static_cast<long long>(bucket); ++j)
-        dir[j] = pos;
+      for (long long j = prev + 1; j <= bucket; ++j) dir[j] = pos;
     }
     if (t <= nb && static_cast<long long>(t) > last_bucket) dir[t] = count;
   }
@@ -424,27 +497,30 @@ static __global__ void build_directory_kernel(const Row *__restrict__ rows,
 // --------------------------------------------------------------- merging
 constexpr int kMergeThreads = 256;
 
+template <class T>
 struct MergeSide {
-  const Row *rows;
+  const typename T::Row *rows;
   const RunMeta *meta;
 };
 
-__device__ __forceinline__ u32 side_key(const MergeSide &s, u32 pos,
-                                        u32 split) {
-  return decode_key(s.rows[pos].x, pos, split);
+template <class T>
+__device__ __forceinline__ typename T::K side_key(const MergeSide<T> &s,
+                                                  u32 pos, u32 split) {
+  return decode_key<T>(T::kw(s.rows[pos]), pos, split);
 }
 
 // Number of A elements among the first `diag` merged elements, where A (the
 // newer run) precedes B on equal keys.
-__device__ __forceinline__ u32 merge_path(const MergeSide &a, u32 ca, u32 sa,
-                                          const MergeSide &b, u32 cb, u32 sb,
-                                          u64 diag) {
+template <class T>
+__device__ __forceinline__ u32 merge_path(const MergeSide<T> &a, u32 ca,
+                                          u32 sa, const MergeSide<T> &b,
+                                          u32 cb, u32 sb, u64 diag) {
   u64 lo = diag > cb ? diag - cb : 0;
   u64 hi = diag < ca ? diag : ca;
   while (lo < hi) {
     const u64 mid = (lo + hi) >> 1;
-    const u32 ka = side_key(a, static_cast<u32>(mid), sa);
-    const u32 kb = side_key(b, static_cast<u32>(diag - mid - 1), sb);
+    const auto ka = side_key(a, static_cast<u32>(mid), sa);
+    const auto kb = side_key(b, static_cast<u32>(diag - mid - 1), sb);
     if (ka <= kb) lo = mid + 1;
     else hi = mid;
   }
@@ -453,11 +529,11 @@ __device__ __forceinline__ u32 merge_path(const MergeSide &a, u32 ca, u32 sa,
 
 // Tile split points of the merge; also clears the merge's chained-scan
 // state and the output run's header (one launch instead of three).
-static __global__ void merge_partition_kernel(MergeSide a, MergeSide b,
-                                              u32 tiles, u32 tile_rows,
-                                              u32 *__restrict__ parts,
-                                              u64 *status, u32 *ticket,
-                                              RunMeta *out_meta) {
+template <class T>
+__global__ void merge_partition_kernel(MergeSide<T> a, MergeSide<T> b,
+                                       u32 tiles, u32 tile_rows,
+                                       u32 *__restrict__ parts, u64 *status,
+                                       u32 *ticket, RunMeta *out_meta) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > tiles) return;
   if (t < tiles) status[t] = 0;
@@ -479,18 +555,24 @@ static __global__ void merge_partition_kernel(MergeSide a, MergeSide b,
 // row survives, and tombstones are dropped when `drop_tombstones` (the
 // output becomes the oldest run, GPULSMOpt.cuh:3175-3179).  A tile is
 // 256 * kMergeItems output diagonals.
-template <int kMergeItems>
+template <class T, int kMergeItems>
 __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
-    MergeSide a, MergeSide b, const u32 *__restrict__ parts,
-    Row *__restrict__ out, RunMeta *out_meta, int drop_tombstones,
+    MergeSide<T> a, MergeSide<T> b, const u32 *__restrict__ parts,
+    typename T::Row *__restrict__ out, RunMeta *out_meta, int drop_tombstones,
     u64 *status, u32 *ticket) {
+  using K = typename T::K;
+  using V = typename T::V;
+  using Row = typename T::Row;
   constexpr int kMergeTile = kMergeThreads * kMergeItems;
-  constexpr u32 kStageHalf = (kMergeTile + 2) / 2;  // rows staged in s_key
+  // survivors are staged as rows over the key array, then the value array:
+  // each holds (tile + 2) * sizeof(K) / sizeof(Row) rows
+  constexpr u32 kStageHalf = (kMergeTile + 2) / 2;
+  static_assert(sizeof(Row) == 2 * sizeof(K) && sizeof(K) == sizeof(V),
+                "row = key word + value");
   __shared__ u32 s_tile;
-  // inputs: decoded key, value, tombstone flag; s_key/s_val are reused as
-  // the 8-byte staging of the survivors
-  __shared__ __align__(8) u32 s_key[kMergeTile + 2];
-  __shared__ __align__(8) u32 s_val[kMergeTile + 2];
+  // inputs: decoded key, value, tombstone flag
+  __shared__ __align__(16) K s_key[kMergeTile + 2];
+  __shared__ __align__(16) V s_val[kMergeTile + 2];
   __shared__ u8 s_tomb[kMergeTile + 2];
   __shared__ u32 s_scan[kMergeThreads / kWave + 1];
   __shared__ u64 s_base;
@@ -523,16 +605,16 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     }
     const u32 pos = i0 + t - 1;
     const Row r = a.rows[pos];
-    s_key[t] = decode_key(r.x, pos, sa);
-    s_tomb[t] = static_cast<u8>(r.x >> 31);
-    s_val[t] = r.y;
+    s_key[t] = decode_key<T>(T::kw(r), pos, sa);
+    s_tomb[t] = static_cast<u8>(T::kw(r) >> (T::kBits - 1));
+    s_val[t] = T::val(r);
   }
   for (u32 t = threadIdx.x; t < nb; t += kMergeThreads) {
     const u32 pos = j0 + t;
     const Row r = b.rows[pos];
-    s_key[na + 1 + t] = decode_key(r.x, pos, sb);
-    s_tomb[na + 1 + t] = static_cast<u8>(r.x >> 31);
-    s_val[na + 1 + t] = r.y;
+    s_key[na + 1 + t] = decode_key<T>(T::kw(r), pos, sb);
+    s_tomb[na + 1 + t] = static_cast<u8>(T::kw(r) >> (T::kBits - 1));
+    s_val[na + 1 + t] = T::val(r);
   }
   __syncthreads();
 
@@ -551,7 +633,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     ia = lo;
     ib = dk - lo;
   }
-  u32 ok[kMergeItems], ov[kMergeItems];
+  K ok[kMergeItems];
+  V ov[kMergeItems];
   u8 ot[kMergeItems];
   bool okeep[kMergeItems];
   u32 kept = 0;
@@ -596,20 +679,18 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     if (threadIdx.x == 0) s_base = ex;
   }
   __syncthreads();  // also: every LDS input read is complete
-  // stage survivors as 8-byte rows over the (now free) key/value arrays:
-  // rows [0, kStageHalf) in s_key, the rest in s_val
   Row *stage_lo = reinterpret_cast<Row *>(s_key);
   Row *stage_hi = reinterpret_cast<Row *>(s_val);
   u32 low = 0, tombs = 0, at = local;
 #pragma unroll
   for (int s = 0; s < kMergeItems; ++s) {
     if (okeep[s]) {
-      const Row r = make_uint2((ok[s] & kLowMask) | (ot[s] ? kTombBit : 0u),
-                               ov[s]);
+      const Row r =
+          T::make((ok[s] & T::kLow) | (ot[s] ? T::kTomb : K(0)), ov[s]);
       if (at < kStageHalf) stage_lo[at] = r;
       else stage_hi[at - kStageHalf] = r;
       ++at;
-      low += ok[s] < kTombBit;
+      low += ok[s] < T::kTomb;
       tombs += ot[s];
     }
   }
@@ -632,83 +713,151 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
 }
 
 // --------------------------------------------------------------- queries
-static __global__ void lookup_kernel(const u32 *__restrict__ queries, u64 n,
-                                     u32 *__restrict__ out_values,
-                                     u8 *__restrict__ out_found, RunSet runs) {
+// Query kernels read their inputs and write their results through an IO
+// policy: PlainLookup / PlainRange over contiguous arrays (the C ABI), or
+// the serving engine's piece tables (one launch answers every request of a
+// step straight from and into the requests' own buffers).
+
+// A miss writes all-ones (0xFFFFFFFF for 32-bit values) without a found
+// array, value 0 + found 0 with one (GPULSMOpt.cuh:4714-4806).
+template <class T>
+struct PlainLookup {
+  const typename T::K *q;
+  typename T::V *out;
+  u8 *found;
+  struct Slot {
+    u64 i;
+  };
+  __device__ __forceinline__ typename T::K load(u64 i, Slot &s) const {
+    s.i = i;
+    return q[i];
+  }
+  __device__ __forceinline__ void put(const Slot &s, bool live,
+                                      typename T::V value) const {
+    if (found) {
+      out[s.i] = live ? value : typename T::V(0);
+      found[s.i] = live ? 1 : 0;
+    } else {
+      out[s.i] = live ? value : T::kAllOnes;
+    }
+  }
+};
+
+// Newest visible version of `key`: live + value, or not live.
+template <class T>
+__device__ __forceinline__ bool lookup_one(const RunSet<T> &runs,
+                                           typename T::K key,
+                                           typename T::V &value) {
+  for (u32 r = 0; r < runs.count; ++r) {
+    const RunView<T> &rv = runs.run[r];
+    typename T::Row row;
+    if (run_find(rv, rv.meta->dir_bits, key, row) != 0xffffffffu) {
+      value = T::val(row);
+      return !(T::kw(row) & T::kTomb);
+    }
+  }
+  return false;
+}
+
+template <class T, class IO>
+__global__ void lookup_kernel(IO io, u64 n, RunSet<T> runs) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride) {
-    const u32 key = queries[i];
-    bool live = false;
-    u32 value = 0;
-    for (u32 r = 0; r < runs.count; ++r) {
-      const RunView &rv = runs.run[r];
-      const u32 b = rv.meta->dir_bits;
-      Row row;
-      if (run_find(rv, b, key, row) != kInvalid) {
-        live = !(row.x & kTombBit);
-        value = row.y;
-        break;
-      }
-    }
-    if (out_found) {
-      out_values[i] = live ? value : 0u;
-      out_found[i] = live ? 1 : 0;
-    } else {
-      out_values[i] = live ? value : kInvalid;
-    }
+    typename IO::Slot slot;
+    const typename T::K key = io.load(i, slot);
+    typename T::V value = 0;
+    const bool live = lookup_one(runs, key, value);
+    io.put(slot, live, value);
   }
 }
 
 // Lookups over queries partitioned by their top key byte (qk = grouped
-// keys, qp = their original positions), each result written back to its
-// original position.  Workgroups are dealt round-robin to the 8 XCDs, so
-// block b serves the contiguous chunk (b % 8) * (G / 8) + b / 8: every XCD
-// walks its own eighth of the grouped queries, i.e. its own eighth of the
-// key space, and the run rows it needs stay in that XCD's L2.
-static __global__ void lookup_grouped_kernel(const u32 *__restrict__ qk,
-                                             const u32 *__restrict__ qp,
-                                             u64 n,
-                                             u32 *__restrict__ out_values,
-                                             u8 *__restrict__ out_found,
-                                             RunSet runs) {
+// keys).  Results are written in the grouped order (res_values[p],
+// res_found[p]), so every store is coalesced; lookup_gather_kernel then
+// brings them back to query order.  Scattering each result straight to its
+// query position would dirty one cache line per query and evict the run
+// rows the grouping keeps in L2.  Workgroups are dealt round-robin to the
+// 8 XCDs, so block b serves the contiguous chunk (b % 8) * (G / 8) + b / 8:
+// every XCD walks its own eighth of the grouped queries, i.e. its own eighth
+// of the key space, and the run rows it needs stay in that XCD's L2.
+template <class T>
+__global__ void lookup_grouped_kernel(const typename T::K *__restrict__ qk,
+                                      u64 n,
+                                      typename T::V *__restrict__ res_values,
+                                      u8 *__restrict__ res_found,
+                                      RunSet<T> runs) {
   const u32 G = gridDim.x;  // a multiple of 8
   const u32 logical = (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
   const u64 chunk = (n + G - 1) / G;
   const u64 begin = u64(logical) * chunk;
   const u64 end = begin + chunk < n ? begin + chunk : n;
   for (u64 p = begin + threadIdx.x; p < end; p += blockDim.x) {
-    const u32 key = qk[p];
-    const u32 i = qp[p];
-    bool live = false;
-    u32 value = 0;
-    for (u32 r = 0; r < runs.count; ++r) {
-      const RunView &rv = runs.run[r];
-      const u32 b = rv.meta->dir_bits;
-      Row row;
-      if (run_find(rv, b, key, row) != kInvalid) {
-        live = !(row.x & kTombBit);
-        value = row.y;
-        break;
-      }
-    }
-    if (out_found) {
-      out_values[i] = live ? value : 0u;
-      out_found[i] = live ? 1 : 0;
-    } else {
-      out_values[i] = live ? value : kInvalid;
-    }
+    typename T::V value = 0;
+    const bool live = lookup_one(runs, qk[p], value);
+    res_values[p] = live ? value : typename T::V(0);
+    res_found[p] = live ? 1 : 0;
   }
 }
 
+// Result i = grouped result dest[i], written through the IO policy: dest
+// (the partition pass's destination of query i) is read in query order and
+// the grouped results come from 256 streams (one per key byte) that each
+// advance through their own segment, so reads and stores both stream.
+template <class T, class IO>
+__global__ void lookup_gather_kernel(IO io, const u32 *__restrict__ dest,
+                                     u64 n,
+                                     const typename T::V *__restrict__ res_values,
+                                     const u8 *__restrict__ res_found) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    typename IO::Slot slot;
+    (void)io.load(i, slot);
+    const u32 d = dest[i];
+    io.put(slot, res_found[d] != 0, res_values[d]);
+  }
+}
+
+// Workgroups of a grouped lookup over n queries: a multiple of 8 (one
+// share per XCD), about four queries per thread.
+inline u32 grouped_lookup_blocks(u64 n) {
+  const u64 g = (n + 1023) / 1024;
+  const u64 c = g < 1 ? 1 : (g > (1u << 16) ? (1u << 16) : g);
+  return static_cast<u32>((c + 7) & ~u64(7));
+}
+
 // Wave per query.  Rows of run r count when they are live and no newer run
-// holds the same key (newer versions and tombstones shadow them).
+// holds the same key (newer versions and tombstones shadow them).  Sums
+// accumulate in 64 bits and are truncated to the value width.
 constexpr int kRangeThreads = 256;
 constexpr int kRangeWaves = kRangeThreads / kWave;
 
-static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
-    const u32 *__restrict__ lo, const u32 *__restrict__ hi, u64 n,
-    u32 *__restrict__ out, int count_only, RunSet runs) {
+template <class T>
+struct PlainRange {
+  const typename T::K *lo, *hi;
+  typename T::V *out;
+  int count_only;
+  struct Slot {
+    u64 q;
+  };
+  // bounds of query q; `count` selects count (1 per live row) or sum
+  __device__ __forceinline__ void load(u64 q, Slot &s, typename T::K &a,
+                                       typename T::K &z, bool &count) const {
+    s.q = q;
+    a = lo[q];
+    z = hi[q];
+    count = count_only != 0;
+  }
+  __device__ __forceinline__ void put(const Slot &s, typename T::V v) const {
+    out[s.q] = v;
+  }
+};
+
+template <class T, class IO>
+__global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io, u64 n,
+                                                              RunSet<T> runs) {
+  using K = typename T::K;
   __shared__ u32 s_lo[kRangeWaves][kMaxRuns];
   __shared__ u32 s_hi[kRangeWaves][kMaxRuns];
   const u32 lane = lane_id();
@@ -716,24 +865,27 @@ static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
   const u64 wave_stride = u64(gridDim.x) * kRangeWaves;
   const u32 nr = runs.count;
   for (u64 q = u64(blockIdx.x) * kRangeWaves + wave; q < n; q += wave_stride) {
-    const u32 a = lo[q], z = hi[q];
+    typename IO::Slot slot;
+    K a, z;
+    bool count_only;
+    io.load(q, slot, a, z, count_only);
     if (a > z) {
-      if (lane == 0) out[q] = 0;
+      if (lane == 0) io.put(slot, typename T::V(0));
       continue;
     }
     // segment bounds of every run, one lane per run
     for (u32 r = lane; r < nr; r += kWave) {
-      const RunView &rv = runs.run[r];
+      const RunView<T> &rv = runs.run[r];
       const u32 b = rv.meta->dir_bits, c = rv.meta->count;
-      s_lo[wave][r] = run_lower_bound(rv, b, c, a);
-      s_hi[wave][r] = run_lower_bound(rv, b, c, u64(z) + 1);
+      s_lo[wave][r] = run_lower_bound(rv, b, a);
+      s_hi[wave][r] = run_upper_bound(rv, b, c, z);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     u64 acc = 0;
     bool newer_rows = false;
     for (u32 r = 0; r < nr; ++r) {
-      const RunView &rv = runs.run[r];
+      const RunView<T> &rv = runs.run[r];
       const u32 L = s_lo[wave][r], H = s_hi[wave][r];
       if (L >= H) continue;
       const u32 tombs = rv.meta->tombs;
@@ -744,70 +896,78 @@ static __global__ __launch_bounds__(kRangeThreads) void range_kernel(
       }
       const u32 split = rv.meta->split;
       for (u32 idx = L + lane; idx < H; idx += kWave) {
-        const Row w = rv.rows[idx];
-        if (w.x & kTombBit) continue;
+        const typename T::Row w = rv.rows[idx];
+        if (T::kw(w) & T::kTomb) continue;
         bool shadowed = false;
         if (newer_rows) {
-          const u32 key = decode_key(w.x, idx, split);
+          const K key = decode_key<T>(T::kw(w), idx, split);
           for (u32 s = 0; s < r && !shadowed; ++s) {
             u32 l2 = s_lo[wave][s], h2 = s_hi[wave][s];
             if (l2 >= h2) continue;
-            const RunView &nv = runs.run[s];
+            const RunView<T> &nv = runs.run[s];
             const u32 ns = nv.meta->split;
             while (l2 < h2) {
               const u32 mid = (l2 + h2) >> 1;
-              if (decode_key(nv.rows[mid].x, mid, ns) < key) l2 = mid + 1;
-              else h2 = mid;
+              if (decode_key<T>(T::kw(nv.rows[mid]), mid, ns) < key)
+                l2 = mid + 1;
+              else
+                h2 = mid;
             }
             if (l2 < s_hi[wave][s] &&
-                decode_key(nv.rows[l2].x, l2, ns) == key)
+                decode_key<T>(T::kw(nv.rows[l2]), l2, ns) == key)
               shadowed = true;
           }
         }
-        if (!shadowed) acc += count_only ? 1u : w.y;
+        if (!shadowed) acc += count_only ? u64(1) : u64(T::val(w));
       }
       newer_rows = true;
     }
     acc = wave_sum_u64(acc);
-    if (lane == 0) out[q] = static_cast<u32>(acc);
+    if (lane == 0) io.put(slot, static_cast<typename T::V>(acc));
     __builtin_amdgcn_wave_barrier();
   }
 }
 
-static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
-                                        u32 *__restrict__ out, RunSet runs) {
+// Smallest live key >= query, all-ones when there is none.
+template <class T>
+__global__ void successor_kernel(const typename T::K *__restrict__ queries,
+                                 u64 n, typename T::K *__restrict__ out,
+                                 RunSet<T> runs) {
+  using K = typename T::K;
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride) {
-    u64 lower = queries[i];
-    u32 result = kInvalid;
-    while (lower <= 0xffffffffull) {
-      u64 best = 0x100000000ull;
+    K lower = queries[i];
+    K result = T::kAllOnes;
+    while (true) {
+      bool any = false;
+      K best = 0;
       for (u32 r = 0; r < runs.count; ++r) {
-        const RunView &rv = runs.run[r];
+        const RunView<T> &rv = runs.run[r];
         const u32 b = rv.meta->dir_bits, c = rv.meta->count;
-        const u32 p = run_lower_bound(rv, b, c, lower);
+        const u32 p = run_lower_bound(rv, b, lower);
         if (p < c) {
-          const u64 k = decode_key(rv.rows[p].x, p, rv.meta->split);
-          if (k < best) best = k;
+          const K k = decode_key<T>(T::kw(rv.rows[p]), p, rv.meta->split);
+          if (!any || k < best) best = k;
+          any = true;
         }
       }
-      if (best > 0xffffffffull) break;
+      if (!any) break;
       // the newest run that holds `best` decides its visibility
       bool live = false;
       for (u32 r = 0; r < runs.count; ++r) {
-        const RunView &rv = runs.run[r];
-        Row row;
-        if (run_find(rv, rv.meta->dir_bits, static_cast<u32>(best), row) !=
-            kInvalid) {
-          live = !(row.x & kTombBit);
+        const RunView<T> &rv = runs.run[r];
+        typename T::Row row;
+        if (run_find(rv, rv.meta->dir_bits, best, row) != 0xffffffffu) {
+          live = !(T::kw(row) & T::kTomb);
           break;
         }
       }
       if (live) {
-        result = static_cast<u32>(best);
+        result = best;
         break;
       }
+      if (best == T::kAllOnes) break;
       lower = best + 1;
     }
     out[i] = result;
@@ -815,18 +975,19 @@ static __global__ void successor_kernel(const u32 *__restrict__ queries, u64 n,
 }
 
 // Writes a tombstone-free run as plain (key, value) arrays.
-static __global__ void decode_run_kernel(const Row *__restrict__ rows,
-                                         const RunMeta *meta,
-                                         u32 *__restrict__ ok,
-                                         u32 *__restrict__ ov) {
+template <class T>
+__global__ void decode_run_kernel(const typename T::Row *__restrict__ rows,
+                                  const RunMeta *meta,
+                                  typename T::K *__restrict__ ok,
+                                  typename T::V *__restrict__ ov) {
   const u32 count = meta->count, split = meta->split;
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
        i += stride) {
     const u32 pos = static_cast<u32>(i);
-    const Row r = rows[pos];
-    ok[i] = decode_key(r.x, pos, split);
-    ov[i] = r.y;
+    const typename T::Row r = rows[pos];
+    ok[i] = decode_key<T>(T::kw(r), pos, split);
+    ov[i] = T::val(r);
   }
 }
 

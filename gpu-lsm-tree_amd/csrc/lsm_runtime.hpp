@@ -17,7 +17,9 @@
 #include <algorithm>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -33,7 +35,9 @@ constexpr u32 kBatchesPerEpoch = 16;                  // GPULSMOpt.cuh:42
 constexpr u64 kMaximumRows = 0xffffffffull;           // GPULSMOpt.cuh:59-60
 
 
-struct Run {
+template <class T>
+struct RunT {
+  using Row = typename T::Row;
   Row *rows = nullptr;
   u32 *dir = nullptr;  // RunMeta header, then the directory entries
   u64 rows_ub = 0;     // upper bound on the rows (exact after a refresh)
@@ -41,7 +45,7 @@ struct Run {
 
   RunMeta *meta() const { return reinterpret_cast<RunMeta *>(dir); }
   u32 *entries() const { return dir + kMetaBytes / sizeof(u32); }
-  RunView view() const { return {rows, entries(), meta()}; }
+  RunView<T> view() const { return {rows, entries(), meta()}; }
 };
 
 static __global__ void zero_two_kernel(u32 *a, u64 na, u32 *b, u64 nb) {
@@ -53,8 +57,14 @@ static __global__ void zero_two_kernel(u32 *a, u64 na, u32 *b, u64 nb) {
   }
 }
 
-class Lsm : public HandleBase {
+template <class T>
+class LsmT : public HandleBase {
  public:
+  using K = typename T::K;
+  using V = typename T::V;
+  using Row = typename T::Row;
+  using Run = RunT<T>;
+
   // GPULSMOpt.cuh:5606-5616 and :255-270
   static u64 batch_capacity_for(const gd_config &c) {
     return std::min<u64>(kMaximumOperationTile,
@@ -69,10 +79,10 @@ class Lsm : public HandleBase {
     return cap;
   }
 
-  explicit Lsm(const gd_config &c)
+  explicit LsmT(const gd_config &c)
       : batch_capacity_(batch_capacity_for(c)), capacity_(capacity_for(c)) {}
 
-  ~Lsm() {
+  ~LsmT() {
     try {
       order_.wait_host();
       reclaim(/*wait=*/true);
@@ -80,6 +90,8 @@ class Lsm : public HandleBase {
     }
     clear_profile();
     if (prof_meta_) (void)hipHostFree(prof_meta_);
+    for (auto &kv : pending_) (void)hipEventDestroy(kv.second.done);
+    pending_.clear();
     defer_ = false;
     for (auto &r : runs_) free_run(r);
     runs_.clear();
@@ -93,7 +105,7 @@ class Lsm : public HandleBase {
     order_.end(s);
   }
 
-  void bulk_build(const u32 *keys, const u32 *values, u64 n, hipStream_t s) {
+  void bulk_build(const K *keys, const V *values, u64 n, hipStream_t s) {
     if ((n && (!keys || !values)) || n > kMaximumRows)
       throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt initial input");
     order_.begin(s);
@@ -117,7 +129,7 @@ class Lsm : public HandleBase {
     order_.end(s);
   }
 
-  void admit(const u32 *keys, const u32 *values, const u8 *flags, u64 n,
+  void admit(const K *keys, const V *values, const u8 *flags, u64 n,
              int mode, hipStream_t s) {
     order_.begin(s);
     try {
@@ -129,7 +141,7 @@ class Lsm : public HandleBase {
     order_.end(s);
   }
 
-  void admit_tiles(const u32 *keys, const u32 *values, const u8 *flags, u64 n,
+  void admit_tiles(const K *keys, const V *values, const u8 *flags, u64 n,
                    int mode, hipStream_t s) {
     if (failed_) {
       throw Error(GD_PUBLICATION_FAILED,
@@ -151,10 +163,20 @@ class Lsm : public HandleBase {
                          flags ? flags + consumed : nullptr, tile, mode, s);
       runs_.insert(runs_.begin(), r);
       consumed += tile;
-      rebalance(s);
+      hipStream_t ms = s;
+      if (ms_) {
+        // merges wait for this admission, admissions never wait for merges
+        hipEvent_t admitted;
+        GD_HIP(hipEventCreateWithFlags(&admitted, hipEventDisableTiming));
+        GD_HIP(hipEventRecord(admitted, s));
+        GD_HIP(hipStreamWaitEvent(ms_, admitted, 0));
+        (void)hipEventDestroy(admitted);
+        ms = ms_;
+      }
+      rebalance(ms);
       if (++pending_batches_ == kBatchesPerEpoch) {
         pending_batches_ = 0;
-        epoch_capacity_check(s);
+        epoch_capacity_check(ms);
       }
       if (consumed < n && failed_) {
         incomplete = true;
@@ -169,34 +191,41 @@ class Lsm : public HandleBase {
   }
 
   // -------------------------------------------------------------- reads
-  void lookup(const u32 *q, u64 n, u32 *out_values, u8 *out_found,
+  void lookup(const K *q, u64 n, V *out_values, u8 *out_found,
               hipStream_t s) {
     if (!n) return;
     if (!q || !out_values)
       throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt lookup");
     order_.begin(s);
     try {
-      const RunSet set = run_set();
+      const RunSet<T> set = run_set();
       if (n >= grouped_lookup_min() && n <= 0xffffffffull && !runs_.empty()) {
-        // group the queries by their top key byte first (one radix pass)
+        // group the queries by their top key byte first (one radix pass
+        // that also records where each query went), look up in grouped
+        // order, then gather the results back to query order
         const u32 nn = static_cast<u32>(n);
-        Lease<u32> qk(pool_, n), qp(pool_, n);
+        Lease<K> qk(pool_, n);
+        Lease<u32> dest(pool_, n);
+        Lease<V> rv(pool_, n);
+        Lease<u8> rf(pool_, n);
         Lease<u32> ctrl(pool_, rs::control_words(n, 1));
-        GD_HIP(rs::partition_pairs<u32>(q, nn, 24, qk.get(), qp.get(),
-                                        ctrl.get(), s, tuning_.sort_block));
-        // a multiple of 8 workgroups (one share per XCD), ~4 queries/thread
-        const u32 blocks =
-            (std::min<u32>(grid_for(n, 1024, 1u << 16), 1u << 16) + 7u) & ~7u;
+        GD_HIP(rs::partition_pairs<K>(q, nn, T::kBits - 8, qk.get(), nullptr,
+                                        ctrl.get(), s, tuning_.sort_block,
+                                        dest.get()));
         ProfRec rec{};
         if (profiling_) rec = prof_begin(1, s, n);
-        lookup_grouped_kernel<<<blocks, 256, 0, s>>>(
-            qk.get(), qp.get(), n, out_values, out_found, set);
+        lookup_grouped_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
+            qk.get(), n, rv.get(), rf.get(), set);
         if (profiling_) prof_end(rec, s);
+        GD_LAUNCH_CHECK();
+        lookup_gather_kernel<T><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            PlainLookup<T>{q, out_values, out_found}, dest.get(), n, rv.get(),
+            rf.get());
       } else {
         ProfRec rec{};
         if (profiling_) rec = prof_begin(1, s, n);
-        lookup_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
-            q, n, out_values, out_found, set);
+        lookup_kernel<T><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            PlainLookup<T>{q, out_values, out_found}, n, set);
         if (profiling_) prof_end(rec, s);
       }
       GD_LAUNCH_CHECK();
@@ -207,32 +236,33 @@ class Lsm : public HandleBase {
     order_.end(s);
   }
 
-  void range(const u32 *lo, const u32 *hi, u64 n, u32 *out, bool count_only,
+  void range(const K *lo, const K *hi, u64 n, V *out, bool count_only,
              hipStream_t s) {
     if (!n) return;
     if (!lo || !hi || !out)
       throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt range");
     order_.begin(s);
-    const RunSet set = run_set();
-    range_kernel<<<grid_for(n, kRangeWaves, 1u << 16), kRangeThreads, 0, s>>>(
-        lo, hi, n, out, count_only ? 1 : 0, set);
+    const RunSet<T> set = run_set();
+    range_kernel<T><<<grid_for(n, kRangeWaves, 1u << 16), kRangeThreads, 0,
+                      s>>>(PlainRange<T>{lo, hi, out, count_only ? 1 : 0}, n,
+                           set);
     GD_LAUNCH_CHECK();
     order_.end(s);
   }
 
-  void successor(const u32 *q, u64 n, u32 *out, hipStream_t s) {
+  void successor(const K *q, u64 n, K *out, hipStream_t s) {
     if (!n) return;
     if (!q || !out)
       throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt successor");
     order_.begin(s);
-    const RunSet set = run_set();
-    successor_kernel<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(q, n, out,
-                                                                 set);
+    const RunSet<T> set = run_set();
+    successor_kernel<T><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(q, n, out,
+                                                                    set);
     GD_LAUNCH_CHECK();
     order_.end(s);
   }
 
-  u64 dump(u32 *out_keys, u32 *out_vals, u64 capacity, hipStream_t s) {
+  u64 dump(K *out_keys, V *out_vals, u64 capacity, hipStream_t s) {
     order_.begin(s);
     u64 count = 0;
     try {
@@ -244,7 +274,7 @@ class Lsm : public HandleBase {
             free_run(merged);
             throw Error(GD_INVALID_ARGUMENT, "null dump output");
           }
-          decode_run_kernel<<<grid_for(count, 256, 1u << 16), 256, 0, s>>>(
+          decode_run_kernel<T><<<grid_for(count, 256, 1u << 16), 256, 0, s>>>(
               merged.rows, merged.meta(), out_keys, out_vals);
           GD_LAUNCH_CHECK();
         }
@@ -275,43 +305,129 @@ class Lsm : public HandleBase {
   // event chaining.  Updates run on one stream while earlier queries may
   // still read the previous runs on another: runs released by updates are
   // kept (deferred) until an event recorded on the query stream completes.
-  void admit_on(const u32 *keys, const u32 *values, const u8 *flags, u64 n,
+  void admit_on(const K *keys, const V *values, const u8 *flags, u64 n,
                 int mode, hipStream_t s) {
     admit_tiles(keys, values, flags, n, mode, s);
   }
 
-  RunSet snapshot() const { return run_set(); }
+  RunSet<T> snapshot() const { return run_set(); }
   bool empty() const { return runs_.empty(); }
 
-  void set_deferred_reclaim(bool on) { defer_ = on; }
+  // Serving mode.  Admissions stay on the caller's stream; the merges that
+  // rebalance the stack run on `merge_stream` (after an event of the
+  // admission), and queries on `query_stream`.  A query snapshot reads the
+  // two inputs of every merge that has not completed instead of its output
+  // (snapshot_for), so queries never wait for merges.  Released runs stay
+  // allocated until (1) every merge enqueued before their release has
+  // completed, after which no new snapshot can name them, and then (2) every
+  // query enqueued up to that moment has completed.  Merges allocate from
+  // their own pool: a pool hands out buffers whose last use was enqueued on
+  // its own stream only.
+  void set_async(hipStream_t merge_stream, hipStream_t query_stream) {
+    ms_ = merge_stream;
+    qs_ = query_stream;
+    defer_ = true;
+  }
 
-  // Releases held runs once every query enqueued so far on `query_stream`
-  // has finished.
-  void retire_pending(hipStream_t query_stream) {
+  RunSet<T> snapshot_for(hipStream_t qs) {
+    RunSet<T> set{};
+    bool too_many = false;
+    // newest first: a pending merge of (newer a, older b) reads as a, b
+    std::function<void(const Run &)> add = [&](const Run &r) {
+      auto it = pending_.find(r.rows);
+      if (it != pending_.end()) {
+        const hipError_t q = hipEventQuery(it->second.done);
+        if (q == hipSuccess) {
+          (void)hipEventDestroy(it->second.done);
+          pending_.erase(it);
+        } else {
+          if (q != hipErrorNotReady) GD_HIP(q);
+          const Run a = it->second.a, b = it->second.b;
+          add(a);
+          add(b);
+          return;
+        }
+      }
+      if (set.count < static_cast<u32>(kMaxRuns)) set.run[set.count] = r.view();
+      else too_many = true;
+      ++set.count;
+    };
+    for (const Run &r : runs_) add(r);
+    if (too_many) {
+      // more runs than a snapshot holds: wait for the merges instead
+      hipEvent_t merged;
+      GD_HIP(hipEventCreateWithFlags(&merged, hipEventDisableTiming));
+      GD_HIP(hipEventRecord(merged, ms_));
+      GD_HIP(hipStreamWaitEvent(qs, merged, 0));
+      (void)hipEventDestroy(merged);
+      return run_set();
+    }
+    return set;
+  }
+
+  // Ends serving mode: completes every retirement while the streams still
+  // exist (call after synchronising them).
+  void detach_streams() {
+    reclaim(/*wait=*/true);
+    for (auto &kv : pending_) (void)hipEventDestroy(kv.second.done);
+    pending_.clear();
+    ms_ = qs_ = nullptr;
+    defer_ = false;
+  }
+
+  // Moves the runs released so far into a retirement (serving mode).
+  void retire_pending() {
     if (pending_free_.empty()) return;
     Deferred d;
-    GD_HIP(hipEventCreateWithFlags(&d.done, hipEventDisableTiming));
-    GD_HIP(hipEventRecord(d.done, query_stream));
+    GD_HIP(hipEventCreateWithFlags(&d.merges_done, hipEventDisableTiming));
+    GD_HIP(hipEventRecord(d.merges_done, ms_));
     d.buffers.swap(pending_free_);
     deferred_.push_back(std::move(d));
   }
 
-  // Returns the buffers of completed retirements to the pool.
+  // Returns the buffers of completed retirements to their pools, and drops
+  // the records of completed merges.
   void reclaim(bool wait = false) {
     while (!deferred_.empty()) {
       Deferred &d = deferred_.front();
+      if (d.merges_done) {
+        if (wait) {
+          GD_HIP(hipEventSynchronize(d.merges_done));
+        } else {
+          const hipError_t q = hipEventQuery(d.merges_done);
+          if (q == hipErrorNotReady) break;
+          GD_HIP(q);
+        }
+        (void)hipEventDestroy(d.merges_done);
+        d.merges_done = nullptr;
+        GD_HIP(hipEventCreateWithFlags(&d.queries_done,
+                                       hipEventDisableTiming));
+        GD_HIP(hipEventRecord(d.queries_done, qs_));
+      }
       if (wait) {
-        GD_HIP(hipEventSynchronize(d.done));
+        GD_HIP(hipEventSynchronize(d.queries_done));
       } else {
-        const hipError_t q = hipEventQuery(d.done);
+        const hipError_t q = hipEventQuery(d.queries_done);
         if (q == hipErrorNotReady) break;
         GD_HIP(q);
       }
-      for (void *p : d.buffers) pool_.put(p);
-      (void)hipEventDestroy(d.done);
+      for (void *p : d.buffers) {
+        pool_.put(p);
+        merge_pool_.put(p);  // each buffer belongs to exactly one of them
+      }
+      (void)hipEventDestroy(d.queries_done);
       deferred_.pop_front();
     }
+    for (auto it = pending_.begin(); it != pending_.end();) {
+      if (hipEventQuery(it->second.done) == hipSuccess) {
+        (void)hipEventDestroy(it->second.done);
+        it = pending_.erase(it);
+      } else {
+        ++it;
+      }
+    }
   }
+
 
   bool failed() const { return failed_; }
 
@@ -331,7 +447,7 @@ class Lsm : public HandleBase {
     if (t.grouped_lookup_min_log)
       v.grouped_lookup_min_log = std::min<u32>(t.grouped_lookup_min_log, 63);
     if (t.small_batch_max)
-      v.small_batch_max = std::min<u32>(t.small_batch_max, kSmallTile);
+      v.small_batch_max = std::min<u32>(t.small_batch_max, T::kSmallTile);
     if (t.max_runs)  // max_runs
       v.max_runs = std::max<u32>(2, std::min<u32>(t.max_runs,
                                                      kMaxRuns - 1));
@@ -381,7 +497,9 @@ class Lsm : public HandleBase {
   }
 
   u32 carry_status() const { return status_; }
-  u64 resident_bytes() const { return pool_.held_bytes(); }
+  u64 resident_bytes() const {
+    return pool_.held_bytes() + merge_pool_.held_bytes();
+  }
   // Synchronous: exact rows per run (refreshes the host upper bounds).
   void shape(u32 *runs, u64 *rows) {
     order_.wait_host();
@@ -404,15 +522,21 @@ class Lsm : public HandleBase {
     status_ = 0;
   }
 
-  Run alloc_run(u64 rows_ub) {
+  // The pool for work on stream s (merges in serving mode have their own).
+  DevicePool &pool_for(hipStream_t s) {
+    return (ms_ && s == ms_) ? merge_pool_ : pool_;
+  }
+
+  Run alloc_run(u64 rows_ub, DevicePool &pool) {
     Run r;
     r.rows_ub = rows_ub;
     r.max_bits = dir_bits_for(rows_ub, kMaxDirBits, tuning_.dir_rows_log);
-    r.rows = pool_.get_n<Row>(std::max<u64>(rows_ub, 1));
+    r.rows = pool.get_n<Row>(std::max<u64>(rows_ub, 1));
     r.dir = static_cast<u32 *>(
-        pool_.get(kMetaBytes + ((u64(1) << r.max_bits) + 1) * sizeof(u32)));
+        pool.get(kMetaBytes + ((u64(1) << r.max_bits) + 1) * sizeof(u32)));
     return r;
   }
+  Run alloc_run(u64 rows_ub) { return alloc_run(rows_ub, pool_); }
 
   void free_run(Run &r) {
     if (defer_) {
@@ -433,35 +557,36 @@ class Lsm : public HandleBase {
 
   void launch_directory(const Run &r, hipStream_t s) {
     const u64 items = std::max<u64>(r.rows_ub, (u64(1) << r.max_bits) + 1);
-    build_directory_kernel<<<grid_for(items, 256, 8192), 256, 0, s>>>(
+    build_directory_kernel<T><<<grid_for(items, 256, 8192), 256, 0, s>>>(
         r.rows, r.meta(), r.entries(), r.max_bits, tuning_.dir_rows_log);
     GD_LAUNCH_CHECK();
   }
 
   // Sorts one tile of updates into a new run.
-  Run sorted_run(const u32 *keys, const u32 *values, const u8 *flags, u32 n,
+  Run sorted_run(const K *keys, const V *values, const u8 *flags, u32 n,
                  int mode, hipStream_t s) {
     Run r = alloc_run(n);
     if (n <= tuning_.small_batch_max) {
       // one workgroup sorts, deduplicates and indexes the whole batch
-      small_batch_run_kernel<<<1, kSmallThreads, 0, s>>>(
+      small_batch_run_kernel<T><<<1, kSmallThreads, 0, s>>>(
           keys, n, values, flags, mode, r.rows, r.meta(), r.entries(),
           r.max_bits, tuning_.dir_rows_log);
       GD_LAUNCH_CHECK();
       return r;
     }
     const u32 fin_tiles = (n + kFinTile - 1) / kFinTile;
-    const u64 sort_words = rs::control_words(n);
+    const u64 sort_words = rs::control_words(n, rs::passes_for<K>());
     const u64 fin_words = 64 + 2ull * fin_tiles;  // ticket pad + status u64
-    Lease<u32> ka(pool_, n), va(pool_, n), kb(pool_, n), vb(pool_, n);
+    Lease<K> ka(pool_, n), kb(pool_, n);
+    Lease<u32> va(pool_, n), vb(pool_, n);
     Lease<u32> ctrl(pool_, sort_words + fin_words);
     u32 *fin_ctrl = ctrl.get() + sort_words;
     zero_two_kernel<<<grid_for(fin_words + 4, 256, 1024), 256, 0, s>>>(
         reinterpret_cast<u32 *>(r.meta()), 4, fin_ctrl, fin_words);
     GD_LAUNCH_CHECK();
-    GD_HIP(rs::sort_pairs<u32>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
-                               ctrl.get(), s, tuning_.sort_block));
-    finalize_batch_kernel<<<fin_tiles, kFinThreads, 0, s>>>(
+    GD_HIP(rs::sort_pairs<K>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
+                             ctrl.get(), s, tuning_.sort_block));
+    finalize_batch_kernel<T><<<fin_tiles, kFinThreads, 0, s>>>(
         ka.get(), va.get(), n, values, flags, mode, r.rows, r.meta(),
         reinterpret_cast<u64 *>(fin_ctrl + 64), fin_ctrl);
     GD_LAUNCH_CHECK();
@@ -474,30 +599,40 @@ class Lsm : public HandleBase {
   Run merge_two(const Run &a, const Run &b, bool drop_tombstones,
                 hipStream_t s) {
     const u64 ub = a.rows_ub + b.rows_ub;
-    Run out = alloc_run(ub);
-    const u32 tile_rows = tuning_.merge_tile_log == 11 ? 2048u : 4096u;
+    DevicePool &pool = pool_for(s);
+    Run out = alloc_run(ub, pool);
+    const u32 tile_rows =
+        (tuning_.merge_tile_log == 11 || T::kMergeItemsMax < 16) ? 2048u
+                                                                  : 4096u;
     const u32 tiles =
         static_cast<u32>(std::max<u64>(1, (ub + tile_rows - 1) / tile_rows));
     const u64 ctrl_words = 64 + 2ull * tiles;
-    Lease<u32> parts(pool_, u64(tiles) + 1);
-    Lease<u32> ctrl(pool_, ctrl_words);
+    Lease<u32> parts(pool, u64(tiles) + 1);
+    Lease<u32> ctrl(pool, ctrl_words);
     u64 *status = reinterpret_cast<u64 *>(ctrl.get() + 64);
-    const MergeSide sa{a.rows, a.meta()};
-    const MergeSide sb{b.rows, b.meta()};
-    merge_partition_kernel<<<grid_for(u64(tiles) + 1, 256), 256, 0, s>>>(
+    const MergeSide<T> sa{a.rows, a.meta()};
+    const MergeSide<T> sb{b.rows, b.meta()};
+    merge_partition_kernel<T><<<grid_for(u64(tiles) + 1, 256), 256, 0, s>>>(
         sa, sb, tiles, tile_rows, parts.get(), status, ctrl.get(),
         out.meta());
     GD_LAUNCH_CHECK();
     ProfRec rec{};
     if (profiling_) rec = prof_begin(0, s, 0);
-    if (tile_rows == 2048)
-      merge_kernel<8><<<tiles, kMergeThreads, 0, s>>>(
+    if constexpr (T::kMergeItemsMax >= 16) {
+      if (tile_rows == 4096) {
+        merge_kernel<T, 16><<<tiles, kMergeThreads, 0, s>>>(
+            sa, sb, parts.get(), out.rows, out.meta(),
+            drop_tombstones ? 1 : 0, status, ctrl.get());
+      } else {
+        merge_kernel<T, 8><<<tiles, kMergeThreads, 0, s>>>(
+            sa, sb, parts.get(), out.rows, out.meta(),
+            drop_tombstones ? 1 : 0, status, ctrl.get());
+      }
+    } else {
+      merge_kernel<T, 8><<<tiles, kMergeThreads, 0, s>>>(
           sa, sb, parts.get(), out.rows, out.meta(), drop_tombstones ? 1 : 0,
           status, ctrl.get());
-    else
-      merge_kernel<16><<<tiles, kMergeThreads, 0, s>>>(
-          sa, sb, parts.get(), out.rows, out.meta(), drop_tombstones ? 1 : 0,
-          status, ctrl.get());
+    }
     GD_LAUNCH_CHECK();
     if (profiling_) {
       prof_end(rec, s);
@@ -514,7 +649,7 @@ class Lsm : public HandleBase {
   }
 
   Run empty_run(hipStream_t s) {
-    Run r = alloc_run(0);
+    Run r = alloc_run(0, pool_for(s));
     zero_two_kernel<<<1, 256, 0, s>>>(reinterpret_cast<u32 *>(r.meta()), 4,
                                       r.entries(), 3);
     GD_LAUNCH_CHECK();
@@ -578,6 +713,17 @@ class Lsm : public HandleBase {
       }
       const bool oldest = i + 2 == runs_.size();
       Run out = merge_two(runs_[i], runs_[i + 1], oldest, s);
+      if (ms_ && s == ms_) {
+        PendingMerge pm{nullptr, runs_[i], runs_[i + 1]};
+        GD_HIP(hipEventCreateWithFlags(&pm.done, hipEventDisableTiming));
+        GD_HIP(hipEventRecord(pm.done, s));
+        auto old = pending_.find(out.rows);  // a stale, completed record
+        if (old != pending_.end()) {
+          (void)hipEventDestroy(old->second.done);
+          pending_.erase(old);
+        }
+        pending_.emplace(out.rows, pm);
+      }
       free_run(runs_[i]);
       free_run(runs_[i + 1]);
       runs_.erase(runs_.begin() + i, runs_.begin() + i + 2);
@@ -615,21 +761,27 @@ class Lsm : public HandleBase {
     }
   }
 
-  RunSet run_set() const {
-    RunSet set{};
+  RunSet<T> run_set() const {
+    RunSet<T> set{};
     set.count = static_cast<u32>(runs_.size());
     for (std::size_t i = 0; i < runs_.size(); ++i) set.run[i] = runs_[i].view();
     return set;
   }
 
   struct Deferred {
-    hipEvent_t done = nullptr;
+    hipEvent_t merges_done = nullptr;   // phase 1 (merge stream)
+    hipEvent_t queries_done = nullptr;  // phase 2 (query stream)
     std::vector<void *> buffers;
+  };
+  struct PendingMerge {
+    hipEvent_t done;
+    Run a, b;  // inputs, newer first
   };
 
   u64 batch_capacity_ = 1;
   u64 capacity_ = 0;
-  gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(kSmallTile), 10, {0, 0}};
+  gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(T::kSmallTile), 10,
+                        {0, 0}};
 
   struct ProfRec {
     int kind;  // 0 merge, 1 lookup
@@ -663,13 +815,19 @@ class Lsm : public HandleBase {
   bool defer_ = false;
   std::vector<void *> pending_free_;
   std::deque<Deferred> deferred_;
+  hipStream_t ms_ = nullptr, qs_ = nullptr;  // serving mode streams
+  std::unordered_map<const void *, PendingMerge> pending_;
   DevicePool pool_;
+  DevicePool merge_pool_;
   OperationOrder order_;
   std::vector<Run> runs_;  // newest first
   u32 pending_batches_ = 0;
   bool failed_ = false;
   u32 status_ = 0;
 };
+
+using Lsm = LsmT<KV<u32>>;
+using Lsm64 = LsmT<KV<u64>>;
 
 }  // namespace lsm
 }  // namespace gdict

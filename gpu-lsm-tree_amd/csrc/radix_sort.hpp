@@ -26,52 +26,68 @@ constexpr int passes_for() {
   return static_cast<int>(sizeof(K));
 }
 
-// Histogram blocks: one partial histogram row per block, written whole (no
-// global atomics: a few hundred blocks adding into the same 1024 counters
-// serialise on those addresses).  16K keys per block, at most 256 rows.
+// Digit histograms.  A block counts 16 keys per thread per round in LDS
+// (the 16 loads are issued together), then adds its counts into one of
+// kHistRows partial rows (row = block % kHistRows), so at most
+// blocks / kHistRows atomics meet on one address; scan_kernel sums the rows.
+constexpr int kHistItems = 16;
+constexpr int kHistRows = 32;
+
 inline u32 hist_blocks_for(u64 n) {
-  const u64 b = (n + 16383) / 16384;
-  return static_cast<u32>(b < 1 ? 1 : (b > 256 ? 256 : b));
+  const u64 b = (n + kThreads * kHistItems - 1) / (kThreads * kHistItems);
+  return static_cast<u32>(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
 
-// Per-block digit histograms of `passes` digits starting at digit `first`
-// (digit p = (key >> 8p) & 0xff).  rows[blockIdx][p * kRadix + d].  The
-// blocks also clear the look-back state of the passes that follow:
-// `zero` (u32 words) is cleared grid-stride.
-template <class K>
+// rows[kHistRows][kPasses][kRadix] (zeroed) += counts of digits
+// first .. first + kPasses - 1, digit p = (key >> 8p) & 0xff.
+template <class K, int kPasses>
 static __global__ __launch_bounds__(kThreads) void histogram_kernel(
-    const K *__restrict__ keys, u32 n, u32 first, u32 passes,
-    u32 *__restrict__ rows, u32 *__restrict__ zero, u64 zero_words) {
-  __shared__ u32 s_hist[kMaxPasses][kRadix];
-  for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kThreads)
+    const K *__restrict__ keys, u32 n, u32 first, u32 *__restrict__ rows) {
+  __shared__ u32 s_hist[kPasses][kRadix];
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads)
     (&s_hist[0][0])[i] = 0;
   __syncthreads();
-  const u64 stride = u64(gridDim.x) * kThreads;
-  for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
-    const K k = keys[i];
-    for (u32 p = 0; p < passes; ++p)
-      atomicAdd(&s_hist[p][static_cast<u32>(k >> (8 * (first + p))) & 0xffu],
-                1u);
+  const u64 round = u64(kThreads) * kHistItems;
+  for (u64 base = u64(blockIdx.x) * round; base < n;
+       base += u64(gridDim.x) * round) {
+    K k[kHistItems];
+#pragma unroll
+    for (int j = 0; j < kHistItems; ++j) {
+      const u64 i = base + u64(j) * kThreads + threadIdx.x;
+      k[j] = i < n ? keys[i] : K(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kHistItems; ++j) {
+      if (base + u64(j) * kThreads + threadIdx.x < n) {
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p)
+          atomicAdd(&s_hist[p][static_cast<u32>(k[j] >> (8 * (first + p))) &
+                               0xffu],
+                    1u);
+      }
+    }
   }
-  for (u64 i = u64(blockIdx.x) * kThreads + threadIdx.x; i < zero_words;
-       i += stride)
-    zero[i] = 0;
   __syncthreads();
-  u32 *row = rows + u64(blockIdx.x) * passes * kRadix;
-  for (u32 i = threadIdx.x; i < passes * kRadix; i += kThreads)
-    row[i] = (&s_hist[0][0])[i];
+  u32 *row = rows + u64(blockIdx.x % kHistRows) * kPasses * kRadix;
+  for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads) {
+    const u32 c = (&s_hist[0][0])[i];
+    if (c) atomicAdd(&row[i], c);
+  }
 }
 
-// Sums the per-block rows of pass blockIdx.x and writes the exclusive scan
-// over digits -> global digit offsets.
+// Sums the kHistRows partial rows of pass blockIdx.x and writes the
+// exclusive scan over digits -> global digit offsets.
 static __global__ __launch_bounds__(kRadix) void scan_kernel(
-    const u32 *__restrict__ rows, u32 n_rows, u32 passes,
-    u32 *__restrict__ offsets) {
+    const u32 *__restrict__ rows, u32 passes, u32 *__restrict__ offsets) {
   __shared__ u32 scratch[kRadix / kWave + 1];
   const u32 p = blockIdx.x;
+  u32 c[kHistRows];
+#pragma unroll
+  for (int r = 0; r < kHistRows; ++r)
+    c[r] = rows[(u64(r) * passes + p) * kRadix + threadIdx.x];
   u32 v = 0;
-  for (u32 r = 0; r < n_rows; ++r)
-    v += rows[(u64(r) * passes + p) * kRadix + threadIdx.x];
+#pragma unroll
+  for (int r = 0; r < kHistRows; ++r) v += c[r];
   offsets[p * kRadix + threadIdx.x] = block_exclusive_scan<kRadix>(v, scratch);
 }
 
@@ -84,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K *__restrict__ keys_in, const u32 *__restrict__ vals_in,
     K *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
     u32 shift, const u32 *__restrict__ digit_offsets, u64 *status,
-    u32 *ticket) {
+    u32 *ticket, u32 *__restrict__ dest_out) {
   constexpr int kTile = kBlock * kItems;
   constexpr int kBlockWaves = kBlock / kWave;
   __shared__ u32 s_tile;
@@ -191,6 +207,8 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
       const u32 lp = s_block_start[dj] + s_wave_hist[wave][dj] + rank[j];
       s_keys[lp] = key[j];
       s_vals[lp] = val[j];
+      // destination of input element idx, written in input order
+      if (dest_out) dest_out[idx] = s_global_base[dj] + lp;
     }
   }
   __syncthreads();
@@ -202,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K k = s_keys[i];
     const u32 pos = s_global_base[static_cast<u32>(k >> shift) & 0xffu] + i;
     keys_out[pos] = k;
-    vals_out[pos] = s_vals[i];
+    if (vals_out) vals_out[pos] = s_vals[i];
   }
 }
 
@@ -225,49 +243,50 @@ inline u32 tiles_for(u64 n) {
 
 // Workspace (u32 words) of a sort of n keys in `passes` digit passes, not
 // counting the ping-pong buffers:
-//   rows    [hist_blocks][passes][256]   per-block histograms
+//   rows    [kHistRows][passes][256]     partial histograms
 //   offsets [passes][256]                global digit offsets
 //   tickets [64]                         tile tickets, one per pass
 //   status  u64 [passes][tiles][256]     look-back words
+// Everything but the offsets starts at zero (one clearing launch).
 struct Control {
   u32 *rows, *offsets, *tickets;
   u64 *status;
-  u32 hist_blocks;
 };
 
-inline u64 control_prefix_words(u64 n, int passes) {
-  const u64 w = u64(hist_blocks_for(n)) * passes * kRadix +
-                u64(passes) * kRadix + 64;
-  return (w + 1) & ~u64(1);  // status is 8-byte aligned
-}
-
 inline u64 control_words(u64 n, int passes = 4) {
-  return control_prefix_words(n, passes) +
+  return u64(kHistRows) * passes * kRadix + u64(passes) * kRadix + 64 +
          2ull * passes * u64(tiles_for(n)) * kRadix;
 }
 
-inline Control control_layout(u32 *control, u64 n, int passes) {
+inline Control control_layout(u32 *control, int passes) {
   Control c;
-  c.hist_blocks = hist_blocks_for(n);
   c.rows = control;
-  c.offsets = control + u64(c.hist_blocks) * passes * kRadix;
+  c.offsets = control + u64(kHistRows) * passes * kRadix;
   c.tickets = c.offsets + passes * kRadix;
-  c.status = reinterpret_cast<u64 *>(control +
-                                     control_prefix_words(n, passes));
+  c.status = reinterpret_cast<u64 *>(c.tickets + 64);  // even word offset
   return c;
+}
+
+inline void clear_control(u32 *control, u64 n, int passes,
+                          hipStream_t stream) {
+  const u64 words = control_words(n, passes);
+  zero_u32_kernel<<<static_cast<u32>(std::min<u64>((words + 255) / 256,
+                                                   4096)),
+                    256, 0, stream>>>(control, words);
 }
 
 template <class K, bool kFirst>
 inline void launch_pass(int block, u32 tiles, const K *src_k,
                         const u32 *src_v, K *dst_k, u32 *dst_v, u32 n,
                         u32 shift, const u32 *offsets, u64 *status,
-                        u32 *ticket, hipStream_t stream) {
+                        u32 *ticket, hipStream_t stream,
+                        u32 *dest = nullptr) {
   if (block == 512)
     onesweep_kernel<K, kFirst, 8, 512><<<tiles, 512, 0, stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket);
+        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest);
   else
     onesweep_kernel<K, kFirst, 16, 256><<<tiles, 256, 0, stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket);
+        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest);
 }
 
 // Sorts n (key, position) pairs stably.  keys_in is read only; the sorted
@@ -282,13 +301,11 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   if (n == 0) return hipSuccess;
   const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
-  const Control c = control_layout(control, n, kPasses);
-  // the histogram blocks clear tickets and look-back words for the passes
-  const u64 zero_words = control_words(n, kPasses) - (c.tickets - control);
-  histogram_kernel<K><<<c.hist_blocks, kThreads, 0, stream>>>(
-      keys_in, n, 0, kPasses, c.rows, c.tickets, zero_words);
-  scan_kernel<<<kPasses, kRadix, 0, stream>>>(c.rows, c.hist_blocks, kPasses,
-                                              c.offsets);
+  const Control c = control_layout(control, kPasses);
+  clear_control(control, n, kPasses, stream);
+  histogram_kernel<K, kPasses><<<hist_blocks_for(n), kThreads, 0, stream>>>(
+      keys_in, n, 0, c.rows);
+  scan_kernel<<<kPasses, kRadix, 0, stream>>>(c.rows, kPasses, c.offsets);
   u32 *tickets = c.tickets;
   u64 *status = c.status;
   const u32 *offsets = c.offsets;
@@ -319,27 +336,29 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   return hipGetLastError();
 }
 
-// Stable partition of n (key, position) pairs by one 8-bit digit
-// (key >> shift) & 0xff: one histogram, one scan and one onesweep pass.
-// Results land in (keys_out, pos_out).  `control` holds
-// control_words(n, 1) words.
+// Stable partition of n keys by one 8-bit digit (key >> shift) & 0xff: one
+// histogram, one scan and one onesweep pass.  The partitioned keys land in
+// keys_out; pos_out (nullable) receives each one's input position and dest
+// (nullable) each input element's partitioned position, in input order.
+// `control` holds control_words(n, 1) words.
 template <class K>
 inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
                                   K *keys_out, u32 *pos_out, u32 *control,
-                                  hipStream_t stream, int block = 0) {
+                                  hipStream_t stream, int block = 0,
+                                  u32 *dest = nullptr) {
   if (n == 0) return hipSuccess;
   const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
-  const Control c = control_layout(control, n, 1);
-  const u64 zero_words = control_words(n, 1) - (c.tickets - control);
-  histogram_kernel<K><<<c.hist_blocks, kThreads, 0, stream>>>(
-      keys_in, n, shift / 8, 1, c.rows, c.tickets, zero_words);
-  scan_kernel<<<1, kRadix, 0, stream>>>(c.rows, c.hist_blocks, 1, c.offsets);
+  const Control c = control_layout(control, 1);
+  clear_control(control, n, 1, stream);
+  histogram_kernel<K, 1><<<hist_blocks_for(n), kThreads, 0, stream>>>(
+      keys_in, n, shift / 8, c.rows);
+  scan_kernel<<<1, kRadix, 0, stream>>>(c.rows, 1, c.offsets);
   const u32 *offsets = c.offsets;
   u32 *tickets = c.tickets;
   u64 *status = c.status;
   launch_pass<K, true>(shape, tiles, keys_in, nullptr, keys_out, pos_out, n,
-                       shift, offsets, status, tickets, stream);
+                       shift, offsets, status, tickets, stream, dest);
   return hipGetLastError();
 }
 

@@ -27,8 +27,10 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -39,7 +41,8 @@
 namespace gdict {
 namespace serve {
 
-using lsm::Lsm;
+using lsm::KV;
+using lsm::LsmT;
 using lsm::RunSet;
 
 struct Piece {
@@ -61,50 +64,107 @@ __device__ __forceinline__ u32 piece_of(const Piece *pieces, u32 np, u64 i) {
   return lo;
 }
 
-static __global__ void gather_kernel(const Piece *pieces, u32 np, u64 total,
-                                     u32 *out_a, u32 *out_b, u8 *out_flag) {
+template <class K, class V>
+__global__ void gather_kernel(const Piece *pieces, u32 np, u64 total,
+                              K *out_a, V *out_b, u8 *out_flag) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += stride) {
     const Piece &p = pieces[piece_of(pieces, np, i)];
     const u64 j = i - p.offset;
-    out_a[i] = static_cast<const u32 *>(p.a)[j];
-    if (out_b) out_b[i] = p.b ? static_cast<const u32 *>(p.b)[j] : 0u;
+    out_a[i] = static_cast<const K *>(p.a)[j];
+    if (out_b) out_b[i] = p.b ? static_cast<const V *>(p.b)[j] : V(0);
     if (out_flag) out_flag[i] = static_cast<u8>(p.flag);
   }
 }
 
-static __global__ void scatter_kernel(const Piece *pieces, u32 np, u64 total,
-                                      const u32 *src_a, const u8 *src_b) {
+// A lookup or range request inside a step's coalesced query batch.
+struct QueryPiece {
+  const void *a;   // queries / lo (keys)
+  const void *b;   // hi (ranges)
+  void *out;       // values / counts / sums (value width)
+  u8 *found;       // lookups: nullable found flags
+  u64 offset;      // position of the request in the batch
+  u32 count_only;  // ranges: count (1) or sum (0)
+  u32 pad;
+};
+
+__device__ __forceinline__ u32 qpiece_of(const QueryPiece *pieces, u32 np,
+                                         u64 i) {
+  u32 lo = 0, hi = np;  // last piece with offset <= i
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (pieces[mid].offset <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// IO policies of the query kernels: read each query from its request and
+// write its result into the request's own output (and found) buffer, with
+// the miss convention of a plain lookup (GPULSMOpt.cuh:4714-4806).
+template <class T>
+struct PieceLookup {
+  using K = typename T::K;
+  using V = typename T::V;
+  const QueryPiece *pieces;
+  u32 np;
+  struct Slot {
+    V *out;
+    u8 *found;
+    u64 j;
+  };
+  __device__ __forceinline__ K load(u64 i, Slot &s) const {
+    const QueryPiece &p = pieces[qpiece_of(pieces, np, i)];
+    s.out = static_cast<V *>(p.out);
+    s.found = p.found;
+    s.j = i - p.offset;
+    return static_cast<const K *>(p.a)[s.j];
+  }
+  __device__ __forceinline__ void put(const Slot &s, bool live,
+                                      V value) const {
+    if (s.found) {
+      s.out[s.j] = live ? value : V(0);
+      s.found[s.j] = live ? 1 : 0;
+    } else {
+      s.out[s.j] = live ? value : T::kAllOnes;
+    }
+  }
+};
+
+template <class T>
+struct PieceRange {
+  using K = typename T::K;
+  using V = typename T::V;
+  const QueryPiece *pieces;
+  u32 np;
+  struct Slot {
+    V *out;
+    u64 j;
+  };
+  __device__ __forceinline__ void load(u64 q, Slot &s, K &a, K &z,
+                                       bool &count) const {
+    const QueryPiece &p = pieces[qpiece_of(pieces, np, q)];
+    s.out = static_cast<V *>(p.out);
+    s.j = q - p.offset;
+    a = static_cast<const K *>(p.a)[s.j];
+    z = static_cast<const K *>(p.b)[s.j];
+    count = p.count_only != 0;
+  }
+  __device__ __forceinline__ void put(const Slot &s, V v) const {
+    s.out[s.j] = v;
+  }
+};
+
+// Queries of a step's lookups as one contiguous array (grouped path).
+template <class K>
+__global__ void query_gather_kernel(const QueryPiece *pieces, u32 np,
+                                    u64 total, K *out) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += stride) {
-    const Piece &p = pieces[piece_of(pieces, np, i)];
-    const u64 j = i - p.offset;
-    static_cast<u32 *>(const_cast<void *>(p.a))[j] = src_a[i];
-    if (p.b && src_b) static_cast<u8 *>(const_cast<void *>(p.b))[j] = src_b[i];
-  }
-}
-
-// A mixed range step computes counts and sums; keep the one each query
-// asked for.
-static __global__ void select_kernel(const u8 *is_count, u32 *out,
-                                     const u32 *sums, u64 n) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    if (!is_count[i]) out[i] = sums[i];
-}
-
-// Lookup results of requests without a found array get the reference's
-// miss sentinel 0xFFFFFFFF (GPULSMOpt.cuh:4729-4744); with one, 0 + found 0.
-static __global__ void lookup_fix_kernel(const Piece *pieces, u32 np, u64 n,
-                                         const u32 *outv, const u8 *outf,
-                                         u32 *fixed) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += stride) {
-    const Piece &p = pieces[piece_of(pieces, np, i)];
-    fixed[i] = (outf[i] || p.b) ? outv[i] : 0xffffffffu;
+    const QueryPiece &p = pieces[qpiece_of(pieces, np, i)];
+    out[i] = static_cast<const K *>(p.a)[i - p.offset];
   }
 }
 
@@ -156,21 +216,40 @@ struct Request {
 struct Step {
   u64 id = 0;
   std::vector<std::shared_ptr<Request>> reqs;
-  hipEvent_t queries_done = nullptr;
+  hipEvent_t updates_done = nullptr;  // admissions of the step (us_)
+  hipEvent_t queries_done = nullptr;  // its queries (qs_)
+  bool updates_stamped = false;
   std::vector<void *> scratch;  // engine-pool buffers released at completion
-  Piece *host_pieces = nullptr;  // pinned piece tables
+  char *host_pieces = nullptr;  // pinned piece tables
   u64 staged_bytes = 0;
   gd_status update_status = GD_OK;
 };
 
-class Server : public HandleBase {
+// Type-erased engine behind the C ABI (32- or 64-bit keys and values).
+class ServerBase {
  public:
+  virtual ~ServerBase() = default;
+  virtual u64 submit(gd_op op, const void *a, const void *b, u64 count,
+                     void *out, u8 *out_found) = 0;
+  virtual gd_status wait(u64 ticket, u64 *step_out) = 0;
+  virtual void drain() = 0;
+  virtual void stats(gd_serve_stats *out) = 0;
+  virtual void reset_stats() = 0;
+};
+
+template <class T>
+class Server : public ServerBase {
+ public:
+  using K = typename T::K;
+  using V = typename T::V;
+
   explicit Server(const gd_serve_config &c)
       : cfg_(c), lsm_(c.lsm) {
-    lsm_.set_deferred_reclaim(true);
     if (c.tuning) lsm_.set_tuning(*c.tuning);
     GD_HIP(hipStreamCreateWithFlags(&us_, hipStreamNonBlocking));
     GD_HIP(hipStreamCreateWithFlags(&qs_, hipStreamNonBlocking));
+    GD_HIP(hipStreamCreateWithFlags(&ms_, hipStreamNonBlocking));
+    lsm_.set_async(ms_, qs_);
     max_inflight_ = c.max_inflight_steps ? c.max_inflight_steps : 2;
     max_step_updates_ = c.max_step_keys ? c.max_step_keys : (u64(1) << 22);
     max_step_queries_ = c.max_step_keys ? c.max_step_keys : (u64(1) << 24);
@@ -178,7 +257,7 @@ class Server : public HandleBase {
     worker_ = std::thread([this] { run(); });
   }
 
-  ~Server() {
+  ~Server() override {
     {
       std::lock_guard<std::mutex> lk(mu_);
       stop_ = true;
@@ -187,17 +266,20 @@ class Server : public HandleBase {
     if (worker_.joinable()) worker_.join();
     try {
       GD_HIP(hipStreamSynchronize(us_));
+      GD_HIP(hipStreamSynchronize(ms_));
       GD_HIP(hipStreamSynchronize(qs_));
+      lsm_.detach_streams();
     } catch (...) {
     }
     for (auto &st : inflight_) release_step(st);
     inflight_.clear();
     (void)hipStreamDestroy(us_);
+    (void)hipStreamDestroy(ms_);
     (void)hipStreamDestroy(qs_);
   }
 
   u64 submit(gd_op op, const void *a, const void *b, u64 count, void *out,
-             u8 *out_found) {
+             u8 *out_found) override {
     if (op > GD_OP_RANGE_SUM)
       throw Error(GD_INVALID_ARGUMENT, "unknown request operation");
     if (count && !a) throw Error(GD_INVALID_ARGUMENT, "null request keys");
@@ -231,7 +313,7 @@ class Server : public HandleBase {
   }
 
   // Waits for one request; returns its status.
-  gd_status wait(u64 ticket, u64 *step_out) {
+  gd_status wait(u64 ticket, u64 *step_out) override {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = table_.find(ticket);
     if (it == table_.end())
@@ -244,7 +326,7 @@ class Server : public HandleBase {
     return r->status;
   }
 
-  void drain() {
+  void drain() override {
     std::unique_lock<std::mutex> lk(mu_);
     done_cv_.wait(lk, [&] {
       return (queue_.empty() && busy_ == 0) || worker_error_ != GD_OK;
@@ -252,7 +334,7 @@ class Server : public HandleBase {
     if (worker_error_ != GD_OK) throw Error(worker_error_, worker_msg_);
   }
 
-  void stats(gd_serve_stats *out) {
+  void stats(gd_serve_stats *out) override {
     std::lock_guard<std::mutex> lk(mu_);
     out->requests = completed_;
     out->steps = steps_done_;
@@ -274,7 +356,7 @@ class Server : public HandleBase {
     out->resident_bytes = resident_bytes_.load();
   }
 
-  void reset_stats() {
+  void reset_stats() override {
     std::lock_guard<std::mutex> lk(mu_);
     latencies_.clear();
     completed_ = 0;
@@ -287,14 +369,18 @@ class Server : public HandleBase {
  private:
   static u64 request_bytes(gd_op op, u64 count) {
     // staging of the coalesced batch: keys + values + flags, or queries +
-    // results (+ found), plus the grouped-lookup scratch (2 x 4 B)
+    // results (+ found), plus the grouped-lookup scratch (queries, grouped
+    // keys, destinations, grouped results and found flags)
+    constexpr u64 kb = sizeof(K), vb = sizeof(V);
     switch (op) {
       case GD_OP_INSERT:
-      case GD_OP_ERASE: return count * 9 + 4 * count * 8;  // + sort scratch
-      case GD_OP_LOOKUP: return count * (4 + 4 + 1 + 8);
-      default: return count * (4 + 4 + 4);
+      case GD_OP_ERASE:  // + sort scratch (two key and two position arrays)
+        return count * (kb + vb + 1) + count * (2 * kb + 8) + count * 8;
+      case GD_OP_LOOKUP: return count * (kb + vb + 1 + 2 * kb + 4 + vb + 1);
+      default: return count * (2 * kb + vb);
     }
   }
+
 
   void run() {
     while (true) {
@@ -346,9 +432,9 @@ class Server : public HandleBase {
     done_cv_.notify_all();
   }
 
-  template <class T>
-  T *scratch(Step &st, u64 n) {
-    T *p = pool_.get_n<T>(std::max<u64>(n, 1));
+  template <class E>
+  E *scratch(Step &st, u64 n) {
+    E *p = pool_.get_n<E>(std::max<u64>(n, 1));
     st.scratch.push_back(p);
     return p;
   }
@@ -357,7 +443,8 @@ class Server : public HandleBase {
     Step st;
     st.id = ++step_counter_;
     st.staged_bytes = staged;
-    std::vector<Piece> upd, look, rng;
+    std::vector<Piece> upd;
+    std::vector<QueryPiece> look, rng;
     u64 n_upd = 0, n_look = 0, n_rng = 0;
     bool any_erase = false, any_insert = false;
     for (auto &r : reqs) {
@@ -369,59 +456,47 @@ class Server : public HandleBase {
         n_upd += r->count;
         (r->op == GD_OP_ERASE ? any_erase : any_insert) = true;
       } else if (r->op == GD_OP_LOOKUP) {
-        look.push_back({r->a, nullptr, r->count, n_look, 0, 0});
+        look.push_back({r->a, nullptr, r->out, r->out_found, n_look, 0, 0});
         n_look += r->count;
       } else {
-        rng.push_back({r->a, r->b, r->count, n_rng,
+        rng.push_back({r->a, r->b, r->out, nullptr, n_rng,
                        r->op == GD_OP_RANGE_COUNT ? 1u : 0u, 0});
         n_rng += r->count;
       }
     }
-    // one pinned piece table per step: [upd | look | look-out | rng | rng-out]
-    const std::size_t total_pieces =
-        upd.size() + 2 * look.size() + 2 * rng.size();
-    if (total_pieces)
-      st.host_pieces =
-          static_cast<Piece *>(pinned_.get(total_pieces * sizeof(Piece)));
-    Piece *dev_pieces =
-        total_pieces ? scratch<Piece>(st, total_pieces) : nullptr;
-    std::size_t at = 0;
-    auto put = [&](const std::vector<Piece> &v) {
-      std::copy(v.begin(), v.end(), st.host_pieces + at);
-      const std::size_t first = at;
-      at += v.size();
-      return first;
-    };
-    const std::size_t upd_at = put(upd);
-    const std::size_t look_at = put(look);
-    std::vector<Piece> look_out, rng_out;
-    {
-      std::size_t k = 0;
-      for (auto &r : reqs)
-        if (r->count && r->op == GD_OP_LOOKUP)
-          look_out.push_back({r->out, r->out_found, r->count,
-                              look[k++].offset, 0, 0});
-      k = 0;
-      for (auto &r : reqs)
-        if (r->count && r->op >= GD_OP_RANGE_COUNT)
-          rng_out.push_back({r->out, nullptr, r->count, rng[k++].offset, 0, 0});
-    }
-    const std::size_t look_out_at = put(look_out);
-    const std::size_t rng_at = put(rng);
-    const std::size_t rng_out_at = put(rng_out);
-    if (total_pieces)
-      GD_HIP(hipMemcpyAsync(dev_pieces, st.host_pieces,
-                            total_pieces * sizeof(Piece),
+    // one pinned table per step: [update pieces | lookups | ranges]
+    static_assert(sizeof(Piece) % 8 == 0 && sizeof(QueryPiece) % 8 == 0,
+                  "piece tables are packed back to back");
+    const std::size_t upd_bytes = upd.size() * sizeof(Piece);
+    const std::size_t look_bytes = look.size() * sizeof(QueryPiece);
+    const std::size_t table_bytes =
+        upd_bytes + look_bytes + rng.size() * sizeof(QueryPiece);
+    char *dev_table = nullptr;
+    if (table_bytes) {
+      st.host_pieces = static_cast<char *>(pinned_.get(table_bytes));
+      std::memcpy(st.host_pieces, upd.data(), upd_bytes);
+      std::memcpy(st.host_pieces + upd_bytes, look.data(), look_bytes);
+      std::memcpy(st.host_pieces + upd_bytes + look_bytes, rng.data(),
+                  rng.size() * sizeof(QueryPiece));
+      dev_table = scratch<char>(st, table_bytes);
+      GD_HIP(hipMemcpyAsync(dev_table, st.host_pieces, table_bytes,
                             hipMemcpyHostToDevice, us_));
+    }
+    const Piece *dev_upd = reinterpret_cast<const Piece *>(dev_table);
+    const QueryPiece *dev_look =
+        reinterpret_cast<const QueryPiece *>(dev_table + upd_bytes);
+    const QueryPiece *dev_rng = reinterpret_cast<const QueryPiece *>(
+        dev_table + upd_bytes + look_bytes);
 
-    // ---------------------------------------------------------- updates
+    // ------------------------------------------------------------ updates
+    // Admission only: the merges it triggers run on the merge stream and
+    // neither this step's queries nor later admissions wait for them.
     if (n_upd) {
-      u32 *keys = scratch<u32>(st, n_upd);
-      u32 *vals = scratch<u32>(st, n_upd);
+      K *keys = scratch<K>(st, n_upd);
+      V *vals = scratch<V>(st, n_upd);
       u8 *flags = scratch<u8>(st, n_upd);
-      gather_kernel<<<grid_for(n_upd, 256, 1u << 14), 256, 0, us_>>>(
-          dev_pieces + upd_at, static_cast<u32>(upd.size()), n_upd, keys,
-          vals, flags);
+      gather_kernel<K, V><<<grid_for(n_upd, 256, 1u << 14), 256, 0, us_>>>(
+          dev_upd, static_cast<u32>(upd.size()), n_upd, keys, vals, flags);
       GD_LAUNCH_CHECK();
       const int mode = any_erase ? (any_insert ? lsm::kModeMixed
                                                : lsm::kModeErase)
@@ -432,82 +507,54 @@ class Server : public HandleBase {
         if (e.status == GD_HIP_ERROR) throw;
         st.update_status = e.status;  // reported on this step's updates
       }
-      // runs released by these updates may still be read by queries
-      // already enqueued on the query stream
-      lsm_.retire_pending(qs_);
     }
-    hipEvent_t updates_done;
-    GD_HIP(hipEventCreateWithFlags(&updates_done, hipEventDisableTiming));
-    GD_HIP(hipEventRecord(updates_done, us_));
-    GD_HIP(hipStreamWaitEvent(qs_, updates_done, 0));
-    (void)hipEventDestroy(updates_done);
+    GD_HIP(hipEventCreateWithFlags(&st.updates_done, hipEventDisableTiming));
+    GD_HIP(hipEventRecord(st.updates_done, us_));
+    GD_HIP(hipStreamWaitEvent(qs_, st.updates_done, 0));
 
-    // ---------------------------------------------------------- queries
-    const RunSet snap = lsm_.snapshot();
+    // ------------------------------------------------------------ queries
+    // the runs after this step's admissions; merges still in flight read
+    // as their inputs
+    const RunSet<T> snap = lsm_.snapshot_for(qs_);
     if (n_look) {
-      u32 *q = scratch<u32>(st, n_look);
-      u32 *outv = scratch<u32>(st, n_look);
-      u8 *outf = scratch<u8>(st, n_look);
-      gather_kernel<<<grid_for(n_look, 256, 1u << 14), 256, 0, qs_>>>(
-          dev_pieces + look_at, static_cast<u32>(look.size()), n_look, q,
-          nullptr, nullptr);
-      GD_LAUNCH_CHECK();
-      if (n_look >= lsm_.grouped_lookup_min() && !lsm_.empty()) {
-        u32 *qk = scratch<u32>(st, n_look);
-        u32 *qp = scratch<u32>(st, n_look);
+      const PieceLookup<T> io{dev_look, static_cast<u32>(look.size())};
+      if (n_look >= lsm_.grouped_lookup_min() && snap.count) {
+        K *q = scratch<K>(st, n_look);
+        K *qk = scratch<K>(st, n_look);
+        u32 *dest = scratch<u32>(st, n_look);
+        V *rv = scratch<V>(st, n_look);
+        u8 *rf = scratch<u8>(st, n_look);
         u32 *ctrl = scratch<u32>(st, rs::control_words(n_look, 1));
-        GD_HIP(rs::partition_pairs<u32>(q, static_cast<u32>(n_look), 24, qk,
-                                        qp, ctrl, qs_,
-                                        lsm_.tuning().sort_block));
-        const u32 blocks =
-            (std::min<u32>(grid_for(n_look, 1024, 1u << 16), 1u << 16) + 7u) &
-            ~7u;
-        lsm::lookup_grouped_kernel<<<blocks, 256, 0, qs_>>>(
-            qk, qp, n_look, outv, outf, snap);
+        query_gather_kernel<K><<<grid_for(n_look, 256, 1u << 14), 256, 0,
+                                  qs_>>>(
+            dev_look, static_cast<u32>(look.size()), n_look, q);
+        GD_HIP(rs::partition_pairs<K>(q, static_cast<u32>(n_look),
+                                      T::kBits - 8, qk,
+                                        nullptr, ctrl, qs_,
+                                        lsm_.tuning().sort_block, dest));
+        lsm::lookup_grouped_kernel<T><<<lsm::grouped_lookup_blocks(n_look), 256,
+                                     0, qs_>>>(qk, n_look, rv, rf, snap);
+        GD_LAUNCH_CHECK();
+        lsm::lookup_gather_kernel<T><<<grid_for(n_look, 256, 1u << 16), 256, 0,
+                                    qs_>>>(io, dest, n_look, rv, rf);
       } else {
-        lsm::lookup_kernel<<<grid_for(n_look, 256, 1u << 16), 256, 0, qs_>>>(
-            q, n_look, outv, outf, snap);
+        lsm::lookup_kernel<T><<<grid_for(n_look, 256, 1u << 16), 256, 0, qs_>>>(
+            io, n_look, snap);
       }
       GD_LAUNCH_CHECK();
-      // requests without a found array get the reference's miss sentinel
-      scatter_lookup(st, dev_pieces + look_out_at,
-                     static_cast<u32>(look_out.size()), n_look, outv, outf);
     }
     if (n_rng) {
-      u32 *lo = scratch<u32>(st, n_rng);
-      u32 *hi = scratch<u32>(st, n_rng);
-      u8 *is_count = scratch<u8>(st, n_rng);
-      u32 *out = scratch<u32>(st, n_rng);
-      gather_kernel<<<grid_for(n_rng, 256, 1u << 14), 256, 0, qs_>>>(
-          dev_pieces + rng_at, static_cast<u32>(rng.size()), n_rng, lo, hi,
-          is_count);
-      GD_LAUNCH_CHECK();
-      // counts and sums share the kernel: a count query sums 1 per row
-      bool all_count = true, all_sum = true;
-      for (auto &p : rng) (p.flag ? all_sum : all_count) = false;
-      if (all_count || all_sum) {
-        lsm::range_kernel<<<grid_for(n_rng, lsm::kRangeWaves, 1u << 16),
-                            lsm::kRangeThreads, 0, qs_>>>(
-            lo, hi, n_rng, out, all_count ? 1 : 0, snap);
-      } else {
-        u32 *out2 = scratch<u32>(st, n_rng);
-        lsm::range_kernel<<<grid_for(n_rng, lsm::kRangeWaves, 1u << 16),
-                            lsm::kRangeThreads, 0, qs_>>>(lo, hi, n_rng, out,
-                                                          1, snap);
-        lsm::range_kernel<<<grid_for(n_rng, lsm::kRangeWaves, 1u << 16),
-                            lsm::kRangeThreads, 0, qs_>>>(lo, hi, n_rng, out2,
-                                                          0, snap);
-        select_kernel<<<grid_for(n_rng, 256, 1u << 14), 256, 0, qs_>>>(
-            is_count, out, out2, n_rng);
-      }
-      GD_LAUNCH_CHECK();
-      scatter_kernel<<<grid_for(n_rng, 256, 1u << 14), 256, 0, qs_>>>(
-          dev_pieces + rng_out_at, static_cast<u32>(rng_out.size()), n_rng,
-          out, nullptr);
+      // counts and sums share the kernel: each request chooses its own
+      lsm::range_kernel<T><<<grid_for(n_rng, lsm::kRangeWaves, 1u << 16),
+                             lsm::kRangeThreads, 0, qs_>>>(
+          PieceRange<T>{dev_rng, static_cast<u32>(rng.size())}, n_rng, snap);
       GD_LAUNCH_CHECK();
     }
     GD_HIP(hipEventCreateWithFlags(&st.queries_done, hipEventDisableTiming));
     GD_HIP(hipEventRecord(st.queries_done, qs_));
+    // runs released by this step's merges stay until the queries that may
+    // read them (this step's and earlier) are done
+    lsm_.retire_pending();
     st.reqs = std::move(reqs);
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -519,19 +566,34 @@ class Server : public HandleBase {
     inflight_.push_back(std::move(st));
   }
 
-  void scatter_lookup(Step &st, const Piece *pieces, u32 np, u64 n,
-                      const u32 *outv, const u8 *outf) {
-    // miss value: 0 with a found array, 0xFFFFFFFF without
-    // (GPULSMOpt.cuh:4729-4744); outv already holds 0 for misses
-    u32 *fixed = scratch<u32>(st, n);
-    lookup_fix_kernel<<<grid_for(n, 256, 1u << 14), 256, 0, qs_>>>(
-        pieces, np, n, outv, outf, fixed);
-    scatter_kernel<<<grid_for(n, 256, 1u << 14), 256, 0, qs_>>>(
-        pieces, np, n, fixed, outf);
-    GD_LAUNCH_CHECK();
+  // Stamps the requests of `st` that `updates` selects as complete.
+  void finish_requests(Step &st, bool updates, Clock::time_point now) {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &r : st.reqs) {
+      if ((r->op <= GD_OP_ERASE) != updates) continue;
+      r->completed = now;
+      r->status = updates ? st.update_status : GD_OK;
+      r->done = true;
+      latencies_.push_back(
+          std::chrono::duration<double, std::micro>(now - r->submitted)
+              .count());
+      ++completed_;
+      --busy_;
+    }
   }
 
   void complete_steps() {
+    // updates complete once admitted (later queries see them from then on,
+    // whether or not the merges they triggered have finished)
+    for (auto &st : inflight_) {
+      if (st.updates_stamped) continue;
+      const hipError_t q = hipEventQuery(st.updates_done);
+      if (q == hipErrorNotReady) break;
+      GD_HIP(q);
+      finish_requests(st, true, Clock::now());
+      st.updates_stamped = true;
+    }
+    bool any = false;
     while (!inflight_.empty()) {
       Step &st = inflight_.front();
       const bool idle = [&] {
@@ -547,24 +609,22 @@ class Server : public HandleBase {
         GD_HIP(hipEventSynchronize(st.queries_done));
       }
       const auto now = Clock::now();
+      if (!st.updates_stamped) {
+        finish_requests(st, true, now);
+        st.updates_stamped = true;
+      }
+      finish_requests(st, false, now);
       {
         std::lock_guard<std::mutex> lk(mu_);
-        for (auto &r : st.reqs) {
-          r->completed = now;
-          r->status = r->op <= GD_OP_ERASE ? st.update_status : GD_OK;
-          r->done = true;
-          latencies_.push_back(
-              std::chrono::duration<double, std::micro>(now - r->submitted)
-                  .count());
-          ++completed_;
-        }
-        busy_ -= st.reqs.size();
         in_flight_bytes_ -= st.staged_bytes;
         ++steps_done_;
       }
-      done_cv_.notify_all();
+      any = true;
       release_step(st);
       inflight_.pop_front();
+    }
+    done_cv_.notify_all();
+    if (any) {
       lsm_.reclaim();
       resident_bytes_.store(lsm_.resident_bytes() + pool_.held_bytes());
     }
@@ -576,15 +636,17 @@ class Server : public HandleBase {
     pinned_.put(st.host_pieces);
     st.host_pieces = nullptr;
     if (st.queries_done) (void)hipEventDestroy(st.queries_done);
+    if (st.updates_done) (void)hipEventDestroy(st.updates_done);
     st.queries_done = nullptr;
+    st.updates_done = nullptr;
   }
 
   gd_serve_config cfg_;
-  Lsm lsm_;
+  LsmT<T> lsm_;
   DevicePool pool_;  // step staging, released when the step completes
   PinnedCache pinned_;
   std::atomic<u64> resident_bytes_{0};
-  hipStream_t us_ = nullptr, qs_ = nullptr;
+  hipStream_t us_ = nullptr, ms_ = nullptr, qs_ = nullptr;
   u32 max_inflight_ = 2;
   u64 max_step_updates_ = 0, max_step_queries_ = 0;
   u64 budget_ = 0;
@@ -610,10 +672,9 @@ class Server : public HandleBase {
 }  // namespace gdict
 
 using gdict::guarded;
-using gdict::serve::Server;
 
-struct gd_server : Server {
-  using Server::Server;
+struct gd_server : gdict::HandleBase {
+  std::unique_ptr<gdict::serve::ServerBase> impl;
 };
 
 extern "C" {
@@ -626,7 +687,17 @@ gd_status gd_serve_create(const gd_serve_config *config, gd_server **out) {
   *out = nullptr;
   return guarded(nullptr, [&] {
     (void)gdict::lsm::Lsm::capacity_for(config->lsm);
-    *out = new gd_server(*config);
+    auto h = std::make_unique<gd_server>();
+    if (config->key_bytes == 0 || config->key_bytes == 4) {
+      h->impl = std::make_unique<
+          gdict::serve::Server<gdict::lsm::KV<gdict::u32>>>(*config);
+    } else if (config->key_bytes == 8) {
+      h->impl = std::make_unique<
+          gdict::serve::Server<gdict::lsm::KV<gdict::u64>>>(*config);
+    } else {
+      throw gdict::Error(GD_INVALID_ARGUMENT, "key_bytes must be 4 or 8");
+    }
+    *out = h.release();
   });
 }
 
@@ -640,14 +711,15 @@ gd_status gd_serve_submit(gd_server *h, gd_op op, const void *a,
                           uint8_t *out_found, uint64_t *out_ticket) {
   if (!h || !out_ticket) return GD_INVALID_ARGUMENT;
   return guarded(h, [&] {
-    *out_ticket = h->submit(op, a, b, count, out, out_found);
+    *out_ticket = h->impl->submit(op, a, b, count, out, out_found);
   });
 }
 
 gd_status gd_serve_wait(gd_server *h, uint64_t ticket, uint64_t *out_step) {
   if (!h) return GD_INVALID_ARGUMENT;
   gd_status result = GD_OK;
-  gd_status st = guarded(h, [&] { result = h->wait(ticket, out_step); });
+  gd_status st =
+      guarded(h, [&] { result = h->impl->wait(ticket, out_step); });
   if (st != GD_OK) return st;
   if (result != GD_OK)
     h->last_error = "the request's update step failed (status " +
@@ -657,14 +729,14 @@ gd_status gd_serve_wait(gd_server *h, uint64_t ticket, uint64_t *out_step) {
 
 gd_status gd_serve_drain(gd_server *h) {
   if (!h) return GD_INVALID_ARGUMENT;
-  return guarded(h, [&] { h->drain(); });
+  return guarded(h, [&] { h->impl->drain(); });
 }
 
 gd_status gd_serve_get_stats(gd_server *h, gd_serve_stats *out, int reset) {
   if (!h || !out) return GD_INVALID_ARGUMENT;
   return guarded(h, [&] {
-    h->stats(out);
-    if (reset) h->reset_stats();
+    h->impl->stats(out);
+    if (reset) h->impl->reset_stats();
   });
 }
 

@@ -242,18 +242,40 @@ class FlixWorkload:
         self.seed, self.n_build, self.batch = seed, n_build, batch
         self.lookups, self.ranges = lookups, ranges
 
+    # keys live in [2, 2^64 - 4]: 0 and 1 are reserved by FliX and the top
+    # values are its sentinels (definitions.cuh:86-103)
+    KEY_LO = 2
+    KEY_SPAN = (1 << 64) - 4 - 2
+
     def build_keys(self):
         runs = max(1, self.n_build >> 12)
         st = stream(sub_seed(self.seed, 20), runs)
-        starts = (st >> np.uint64(2)) + np.uint64(2)  # < 2^62 + 2
+        # run starts uniform over the key range (a run of 4096 must fit);
+        # overlapping runs are re-drawn from the next counter values
+        span = np.uint64(self.KEY_SPAN - 4096)
+        starts = st % span + np.uint64(self.KEY_LO)
+        extra = 0
+        while True:
+            order = np.sort(starts)
+            clash = np.nonzero(np.diff(order) < np.uint64(4096))[0]
+            if len(clash) == 0:
+                break
+            bad = np.isin(starts, order[clash + 1])
+            fresh = stream(sub_seed(self.seed, 23), int(bad.sum()), extra)
+            extra += int(bad.sum())
+            starts[bad] = fresh % span + np.uint64(self.KEY_LO)
         keys = (starts[:, None] + np.arange(4096, dtype=np.uint64)).ravel()
-        return keys[: self.n_build]
+        return np.sort(keys[: self.n_build])
 
     def batch_ops(self, t, built):
+        """75% inserts of uniform keys over the key range (a uniform 64-bit
+        key hits one of the built keys with probability < 2^-36, and an
+        insert of a present key is a no-op in FliX and in the oracle alike),
+        25% deletes of built keys."""
         n = self.batch
         x = stream(sub_seed(self.seed, 21), 2 * n, t * 2 * n)
         n_ins = (3 * n) // 4
-        ins = (x[:n_ins] >> np.uint64(1)) | np.uint64(1) << np.uint64(62)
+        ins = x[:n_ins] % np.uint64(self.KEY_SPAN) + np.uint64(self.KEY_LO)
         pick = (x[n_ins:n] % np.uint64(len(built))).astype(np.int64)
         dels = built[pick]
         vals = splitmix64(ins)
@@ -266,7 +288,8 @@ class FlixWorkload:
         r = x[self.lookups:]
         lo = built[(r[: self.ranges] % np.uint64(len(built))).astype(np.int64)]
         w = log_uniform(r[self.ranges:], 16, 1024)
-        hi = lo + w - np.uint64(1)
+        room = np.uint64(self.KEY_LO + self.KEY_SPAN) - lo  # no wrap-around
+        hi = lo + np.minimum(w - np.uint64(1), room)
         return q, lo, hi
 
 

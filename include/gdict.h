@@ -174,6 +174,56 @@ gd_status gd_lsm_read_profile(gd_lsm *h, gd_lsm_profile *out, int reset);
  * Synchronous. */
 gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub);
 
+/* ---------------------------------------------------------- LSM, 64-bit */
+
+/* No reference counterpart: the reference LSM is uint32-only
+ * (FliX/impl_gpulsmopt.cuh:145-148, GPULSMOpt.cuh's 16/16-bit row split).
+ * The same dictionary over uint64 keys and uint64 values, with the gd_lsm_*
+ * semantics: newest update wins, tombstones mask older runs, a lookup miss
+ * writes all-ones (value 0 + found 0 with out_found), range sums wrap
+ * modulo 2^64, range counts are uint64, successor returns all-ones when no
+ * live key is >= the query.  Capacity and tiling are those of gd_config
+ * (at most 2^32 - 1 rows). */
+typedef struct gd_lsm64 gd_lsm64;
+
+gd_status gd_lsm64_create(const gd_config *config, gd_lsm64 **out);
+gd_status gd_lsm64_destroy(gd_lsm64 *h);
+gd_status gd_lsm64_clear(gd_lsm64 *h, hipStream_t stream);
+gd_status gd_lsm64_bulk_build(gd_lsm64 *h, const uint64_t *keys,
+                              const uint64_t *values, uint64_t count,
+                              hipStream_t stream);
+gd_status gd_lsm64_insert(gd_lsm64 *h, const uint64_t *keys,
+                          const uint64_t *values, uint64_t count,
+                          hipStream_t stream);
+gd_status gd_lsm64_erase(gd_lsm64 *h, const uint64_t *keys, uint64_t count,
+                         hipStream_t stream);
+gd_status gd_lsm64_apply_mixed(gd_lsm64 *h, const uint64_t *keys,
+                               const uint64_t *values,
+                               const uint8_t *is_erase, uint64_t count,
+                               hipStream_t stream);
+gd_status gd_lsm64_lookup(gd_lsm64 *h, const uint64_t *queries,
+                          uint64_t count, uint64_t *out_values,
+                          uint8_t *out_found, hipStream_t stream);
+gd_status gd_lsm64_range_sum(gd_lsm64 *h, const uint64_t *lo,
+                             const uint64_t *hi, uint64_t count,
+                             uint64_t *out_sums, hipStream_t stream);
+gd_status gd_lsm64_range_count(gd_lsm64 *h, const uint64_t *lo,
+                               const uint64_t *hi, uint64_t count,
+                               uint64_t *out_counts, hipStream_t stream);
+gd_status gd_lsm64_successor(gd_lsm64 *h, const uint64_t *queries,
+                             uint64_t count, uint64_t *out_keys,
+                             hipStream_t stream);
+gd_status gd_lsm64_dump_sorted(gd_lsm64 *h, uint64_t *out_keys,
+                               uint64_t *out_values, uint64_t capacity,
+                               uint64_t *out_count, hipStream_t stream);
+gd_status gd_lsm64_compact(gd_lsm64 *h, hipStream_t stream);
+gd_status gd_lsm64_carry_status(gd_lsm64 *h, uint32_t *out_status);
+gd_status gd_lsm64_resident_bytes(gd_lsm64 *h, uint64_t *out_bytes);
+gd_status gd_lsm64_set_tuning(gd_lsm64 *h, const gd_lsm_tuning *tuning);
+gd_status gd_lsm64_get_tuning(gd_lsm64 *h, gd_lsm_tuning *out);
+gd_status gd_lsm64_shape(gd_lsm64 *h, uint32_t *out_runs,
+                         uint64_t *out_rows_ub);
+
 /* -------------------------------------------------------------- serving */
 
 /* No reference counterpart: a continuous-batching engine over one LSM
@@ -201,7 +251,8 @@ typedef struct gd_serve_config {
   uint64_t hbm_budget_bytes;   /* dictionary + in-flight staging cap; 0 = none */
   uint64_t max_step_keys;      /* keys coalesced per step; 0 = 2^22 / 2^24 */
   uint32_t max_inflight_steps; /* pipelined steps; 0 = 2                   */
-  uint32_t reserved;
+  uint32_t key_bytes;          /* 4 (0) or 8: uint32 or uint64 keys and
+                                  values (the gd_lsm / gd_lsm64 dictionary) */
   const gd_lsm_tuning *tuning; /* nullable                                 */
 } gd_serve_config;
 

@@ -47,6 +47,20 @@ _SIGS = {
     "orc_lsm_status": ([_P], _U32),
     "orc_lsm_capacity": ([_P], _U64),
     "orc_lsm_message": ([_P], ctypes.c_char_p),
+    "orc_lsm64_create": ([_U64, _U64], _P),
+    "orc_lsm64_destroy": ([_P], None),
+    "orc_lsm64_clear": ([_P], None),
+    "orc_lsm64_bulk_build": ([_P, _P, _P, _U64], ctypes.c_int),
+    "orc_lsm64_insert": ([_P, _P, _P, _U64], ctypes.c_int),
+    "orc_lsm64_erase": ([_P, _P, _U64], ctypes.c_int),
+    "orc_lsm64_apply_mixed": ([_P, _P, _P, _P, _U64], ctypes.c_int),
+    "orc_lsm64_lookup": ([_P, _P, _U64, _P, _P], None),
+    "orc_lsm64_range_sum": ([_P, _P, _P, _U64, _P], None),
+    "orc_lsm64_range_count": ([_P, _P, _P, _U64, _P], None),
+    "orc_lsm64_successor": ([_P, _P, _U64, _P], None),
+    "orc_lsm64_size": ([_P], _U64),
+    "orc_lsm64_dump": ([_P, _P, _P], None),
+    "orc_lsm64_status": ([_P], _U32),
     "orc_flix_create": ([ctypes.c_int, ctypes.c_int], _P),
     "orc_flix_destroy": ([_P], None),
     "orc_flix_build": ([_P, _P, _U64], None),
@@ -135,84 +149,99 @@ def range_digest(results, query_offset=0):
 
 # ---------------------------------------------------------------- LSM oracle
 class LsmOracle:
-    """Replays GPULSMOpt's observable semantics (see oracle.cpp)."""
+    """Replays GPULSMOpt's observable semantics (see oracle.cpp).  key_bytes
+    = 8 selects the 64-bit extension (uint64 keys and values)."""
 
     def __init__(self, max_elements=0, batch_capacity=0,
-                 level_zero_capacity=0):
-        self._h = lib().orc_lsm_create(max_elements, batch_capacity,
-                                       level_zero_capacity)
+                 level_zero_capacity=0, key_bytes=4):
+        self.key_bytes = key_bytes
+        self.kt = np.uint32 if key_bytes == 4 else np.uint64
+        self._p = "orc_lsm_" if key_bytes == 4 else "orc_lsm64_"
+        if key_bytes == 4:
+            self._h = lib().orc_lsm_create(max_elements, batch_capacity,
+                                           level_zero_capacity)
+        else:
+            self._h = lib().orc_lsm64_create(max_elements, batch_capacity)
         if not self._h:
             raise ValueError("GPULSMOpt capacity exceeds 32-bit key space")
 
+    def _f(self, name):
+        return getattr(lib(), self._p + name)
+
+    def _k(self, a):
+        return np.ascontiguousarray(a, dtype=self.kt)
+
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_lsm_destroy(self._h)
+            self._f("destroy")(self._h)
             self._h = None
 
     def clear(self):
-        lib().orc_lsm_clear(self._h)
+        self._f("clear")(self._h)
 
     def bulk_build(self, keys, values):
-        k, v = u32(keys), u32(values)
-        return lib().orc_lsm_bulk_build(self._h, _a(k), _a(v), len(k))
+        k, v = self._k(keys), self._k(values)
+        return self._f("bulk_build")(self._h, _a(k), _a(v), len(k))
 
     def insert(self, keys, values):
-        k, v = u32(keys), u32(values)
-        return lib().orc_lsm_insert(self._h, _a(k), _a(v), len(k))
+        k, v = self._k(keys), self._k(values)
+        return self._f("insert")(self._h, _a(k), _a(v), len(k))
 
     def erase(self, keys):
-        k = u32(keys)
-        return lib().orc_lsm_erase(self._h, _a(k), len(k))
+        k = self._k(keys)
+        return self._f("erase")(self._h, _a(k), len(k))
 
     def apply_mixed(self, keys, values, is_erase):
-        k, v = u32(keys), u32(values)
+        k, v = self._k(keys), self._k(values)
         e = np.ascontiguousarray(is_erase, dtype=np.uint8)
-        return lib().orc_lsm_apply_mixed(self._h, _a(k), _a(v), _a(e), len(k))
+        return self._f("apply_mixed")(self._h, _a(k), _a(v), _a(e), len(k))
 
     def lookup(self, queries, with_found=False):
-        q = u32(queries)
-        out = np.empty(len(q), dtype=np.uint32)
+        q = self._k(queries)
+        out = np.empty(len(q), dtype=self.kt)
         found = np.empty(len(q), dtype=np.uint8) if with_found else None
-        lib().orc_lsm_lookup(self._h, _a(q), len(q), _a(out), _a(found))
+        self._f("lookup")(self._h, _a(q), len(q), _a(out), _a(found))
         return (out, found) if with_found else out
 
     def range_sum(self, lo, hi):
-        lo, hi = u32(lo), u32(hi)
-        out = np.empty(len(lo), dtype=np.uint32)
-        lib().orc_lsm_range_sum(self._h, _a(lo), _a(hi), len(lo), _a(out))
+        lo, hi = self._k(lo), self._k(hi)
+        out = np.empty(len(lo), dtype=self.kt)
+        self._f("range_sum")(self._h, _a(lo), _a(hi), len(lo), _a(out))
         return out
 
     def range_count(self, lo, hi):
-        lo, hi = u32(lo), u32(hi)
-        out = np.empty(len(lo), dtype=np.uint32)
-        lib().orc_lsm_range_count(self._h, _a(lo), _a(hi), len(lo), _a(out))
+        lo, hi = self._k(lo), self._k(hi)
+        out = np.empty(len(lo), dtype=self.kt)
+        self._f("range_count")(self._h, _a(lo), _a(hi), len(lo), _a(out))
         return out
 
     def successor(self, queries):
-        q = u32(queries)
-        out = np.empty(len(q), dtype=np.uint32)
-        lib().orc_lsm_successor(self._h, _a(q), len(q), _a(out))
+        q = self._k(queries)
+        out = np.empty(len(q), dtype=self.kt)
+        self._f("successor")(self._h, _a(q), len(q), _a(out))
         return out
 
     def size(self):
-        return int(lib().orc_lsm_size(self._h))
+        return int(self._f("size")(self._h))
 
     def dump(self):
         n = self.size()
-        k = np.empty(n, dtype=np.uint32)
-        v = np.empty(n, dtype=np.uint32)
+        k = np.empty(n, dtype=self.kt)
+        v = np.empty(n, dtype=self.kt)
         if n:
-            lib().orc_lsm_dump(self._h, _a(k), _a(v))
+            self._f("dump")(self._h, _a(k), _a(v))
         return k, v
 
     def status(self):
-        return int(lib().orc_lsm_status(self._h))
+        return int(self._f("status")(self._h))
 
     def capacity(self):
-        return int(lib().orc_lsm_capacity(self._h))
+        return int(lib().orc_lsm_capacity(self._h)) if self.key_bytes == 4 \
+            else None
 
     def message(self):
-        return lib().orc_lsm_message(self._h).decode()
+        return lib().orc_lsm_message(self._h).decode() \
+            if self.key_bytes == 4 else ""
 
 
 # --------------------------------------------------------------- FliX oracle

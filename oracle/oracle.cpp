@@ -183,7 +183,6 @@ struct Zipf {
 };
 
 // ----------------------------------------------------------------- LSM oracle
-constexpr std::uint32_t kInvalid = 0xffffffffu;
 constexpr std::uint32_t kBatchesPerEpoch = 16u;
 constexpr std::uint64_t kMaximumOperationTile = 1ull << 20;
 constexpr std::uint32_t kStatusOutputOverflow = 1u << 3;
@@ -194,24 +193,29 @@ enum OracleCode : int {
   ORC_PUBLICATION_FAILED = 4,
 };
 
+// K is the key and value type (uint32 for GPULSMOpt; uint64 for the 64-bit
+// extension, where sums wrap modulo 2^64 and the sentinel is all-ones).
+template <class K>
 struct Update {
-  std::uint32_t key;
-  std::uint32_t value;
+  K key;
+  K value;
   bool tombstone;
 };
 
-struct LsmOracle {
+template <class K>
+struct LsmOracleT {
+  using Entry = std::pair<K, K>;
   std::uint64_t batch_capacity = 1;
   std::uint64_t capacity = 0;
   // Live contents, sorted by key.
-  std::vector<std::pair<std::uint32_t, std::uint32_t>> live;
+  std::vector<Entry> live;
   std::uint32_t pending_batches = 0;
   bool failed = false;
   std::uint32_t status = 0;
   std::string message;
 
   // Apply one admitted batch: later positions override earlier ones.
-  void apply_tile(const Update *u, std::size_t n) {
+  void apply_tile(const Update<K> *u, std::size_t n) {
     std::vector<std::size_t> order(n);
     std::iota(order.begin(), order.end(), std::size_t{0});
     std::stable_sort(order.begin(), order.end(),
@@ -219,13 +223,13 @@ struct LsmOracle {
                        return u[a].key < u[b].key;
                      });
     // keep the last occurrence of every key (the newest age)
-    std::vector<Update> winners;
+    std::vector<Update<K>> winners;
     winners.reserve(n);
     for (std::size_t i = 0; i < n; ++i) {
       if (i + 1 < n && u[order[i]].key == u[order[i + 1]].key) continue;
       winners.push_back(u[order[i]]);
     }
-    std::vector<std::pair<std::uint32_t, std::uint32_t>> merged;
+    std::vector<Entry> merged;
     merged.reserve(live.size() + winners.size());
     std::size_t a = 0, b = 0;
     while (a < live.size() || b < winners.size()) {
@@ -242,7 +246,7 @@ struct LsmOracle {
     live.swap(merged);
   }
 
-  int admit(const Update *u, std::size_t n) {
+  int admit(const Update<K> *u, std::size_t n) {
     if (failed) {
       message =
           "GPULSMOpt publication failed with status " +
@@ -273,16 +277,130 @@ struct LsmOracle {
     return ORC_OK;
   }
 
-  const std::pair<std::uint32_t, std::uint32_t> *find(std::uint32_t k) const {
+  const Entry *find(K k) const {
     auto it = std::lower_bound(
         live.begin(), live.end(), k,
-        [](const std::pair<std::uint32_t, std::uint32_t> &e, std::uint32_t x) {
-          return e.first < x;
-        });
+        [](const Entry &e, K x) { return e.first < x; });
     if (it == live.end() || it->first != k) return nullptr;
     return &*it;
   }
 };
+
+using LsmOracle = LsmOracleT<std::uint32_t>;
+using LsmOracle64 = LsmOracleT<std::uint64_t>;
+
+// --------------------------------------------------- LSM oracle operations
+template <class K>
+LsmOracleT<K> *lsm_create(std::uint64_t max_elements,
+                          std::uint64_t batch_capacity) {
+  auto *o = new LsmOracleT<K>();
+  o->batch_capacity = std::min<std::uint64_t>(
+      kMaximumOperationTile, std::max<std::uint64_t>(1u, batch_capacity));
+  o->capacity = std::max<std::uint64_t>(max_elements,
+                                        o->batch_capacity * kBatchesPerEpoch);
+  if (o->capacity > 0xffffffffull) {
+    delete o;
+    return nullptr;
+  }
+  return o;
+}
+
+template <class K>
+void lsm_clear(LsmOracleT<K> *o) {
+  o->live.clear();
+  o->pending_batches = 0;
+  o->failed = false;
+  o->status = 0;
+}
+
+template <class K>
+int lsm_bulk_build(LsmOracleT<K> *o, const K *keys, const K *values,
+                   std::uint64_t n) {
+  if (n > 0xffffffffull || (n && (!keys || !values)))
+    return ORC_INVALID_ARGUMENT;
+  lsm_clear(o);
+  std::vector<Update<K>> u(n);
+  for (std::uint64_t i = 0; i < n; ++i) u[i] = {keys[i], values[i], false};
+  o->apply_tile(u.data(), n);  // one stable pass, last occurrence wins
+  return ORC_OK;
+}
+
+// Mixed batch: entry i is an erase when is_erase[i] != 0 (all erases when
+// values is null and is_erase is null); age = position.
+template <class K>
+int lsm_admit(LsmOracleT<K> *o, const K *keys, const K *values,
+              const std::uint8_t *is_erase, bool all_erase, std::uint64_t n) {
+  if (!n) return ORC_OK;
+  if (!keys || (!all_erase && !values)) return ORC_INVALID_ARGUMENT;
+  std::vector<Update<K>> u(n);
+  for (std::uint64_t i = 0; i < n; ++i) {
+    const bool e = all_erase || (is_erase && is_erase[i] != 0);
+    u[i] = {keys[i], e ? K(0) : values[i], e};
+  }
+  return o->admit(u.data(), n);
+}
+
+template <class K>
+void lsm_lookup(LsmOracleT<K> *o, const K *queries, std::uint64_t n,
+                K *out_values, std::uint8_t *out_found) {
+  for (std::uint64_t i = 0; i < n; ++i) {
+    const auto *e = o->find(queries[i]);
+    if (e) {
+      out_values[i] = e->second;
+      if (out_found) out_found[i] = 1;
+    } else {
+      out_values[i] = out_found ? K(0) : static_cast<K>(~K(0));
+      if (out_found) out_found[i] = 0;
+    }
+  }
+}
+
+// Sums accumulate in uint64 (wrapping for 64-bit values) and are truncated
+// to K.
+template <class K>
+void lsm_range(LsmOracleT<K> *o, const K *lo, const K *hi, std::uint64_t n,
+               K *out, bool count_only) {
+  // prefix sums over the live snapshot give O(log n) queries
+  std::vector<std::uint64_t> prefix(o->live.size() + 1, 0);
+  for (std::size_t i = 0; i < o->live.size(); ++i)
+    prefix[i + 1] = prefix[i] + (count_only ? 1u : o->live[i].second);
+  using Entry = typename LsmOracleT<K>::Entry;
+  for (std::uint64_t i = 0; i < n; ++i) {
+    if (lo[i] > hi[i]) {
+      out[i] = 0;
+      continue;
+    }
+    const std::size_t a = std::size_t(
+        std::lower_bound(o->live.begin(), o->live.end(), lo[i],
+                         [](const Entry &e, K x) { return e.first < x; }) -
+        o->live.begin());
+    const std::size_t b = std::size_t(
+        std::upper_bound(o->live.begin(), o->live.end(), hi[i],
+                         [](K x, const Entry &e) { return x < e.first; }) -
+        o->live.begin());
+    out[i] = static_cast<K>(prefix[b] - prefix[a]);
+  }
+}
+
+template <class K>
+void lsm_successor(LsmOracleT<K> *o, const K *queries, std::uint64_t n,
+                   K *out) {
+  using Entry = typename LsmOracleT<K>::Entry;
+  for (std::uint64_t i = 0; i < n; ++i) {
+    auto it = std::lower_bound(
+        o->live.begin(), o->live.end(), queries[i],
+        [](const Entry &e, K x) { return e.first < x; });
+    out[i] = it == o->live.end() ? static_cast<K>(~K(0)) : it->first;
+  }
+}
+
+template <class K>
+void lsm_dump(LsmOracleT<K> *o, K *keys, K *values) {
+  for (std::size_t i = 0; i < o->live.size(); ++i) {
+    keys[i] = o->live[i].first;
+    values[i] = o->live[i].second;
+  }
+}
 
 // ---------------------------------------------------------------- FliX oracle
 template <class K, class V>
@@ -429,110 +547,40 @@ void orc_range_digest(const std::uint32_t *results, std::uint64_t n,
 void *orc_lsm_create(std::uint64_t max_elements, std::uint64_t batch_capacity,
                      std::uint64_t level_zero_capacity) {
   (void)level_zero_capacity;  // layout-only in the reference
-  auto *o = new LsmOracle();
-  o->batch_capacity = std::min<std::uint64_t>(
-      kMaximumOperationTile, std::max<std::uint64_t>(1u, batch_capacity));
-  o->capacity = std::max<std::uint64_t>(max_elements,
-                                        o->batch_capacity * kBatchesPerEpoch);
-  if (o->capacity > 0xffffffffull) {
-    delete o;
-    return nullptr;
-  }
-  return o;
+  return lsm_create<std::uint32_t>(max_elements, batch_capacity);
 }
 
 void orc_lsm_destroy(void *h) { delete static_cast<LsmOracle *>(h); }
 
-void orc_lsm_clear(void *h) {
-  auto *o = static_cast<LsmOracle *>(h);
-  o->live.clear();
-  o->pending_batches = 0;
-  o->failed = false;
-  o->status = 0;
-}
+void orc_lsm_clear(void *h) { lsm_clear(static_cast<LsmOracle *>(h)); }
 
 int orc_lsm_bulk_build(void *h, const std::uint32_t *keys,
                        const std::uint32_t *values, std::uint64_t n) {
-  auto *o = static_cast<LsmOracle *>(h);
-  if (n > 0xffffffffull || (n && (!keys || !values)))
-    return ORC_INVALID_ARGUMENT;
-  orc_lsm_clear(h);
-  std::vector<Update> u(n);
-  for (std::uint64_t i = 0; i < n; ++i) u[i] = {keys[i], values[i], false};
-  o->apply_tile(u.data(), n);  // one stable pass, last occurrence wins
-  return ORC_OK;
+  return lsm_bulk_build(static_cast<LsmOracle *>(h), keys, values, n);
 }
 
 int orc_lsm_insert(void *h, const std::uint32_t *keys,
                    const std::uint32_t *values, std::uint64_t n) {
-  auto *o = static_cast<LsmOracle *>(h);
-  if (!n) return ORC_OK;
-  if (!keys || !values) return ORC_INVALID_ARGUMENT;
-  std::vector<Update> u(n);
-  for (std::uint64_t i = 0; i < n; ++i) u[i] = {keys[i], values[i], false};
-  return o->admit(u.data(), n);
+  return lsm_admit(static_cast<LsmOracle *>(h), keys, values, nullptr, false,
+                   n);
 }
 
 int orc_lsm_erase(void *h, const std::uint32_t *keys, std::uint64_t n) {
-  auto *o = static_cast<LsmOracle *>(h);
-  if (!n) return ORC_OK;
-  if (!keys) return ORC_INVALID_ARGUMENT;
-  std::vector<Update> u(n);
-  for (std::uint64_t i = 0; i < n; ++i) u[i] = {keys[i], 0u, true};
-  return o->admit(u.data(), n);
+  return lsm_admit<std::uint32_t>(static_cast<LsmOracle *>(h), keys, nullptr,
+                                  nullptr, true, n);
 }
 
-// Mixed batch: entry i is an erase when is_erase[i] != 0; age = position.
 int orc_lsm_apply_mixed(void *h, const std::uint32_t *keys,
                         const std::uint32_t *values,
                         const std::uint8_t *is_erase, std::uint64_t n) {
-  auto *o = static_cast<LsmOracle *>(h);
-  if (!n) return ORC_OK;
-  if (!keys || !values || !is_erase) return ORC_INVALID_ARGUMENT;
-  std::vector<Update> u(n);
-  for (std::uint64_t i = 0; i < n; ++i)
-    u[i] = {keys[i], is_erase[i] ? 0u : values[i], is_erase[i] != 0};
-  return o->admit(u.data(), n);
+  if (n && !is_erase) return ORC_INVALID_ARGUMENT;
+  return lsm_admit(static_cast<LsmOracle *>(h), keys, values, is_erase, false,
+                   n);
 }
 
 void orc_lsm_lookup(void *h, const std::uint32_t *queries, std::uint64_t n,
                     std::uint32_t *out_values, std::uint8_t *out_found) {
-  auto *o = static_cast<LsmOracle *>(h);
-  for (std::uint64_t i = 0; i < n; ++i) {
-    const auto *e = o->find(queries[i]);
-    if (e) {
-      out_values[i] = e->second;
-      if (out_found) out_found[i] = 1;
-    } else {
-      out_values[i] = out_found ? 0u : kInvalid;
-      if (out_found) out_found[i] = 0;
-    }
-  }
-}
-
-static void lsm_range(LsmOracle *o, const std::uint32_t *lo,
-                      const std::uint32_t *hi, std::uint64_t n,
-                      std::uint32_t *out, bool count_only) {
-  // prefix sums over the live snapshot give O(log n) queries
-  std::vector<std::uint64_t> prefix(o->live.size() + 1, 0);
-  for (std::size_t i = 0; i < o->live.size(); ++i)
-    prefix[i + 1] = prefix[i] + (count_only ? 1u : o->live[i].second);
-  auto pos = [&](std::uint64_t k) {  // first index with key >= k
-    return std::size_t(
-        std::lower_bound(o->live.begin(), o->live.end(), k,
-                         [](const std::pair<std::uint32_t, std::uint32_t> &e,
-                            std::uint64_t x) { return e.first < x; }) -
-        o->live.begin());
-  };
-  for (std::uint64_t i = 0; i < n; ++i) {
-    if (lo[i] > hi[i]) {
-      out[i] = 0;
-      continue;
-    }
-    const std::size_t a = pos(lo[i]);
-    const std::size_t b = pos(std::uint64_t(hi[i]) + 1u);
-    out[i] = static_cast<std::uint32_t>(prefix[b] - prefix[a]);
-  }
+  lsm_lookup(static_cast<LsmOracle *>(h), queries, n, out_values, out_found);
 }
 
 void orc_lsm_range_sum(void *h, const std::uint32_t *lo,
@@ -549,15 +597,7 @@ void orc_lsm_range_count(void *h, const std::uint32_t *lo,
 
 void orc_lsm_successor(void *h, const std::uint32_t *queries, std::uint64_t n,
                        std::uint32_t *out) {
-  auto *o = static_cast<LsmOracle *>(h);
-  for (std::uint64_t i = 0; i < n; ++i) {
-    auto it = std::lower_bound(
-        o->live.begin(), o->live.end(), queries[i],
-        [](const std::pair<std::uint32_t, std::uint32_t> &e, std::uint32_t x) {
-          return e.first < x;
-        });
-    out[i] = it == o->live.end() ? kInvalid : it->first;
-  }
+  lsm_successor(static_cast<LsmOracle *>(h), queries, n, out);
 }
 
 std::uint64_t orc_lsm_size(void *h) {
@@ -565,11 +605,7 @@ std::uint64_t orc_lsm_size(void *h) {
 }
 
 void orc_lsm_dump(void *h, std::uint32_t *keys, std::uint32_t *values) {
-  auto *o = static_cast<LsmOracle *>(h);
-  for (std::size_t i = 0; i < o->live.size(); ++i) {
-    keys[i] = o->live[i].first;
-    values[i] = o->live[i].second;
-  }
+  lsm_dump(static_cast<LsmOracle *>(h), keys, values);
 }
 
 std::uint32_t orc_lsm_status(void *h) {
@@ -582,6 +618,75 @@ std::uint64_t orc_lsm_capacity(void *h) {
 
 const char *orc_lsm_message(void *h) {
   return static_cast<LsmOracle *>(h)->message.c_str();
+}
+
+// ----------------------------------------------- LSM ABI, 64-bit extension
+void *orc_lsm64_create(std::uint64_t max_elements,
+                       std::uint64_t batch_capacity) {
+  return lsm_create<std::uint64_t>(max_elements, batch_capacity);
+}
+
+void orc_lsm64_destroy(void *h) { delete static_cast<LsmOracle64 *>(h); }
+
+void orc_lsm64_clear(void *h) { lsm_clear(static_cast<LsmOracle64 *>(h)); }
+
+int orc_lsm64_bulk_build(void *h, const std::uint64_t *keys,
+                         const std::uint64_t *values, std::uint64_t n) {
+  return lsm_bulk_build(static_cast<LsmOracle64 *>(h), keys, values, n);
+}
+
+int orc_lsm64_insert(void *h, const std::uint64_t *keys,
+                     const std::uint64_t *values, std::uint64_t n) {
+  return lsm_admit(static_cast<LsmOracle64 *>(h), keys, values, nullptr,
+                   false, n);
+}
+
+int orc_lsm64_erase(void *h, const std::uint64_t *keys, std::uint64_t n) {
+  return lsm_admit<std::uint64_t>(static_cast<LsmOracle64 *>(h), keys,
+                                  nullptr, nullptr, true, n);
+}
+
+int orc_lsm64_apply_mixed(void *h, const std::uint64_t *keys,
+                          const std::uint64_t *values,
+                          const std::uint8_t *is_erase, std::uint64_t n) {
+  if (n && !is_erase) return ORC_INVALID_ARGUMENT;
+  return lsm_admit(static_cast<LsmOracle64 *>(h), keys, values, is_erase,
+                   false, n);
+}
+
+void orc_lsm64_lookup(void *h, const std::uint64_t *queries, std::uint64_t n,
+                      std::uint64_t *out_values, std::uint8_t *out_found) {
+  lsm_lookup(static_cast<LsmOracle64 *>(h), queries, n, out_values,
+             out_found);
+}
+
+void orc_lsm64_range_sum(void *h, const std::uint64_t *lo,
+                         const std::uint64_t *hi, std::uint64_t n,
+                         std::uint64_t *out) {
+  lsm_range(static_cast<LsmOracle64 *>(h), lo, hi, n, out, false);
+}
+
+void orc_lsm64_range_count(void *h, const std::uint64_t *lo,
+                           const std::uint64_t *hi, std::uint64_t n,
+                           std::uint64_t *out) {
+  lsm_range(static_cast<LsmOracle64 *>(h), lo, hi, n, out, true);
+}
+
+void orc_lsm64_successor(void *h, const std::uint64_t *queries,
+                         std::uint64_t n, std::uint64_t *out) {
+  lsm_successor(static_cast<LsmOracle64 *>(h), queries, n, out);
+}
+
+std::uint64_t orc_lsm64_size(void *h) {
+  return static_cast<LsmOracle64 *>(h)->live.size();
+}
+
+void orc_lsm64_dump(void *h, std::uint64_t *keys, std::uint64_t *values) {
+  lsm_dump(static_cast<LsmOracle64 *>(h), keys, values);
+}
+
+std::uint32_t orc_lsm64_status(void *h) {
+  return static_cast<LsmOracle64 *>(h)->status;
 }
 
 // ----------------------------------------------------------------- FliX ABI

@@ -172,6 +172,74 @@ def test_lsm_oracle_against_dict_replay(orc, seed):
     assert v.tolist() == [ref.d[x] for x in sorted(ref.d)]
 
 
+ALL64 = (1 << 64) - 1
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_lsm64_oracle_against_dict_replay(orc, seed):
+    """The 64-bit extension: dict replay with sums modulo 2^64, all-ones
+    misses and successor sentinel, keys in both halves of the key space."""
+    rng = np.random.default_rng(200 + seed)
+    span = [64, 1 << 33, 1 << 63, 0][seed]
+    o = orc.LsmOracle(max_elements=1 << 16, batch_capacity=5, key_bytes=8)
+    d = {}
+
+    def draw(n):
+        if span:
+            return rng.integers(0, span, n, dtype=np.uint64) | (
+                np.uint64(1 << 63) if seed == 2 else np.uint64(0))
+        return rng.integers(0, 2**64, n, dtype=np.uint64, endpoint=False)
+
+    for b in range(30):
+        n = int(rng.integers(1, 30))
+        k = draw(n)
+        if b == 0:
+            k[:2] = [0, ALL64][:n] if n >= 2 else [ALL64]
+        v = rng.integers(0, 2**64, n, dtype=np.uint64, endpoint=False)
+        e = (rng.random(n) < 0.3).astype(np.uint8)
+        if b % 3 == 0:
+            assert o.insert(k, v) == 0
+            for kk, vv in zip(k.tolist(), v.tolist()):
+                d[kk] = vv
+        elif b % 3 == 1:
+            assert o.erase(k) == 0
+            for kk in k.tolist():
+                d.pop(kk, None)
+        else:
+            assert o.apply_mixed(k, v, e) == 0
+            for kk, vv, ee in zip(k.tolist(), v.tolist(), e.tolist()):
+                if ee:
+                    d.pop(kk, None)
+                else:
+                    d[kk] = vv
+    q = draw(200)
+    if d:
+        q[:80] = rng.choice(np.array(list(d), np.uint64), 80)
+    q[80:82] = [0, ALL64]
+    expect = np.array([d.get(x, ALL64) for x in q.tolist()], np.uint64)
+    assert np.array_equal(o.lookup(q), expect)
+    vals, found = o.lookup(q, with_found=True)
+    assert found.tolist() == [int(x in d) for x in q.tolist()]
+    assert vals.tolist() == [d.get(x, 0) for x in q.tolist()]
+    lo = draw(50)
+    w = rng.integers(0, 2**62, 50, dtype=np.uint64)
+    hi = lo + np.minimum(w, np.uint64(ALL64) - lo)
+    lo[:4], hi[:4] = hi[:4].copy(), lo[:4].copy()
+    lo[4], hi[4] = 0, ALL64
+    sums = [sum(v for k, v in d.items() if a <= k <= z) % (1 << 64)
+            for a, z in zip(lo.tolist(), hi.tolist())]
+    counts = [sum(1 for k in d if a <= k <= z)
+              for a, z in zip(lo.tolist(), hi.tolist())]
+    assert o.range_sum(lo, hi).tolist() == sums
+    assert o.range_count(lo, hi).tolist() == counts
+    ks = sorted(d)
+    succ = [next((k for k in ks if k >= x), ALL64) for x in q.tolist()]
+    assert o.successor(q).tolist() == succ
+    k, v = o.dump()
+    assert k.tolist() == ks
+    assert v.tolist() == [d[x] for x in ks]
+
+
 def test_lsm_oracle_bulk_build_last_wins(orc):
     keys = np.array([5, 3, 5, 9, 3, 5], np.uint32)
     vals = np.arange(6, dtype=np.uint32)

@@ -114,14 +114,18 @@ def _check_against_replay(gd, orc, io, reqs, served, cfg):
                 assert np.array_equal(io.down(alone, len(a)), got)
 
 
-@pytest.mark.parametrize("seed,span,max_size,inflight", [
-    (1, 1 << 12, 300, 2), (2, 1 << 20, 5000, 3), (3, 1 << 32, 20000, 1)])
+@pytest.mark.parametrize("seed,span,max_size,inflight,n_req", [
+    (1, 1 << 12, 300, 2, 300), (2, 1 << 20, 5000, 3, 300),
+    (3, 1 << 32, 20000, 1, 300),
+    # large inserts: merges of several 2^16-row runs are still running on
+    # the merge stream while later steps query their inputs
+    (4, 1 << 32, 1 << 16, 3, 120)])
 def test_served_results_equal_step_replay(gd, orc, io, seed, span, max_size,
-                                          inflight):
+                                          inflight, n_req):
     rng = np.random.default_rng(seed)
     cfg = dict(max_elements=1 << 22, batch_capacity=1 << 16)
     srv = gd.Server(**cfg, max_inflight_steps=inflight)
-    reqs = _make_requests(rng, 300, span, max_size)
+    reqs = _make_requests(rng, n_req, span, max_size)
     handles = _submit_all(gd, io, srv, reqs)
     served = []
     for order, (idx, t, out, found) in enumerate(handles):
