@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--serve-rate-x", type=float, default=2.0,
                     help="serving arrival rate as a multiple of the plain "
                          "path's request rate")
+    ap.add_argument("--c5-scale-log", type=int, default=-1,
+                    help="run the config-5 serving sweep on a 2^(31-s)-key "
+                         "uint64 index (-1 = skip)")
+    ap.add_argument("--c5-requests", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run an int32 negation over BYTES bytes (a "
                          "plain copy of known size that calibrates "
@@ -197,14 +201,14 @@ def pmc_report(args):
 
 
 def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
-    """Plain bulk vs serving on a mixed request stream over the 2^26-key
-    index.  Plain: every request is one synchronous call, back to back
+    """Plain bulk vs serving on the config-4 request stream over the
+    2^26-key index.  Plain: every request is one synchronous call, back to back
     (latency = call time, no queueing).  Serving: the same requests arrive
     as an open-loop Poisson stream at serve_rate_x times the plain request
     rate (latency = completion - submission, queueing included), then all
     at once (saturated throughput)."""
     import threading
-    rs = wl.RequestStream(wl.SEEDS[4], args.requests).requests(inserted)
+    rs = wl.config4(args.requests).requests()
     ops = int(sum(len(r[1]) for r in rs))
 
     def up(a):
@@ -322,9 +326,11 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
     srv.close()
     d.close()
     return {
-        "workload": "%d requests, sizes log-uniform [32, 2^18], 40%% insert "
-                    "/ 10%% erase / 40%% lookup / 10%% range count, over the "
-                    "config-2 dictionary (2^26 keys); %d ops" % (len(rs), ops),
+        "workload": "config4 stream (%d of its 10^4 requests): sizes "
+                    "log-uniform [32, 2^18], 40%% insert / 10%% erase / 40%% "
+                    "lookup / 10%% range count, keys Zipf(0.99) over 2^30 "
+                    "scrambled, applied to the config-2 dictionary (2^26 "
+                    "keys); %d ops" % (len(rs), ops),
         "plain": plain, "serving_open_loop": serving,
         "serving_saturated": saturated,
         "throughput_ratio_open_loop": round(serving["mops"] / plain["mops"],
@@ -333,6 +339,97 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
                                             3),
         "p99_ratio_open_loop": round(serving["p99_us"] / plain["p99_us"], 3),
     }
+
+
+def c5_measurement(args, pkg, wl, torch, dev):
+    """Config 5: a uint64 index grown to 2^(31-s) keys through the serving
+    engine (200 GB HBM budget), then the config-5 request stream (Zipf(0.8)
+    keys, sizes log-uniform [1, 2^16]) submitted all at once (saturated)
+    and as open-loop Poisson arrivals at fractions of the saturated request
+    rate.  Latency = completion - submission."""
+    universe = 1 << (31 - args.c5_scale_log)
+    srv = pkg.Server(max_elements=universe + (universe >> 1),
+                     batch_capacity=1 << 20, hbm_budget_bytes=200 << 30,
+                     max_inflight_steps=3, key_bytes=8)
+    S = pkg.Server
+    # grow: keys splitmix64(0 .. universe-1), generated on the device
+    chunk = 1 << 22
+    t0 = time.perf_counter()
+    for start in range(0, universe, chunk):
+        n = min(chunk, universe - start)
+        k = wl.device_splitmix64(torch.arange(start, start + n,
+                                              dtype=torch.int64, device=dev))
+        v = wl.device_splitmix64(k)
+        torch.cuda.synchronize()
+        srv.wait(srv.submit(S.INSERT, k, v, count=n))
+        del k, v
+    srv.drain()
+    grow_s = time.perf_counter() - t0
+    progress("config-5 index grown to %d keys in %.1f s" % (universe, grow_s))
+
+    rs = wl.config5(args.c5_requests, args.c5_scale_log).requests()
+    ops = int(sum(len(r[1]) for r in rs))
+    opmap = {"insert": S.INSERT, "erase": S.ERASE, "lookup": S.LOOKUP,
+             "range": S.RANGE_SUM}
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)) \
+            .to(dev)
+
+    staged = []
+    for kind, a, b in rs:
+        da = up(a)
+        db = up(b) if b is not None else None
+        out = torch.empty(len(a), dtype=torch.int64, device=dev) \
+            if kind in ("lookup", "range") else None
+        staged.append((opmap[kind], da, db, out, len(a)))
+    torch.cuda.synchronize()
+
+    def run(rate):
+        srv.stats(reset=True)
+        tickets = []
+        t0 = time.perf_counter()
+        if rate is None:
+            for op, da, db, out, n in staged:
+                tickets.append(srv.submit(op, da, db, out, None, count=n))
+        else:
+            gaps = -np.log(np.maximum(
+                (wl.stream(wl.SEEDS[5] ^ 0x55, len(staged)) >>
+                 np.uint64(11)).astype(np.float64) * 2.0 ** -53,
+                1e-300)) / rate
+            arrivals = t0 + np.cumsum(gaps)
+            for i, (op, da, db, out, n) in enumerate(staged):
+                while time.perf_counter() < arrivals[i]:
+                    pass
+                tickets.append(srv.submit(op, da, db, out, None, count=n))
+        srv.drain()
+        secs = time.perf_counter() - t0
+        st = srv.stats()
+        for t in tickets:
+            srv.wait(t)
+        return {"offered_requests_per_s": None if rate is None
+                else round(rate, 1),
+                "mops": round(ops / secs / 1e6, 3),
+                "requests_per_s": round(len(staged) / secs, 1),
+                "p50_us": round(st["latency_p50_us"], 1),
+                "p99_us": round(st["latency_p99_us"], 1),
+                "p999_us": round(st["latency_p999_us"], 1),
+                "steps": int(st["steps"]), "seconds": round(secs, 4)}
+
+    saturated = run(None)
+    sweep = [run(f * saturated["requests_per_s"]) for f in (0.25, 0.5, 0.75)]
+    resident = srv.stats()["resident_bytes"]
+    srv.close()
+    return {
+        "workload": "config5: uint64 keys/values, index grown to %d keys, "
+                    "%d requests (sizes log-uniform [1, 2^16], 50%% lookup "
+                    "/ 30%% insert / 10%% erase / 10%% range sum, Zipf(0.8) "
+                    "keys), %d ops, 200 GB HBM budget" % (universe, len(rs),
+                                                          ops),
+        "grow_seconds": round(grow_s, 2),
+        "grow_mkeys_per_s": round(universe / grow_s / 1e6, 1),
+        "resident_gb": round(resident / 2 ** 30, 2),
+        "saturated": saturated, "poisson_sweep": sweep}
 
 
 def flix_measurement(args, pkg, wl, torch, dev):
@@ -646,6 +743,14 @@ def main():
     if args.flix_batches > 0:
         d.close()
         result["flix"] = flix_measurement(args, pkg, wl, torch, dev)
+        progress("flix done")
+    if args.c5_scale_log >= 0:
+        d.close()
+        try:
+            result["config5"] = c5_measurement(args, pkg, wl, torch, dev)
+        except Exception as e:  # the main line must still print
+            result["config5"] = {"error": repr(e)}
+        progress("config-5 sweep done")
     print(json.dumps(result))
 
 

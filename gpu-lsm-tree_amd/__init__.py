@@ -194,6 +194,13 @@ SIGNATURES = {
     "gd_lsm_set_profiling": [_P, _I],
     "gd_lsm_read_profile": [_P, ctypes.POINTER(_Profile), _I],
     "gd_lsm_get_tuning": [_P, ctypes.POINTER(_Tuning)],
+    "gd_lsm_snapshot_save": [_P, ctypes.c_char_p, ctypes.POINTER(_U64), _P],
+    "gd_lsm_snapshot_restore": [_P, ctypes.c_char_p, ctypes.POINTER(_U64),
+                                _P],
+    "gd_lsm64_snapshot_save": [_P, ctypes.c_char_p, ctypes.POINTER(_U64),
+                               _P],
+    "gd_lsm64_snapshot_restore": [_P, ctypes.c_char_p, ctypes.POINTER(_U64),
+                                  _P],
     "gd_lsm64_create": [ctypes.POINTER(_Config), ctypes.POINTER(_P)],
     "gd_lsm64_destroy": [_P],
     "gd_lsm64_clear": [_P, _P],
@@ -407,6 +414,20 @@ class GPULSMOpt:
 
     def compact(self, stream=None):
         self._call("compact", _stream(stream))
+
+    def save(self, path, stream=None):
+        """Writes the live contents to a snapshot file; returns the count."""
+        n = _U64(0)
+        self._call("snapshot_save", os.fsencode(path), ctypes.byref(n),
+                   _stream(stream))
+        return int(n.value)
+
+    def restore(self, path, stream=None):
+        """Replaces the contents with a snapshot's; returns the count."""
+        n = _U64(0)
+        self._call("snapshot_restore", os.fsencode(path), ctypes.byref(n),
+                   _stream(stream))
+        return int(n.value)
 
     def canonical_carry_status(self):
         v = ctypes.c_uint32(0)

@@ -172,6 +172,22 @@ gd_status gd_lsm_shape(gd_lsm *h, uint32_t *out_runs, uint64_t *out_rows_ub) {
   GD_LSM_CALL(h, h->shape(out_runs, out_rows_ub));
 }
 
+gd_status gd_lsm_snapshot_save(gd_lsm *h, const char *path,
+                               uint64_t *out_count, hipStream_t s) {
+  GD_LSM_CALL(h, {
+    const uint64_t n = h->save(path, s);
+    if (out_count) *out_count = n;
+  });
+}
+
+gd_status gd_lsm_snapshot_restore(gd_lsm *h, const char *path,
+                                  uint64_t *out_count, hipStream_t s) {
+  GD_LSM_CALL(h, {
+    const uint64_t n = h->restore(path, s);
+    if (out_count) *out_count = n;
+  });
+}
+
 // ------------------------------------------------------------ 64-bit LSM
 gd_status gd_lsm64_create(const gd_config *config, gd_lsm64 **out) {
   if (!config || !out) {
@@ -273,6 +289,22 @@ gd_status gd_lsm64_set_tuning(gd_lsm64 *h, const gd_lsm_tuning *tuning) {
 gd_status gd_lsm64_get_tuning(gd_lsm64 *h, gd_lsm_tuning *out) {
   if (!out) return GD_INVALID_ARGUMENT;
   GD_LSM_CALL(h, *out = h->tuning());
+}
+
+gd_status gd_lsm64_snapshot_save(gd_lsm64 *h, const char *path,
+                                 uint64_t *out_count, hipStream_t s) {
+  GD_LSM_CALL(h, {
+    const uint64_t n = h->save(path, s);
+    if (out_count) *out_count = n;
+  });
+}
+
+gd_status gd_lsm64_snapshot_restore(gd_lsm64 *h, const char *path,
+                                    uint64_t *out_count, hipStream_t s) {
+  GD_LSM_CALL(h, {
+    const uint64_t n = h->restore(path, s);
+    if (out_count) *out_count = n;
+  });
 }
 
 gd_status gd_lsm64_shape(gd_lsm64 *h, uint32_t *out_runs,

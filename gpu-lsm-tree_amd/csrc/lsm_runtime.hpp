@@ -15,6 +15,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -299,6 +300,102 @@ class LsmT : public HandleBase {
     order_.end(s);
   }
 
+  // ------------------------------------------------- snapshot / restore
+  // File format (little endian):
+  //   header  magic "GDLSMSN1", u32 version (1), u32 key bytes (4 | 8),
+  //           u64 live count, u64 digest of the keys, u64 digest of the
+  //           values (word-wise FNV-1a over the 64-bit words of each array,
+  //           a zero-padded tail word for 32-bit arrays of odd length)
+  //   body    keys[count] ascending, then values[count]
+  // A snapshot holds the live contents only (tombstones and shadowed
+  // versions are resolved), so restoring it is a bulk build.
+  struct SnapshotHeader {
+    char magic[8];
+    u32 version;
+    u32 key_bytes;
+    u64 count;
+    u64 key_digest;
+    u64 value_digest;
+  };
+
+  u64 save(const char *path, hipStream_t s) {
+    if (!path) throw Error(GD_INVALID_ARGUMENT, "null snapshot path");
+    order_.begin(s);
+    u64 count = 0;
+    try {
+      Run merged{};
+      bool have = false;
+      if (!runs_.empty()) {
+        merged = merge_all(runs_, /*consume=*/false, s);
+        have = true;
+        count = read_count(merged, s);
+      }
+      Lease<K> k(pool_, std::max<u64>(count, 1));
+      Lease<V> v(pool_, std::max<u64>(count, 1));
+      if (count) {
+        decode_run_kernel<T><<<grid_for(count, 256, 1u << 16), 256, 0, s>>>(
+            merged.rows, merged.meta(), k.get(), v.get());
+        GD_LAUNCH_CHECK();
+      }
+      FileCloser f(std::fopen(path, "wb"));
+      if (!f.p)
+        throw Error(GD_INVALID_ARGUMENT,
+                    std::string("cannot open snapshot for writing: ") + path);
+      SnapshotHeader h{};
+      std::memcpy(h.magic, "GDLSMSN1", 8);
+      h.version = 1;
+      h.key_bytes = sizeof(K);
+      h.count = count;
+      // header first with zero digests, body, then the final header
+      write_all(f.p, &h, sizeof(h));
+      h.key_digest = stream_out(f.p, k.get(), count, s);
+      h.value_digest = stream_out(f.p, v.get(), count, s);
+      if (std::fseek(f.p, 0, SEEK_SET) != 0)
+        throw Error(GD_INVALID_ARGUMENT, "snapshot seek failed");
+      write_all(f.p, &h, sizeof(h));
+      if (have) free_run(merged);
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+    return count;
+  }
+
+  u64 restore(const char *path, hipStream_t s) {
+    if (!path) throw Error(GD_INVALID_ARGUMENT, "null snapshot path");
+    FileCloser f(std::fopen(path, "rb"));
+    if (!f.p)
+      throw Error(GD_INVALID_ARGUMENT,
+                  std::string("cannot open snapshot: ") + path);
+    SnapshotHeader h{};
+    if (std::fread(&h, sizeof(h), 1, f.p) != 1 ||
+        std::memcmp(h.magic, "GDLSMSN1", 8) != 0 || h.version != 1)
+      throw Error(GD_INVALID_ARGUMENT, "not a gdict LSM snapshot");
+    if (h.key_bytes != sizeof(K))
+      throw Error(GD_INVALID_ARGUMENT,
+                  "snapshot key width " + std::to_string(h.key_bytes) +
+                      " does not match this dictionary");
+    if (h.count > capacity_)
+      throw Error(GD_OUT_OF_MEMORY,
+                  "bad_alloc: snapshot of " + std::to_string(h.count) +
+                      " keys exceeds capacity " + std::to_string(capacity_));
+    Lease<K> k(pool_, std::max<u64>(h.count, 1));
+    Lease<V> v(pool_, std::max<u64>(h.count, 1));
+    order_.begin(s);
+    try {
+      if (stream_in(f.p, k.get(), h.count, s) != h.key_digest ||
+          stream_in(f.p, v.get(), h.count, s) != h.value_digest)
+        throw Error(GD_INVALID_ARGUMENT, "snapshot digest mismatch");
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+    bulk_build(k.get(), v.get(), h.count, s);
+    return h.count;
+  }
+
   // ----------------------------------------------- serving-engine access
   // The serving engine owns its dictionary exclusively and orders work with
   // its own streams and events, so these entry points skip the per-call
@@ -516,6 +613,85 @@ class LsmT : public HandleBase {
 
  private:
   // ----------------------------------------------------------- helpers
+  struct FileCloser {
+    std::FILE *p;
+    explicit FileCloser(std::FILE *f) : p(f) {}
+    ~FileCloser() {
+      if (p) std::fclose(p);
+    }
+  };
+
+  static void write_all(std::FILE *f, const void *data, std::size_t bytes) {
+    if (bytes && std::fwrite(data, 1, bytes, f) != bytes)
+      throw Error(GD_INVALID_ARGUMENT, "snapshot write failed");
+  }
+
+  // Word-wise FNV-1a over 64-bit words (a zero-padded last word).
+  static u64 digest_words(u64 h, const void *data, std::size_t bytes) {
+    const unsigned char *b = static_cast<const unsigned char *>(data);
+    std::size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+      u64 w;
+      std::memcpy(&w, b + i, 8);
+      h = (h ^ w) * 0x100000001b3ull;
+    }
+    if (i < bytes) {
+      u64 w = 0;
+      std::memcpy(&w, b + i, bytes - i);
+      h = (h ^ w) * 0x100000001b3ull;
+    }
+    return h;
+  }
+
+  static constexpr std::size_t kChunkBytes = std::size_t(64) << 20;
+
+  struct PinnedChunk {
+    void *p = nullptr;
+    PinnedChunk() { GD_HIP(hipHostMalloc(&p, kChunkBytes, hipHostMallocDefault)); }
+    ~PinnedChunk() {
+      if (p) (void)hipHostFree(p);
+    }
+  };
+
+  // Device array -> file through a pinned chunk; returns the digest.
+  template <class E>
+  u64 stream_out(std::FILE *f, const E *dev, u64 n, hipStream_t s) {
+    u64 h = 0xcbf29ce484222325ull;
+    if (!n) return h;
+    PinnedChunk buf;
+    const u64 total = n * sizeof(E);
+    for (u64 off = 0; off < total; off += kChunkBytes) {
+      const std::size_t len = static_cast<std::size_t>(
+          std::min<u64>(kChunkBytes, total - off));
+      GD_HIP(hipMemcpyAsync(buf.p, reinterpret_cast<const char *>(dev) + off,
+                            len, hipMemcpyDeviceToHost, s));
+      GD_HIP(hipStreamSynchronize(s));
+      h = digest_words(h, buf.p, len);
+      write_all(f, buf.p, len);
+    }
+    return h;
+  }
+
+  // File -> device array through a pinned chunk; returns the digest.
+  template <class E>
+  u64 stream_in(std::FILE *f, E *dev, u64 n, hipStream_t s) {
+    u64 h = 0xcbf29ce484222325ull;
+    if (!n) return h;
+    PinnedChunk buf;
+    const u64 total = n * sizeof(E);
+    for (u64 off = 0; off < total; off += kChunkBytes) {
+      const std::size_t len = static_cast<std::size_t>(
+          std::min<u64>(kChunkBytes, total - off));
+      if (std::fread(buf.p, 1, len, f) != len)
+        throw Error(GD_INVALID_ARGUMENT, "snapshot is truncated");
+      h = digest_words(h, buf.p, len);
+      GD_HIP(hipMemcpyAsync(reinterpret_cast<char *>(dev) + off, buf.p, len,
+                            hipMemcpyHostToDevice, s));
+      GD_HIP(hipStreamSynchronize(s));  // the chunk is reused
+    }
+    return h;
+  }
+
   void reset_state() {
     pending_batches_ = 0;
     failed_ = false;

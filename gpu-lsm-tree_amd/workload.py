@@ -172,6 +172,189 @@ def config2(scale_log=0):
                        1 << (20 - s), 1 << 4, 1 << 20, 0.1, 0.7)
 
 
+class Zipf:
+    """Zipf(s) ranks over 1..n by rejection-inversion (W. Hormann and
+    G. Derflinger, ACM TOMACS 6(3), 1996).  Sample i draws its uniforms from
+    the counter stream splitmix64(splitmix64(seed + i) + j), j = 0, 1, ...
+    until one is accepted, so every sample is independent of the others
+    and of the batch it is drawn in."""
+
+    def __init__(self, s, n):
+        self.s, self.n = float(s), float(n)
+        self.h_x1 = self.H(1.5) - 1.0
+        self.h_n = self.H(self.n + 0.5)
+        self.sv = 2.0 - self.H_inv(self.H(2.5) - self.h(2.0))
+
+    @staticmethod
+    def _helper1(x):  # log1p(x) / x
+        x = np.asarray(x, np.float64)
+        small = np.abs(x) <= 1e-8
+        safe = np.where(small, 1.0, x)
+        return np.where(small, 1.0 - x * (0.5 - x / 3.0),
+                        np.log1p(safe) / safe)
+
+    @staticmethod
+    def _helper2(x):  # expm1(x) / x
+        x = np.asarray(x, np.float64)
+        small = np.abs(x) <= 1e-8
+        safe = np.where(small, 1.0, x)
+        return np.where(small, 1.0 + x * 0.5 * (1.0 + x / 3.0),
+                        np.expm1(safe) / safe)
+
+    def h(self, x):
+        return np.exp(-self.s * np.log(x))
+
+    def H(self, x):
+        lx = np.log(x)
+        return self._helper2((1.0 - self.s) * lx) * lx
+
+    def H_inv(self, x):
+        t = np.maximum(x * (1.0 - self.s), -1.0)
+        return np.exp(self._helper1(t) * x)
+
+    def ranks(self, seed, count, offset=0):
+        base = splitmix64(np.uint64(seed) +
+                          np.arange(offset, offset + count, dtype=np.uint64))
+        out = np.zeros(count, dtype=np.uint64)
+        todo = np.arange(count)
+        j = np.uint64(0)
+        while len(todo):
+            u01 = (splitmix64(base[todo] + j) >> np.uint64(11)).astype(
+                np.float64) * 2.0 ** -53
+            u = self.h_n + u01 * (self.h_x1 - self.h_n)
+            x = self.H_inv(u)
+            k = np.clip(np.floor(x + 0.5), 1.0, self.n)
+            ok = (k - x <= self.sv) | (u >= self.H(k + 0.5) - self.h(k))
+            out[todo[ok]] = k[ok].astype(np.uint64)
+            todo = todo[~ok]
+            j += np.uint64(1)
+        return out
+
+
+def scrambled_keys(ranks, bits, c=0x5BD1E995):
+    """Rank r -> (r * 0x9E3779B1 + c) mod 2^bits: an odd multiplier makes
+    it a bijection of the bits-wide key space (the paper-sweep key map,
+    FliX/paper_lsmu_sweep.cu:64-67), so hot ranks land on scattered keys."""
+    mask = np.uint64((1 << bits) - 1)
+    with np.errstate(over="ignore"):
+        return (np.asarray(ranks, np.uint64) * np.uint64(0x9E3779B1) +
+                np.uint64(c)) & mask
+
+
+class ZipfRequestStream:
+    """The uneven request stream of config 4: `n` requests with sizes
+    log-uniform in [smin, smax]; 40% inserts (values murmur3 of the key),
+    10% erases, 40% lookups and 10% range counts (widths log-uniform in
+    [2^4, 2^16]); every key is a Zipf(s) rank over 2^bits scrambled
+    bijectively over the bits-wide key space."""
+
+    def __init__(self, seed, n, s=0.99, bits=30, smin=32, smax=1 << 18):
+        self.seed, self.n, self.bits = seed, n, bits
+        self.smin, self.smax = smin, smax
+        self.zipf = Zipf(s, 1 << bits)
+
+    def requests(self):
+        r = stream(sub_seed(self.seed, 30), self.n)
+        sizes = log_uniform(stream(sub_seed(self.seed, 31), self.n),
+                            self.smin, self.smax).astype(np.int64)
+        u = (r >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        total = int(sizes.sum())
+        keys = scrambled_keys(self.zipf.ranks(sub_seed(self.seed, 32), total),
+                              self.bits).astype(np.uint32)
+        widths = log_uniform(stream(sub_seed(self.seed, 33), total), 16,
+                             1 << 16)
+        out = []
+        off = 0
+        top = np.uint64((1 << 32) - 1)
+        for i in range(self.n):
+            n = int(sizes[i])
+            k = keys[off:off + n]
+            if u[i] < 0.4:
+                out.append(("insert", k, murmur3_u32(k)))
+            elif u[i] < 0.5:
+                out.append(("erase", k, None))
+            elif u[i] < 0.9:
+                out.append(("lookup", k, None))
+            else:
+                hi = np.minimum(k.astype(np.uint64) + widths[off:off + n] -
+                                np.uint64(1), top).astype(np.uint32)
+                out.append(("range", k, hi))
+            off += n
+        return out
+
+
+def config4(n_requests=10000, scale_log=0):
+    """Config 4: 10^4 uneven requests (sizes log-uniform [32, 2^18]) over
+    Zipf(0.99) keys on a 2^30 universe; scale_log shrinks the sizes."""
+    return ZipfRequestStream(SEEDS[4], n_requests, 0.99, 30, 32,
+                             max(32, (1 << 18) >> scale_log))
+
+
+def _i64(c):
+    """A uint64 constant as the int64 with the same bits."""
+    return c - (1 << 64) if c >= (1 << 63) else c
+
+
+def device_splitmix64(idx):
+    """splitmix64 of an int64 torch tensor on its device, bit-identical to
+    splitmix64() (int64 products wrap like uint64 ones; right shifts are
+    masked to be logical)."""
+    def lsr(x, k):
+        return (x >> k) & ((1 << (64 - k)) - 1)
+    x = idx + _i64(0x9E3779B97F4A7C15)
+    x = (x ^ lsr(x, 30)) * _i64(0xBF58476D1CE4E5B9)
+    x = (x ^ lsr(x, 27)) * _i64(0x94D049BB133111EB)
+    return x ^ lsr(x, 31)
+
+
+class ServingStream64:
+    """Config 5 requests: sizes log-uniform in [1, 2^16]; 50% lookups, 30%
+    inserts, 10% erases, 10% range sums; uint64 keys are Zipf(0.8) ranks r
+    over `universe` mapped to splitmix64(r - 1), the keys an index grown
+    with splitmix64(0 .. universe-1) holds, so hot ranks are hot stored
+    keys; values splitmix64(key); range widths log-uniform in [2^4, 2^40]
+    in key space."""
+
+    def __init__(self, seed, n, universe, s=0.8, smin=1, smax=1 << 16):
+        self.seed, self.n, self.universe = seed, n, universe
+        self.smin, self.smax = smin, smax
+        self.zipf = Zipf(s, universe)
+
+    def requests(self):
+        r = stream(sub_seed(self.seed, 40), self.n)
+        sizes = log_uniform(stream(sub_seed(self.seed, 41), self.n),
+                            self.smin, self.smax).astype(np.int64)
+        u = (r >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        total = int(sizes.sum())
+        ranks = self.zipf.ranks(sub_seed(self.seed, 42), total)
+        keys = splitmix64(ranks - np.uint64(1))
+        widths = log_uniform(stream(sub_seed(self.seed, 43), total), 16,
+                             1 << 40)
+        out = []
+        off = 0
+        top = np.uint64(0xFFFFFFFFFFFFFFFF)
+        for i in range(self.n):
+            n = int(sizes[i])
+            k = keys[off:off + n]
+            if u[i] < 0.5:
+                out.append(("lookup", k, None))
+            elif u[i] < 0.8:
+                out.append(("insert", k, splitmix64(k)))
+            elif u[i] < 0.9:
+                out.append(("erase", k, None))
+            else:
+                hi = k + np.minimum(widths[off:off + n] - np.uint64(1),
+                                    top - k)
+                out.append(("range", k, hi))
+            off += n
+        return out
+
+
+def config5(n_requests=2000, scale_log=0):
+    """Config 5: a 2^31-key uint64 index served under Poisson arrivals."""
+    return ServingStream64(SEEDS[5], n_requests, 1 << (31 - scale_log))
+
+
 class RequestStream:
     """Mixed request stream for the serving comparison: `n` requests with
     sizes log-uniform in [smin, smax]; 40% inserts (new uniform 31-bit keys,

@@ -128,6 +128,17 @@ gd_status gd_lsm_dump_sorted(gd_lsm *h, uint32_t *out_keys,
 /* No reference counterpart (LSMu's cleanup, FliX/impl_lsm_tree.cuh:1256):
  * merges every run into one, dropping tombstones and shadowed versions. */
 gd_status gd_lsm_compact(gd_lsm *h, hipStream_t stream);
+/* No reference counterpart (the reference has no on-disk format): writes
+ * the live contents to `path` (header with key width, count and digests,
+ * then the ascending keys and their values) and *out_count (nullable) =
+ * live keys.  restore replaces the contents with a snapshot's (a bulk
+ * build); GD_INVALID_ARGUMENT for a missing, foreign, truncated or
+ * corrupted file or another key width, GD_OUT_OF_MEMORY beyond capacity.
+ * Both synchronise the stream. */
+gd_status gd_lsm_snapshot_save(gd_lsm *h, const char *path,
+                               uint64_t *out_count, hipStream_t stream);
+gd_status gd_lsm_snapshot_restore(gd_lsm *h, const char *path,
+                                  uint64_t *out_count, hipStream_t stream);
 /* canonical_carry_status()  GPULSMOpt.cuh:5872 (synchronous). */
 gd_status gd_lsm_carry_status(gd_lsm *h, uint32_t *out_status);
 /* gpu_resident_bytes()  GPULSMOpt.cuh:6341 */
@@ -223,6 +234,10 @@ gd_status gd_lsm64_set_tuning(gd_lsm64 *h, const gd_lsm_tuning *tuning);
 gd_status gd_lsm64_get_tuning(gd_lsm64 *h, gd_lsm_tuning *out);
 gd_status gd_lsm64_shape(gd_lsm64 *h, uint32_t *out_runs,
                          uint64_t *out_rows_ub);
+gd_status gd_lsm64_snapshot_save(gd_lsm64 *h, const char *path,
+                                 uint64_t *out_count, hipStream_t stream);
+gd_status gd_lsm64_snapshot_restore(gd_lsm64 *h, const char *path,
+                                    uint64_t *out_count, hipStream_t stream);
 
 /* -------------------------------------------------------------- serving */
 

@@ -264,3 +264,19 @@ def test_server_64bit_matches_replay(gd, orc, io):
                 assert np.array_equal(io.down(out, len(a), np.uint64),
                                       o.range_sum(a, b))
     srv.close()
+
+
+def test_snapshot_round_trip_64(gd, orc, io, tmp_path):
+    rng = np.random.default_rng(8)
+    p = Pair64(gd, orc, io, max_elements=1 << 16, batch_capacity=1 << 12)
+    for _ in range(6):
+        k = _keys(rng, 3000, 0)
+        p.mixed(k, ~k, (rng.random(3000) < 0.2).astype(np.uint8))
+    path = str(tmp_path / "lsm64.snap")
+    assert p.d.save(path) == p.o.size()
+    fresh = Pair64(gd, orc, io, max_elements=1 << 16, batch_capacity=1 << 12)
+    fresh.d.restore(path)
+    k, v = p.o.dump()
+    assert fresh.o.bulk_build(k, v) == 0
+    fresh.check_contents()
+    fresh.check_lookup(_queries(rng, k, 4000, 0), True)

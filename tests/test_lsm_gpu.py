@@ -454,3 +454,51 @@ def test_kernel_profile_counts(gd, orc, io):
     again = p.d.read_profile(reset=False)
     assert again["merge_launches"] == 0 and again["lookup_launches"] == 0
     p.d.set_profiling(False)
+
+
+def test_snapshot_save_restore(gd, orc, io, tmp_path):
+    """A snapshot holds the live contents (tombstones resolved); restoring
+    it into a fresh dictionary gives the same contents and answers.
+    Foreign, truncated, corrupted or other-width files are rejected."""
+    rng = np.random.default_rng(31)
+    p = Pair(gd, orc, io, max_elements=1 << 18, batch_capacity=1 << 12)
+    for b in range(12):
+        keys = rng.integers(0, 1 << 32, 6000, dtype=np.uint64).astype(
+            np.uint32)
+        erase = (rng.random(6000) < 0.25).astype(np.uint8)
+        p.mixed(keys, keys ^ np.uint32(0xA5A5A5A5), erase)
+    path = str(tmp_path / "lsm.snap")
+    n = p.d.save(path)
+    assert n == p.o.size()
+    fresh = Pair(gd, orc, io, max_elements=1 << 18, batch_capacity=1 << 12)
+    assert fresh.d.restore(path) == n
+    k, v = p.o.dump()
+    assert fresh.o.bulk_build(k, v) == 0
+    fresh.check_contents()
+    q = _queries(rng, k, 5000, 1 << 32)
+    fresh.check_lookup(q, True)
+    fresh.check_successor(q[:2000])
+    # updates after a restore
+    fresh.insert(q[:3000], q[:3000])
+    fresh.check_contents()
+    # empty dictionary round trip
+    empty = gd.GPULSMOpt(max_elements=1 << 12, batch_capacity=64)
+    epath = str(tmp_path / "empty.snap")
+    assert empty.save(epath) == 0
+    assert empty.restore(epath) == 0
+    assert empty.dump_sorted() == 0
+    # rejections
+    raw = open(path, "rb").read()
+    bad = tmp_path / "bad.snap"
+    for blob in (b"not a snapshot at all", raw[: len(raw) - 5],
+                 raw[:100] + bytes([raw[100] ^ 1]) + raw[101:]):
+        bad.write_bytes(blob)
+        with pytest.raises(gd.InvalidArgument):
+            fresh.d.restore(str(bad))
+    wide = gd.GPULSMOpt(max_elements=1 << 18, batch_capacity=1 << 12,
+                        key_bytes=8)
+    with pytest.raises(gd.InvalidArgument):
+        wide.restore(path)
+    small = gd.GPULSMOpt(max_elements=1 << 10, batch_capacity=16)
+    with pytest.raises(gd.OutOfMemory):
+        small.restore(path)

@@ -75,6 +75,20 @@ def test_zipf_frequencies(orc):
     assert np.max(np.abs(freq - pmf)) < 0.01
 
 
+def test_product_zipf_matches_oracle(orc, gd):
+    """The product's numpy Zipf sampler (config 4/5 keys) draws the same
+    ranks as the oracle's rejection-inversion, sample for sample."""
+    import importlib
+    wl = importlib.import_module("gdict_amd.workload")
+    for s, n, seed in ((1.1, 20, 11), (0.99, 1 << 30, 5), (0.8, 1 << 31, 9)):
+        assert np.array_equal(wl.Zipf(s, n).ranks(seed, 50000),
+                              orc.zipf_ranks(s, n, seed, 50000))
+    # the scrambling is a bijection of the 30-bit key space
+    r = np.arange(1 << 16, dtype=np.uint64) * np.uint64(16411)
+    k = wl.scrambled_keys(r, 30)
+    assert len(np.unique(k)) == len(r) and k.max() < (1 << 30)
+
+
 def test_paper_key_bijection(orc):
     # FliX/paper_lsmu_sweep.cu:64-67: odd multiplier -> bijection on 2^30
     wl = importlib.import_module("gdict_amd.workload")
