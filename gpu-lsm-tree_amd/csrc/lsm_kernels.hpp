@@ -31,7 +31,7 @@ struct RunMeta {
   u32 count;
   u32 split;     // rows whose key has the top bit clear
   u32 dir_bits;  // b, 1 <= b <= kMaxDirBits
-  u32 tombs;     // tombstone rows
+  u32 tombs;     // nonzero when the run holds tombstone rows
 };
 constexpr u64 kMetaBytes = 256;  // header slot in front of the directory
 
@@ -181,14 +181,20 @@ enum FinalizeMode : int {
 // tombstone flag, encode, and compact with a single-pass chained scan.
 // Items are striped (item j of a thread is row base + j*512 + tid), so every
 // load and store of a row j is one coalesced access of the block; ballots
-// give the survivors' ranks.
+// give the survivors' ranks.  The run header is written without atomics:
+// the tile holding the last key writes the count, the tile where keys
+// first reach the top bit (or the last tile) writes the split, and tiles
+// with tombstones set the tombstone flag.  The radix directory (dir_bits
+// bits) is written here too: sorted position i owns the directory entries
+// of the buckets in (bucket(key[i-1]), bucket(key[i])] and writes them the
+// number of survivors before position i (the output row of its key).
 template <class T>
 __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
     const typename T::K *__restrict__ sk, const u32 *__restrict__ sp, u32 n,
     const typename T::V *__restrict__ values,
     const u8 *__restrict__ erase_flags, int mode,
-    typename T::Row *__restrict__ out, RunMeta *meta, u64 *status,
-    u32 *ticket) {
+    typename T::Row *__restrict__ out, RunMeta *meta, u32 *__restrict__ dir,
+    u32 dir_bits, u64 *status, u32 *ticket) {
   using K = typename T::K;
   using V = typename T::V;
   __shared__ u32 s_tile;
@@ -255,6 +261,25 @@ __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
     low += key[j] < T::kTomb;
     tombs += tomb;
   }
+  // directory entries
+  const u32 shift = T::kBits - dir_bits;
+#pragma unroll
+  for (int j = 0; j < kFinItems; ++j) {
+    const u64 i = base + u64(j) * kFinThreads + threadIdx.x;
+    if (i >= n) continue;
+    const long long bk = static_cast<long long>(key[j] >> shift);
+    const long long pbk =
+        i ? static_cast<long long>(sk[i - 1] >> shift) : -1;
+    const u32 before = static_cast<u32>(out_base + s_off[j][wave] +
+                                        popc64(mask[j] & lanemask_lt()));
+    for (long long b = pbk + 1; b <= bk; ++b) dir[b] = before;
+    if (i + 1 == n) {
+      // the last key: the buckets after it end at the row count
+      const u32 count = static_cast<u32>(out_base + tile_total);
+      for (long long b = bk + 1; b <= (1ll << dir_bits); ++b) dir[b] = count;
+      meta->dir_bits = dir_bits;
+    }
+  }
   low = wave_sum_u32(low);
   tombs = wave_sum_u32(tombs);
   if (lane == 0) {
@@ -263,9 +288,15 @@ __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (tile_total) atomicAdd(&meta->count, tile_total);
-    if (s_low) atomicAdd(&meta->split, s_low);
-    if (s_tomb) atomicAdd(&meta->tombs, s_tomb);
+    const u64 in_end = base + kFinTile < n ? base + kFinTile : n;
+    const bool last_tile = in_end == n;
+    // keys before this tile are all below the top bit and some key of the
+    // tile (its last) reaches it, or no key does at all
+    const bool transition = (base == 0 || sk[base - 1] < T::kTomb) &&
+                            (sk[in_end - 1] >= T::kTomb || last_tile);
+    if (last_tile) meta->count = static_cast<u32>(out_base + tile_total);
+    if (transition) meta->split = static_cast<u32>(out_base + s_low);
+    if (s_tomb) meta->tombs = 1;
   }
 }
 
@@ -527,39 +558,75 @@ __device__ __forceinline__ u32 merge_path(const MergeSide<T> &a, u32 ca,
   return static_cast<u32>(lo);
 }
 
-// Tile split points of the merge; also clears the merge's chained-scan
-// state and the output run's header (one launch instead of three).
+// Tile split points of the merge, one wave per split point: a 64-ary
+// search (every lane tests one candidate, a ballot keeps the interval that
+// holds the answer), so a split over 2^26 rows costs 5 rounds of loads
+// instead of 26 dependent ones.  Also clears the merge's chained-scan state
+// and the output run's header (one launch instead of three).
 template <class T>
 __global__ void merge_partition_kernel(MergeSide<T> a, MergeSide<T> b,
                                        u32 tiles, u32 tile_rows,
                                        u32 *__restrict__ parts, u64 *status,
-                                       u32 *ticket, RunMeta *out_meta) {
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+                                       u32 *ticket, RunMeta *out_meta,
+                                       u32 dir_bits) {
+  const u32 lane = lane_id();
+  const u64 t = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   if (t > tiles) return;
-  if (t < tiles) status[t] = 0;
-  if (t == 0) {
+  if (lane == 0 && t < tiles) status[t] = 0;
+  if (t == 0 && lane == 0) {
     *ticket = 0;
     out_meta->count = 0;
     out_meta->split = 0;
-    out_meta->dir_bits = 0;
+    out_meta->dir_bits = dir_bits;
     out_meta->tombs = 0;
   }
   const u32 ca = a.meta->count, cb = b.meta->count;
+  const u32 sa = a.meta->split, sb = b.meta->split;
   const u64 total = u64(ca) + cb;
-  u64 diag = u64(t) * tile_rows;
+  u64 diag = t * tile_rows;
   if (diag > total) diag = total;
-  parts[t] = merge_path(a, ca, a.meta->split, b, cb, b.meta->split, diag);
+  // answer = first m in [lo, hi) with A[m] > B[diag - m - 1], else hi
+  u64 lo = diag > cb ? diag - cb : 0;
+  u64 hi = diag < ca ? diag : ca;
+  while (hi > lo) {
+    const u64 span = hi - lo;
+    const u64 m = span <= kWave ? lo + lane : lo + ((span * lane) >> 6);
+    bool p = false;  // lanes past the interval vote "false"
+    if (m < hi)
+      p = side_key(a, static_cast<u32>(m), sa) <=
+          side_key(b, static_cast<u32>(diag - m - 1), sb);
+    const u64 vote = ballot(p);
+    const u32 c = popc64(vote);  // the true candidates form a prefix
+    if (span <= kWave) {
+      lo += c;
+      break;
+    }
+    const u64 m_prev = lo + ((span * (c ? c - 1 : 0)) >> 6);
+    const u64 m_next = lo + ((span * c) >> 6);
+    if (c == 0) {
+      hi = lo;  // A[lo] > B[...]: the answer is lo
+    } else {
+      lo = m_prev + 1;
+      if (c < kWave) hi = m_next;
+    }
+  }
+  if (lane == 0) parts[t] = static_cast<u32>(lo);
 }
 
 // Merges run A (newer) with run B (older) into `out`: on equal keys the A
 // row survives, and tombstones are dropped when `drop_tombstones` (the
 // output becomes the oldest run, GPULSMOpt.cuh:3175-3179).  A tile is
-// 256 * kMergeItems output diagonals.
+// 256 * kMergeItems output diagonals.  The output's radix directory
+// (dir_bits bits) is written here: every input element, kept or dropped,
+// owns the entries of the buckets in (bucket(previous input key),
+// bucket(its key)] in merged order and writes them the number of survivors
+// before it, which is the output row of the first key >= those bounds.
 template <class T, int kMergeItems>
 __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     MergeSide<T> a, MergeSide<T> b, const u32 *__restrict__ parts,
-    typename T::Row *__restrict__ out, RunMeta *out_meta, int drop_tombstones,
-    u64 *status, u32 *ticket) {
+    typename T::Row *__restrict__ out, RunMeta *out_meta,
+    u32 *__restrict__ dir, u32 dir_bits, int drop_tombstones, u64 *status,
+    u32 *ticket) {
   using K = typename T::K;
   using V = typename T::V;
   using Row = typename T::Row;
@@ -577,6 +644,7 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   __shared__ u32 s_scan[kMergeThreads / kWave + 1];
   __shared__ u64 s_base;
   __shared__ u32 s_low, s_tombs;
+  __shared__ K s_prev_b;  // B[j0 - 1]: with A[i0 - 1], the row before the tile
 
   if (threadIdx.x == 0) {
     s_tile = atomicAdd(ticket, 1u);
@@ -589,7 +657,13 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   const u32 sa = a.meta->split, sb = b.meta->split;
   const u64 total = u64(ca) + cb;
   const u64 d0 = u64(tile) * kMergeTile;
-  if (d0 >= total) return;  // every later ticket is empty too
+  if (d0 >= total) {  // every later ticket is empty too
+    if (total == 0 && tile == 0) {  // empty output: an all-zero directory
+      for (u32 j = threadIdx.x; j <= (1u << dir_bits); j += kMergeThreads)
+        dir[j] = 0;
+    }
+    return;
+  }
   const u64 d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
   const u32 i0 = parts[tile], i1 = parts[tile + 1];
   const u32 j0 = static_cast<u32>(d0 - i0), j1 = static_cast<u32>(d1 - i1);
@@ -616,7 +690,22 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     s_tomb[na + 1 + t] = static_cast<u8>(T::kw(r) >> (T::kBits - 1));
     s_val[na + 1 + t] = T::val(r);
   }
+  if (threadIdx.x == kMergeThreads - 1)
+    s_prev_b = j0 ? decode_key<T>(T::kw(b.rows[j0 - 1]), j0 - 1, sb) : K(0);
   __syncthreads();
+  // Header fields without atomics: the tile reaching the end writes the
+  // count; the first tile whose inputs reach the top key bit (or the last
+  // tile) writes the split, as every row before it is below the top bit and
+  // every row after it is not; tiles with surviving tombstones set the flag.
+  const bool last_tile = d1 == total;
+  bool transition = false;
+  {
+    const bool before_low = (i0 == 0 || s_key[0] < T::kTomb) &&
+                            (j0 == 0 || s_prev_b < T::kTomb);
+    const bool tile_high = (na && s_key[na] >= T::kTomb) ||
+                           (nb && s_key[na + nb] >= T::kTomb);
+    transition = before_low && (tile_high || last_tile);
+  }
 
   // Per-thread merge path inside the tile.
   const u32 tile_len = na + nb;
@@ -632,6 +721,20 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     }
     ia = lo;
     ib = dk - lo;
+  }
+  // the input element before this thread's first one, in merged order
+  bool has_prev = false;
+  K prev = 0;
+  if (dk < tile_len) {
+    if (ia > 0 || i0 > 0) {
+      has_prev = true;
+      prev = s_key[ia];  // LDS index ia holds A[ia - 1] (A[i0 - 1] at 0)
+    }
+    if (ib > 0 || j0 > 0) {
+      const K kb = ib > 0 ? s_key[na + ib] : s_prev_b;
+      prev = has_prev && prev > kb ? prev : kb;
+      has_prev = true;
+    }
   }
   K ok[kMergeItems];
   V ov[kMergeItems];
@@ -698,6 +801,26 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   const u64 obase = s_base;
   for (u32 t = threadIdx.x; t < tile_total; t += kMergeThreads)
     out[obase + t] = t < kStageHalf ? stage_lo[t] : stage_hi[t - kStageHalf];
+  // directory entries
+  {
+    const u32 shift = T::kBits - dir_bits;
+    long long pbk = has_prev ? static_cast<long long>(prev >> shift) : -1;
+    u32 before = static_cast<u32>(obase) + local;
+#pragma unroll
+    for (int s = 0; s < kMergeItems; ++s) {
+      if (static_cast<u32>(s) < steps) {
+        const long long bk = static_cast<long long>(ok[s] >> shift);
+        for (long long j = pbk + 1; j <= bk; ++j) dir[j] = before;
+        pbk = bk > pbk ? bk : pbk;
+        before += okeep[s];
+      }
+    }
+    if (last_tile && steps && dk + steps == tile_len) {
+      // the last input element: the buckets after it end at the row count
+      for (long long j = pbk + 1; j <= (1ll << dir_bits); ++j)
+        dir[j] = static_cast<u32>(obase + tile_total);
+    }
+  }
   low = wave_sum_u32(low);
   tombs = wave_sum_u32(tombs);
   if (lane_id() == 0) {
@@ -706,9 +829,9 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (tile_total) atomicAdd(&out_meta->count, tile_total);
-    if (s_low) atomicAdd(&out_meta->split, s_low);
-    if (s_tombs) atomicAdd(&out_meta->tombs, s_tombs);
+    if (last_tile) out_meta->count = static_cast<u32>(obase + tile_total);
+    if (transition) out_meta->split = static_cast<u32>(obase + s_low);
+    if (s_tombs) out_meta->tombs = 1;
   }
 }
 

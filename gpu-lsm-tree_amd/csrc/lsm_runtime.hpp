@@ -764,9 +764,9 @@ class LsmT : public HandleBase {
                              ctrl.get(), s, tuning_.sort_block));
     finalize_batch_kernel<T><<<fin_tiles, kFinThreads, 0, s>>>(
         ka.get(), va.get(), n, values, flags, mode, r.rows, r.meta(),
-        reinterpret_cast<u64 *>(fin_ctrl + 64), fin_ctrl);
+        r.entries(), r.max_bits, reinterpret_cast<u64 *>(fin_ctrl + 64),
+        fin_ctrl);
     GD_LAUNCH_CHECK();
-    launch_directory(r, s);
     return r;
   }
 
@@ -788,26 +788,26 @@ class LsmT : public HandleBase {
     u64 *status = reinterpret_cast<u64 *>(ctrl.get() + 64);
     const MergeSide<T> sa{a.rows, a.meta()};
     const MergeSide<T> sb{b.rows, b.meta()};
-    merge_partition_kernel<T><<<grid_for(u64(tiles) + 1, 256), 256, 0, s>>>(
+    merge_partition_kernel<T><<<(tiles + 4) / 4, 256, 0, s>>>(
         sa, sb, tiles, tile_rows, parts.get(), status, ctrl.get(),
-        out.meta());
+        out.meta(), out.max_bits);
     GD_LAUNCH_CHECK();
     ProfRec rec{};
     if (profiling_) rec = prof_begin(0, s, 0);
     if constexpr (T::kMergeItemsMax >= 16) {
       if (tile_rows == 4096) {
         merge_kernel<T, 16><<<tiles, kMergeThreads, 0, s>>>(
-            sa, sb, parts.get(), out.rows, out.meta(),
-            drop_tombstones ? 1 : 0, status, ctrl.get());
+            sa, sb, parts.get(), out.rows, out.meta(), out.entries(),
+            out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
       } else {
         merge_kernel<T, 8><<<tiles, kMergeThreads, 0, s>>>(
-            sa, sb, parts.get(), out.rows, out.meta(),
-            drop_tombstones ? 1 : 0, status, ctrl.get());
+            sa, sb, parts.get(), out.rows, out.meta(), out.entries(),
+            out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
       }
     } else {
       merge_kernel<T, 8><<<tiles, kMergeThreads, 0, s>>>(
-          sa, sb, parts.get(), out.rows, out.meta(), drop_tombstones ? 1 : 0,
-          status, ctrl.get());
+          sa, sb, parts.get(), out.rows, out.meta(), out.entries(),
+          out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
     }
     GD_LAUNCH_CHECK();
     if (profiling_) {
@@ -820,7 +820,6 @@ class LsmT : public HandleBase {
       GD_HIP(hipMemcpyAsync(slot + 2, out.meta(), sizeof(RunMeta),
                             hipMemcpyDeviceToHost, s));
     }
-    launch_directory(out, s);
     return out;
   }
 
