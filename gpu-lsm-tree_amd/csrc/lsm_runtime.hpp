@@ -49,14 +49,7 @@ struct RunT {
   RunView<T> view() const { return {rows, entries(), meta()}; }
 };
 
-static __global__ void zero_two_kernel(u32 *a, u64 na, u32 *b, u64 nb) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb;
-       i += stride) {
-    if (i < na) a[i] = 0;
-    else b[i - na] = 0;
-  }
-}
+using rs::zero_two_kernel;
 
 template <class T>
 class LsmT : public HandleBase {
@@ -757,11 +750,12 @@ class LsmT : public HandleBase {
     Lease<u32> va(pool_, n), vb(pool_, n);
     Lease<u32> ctrl(pool_, sort_words + fin_words);
     u32 *fin_ctrl = ctrl.get() + sort_words;
-    zero_two_kernel<<<grid_for(fin_words + 4, 256, 1024), 256, 0, s>>>(
-        reinterpret_cast<u32 *>(r.meta()), 4, fin_ctrl, fin_words);
-    GD_LAUNCH_CHECK();
+    // the sort's clearing launch also clears the finalize state behind its
+    // own and the run header
     GD_HIP(rs::sort_pairs<K>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
-                             ctrl.get(), s, tuning_.sort_block));
+                             ctrl.get(), s, tuning_.sort_block,
+                             sort_words + fin_words,
+                             reinterpret_cast<u32 *>(r.meta()), 4));
     finalize_batch_kernel<T><<<fin_tiles, kFinThreads, 0, s>>>(
         ka.get(), va.get(), n, values, flags, mode, r.rows, r.meta(),
         r.entries(), r.max_bits, reinterpret_cast<u64 *>(fin_ctrl + 64),

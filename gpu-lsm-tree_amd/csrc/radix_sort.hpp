@@ -29,7 +29,9 @@ constexpr int passes_for() {
 // Digit histograms.  A block counts 16 keys per thread per round in LDS
 // (the 16 loads are issued together), then adds its counts into one of
 // kHistRows partial rows (row = block % kHistRows), so at most
-// blocks / kHistRows atomics meet on one address; scan_kernel sums the rows.
+// blocks / kHistRows atomics meet on one address.  The last block to finish
+// (a done counter) sums the rows and writes the exclusive digit offsets, so
+// no separate scan launch is needed.
 constexpr int kHistItems = 16;
 constexpr int kHistRows = 32;
 
@@ -39,11 +41,15 @@ inline u32 hist_blocks_for(u64 n) {
 }
 
 // rows[kHistRows][kPasses][kRadix] (zeroed) += counts of digits
-// first .. first + kPasses - 1, digit p = (key >> 8p) & 0xff.
+// first .. first + kPasses - 1, digit p = (key >> 8p) & 0xff; then
+// offsets[p][d] = exclusive scan over d of the summed rows.  *done is zero.
 template <class K, int kPasses>
 static __global__ __launch_bounds__(kThreads) void histogram_kernel(
-    const K *__restrict__ keys, u32 n, u32 first, u32 *__restrict__ rows) {
+    const K *__restrict__ keys, u32 n, u32 first, u32 *__restrict__ rows,
+    u32 *__restrict__ offsets, u32 *done) {
   __shared__ u32 s_hist[kPasses][kRadix];
+  __shared__ u32 s_last;
+  __shared__ u32 s_scratch[kRadix / kWave + 1];
   for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads)
     (&s_hist[0][0])[i] = 0;
   __syncthreads();
@@ -73,22 +79,28 @@ static __global__ __launch_bounds__(kThreads) void histogram_kernel(
     const u32 c = (&s_hist[0][0])[i];
     if (c) atomicAdd(&row[i], c);
   }
-}
-
-// Sums the kHistRows partial rows of pass blockIdx.x and writes the
-// exclusive scan over digits -> global digit offsets.
-static __global__ __launch_bounds__(kRadix) void scan_kernel(
-    const u32 *__restrict__ rows, u32 passes, u32 *__restrict__ offsets) {
-  __shared__ u32 scratch[kRadix / kWave + 1];
-  const u32 p = blockIdx.x;
-  u32 c[kHistRows];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // the last block: every row is complete; coherent loads of the rows
 #pragma unroll
-  for (int r = 0; r < kHistRows; ++r)
-    c[r] = rows[(u64(r) * passes + p) * kRadix + threadIdx.x];
-  u32 v = 0;
+  for (int p = 0; p < kPasses; ++p) {
+    u32 c[kHistRows];
 #pragma unroll
-  for (int r = 0; r < kHistRows; ++r) v += c[r];
-  offsets[p * kRadix + threadIdx.x] = block_exclusive_scan<kRadix>(v, scratch);
+    for (int r = 0; r < kHistRows; ++r)
+      c[r] = __hip_atomic_load(
+          &rows[(u64(r) * kPasses + p) * kRadix + threadIdx.x],
+          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    u32 v = 0;
+#pragma unroll
+    for (int r = 0; r < kHistRows; ++r) v += c[r];
+    offsets[p * kRadix + threadIdx.x] =
+        block_exclusive_scan<kRadix>(v, s_scratch);
+    __syncthreads();
+  }
 }
 
 // One digit pass.  status has tiles*kRadix words and must be zero;
@@ -224,6 +236,16 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
 }
 
+// Clears a[0..na) and b[0..nb) in one launch.
+static __global__ void zero_two_kernel(u32 *a, u64 na, u32 *b, u64 nb) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb;
+       i += stride) {
+    if (i < na) a[i] = 0;
+    else b[i - na] = 0;
+  }
+}
+
 static __global__ void zero_u32_kernel(u32 *p, u64 n) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -267,12 +289,14 @@ inline Control control_layout(u32 *control, int passes) {
   return c;
 }
 
-inline void clear_control(u32 *control, u64 n, int passes,
-                          hipStream_t stream) {
-  const u64 words = control_words(n, passes);
-  zero_u32_kernel<<<static_cast<u32>(std::min<u64>((words + 255) / 256,
-                                                   4096)),
-                    256, 0, stream>>>(control, words);
+// Clears the control words (at least control_words(n, passes); callers may
+// extend the region with their own state) and an optional second region,
+// in one launch.
+inline void clear_control(u32 *control, u64 words, hipStream_t stream,
+                          u32 *extra = nullptr, u64 extra_words = 0) {
+  const u64 total = words + extra_words;
+  zero_two_kernel<<<static_cast<u32>(std::min<u64>((total + 255) / 256, 4096)),
+                    256, 0, stream>>>(control, words, extra, extra_words);
 }
 
 template <class K, bool kFirst>
@@ -293,19 +317,24 @@ inline void launch_pass(int block, u32 tiles, const K *src_k,
 // keys and positions end in (keys_a, vals_a) (an even number of passes).
 // keys_b/vals_b are scratch.  `control` holds control_words(n, passes) u32
 // words.  All launches go to `stream`.
+// clear_words (0 = control_words(n, passes)) words of `control` and
+// extra[0..extra_words) are cleared by the first launch, so callers can
+// place zero-initialised state of their own behind the sort's.
 template <class K>
 inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
                              K *keys_b, u32 *vals_b, u32 *control,
-                             hipStream_t stream, int block = 0) {
+                             hipStream_t stream, int block = 0,
+                             u64 clear_words = 0, u32 *extra = nullptr,
+                             u64 extra_words = 0) {
   constexpr int kPasses = passes_for<K>();
   if (n == 0) return hipSuccess;
   const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
   const Control c = control_layout(control, kPasses);
-  clear_control(control, n, kPasses, stream);
+  clear_control(control, clear_words ? clear_words : control_words(n, kPasses),
+                stream, extra, extra_words);
   histogram_kernel<K, kPasses><<<hist_blocks_for(n), kThreads, 0, stream>>>(
-      keys_in, n, 0, c.rows);
-  scan_kernel<<<kPasses, kRadix, 0, stream>>>(c.rows, kPasses, c.offsets);
+      keys_in, n, 0, c.rows, c.offsets, c.tickets + 32);
   u32 *tickets = c.tickets;
   u64 *status = c.status;
   const u32 *offsets = c.offsets;
@@ -350,10 +379,9 @@ inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
   const int shape = block ? block : block_for(n);
   const u32 tiles = tiles_for(n);
   const Control c = control_layout(control, 1);
-  clear_control(control, n, 1, stream);
+  clear_control(control, control_words(n, 1), stream);
   histogram_kernel<K, 1><<<hist_blocks_for(n), kThreads, 0, stream>>>(
-      keys_in, n, shift / 8, c.rows);
-  scan_kernel<<<1, kRadix, 0, stream>>>(c.rows, 1, c.offsets);
+      keys_in, n, shift / 8, c.rows, c.offsets, c.tickets + 32);
   const u32 *offsets = c.offsets;
   u32 *tickets = c.tickets;
   u64 *status = c.status;
