@@ -7,9 +7,14 @@
 //   rows[c]  rows of (key word, value): the key word holds the low B-1 bits
 //            of the key with the tombstone flag in its top bit, the value
 //            is 0 for a tombstone (B = key bits)
-//   dir      RunMeta header (count, split, directory bits, tombstones)
-//            followed by a radix directory of 2^b + 1 u32 row offsets:
-//            dir[j] = first row whose key >= j << (B - b).
+//   dir      RunMeta header (count, split, directory bits, tombstones,
+//            bucket window) followed by a radix directory of 2^b + 1 u32
+//            row offsets: dir[j] = first row whose key >= j << (B - b),
+//            valid for j in [lo_bucket, hi_bucket + 1], the buckets of the
+//            run's first and last keys; a key below that window has
+//            lower bound 0, a key above it has lower bound count, so the
+//            empty stretches of key space below and above the run are
+//            never written.
 // The top bit of a key is implied by the row position (rows >= split have
 // it set), which frees it for the tombstone flag, so a row is exactly
 // 2 * sizeof(K) bytes like the reference's 8-byte Row (GPULSMOpt.cuh:
@@ -29,10 +34,13 @@ constexpr u32 kMaxDirBits = 24;
 
 struct RunMeta {
   u32 count;
-  u32 split;     // rows whose key has the top bit clear
-  u32 dir_bits;  // b, 1 <= b <= kMaxDirBits
-  u32 tombs;     // nonzero when the run holds tombstone rows
+  u32 split;      // rows whose key has the top bit clear
+  u32 dir_bits;   // b, 1 <= b <= kMaxDirBits
+  u32 tombs;      // nonzero when the run holds tombstone rows
+  u32 lo_bucket;  // directory window: buckets of the first and last key
+  u32 hi_bucket;  // (lo_bucket = 1, hi_bucket = 0: no rows)
 };
+constexpr u64 kMetaWords = sizeof(RunMeta) / sizeof(u32);
 constexpr u64 kMetaBytes = 256;  // header slot in front of the directory
 
 struct alignas(16) Row64 {
@@ -116,13 +124,33 @@ __host__ __device__ __forceinline__ u32 dir_bits_for(u64 count, u32 max_bits,
   return b;
 }
 
+// Directory bounds [lo, hi) of the rows that may hold key x; false when x
+// lies outside the run's bucket window (then lo = hi = its lower bound).
+template <class T>
+__device__ __forceinline__ bool dir_window(const RunView<T> &r,
+                                           const RunMeta &m, typename T::K x,
+                                           u32 &lo, u32 &hi) {
+  const u32 bucket = static_cast<u32>(x >> (T::kBits - m.dir_bits));
+  if (bucket < m.lo_bucket) {
+    lo = hi = 0;
+    return false;
+  }
+  if (bucket > m.hi_bucket) {
+    lo = hi = m.count;
+    return false;
+  }
+  lo = r.dir[bucket];
+  hi = r.dir[bucket + 1];
+  return true;
+}
+
 // First row whose key is >= x.
 template <class T>
-__device__ __forceinline__ u32 run_lower_bound(const RunView<T> &r, u32 b,
+__device__ __forceinline__ u32 run_lower_bound(const RunView<T> &r,
+                                               const RunMeta &m,
                                                typename T::K x) {
-  const u32 bucket = static_cast<u32>(x >> (T::kBits - b));
-  u32 lo = r.dir[bucket];
-  u32 hi = r.dir[bucket + 1];
+  u32 lo, hi;
+  if (!dir_window(r, m, x, lo, hi)) return lo;
   const typename T::K target = x & T::kLow;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
@@ -134,19 +162,19 @@ __device__ __forceinline__ u32 run_lower_bound(const RunView<T> &r, u32 b,
 
 // First row whose key is > x.
 template <class T>
-__device__ __forceinline__ u32 run_upper_bound(const RunView<T> &r, u32 b,
-                                               u32 count, typename T::K x) {
-  return x == T::kAllOnes ? count : run_lower_bound(r, b, x + 1);
+__device__ __forceinline__ u32 run_upper_bound(const RunView<T> &r,
+                                               const RunMeta &m,
+                                               typename T::K x) {
+  return x == T::kAllOnes ? m.count : run_lower_bound(r, m, x + 1);
 }
 
 // Point search; returns the row position or 0xffffffff (row in `row`).
 template <class T>
-__device__ __forceinline__ u32 run_find(const RunView<T> &r, u32 b,
+__device__ __forceinline__ u32 run_find(const RunView<T> &r, const RunMeta &m,
                                         typename T::K key,
                                         typename T::Row &row) {
-  const u32 bucket = static_cast<u32>(key >> (T::kBits - b));
-  u32 lo = r.dir[bucket];
-  const u32 end = r.dir[bucket + 1];
+  u32 lo, end;
+  if (!dir_window(r, m, key, lo, end)) return 0xffffffffu;
   u32 hi = end;
   const typename T::K target = key & T::kLow;
   while (lo < hi) {
@@ -261,23 +289,26 @@ __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
     low += key[j] < T::kTomb;
     tombs += tomb;
   }
-  // directory entries
+  // directory entries inside the bucket window
   const u32 shift = T::kBits - dir_bits;
 #pragma unroll
   for (int j = 0; j < kFinItems; ++j) {
     const u64 i = base + u64(j) * kFinThreads + threadIdx.x;
     if (i >= n) continue;
-    const long long bk = static_cast<long long>(key[j] >> shift);
-    const long long pbk =
-        i ? static_cast<long long>(sk[i - 1] >> shift) : -1;
+    const u32 bk = static_cast<u32>(key[j] >> shift);
     const u32 before = static_cast<u32>(out_base + s_off[j][wave] +
                                         popc64(mask[j] & lanemask_lt()));
-    for (long long b = pbk + 1; b <= bk; ++b) dir[b] = before;
+    if (i == 0) {
+      meta->lo_bucket = bk;
+      dir[bk] = 0;
+    } else {
+      for (u32 b = static_cast<u32>(sk[i - 1] >> shift) + 1; b <= bk; ++b)
+        dir[b] = before;
+    }
     if (i + 1 == n) {
-      // the last key: the buckets after it end at the row count
-      const u32 count = static_cast<u32>(out_base + tile_total);
-      for (long long b = bk + 1; b <= (1ll << dir_bits); ++b) dir[b] = count;
+      meta->hi_bucket = bk;
       meta->dir_bits = dir_bits;
+      dir[bk + 1] = static_cast<u32>(out_base + tile_total);
     }
   }
   low = wave_sum_u32(low);
@@ -468,29 +499,29 @@ __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   __syncthreads();
   const u32 b = dir_bits_for(count, max_bits, rows_log);
   const u32 shift = T::kBits - b;
-  const u32 nb = 1u << b;
   if (tid == 0) {
+    // count >= 1 (n >= 1)
+    const u32 lo_b = static_cast<u32>(dst_k[0] >> shift);
+    const u32 hi_b = static_cast<u32>(dst_k[count - 1] >> shift);
     meta->count = count;
     meta->split = s_misc[1];
     meta->tombs = s_misc[2];
     meta->dir_bits = b;
+    meta->lo_bucket = lo_b;
+    meta->hi_bucket = hi_b;
+    dir[lo_b] = 0;
+    dir[hi_b + 1] = count;
   }
-  const long long last_bucket =
-      count ? static_cast<long long>(dst_k[count - 1] >> shift) : -1;
-  const u32 items = count > nb + 1 ? count : nb + 1;
-  for (u32 t = tid; t < items; t += kSmallThreads) {
-    if (t < count) {
-      const long long bucket = static_cast<long long>(dst_k[t] >> shift);
-      const long long prev =
-          t ? static_cast<long long>(dst_k[t - 1] >> shift) : -1;
-      for (long long j = prev + 1; j <= bucket; ++j) dir[j] = t;
-    }
-    if (t <= nb && static_cast<long long>(t) > last_bucket) dir[t] = count;
+  for (u32 t = 1 + tid; t < count; t += kSmallThreads) {
+    const u32 bucket = static_cast<u32>(dst_k[t] >> shift);
+    for (u32 j = static_cast<u32>(dst_k[t - 1] >> shift) + 1; j <= bucket;
+         ++j)
+      dir[j] = t;
   }
 }
 
-// Builds the radix directory of a run from its final count/split.  Launch
-// over max(count_ub, 2^max_bits + 1) threads (grid-stride).
+// Builds the radix directory of a run from its final count/split (one
+// thread per row, grid-stride).
 template <class T>
 __global__ void build_directory_kernel(const typename T::Row *__restrict__ rows,
                                        RunMeta *meta, u32 *__restrict__ dir,
@@ -499,29 +530,34 @@ __global__ void build_directory_kernel(const typename T::Row *__restrict__ rows,
   const u32 split = meta->split;
   const u32 b = dir_bits_for(count, max_bits, rows_log);
   const u32 shift = T::kBits - b;
-  const u32 nb = 1u << b;
-  if (blockIdx.x == 0 && threadIdx.x == 0) meta->dir_bits = b;
-  const long long last_bucket =
-      count ? static_cast<long long>(
-                  decode_key<T>(T::kw(rows[count - 1]), count - 1, split) >>
-                  shift)
-            : -1;
-  const u64 items = count > u64(nb) + 1 ? u64(count) : u64(nb) + 1;
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 t = u64(blockIdx.x) * blockDim.x + threadIdx.x; t < items;
-       t += stride) {
-    if (t < count) {
-      const u32 pos = static_cast<u32>(t);
-      const long long bucket = static_cast<long long>(
-          decode_key<T>(T::kw(rows[pos]), pos, split) >> shift);
-      const long long prev =
-          pos ? static_cast<long long>(
-                    decode_key<T>(T::kw(rows[pos - 1]), pos - 1, split) >>
-                    shift)
-              : -1;
-      for (long long j = prev + 1; j <= bucket; ++j) dir[j] = pos;
+  const u64 gid = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid == 0) {
+    meta->dir_bits = b;
+    if (count == 0) {
+      meta->lo_bucket = 1;
+      meta->hi_bucket = 0;
+    } else {
+      const u32 lo_b = static_cast<u32>(
+          decode_key<T>(T::kw(rows[0]), 0, split) >> shift);
+      const u32 hi_b = static_cast<u32>(
+          decode_key<T>(T::kw(rows[count - 1]), count - 1, split) >> shift);
+      meta->lo_bucket = lo_b;
+      meta->hi_bucket = hi_b;
+      dir[lo_b] = 0;
+      dir[hi_b + 1] = count;
     }
-    if (t <= nb && static_cast<long long>(t) > last_bucket) dir[t] = count;
+  }
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 t = gid + 1; t < count; t += stride) {
+    const u32 pos = static_cast<u32>(t);
+    const u32 bucket = static_cast<u32>(
+        decode_key<T>(T::kw(rows[pos]), pos, split) >> shift);
+    for (u32 j = static_cast<u32>(
+                     decode_key<T>(T::kw(rows[pos - 1]), pos - 1, split) >>
+                     shift) +
+                 1;
+         j <= bucket; ++j)
+      dir[j] = pos;
   }
 }
 
@@ -579,6 +615,8 @@ __global__ void merge_partition_kernel(MergeSide<T> a, MergeSide<T> b,
     out_meta->split = 0;
     out_meta->dir_bits = dir_bits;
     out_meta->tombs = 0;
+    out_meta->lo_bucket = 1;  // empty until the first and last input
+    out_meta->hi_bucket = 0;  // element write their buckets
   }
   const u32 ca = a.meta->count, cb = b.meta->count;
   const u32 sa = a.meta->split, sb = b.meta->split;
@@ -657,13 +695,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   const u32 sa = a.meta->split, sb = b.meta->split;
   const u64 total = u64(ca) + cb;
   const u64 d0 = u64(tile) * kMergeTile;
-  if (d0 >= total) {  // every later ticket is empty too
-    if (total == 0 && tile == 0) {  // empty output: an all-zero directory
-      for (u32 j = threadIdx.x; j <= (1u << dir_bits); j += kMergeThreads)
-        dir[j] = 0;
-    }
-    return;
-  }
+  if (d0 >= total) return;  // every later ticket is empty too (an empty
+                            // output keeps the empty bucket window)
   const u64 d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
   const u32 i0 = parts[tile], i1 = parts[tile + 1];
   const u32 j0 = static_cast<u32>(d0 - i0), j1 = static_cast<u32>(d1 - i1);
@@ -801,24 +834,29 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   const u64 obase = s_base;
   for (u32 t = threadIdx.x; t < tile_total; t += kMergeThreads)
     out[obase + t] = t < kStageHalf ? stage_lo[t] : stage_hi[t - kStageHalf];
-  // directory entries
+  // directory entries inside the bucket window (first to last input key)
   {
     const u32 shift = T::kBits - dir_bits;
-    long long pbk = has_prev ? static_cast<long long>(prev >> shift) : -1;
+    u32 pbk = static_cast<u32>(prev >> shift);
     u32 before = static_cast<u32>(obase) + local;
 #pragma unroll
     for (int s = 0; s < kMergeItems; ++s) {
       if (static_cast<u32>(s) < steps) {
-        const long long bk = static_cast<long long>(ok[s] >> shift);
-        for (long long j = pbk + 1; j <= bk; ++j) dir[j] = before;
-        pbk = bk > pbk ? bk : pbk;
+        const u32 bk = static_cast<u32>(ok[s] >> shift);
+        if (s == 0 && !has_prev) {  // the first input element of the merge
+          out_meta->lo_bucket = bk;
+          dir[bk] = 0;
+        } else {
+          for (u32 j = pbk + 1; j <= bk; ++j) dir[j] = before;
+        }
+        pbk = bk;
         before += okeep[s];
       }
     }
     if (last_tile && steps && dk + steps == tile_len) {
-      // the last input element: the buckets after it end at the row count
-      for (long long j = pbk + 1; j <= (1ll << dir_bits); ++j)
-        dir[j] = static_cast<u32>(obase + tile_total);
+      // the last input element closes the window at the row count
+      out_meta->hi_bucket = pbk;
+      dir[pbk + 1] = static_cast<u32>(obase + tile_total);
     }
   }
   low = wave_sum_u32(low);
@@ -874,7 +912,8 @@ __device__ __forceinline__ bool lookup_one(const RunSet<T> &runs,
   for (u32 r = 0; r < runs.count; ++r) {
     const RunView<T> &rv = runs.run[r];
     typename T::Row row;
-    if (run_find(rv, rv.meta->dir_bits, key, row) != 0xffffffffu) {
+    const RunMeta m = *rv.meta;
+    if (run_find(rv, m, key, row) != 0xffffffffu) {
       value = T::val(row);
       return !(T::kw(row) & T::kTomb);
     }
@@ -999,9 +1038,9 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io, u64 n,
     // segment bounds of every run, one lane per run
     for (u32 r = lane; r < nr; r += kWave) {
       const RunView<T> &rv = runs.run[r];
-      const u32 b = rv.meta->dir_bits, c = rv.meta->count;
-      s_lo[wave][r] = run_lower_bound(rv, b, a);
-      s_hi[wave][r] = run_upper_bound(rv, b, c, z);
+      const RunMeta m = *rv.meta;
+      s_lo[wave][r] = run_lower_bound(rv, m, a);
+      s_hi[wave][r] = run_upper_bound(rv, m, z);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1067,10 +1106,10 @@ __global__ void successor_kernel(const typename T::K *__restrict__ queries,
       K best = 0;
       for (u32 r = 0; r < runs.count; ++r) {
         const RunView<T> &rv = runs.run[r];
-        const u32 b = rv.meta->dir_bits, c = rv.meta->count;
-        const u32 p = run_lower_bound(rv, b, lower);
-        if (p < c) {
-          const K k = decode_key<T>(T::kw(rv.rows[p]), p, rv.meta->split);
+        const RunMeta m = *rv.meta;
+        const u32 p = run_lower_bound(rv, m, lower);
+        if (p < m.count) {
+          const K k = decode_key<T>(T::kw(rv.rows[p]), p, m.split);
           if (!any || k < best) best = k;
           any = true;
         }
@@ -1081,7 +1120,7 @@ __global__ void successor_kernel(const typename T::K *__restrict__ queries,
       for (u32 r = 0; r < runs.count; ++r) {
         const RunView<T> &rv = runs.run[r];
         typename T::Row row;
-        if (run_find(rv, rv.meta->dir_bits, best, row) != 0xffffffffu) {
+        if (run_find(rv, *rv.meta, best, row) != 0xffffffffu) {
           live = !(T::kw(row) & T::kTomb);
           break;
         }

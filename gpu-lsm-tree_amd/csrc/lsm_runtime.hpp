@@ -755,7 +755,7 @@ class LsmT : public HandleBase {
     GD_HIP(rs::sort_pairs<K>(keys, n, ka.get(), va.get(), kb.get(), vb.get(),
                              ctrl.get(), s, tuning_.sort_block,
                              sort_words + fin_words,
-                             reinterpret_cast<u32 *>(r.meta()), 4));
+                             reinterpret_cast<u32 *>(r.meta()), kMetaWords));
     finalize_batch_kernel<T><<<fin_tiles, kFinThreads, 0, s>>>(
         ka.get(), va.get(), n, values, flags, mode, r.rows, r.meta(),
         r.entries(), r.max_bits, reinterpret_cast<u64 *>(fin_ctrl + 64),
@@ -819,8 +819,8 @@ class LsmT : public HandleBase {
 
   Run empty_run(hipStream_t s) {
     Run r = alloc_run(0, pool_for(s));
-    zero_two_kernel<<<1, 256, 0, s>>>(reinterpret_cast<u32 *>(r.meta()), 4,
-                                      r.entries(), 3);
+    zero_two_kernel<<<1, 256, 0, s>>>(reinterpret_cast<u32 *>(r.meta()),
+                                      kMetaWords, r.entries(), 3);
     GD_LAUNCH_CHECK();
     launch_directory(r, s);
     return r;
