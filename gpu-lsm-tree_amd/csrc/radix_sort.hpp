@@ -29,9 +29,10 @@ constexpr int passes_for() {
 // Digit histograms.  A block counts 16 keys per thread per round in LDS
 // (the 16 loads are issued together), then adds its counts into one of
 // kHistRows partial rows (row = block % kHistRows), so at most
-// blocks / kHistRows atomics meet on one address.  The last block to finish
-// (a done counter) sums the rows and writes the exclusive digit offsets, so
-// no separate scan launch is needed.
+// blocks / kHistRows atomics meet on one address; scan_kernel sums the rows.
+// (Folding the scan into the histogram's last block through a done counter
+// was measured slower on MI355X: every block's fence and the atomics of all
+// blocks on one counter cost more than the extra launch.)
 constexpr int kHistItems = 16;
 constexpr int kHistRows = 32;
 
@@ -41,15 +42,11 @@ inline u32 hist_blocks_for(u64 n) {
 }
 
 // rows[kHistRows][kPasses][kRadix] (zeroed) += counts of digits
-// first .. first + kPasses - 1, digit p = (key >> 8p) & 0xff; then
-// offsets[p][d] = exclusive scan over d of the summed rows.  *done is zero.
+// first .. first + kPasses - 1, digit p = (key >> 8p) & 0xff.
 template <class K, int kPasses>
 static __global__ __launch_bounds__(kThreads) void histogram_kernel(
-    const K *__restrict__ keys, u32 n, u32 first, u32 *__restrict__ rows,
-    u32 *__restrict__ offsets, u32 *done) {
+    const K *__restrict__ keys, u32 n, u32 first, u32 *__restrict__ rows) {
   __shared__ u32 s_hist[kPasses][kRadix];
-  __shared__ u32 s_last;
-  __shared__ u32 s_scratch[kRadix / kWave + 1];
   for (int i = threadIdx.x; i < kPasses * kRadix; i += kThreads)
     (&s_hist[0][0])[i] = 0;
   __syncthreads();
@@ -79,28 +76,22 @@ static __global__ __launch_bounds__(kThreads) void histogram_kernel(
     const u32 c = (&s_hist[0][0])[i];
     if (c) atomicAdd(&row[i], c);
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // the last block: every row is complete; coherent loads of the rows
+}
+
+// Sums the kHistRows partial rows of pass blockIdx.x and writes the
+// exclusive scan over digits -> global digit offsets.
+static __global__ __launch_bounds__(kRadix) void scan_kernel(
+    const u32 *__restrict__ rows, u32 passes, u32 *__restrict__ offsets) {
+  __shared__ u32 scratch[kRadix / kWave + 1];
+  const u32 p = blockIdx.x;
+  u32 c[kHistRows];
 #pragma unroll
-  for (int p = 0; p < kPasses; ++p) {
-    u32 c[kHistRows];
+  for (int r = 0; r < kHistRows; ++r)
+    c[r] = rows[(u64(r) * passes + p) * kRadix + threadIdx.x];
+  u32 v = 0;
 #pragma unroll
-    for (int r = 0; r < kHistRows; ++r)
-      c[r] = __hip_atomic_load(
-          &rows[(u64(r) * kPasses + p) * kRadix + threadIdx.x],
-          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    u32 v = 0;
-#pragma unroll
-    for (int r = 0; r < kHistRows; ++r) v += c[r];
-    offsets[p * kRadix + threadIdx.x] =
-        block_exclusive_scan<kRadix>(v, s_scratch);
-    __syncthreads();
-  }
+  for (int r = 0; r < kHistRows; ++r) v += c[r];
+  offsets[p * kRadix + threadIdx.x] = block_exclusive_scan<kRadix>(v, scratch);
 }
 
 // One digit pass.  status has tiles*kRadix words and must be zero;
@@ -334,7 +325,8 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   clear_control(control, clear_words ? clear_words : control_words(n, kPasses),
                 stream, extra, extra_words);
   histogram_kernel<K, kPasses><<<hist_blocks_for(n), kThreads, 0, stream>>>(
-      keys_in, n, 0, c.rows, c.offsets, c.tickets + 32);
+      keys_in, n, 0, c.rows);
+  scan_kernel<<<kPasses, kRadix, 0, stream>>>(c.rows, kPasses, c.offsets);
   u32 *tickets = c.tickets;
   u64 *status = c.status;
   const u32 *offsets = c.offsets;
@@ -381,7 +373,8 @@ inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
   const Control c = control_layout(control, 1);
   clear_control(control, control_words(n, 1), stream);
   histogram_kernel<K, 1><<<hist_blocks_for(n), kThreads, 0, stream>>>(
-      keys_in, n, shift / 8, c.rows, c.offsets, c.tickets + 32);
+      keys_in, n, shift / 8, c.rows);
+  scan_kernel<<<1, kRadix, 0, stream>>>(c.rows, 1, c.offsets);
   const u32 *offsets = c.offsets;
   u32 *tickets = c.tickets;
   u64 *status = c.status;
