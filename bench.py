@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--serve-rate-x", type=float, default=2.0,
                     help="serving arrival rate as a multiple of the plain "
                          "path's request rate")
+    ap.add_argument("--lsm-tuning", default="",
+                    help="JSON dict of gd_lsm_tuning knobs for the config-2 "
+                         "dictionary (default: the built-in defaults)")
     ap.add_argument("--c5-scale-log", type=int, default=-1,
                     help="run the config-5 serving sweep on a 2^(31-s)-key "
                          "uint64 index (-1 = skip)")
@@ -566,7 +569,9 @@ def main():
 
     n_upd = len(keys)
     n_ops = n_upd + len(queries) + len(lo)
-    d = pkg.GPULSMOpt(max_elements=w.n_keys, batch_capacity=w.batch)
+    d = pkg.GPULSMOpt(max_elements=w.n_keys, batch_capacity=w.batch,
+                      tuning=json.loads(args.lsm_tuning) if args.lsm_tuning
+                      else None)
     s_ptr = stream.cuda_stream
 
     def ev():
@@ -692,6 +697,7 @@ def main():
             "erase_fraction": 0.1, "lookups": int(Q),
             "lookup_hit_draw": 0.7, "range_counts": int(R),
             "range_width": [w.wmin, w.wmax], "scale_log": args.scale_log,
+            "lsm_tuning": d.get_tuning(),
         },
         "latency_ms": {"p50": round(float(np.percentile(lat_arr, 50)), 4),
                        "p99": round(float(np.percentile(lat_arr, 99)), 4),

@@ -103,12 +103,14 @@ class _Tuning(ctypes.Structure):
         ("grouped_lookup_min_log", ctypes.c_uint32),
         ("small_batch_max", ctypes.c_uint32),
         ("max_runs", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("segmented_admission", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
 TUNING_FIELDS = ("dir_rows_log", "merge_tile_log", "sort_block",
-                 "grouped_lookup_min_log", "small_batch_max", "max_runs")
+                 "grouped_lookup_min_log", "small_batch_max", "max_runs",
+                 "segmented_admission")
 
 
 def _tuning(d):

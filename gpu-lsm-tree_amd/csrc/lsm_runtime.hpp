@@ -27,6 +27,7 @@
 #include "host_common.hpp"
 #include "lsm_kernels.hpp"
 #include "radix_sort.hpp"
+#include "seg_admission.hpp"
 
 namespace gdict {
 namespace lsm {
@@ -541,6 +542,11 @@ class LsmT : public HandleBase {
     if (t.max_runs)  // max_runs
       v.max_runs = std::max<u32>(2, std::min<u32>(t.max_runs,
                                                      kMaxRuns - 1));
+    if (t.segmented_admission) {
+      if (t.segmented_admission > 2)
+        throw Error(GD_INVALID_ARGUMENT, "segmented_admission must be 1 or 2");
+      v.segmented_admission = t.segmented_admission;
+    }
     tuning_ = v;
   }
   const gd_lsm_tuning &tuning() const { return tuning_; }
@@ -743,6 +749,10 @@ class LsmT : public HandleBase {
       GD_LAUNCH_CHECK();
       return r;
     }
+    if constexpr (sizeof(K) == 4) {
+      if (tuning_.segmented_admission == 1 && n > 16384)
+        return segmented_run(keys, values, flags, n, mode, s, r);
+    }
     const u32 fin_tiles = (n + kFinTile - 1) / kFinTile;
     const u64 sort_words = rs::control_words(n, rs::passes_for<K>());
     const u64 fin_words = 64 + 2ull * fin_tiles;  // ticket pad + status u64
@@ -760,6 +770,40 @@ class LsmT : public HandleBase {
         ka.get(), va.get(), n, values, flags, mode, r.rows, r.meta(),
         r.entries(), r.max_bits, reinterpret_cast<u64 *>(fin_ctrl + 64),
         fin_ctrl);
+    GD_LAUNCH_CHECK();
+    return r;
+  }
+
+  // 32-bit batches: partition by the top varying key byte, then one
+  // workgroup per segment sorts, dedupes, encodes and indexes it.
+  Run segmented_run(const K *keys, const V *values, const u8 *flags, u32 n,
+                    int mode, hipStream_t s, Run r) {
+    const u64 part_words = rs::control_words(n, 1);
+    // [partition control | span (64) | shift, pad | status (256 x u64) |
+    //  ticket, pad]
+    const u64 span_at = part_words + (part_words & 1);
+    const u64 shift_at = span_at + kSpanBlocks;
+    const u64 status_at = shift_at + 2;
+    const u64 ticket_at = status_at + 2 * 256;
+    const u64 words = ticket_at + 2;
+    Lease<u32> ctrl(pool_, words);
+    Lease<u32> ka(pool_, n), va(pool_, n), kb(pool_, n), vb(pool_, n);
+    u32 *c = ctrl.get();
+    const rs::Control pc = rs::control_layout(c, 1);
+    rs::clear_control(c, words, s, reinterpret_cast<u32 *>(r.meta()),
+                      kMetaWords);
+    keyspan_kernel<<<kSpanBlocks, 256, 0, s>>>(keys, n, c + span_at);
+    seg_hist_kernel<<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
+        keys, n, c + span_at, pc.rows, c + shift_at);
+    rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
+    rs::launch_pass<u32, true>(
+        tuning_.sort_block ? tuning_.sort_block : rs::block_for(n),
+        rs::tiles_for(n), keys, nullptr, ka.get(), va.get(), n, 0,
+        pc.offsets, pc.status, pc.tickets, s, nullptr, c + shift_at);
+    seg_sort_kernel<T><<<256, kSegThreads, 0, s>>>(
+        keys, ka.get(), va.get(), kb.get(), vb.get(), n, pc.offsets,
+        c + shift_at, values, flags, mode, r.rows, r.meta(), r.entries(),
+        r.max_bits, reinterpret_cast<u64 *>(c + status_at), c + ticket_at);
     GD_LAUNCH_CHECK();
     return r;
   }
@@ -950,7 +994,7 @@ class LsmT : public HandleBase {
   u64 batch_capacity_ = 1;
   u64 capacity_ = 0;
   gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(T::kSmallTile), 10,
-                        {0, 0}};
+                        2, 0};
 
   struct ProfRec {
     int kind;  // 0 merge, 1 lookup

@@ -102,9 +102,12 @@ template <class K, bool kFirst, int kItems, int kBlock>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K *__restrict__ keys_in, const u32 *__restrict__ vals_in,
     K *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
-    u32 shift, const u32 *__restrict__ digit_offsets, u64 *status,
-    u32 *ticket, u32 *__restrict__ dest_out) {
+    u32 shift_arg, const u32 *__restrict__ digit_offsets, u64 *status,
+    u32 *ticket, u32 *__restrict__ dest_out, const u32 *shift_dev) {
   constexpr int kTile = kBlock * kItems;
+  // the digit shift: an argument, or computed on the device by an earlier
+  // launch
+  const u32 shift = shift_dev ? *shift_dev : shift_arg;
   constexpr int kBlockWaves = kBlock / kWave;
   __shared__ u32 s_tile;
   __shared__ u32 s_wave_hist[kBlockWaves][kRadix];
@@ -295,13 +298,16 @@ inline void launch_pass(int block, u32 tiles, const K *src_k,
                         const u32 *src_v, K *dst_k, u32 *dst_v, u32 n,
                         u32 shift, const u32 *offsets, u64 *status,
                         u32 *ticket, hipStream_t stream,
-                        u32 *dest = nullptr) {
+                        u32 *dest = nullptr,
+                        const u32 *shift_dev = nullptr) {
   if (block == 512)
     onesweep_kernel<K, kFirst, 8, 512><<<tiles, 512, 0, stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest);
+        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest,
+        shift_dev);
   else
     onesweep_kernel<K, kFirst, 16, 256><<<tiles, 256, 0, stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest);
+        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest,
+        shift_dev);
 }
 
 // Sorts n (key, position) pairs stably.  keys_in is read only; the sorted

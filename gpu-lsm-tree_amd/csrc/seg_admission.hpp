@@ -1,0 +1,439 @@
+// Segmented admission of a 32-bit batch: the batch becomes a run in two
+// sorting launches instead of four radix passes and a finalize launch.
+//
+//   keyspan_kernel    per-block OR of (key ^ keys[0]): the bits that vary
+//   seg_hist_kernel   s0 = (highest varying bit) - 7; histogram of the
+//                     digit (key >> s0) & 0xff, so the 256 segments split
+//                     the batch's actual key spread, not the key space
+//   scan_kernel       segment starts
+//   onesweep pass     stable partition by that digit (positions as payload)
+//   seg_sort_kernel   one workgroup per segment, in key order (ticket =
+//                     segment): an LDS LSD sort of the bits below s0, the
+//                     newest-wins dedupe, value / tombstone gather, row
+//                     encoding, the chained scan for the output offset, the
+//                     radix-directory entries of the segment's key range
+//                     and the run header.
+//
+// A segment holds up to kSegTile keys in LDS; a larger one (heavy
+// duplication: a hot key) is sorted by its workgroup in global memory
+// (stable LSD, chunk by chunk), which is slower but exact.
+#pragma once
+
+#include "lsm_kernels.hpp"
+#include "radix_sort.hpp"
+
+namespace gdict {
+namespace lsm {
+
+constexpr int kSegThreads = 512;
+constexpr int kSegItems = 16;
+constexpr int kSegTile = kSegThreads * kSegItems;  // 8192
+constexpr int kSegWaves = kSegThreads / kWave;
+constexpr int kSpanBlocks = 256;
+
+// span[blockIdx.x] = OR over the block's keys of (key ^ keys[0]); 16 keys
+// per thread per round, loaded together.
+static __global__ __launch_bounds__(256) void keyspan_kernel(
+    const u32 *__restrict__ keys, u32 n, u32 *__restrict__ span) {
+  __shared__ u32 s_or[256 / kWave];
+  const u32 k0 = keys[0];
+  u32 acc = 0;
+  const u64 stride = u64(gridDim.x) * 256 * 16;
+  for (u64 base = u64(blockIdx.x) * 256 * 16; base < n; base += stride) {
+    u32 k[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const u64 i = base + u64(j) * 256 + threadIdx.x;
+      k[j] = i < n ? keys[i] : k0;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc |= k[j] ^ k0;
+  }
+#pragma unroll
+  for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    acc |= __shfl_xor(acc, offset, kWave);
+  if (lane_id() == 0) s_or[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 v = 0;
+    for (int w = 0; w < 256 / kWave; ++w) v |= s_or[w];
+    span[blockIdx.x] = v;
+  }
+}
+
+// Digit shift from the key spread: the top 8 varying bits.
+__device__ __forceinline__ u32 seg_shift_from(const u32 *span) {
+  // every thread folds the block results (one wave-wide OR per 64)
+  u32 diff = 0;
+  for (int b = static_cast<int>(lane_id()); b < kSpanBlocks; b += kWave)
+    diff |= span[b];
+#pragma unroll
+  for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    diff |= __shfl_xor(diff, offset, kWave);
+  const u32 h = diff ? 31u - static_cast<u32>(__clz(diff)) : 0u;
+  return h >= 7 ? h - 7 : 0u;
+}
+
+// Histogram of the segment digit into kHistRows partial rows (zeroed);
+// block 0 also stores the digit shift for the later launches.
+static __global__ __launch_bounds__(rs::kThreads) void seg_hist_kernel(
+    const u32 *__restrict__ keys, u32 n, const u32 *__restrict__ span,
+    u32 *__restrict__ rows, u32 *shift_out) {
+  __shared__ u32 s_hist[rs::kRadix];
+  const u32 s0 = seg_shift_from(span);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *shift_out = s0;
+  for (int i = threadIdx.x; i < rs::kRadix; i += rs::kThreads) s_hist[i] = 0;
+  __syncthreads();
+  const u64 round = u64(rs::kThreads) * rs::kHistItems;
+  for (u64 base = u64(blockIdx.x) * round; base < n;
+       base += u64(gridDim.x) * round) {
+    u32 k[rs::kHistItems];
+#pragma unroll
+    for (int j = 0; j < rs::kHistItems; ++j) {
+      const u64 i = base + u64(j) * rs::kThreads + threadIdx.x;
+      k[j] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < rs::kHistItems; ++j)
+      if (base + u64(j) * rs::kThreads + threadIdx.x < n)
+        atomicAdd(&s_hist[(k[j] >> s0) & 0xffu], 1u);
+  }
+  __syncthreads();
+  u32 *row = rows + u64(blockIdx.x % rs::kHistRows) * rs::kRadix;
+  for (int i = threadIdx.x; i < rs::kRadix; i += rs::kThreads) {
+    const u32 c = s_hist[i];
+    if (c) atomicAdd(&row[i], c);
+  }
+}
+
+// Shared state of seg_sort_kernel.
+struct SegShared {
+  u32 k[2][kSegTile];
+  u32 p[2][kSegTile];
+  u32 wave_hist[kSegWaves][256];
+  u32 digit_start[256];  // chunk-local start of each digit
+  u32 run[256];          // running global offsets (global-memory sort)
+  u32 scan[kSegWaves + 1];
+  u32 off[kSegItems][kSegWaves];
+  u32 misc[8];
+  u64 base;
+};
+
+// Stable ranking of one chunk of up to kSegTile elements (wave w holds
+// chunk elements [w*1024, (w+1)*1024), item j of lane l is w*1024 + j*64 +
+// l): on return dst_local[j] is each item's position in the chunk sorted
+// by its digit, and sh.digit_start / sh.wave_hist hold the digit layout
+// (count of digit d = digit_start[d+1] - digit_start[d]).
+__device__ __forceinline__ void seg_rank(SegShared &sh, const u32 (&key)[kSegItems],
+                                         u32 count, u32 shift,
+                                         u32 (&dst_local)[kSegItems]) {
+  const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  for (u32 i = tid; i < kSegWaves * 256; i += kSegThreads)
+    (&sh.wave_hist[0][0])[i] = 0;
+  __syncthreads();
+  volatile u32 *my_hist = sh.wave_hist[wave];
+  u32 rank[kSegItems];
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
+    const bool valid = idx < count;
+    const u32 d = (key[j] >> shift) & 0xffu;
+    u64 match = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const u64 vote = ballot(bit);
+      match &= bit ? vote : ~vote;
+    }
+    u32 prior = 0;
+    if (valid) prior = my_hist[d];
+    rank[j] = prior + popc64(match & lanemask_lt());
+    if (valid && (match >> lane) == 1ull) my_hist[d] = prior + popc64(match);
+  }
+  __syncthreads();
+  // per digit: wave prefixes, then digit starts (threads 0..255)
+  u32 total = 0;
+  if (tid < 256) {
+    for (int w = 0; w < kSegWaves; ++w) {
+      const u32 c = sh.wave_hist[w][tid];
+      sh.wave_hist[w][tid] = total;
+      total += c;
+    }
+  }
+  const u32 start = block_exclusive_scan<kSegThreads>(tid < 256 ? total : 0u,
+                                                      sh.scan);
+  if (tid < 256) sh.digit_start[tid] = start;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 d = (key[j] >> shift) & 0xffu;
+    dst_local[j] = sh.digit_start[d] + sh.wave_hist[wave][d] + rank[j];
+  }
+}
+
+// One workgroup per segment (ticket order = key order).  keys_in: the
+// batch (only its first key is read, for the shared high bits); pk/pp: the
+// partitioned keys and input positions; tk/tp: scratch of n entries for
+// segments too large for LDS.  offsets: segment starts (exclusive scan of
+// the segment histogram).  Writes the run (rows, header, directory).
+template <class T>
+__global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
+    const u32 *__restrict__ keys_in, u32 *pk, u32 *pp, u32 *tk, u32 *tp,
+    u32 n, const u32 *__restrict__ offsets,
+    const u32 *shift_in, const u32 *__restrict__ values,
+    const u8 *__restrict__ erase_flags, int mode,
+    typename T::Row *__restrict__ out, RunMeta *meta, u32 *__restrict__ dir,
+    u32 dir_bits, u64 *status, u32 *ticket) {
+  static_assert(sizeof(typename T::K) == 4, "32-bit keys");
+  __shared__ SegShared sh;
+  __shared__ u32 s_ticket;
+  const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const u32 seg = s_ticket;  // grid = 256: one ticket per segment
+  const u32 s0 = *shift_in;
+  const u32 start = offsets[seg];
+  const u32 end = seg == 255 ? n : offsets[seg + 1];
+  const u32 m = end - start;
+  const u32 passes = (s0 + 7) / 8;
+
+  // ---------------------------------------------------------------- sort
+  const u32 *sk;  // sorted keys / positions of the segment (LDS or global)
+  const u32 *sp;
+  if (m <= static_cast<u32>(kSegTile)) {
+    for (u32 i = tid; i < m; i += kSegThreads) {
+      sh.k[0][i] = pk[start + i];
+      sh.p[0][i] = pp[start + i];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (u32 pass = 0; pass < passes; ++pass) {
+      u32 key[kSegItems], pos[kSegItems], dst[kSegItems];
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
+        key[j] = idx < m ? sh.k[cur][idx] : 0xffffffffu;
+        pos[j] = idx < m ? sh.p[cur][idx] : 0u;
+      }
+      seg_rank(sh, key, m, 8 * pass, dst);
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
+        if (idx < m) {
+          sh.k[cur ^ 1][dst[j]] = key[j];
+          sh.p[cur ^ 1][dst[j]] = pos[j];
+        }
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
+    sk = sh.k[cur];
+    sp = sh.p[cur];
+  } else {
+    // stable LSD in global memory, chunk by chunk, ping-pong pk <-> tk
+    u32 *src_k = pk + start, *src_p = pp + start;
+    u32 *dst_k = tk + start, *dst_p = tp + start;
+    for (u32 pass = 0; pass < passes; ++pass) {
+      const u32 shift = 8 * pass;
+      for (u32 i = tid; i < 256; i += kSegThreads) sh.run[i] = 0;
+      __syncthreads();
+      for (u32 i = tid; i < m; i += kSegThreads)
+        atomicAdd(&sh.run[(src_k[i] >> shift) & 0xffu], 1u);
+      __syncthreads();
+      {
+        const u32 v = tid < 256 ? sh.run[tid] : 0u;
+        const u32 ex = block_exclusive_scan<kSegThreads>(v, sh.scan);
+        if (tid < 256) sh.run[tid] = ex;
+        __syncthreads();
+      }
+      for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
+        const u32 cnt = m - c0 < u32(kSegTile) ? m - c0 : u32(kSegTile);
+        u32 key[kSegItems], pos[kSegItems], dst[kSegItems];
+#pragma unroll
+        for (int j = 0; j < kSegItems; ++j) {
+          const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
+          key[j] = idx < cnt ? src_k[c0 + idx] : 0xffffffffu;
+          pos[j] = idx < cnt ? src_p[c0 + idx] : 0u;
+        }
+        seg_rank(sh, key, cnt, shift, dst);
+#pragma unroll
+        for (int j = 0; j < kSegItems; ++j) {
+          const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
+          if (idx < cnt) {
+            const u32 d = (key[j] >> shift) & 0xffu;
+            const u32 o = sh.run[d] + dst[j] - sh.digit_start[d];
+            dst_k[o] = key[j];
+            dst_p[o] = pos[j];
+          }
+        }
+        __syncthreads();
+        // advance the running offsets by this chunk's digit counts
+        if (tid < 256) {
+          const u32 next = tid < 255 ? sh.digit_start[tid + 1] : cnt;
+          sh.run[tid] += next - sh.digit_start[tid];
+        }
+        __syncthreads();
+      }
+      u32 *t0 = src_k, *t1 = src_p;
+      src_k = dst_k;
+      src_p = dst_p;
+      dst_k = t0;
+      dst_p = t1;
+      __threadfence_block();
+      __syncthreads();
+    }
+    sk = src_k;
+    sp = src_p;
+  }
+
+  // ------------------------------------------- survivors: count, offset
+  // keep the last occurrence of each key (stable sort: the newest update)
+  u32 total = 0;
+  for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
+    u32 cnt = 0;
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const u32 i = c0 + j * kSegThreads + tid;
+      cnt += i < m && (i + 1 == m || sk[i + 1] != sk[i]);
+    }
+    total += wave_sum_u32(cnt);
+  }
+  if (lane == 0) sh.scan[wave] = total;
+  __syncthreads();
+  if (tid == 0) {
+    u32 t = 0;
+    for (int w = 0; w < kSegWaves; ++w) t += sh.scan[w];
+    sh.misc[0] = t;
+  }
+  __syncthreads();
+  const u32 seg_total = sh.misc[0];
+  if (tid < kWave) {
+    const u64 ex = wave_lookback(status, seg, seg_total);
+    if (tid == 0) sh.base = ex;
+  }
+  if (tid == 0) {
+    sh.misc[1] = 0;  // survivors below the top key bit
+    sh.misc[2] = 0;  // tombstones
+  }
+  __syncthreads();
+  const u64 base = sh.base;
+
+  // ------------------------------------------- rows and directory entries
+  const u32 shift = 32u - dir_bits;
+  u32 low = 0, tombs = 0;
+  u32 running = 0;  // survivors of earlier chunks
+  for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
+    u64 mask[kSegItems];
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const u32 i = c0 + j * kSegThreads + tid;
+      const bool keep = i < m && (i + 1 == m || sk[i + 1] != sk[i]);
+      mask[j] = ballot(keep);
+      if (lane == 0) sh.off[j][wave] = popc64(mask[j]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      u32 r = running;
+      for (int j = 0; j < kSegItems; ++j)
+        for (int w = 0; w < kSegWaves; ++w) {
+          const u32 c = sh.off[j][w];
+          sh.off[j][w] = r;
+          r += c;
+        }
+      sh.misc[3] = r;
+    }
+    __syncthreads();
+    // the gathers of all items are issued before any row is written, so
+    // their latencies overlap
+    u32 k[kSegItems], prev[kSegItems], pos[kSegItems];
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const u32 i = c0 + j * kSegThreads + tid;
+      k[j] = i < m ? sk[i] : 0u;
+      prev[j] = (i < m && i > 0) ? sk[i - 1] : k[j];
+      pos[j] = (i < m && ((mask[j] >> lane) & 1ull)) ? sp[i] : 0u;
+    }
+    u32 val[kSegItems];
+    u8 fl[kSegItems];
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const bool kept = (mask[j] >> lane) & 1ull;
+      fl[j] = (kept && mode == kModeMixed) ? erase_flags[pos[j]] : u8(0);
+      val[j] = (kept && mode != kModeErase) ? values[pos[j]] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const u32 i = c0 + j * kSegThreads + tid;
+      if (i >= m) continue;
+      // survivors before position i: the output row of key k
+      const u32 before = static_cast<u32>(base) + sh.off[j][wave] +
+                         popc64(mask[j] & lanemask_lt());
+      if (prev[j] != k[j]) {
+        for (u32 b = (prev[j] >> shift) + 1; b <= (k[j] >> shift); ++b)
+          dir[b] = before;
+      }
+      if ((mask[j] >> lane) & 1ull) {
+        const bool tomb = mode == kModeErase || fl[j] != 0;
+        out[before] = T::make((k[j] & T::kLow) | (tomb ? T::kTomb : 0u),
+                              tomb ? 0u : val[j]);
+        low += k[j] < T::kTomb;
+        tombs += tomb;
+      }
+    }
+    running = sh.misc[3];
+    __syncthreads();
+  }
+  low = wave_sum_u32(low);
+  tombs = wave_sum_u32(tombs);
+  if (lane == 0) {
+    if (low) atomicAdd(&sh.misc[1], low);
+    if (tombs) atomicAdd(&sh.misc[2], tombs);
+  }
+  __syncthreads();
+
+  // the segment's key range [lo_key, hi_key) and its directory bounds:
+  // bounds below its first key and above its last key (all of them when
+  // it is empty) take the rows before / after the segment
+  const u64 unit = u64(1) << s0;
+  // every key of the batch shares the bits at and above s0 + 8
+  const u64 common =
+      s0 + 8 >= 32 ? 0 : (u64(keys_in[0]) >> (s0 + 8)) << (s0 + 8);
+  const u64 lo_key = common + u64(seg) * unit;
+  const u64 hi_key = lo_key + unit;  // exclusive, may be 2^32
+  const u64 ub = u64(1) << shift;    // bucket width
+  const u64 first_b = (lo_key + ub - 1) >> shift;  // first bound >= lo_key
+  const u64 end_b = (hi_key + ub - 1) >> shift;    // first bound >= hi_key
+  const u32 seg_end_row = static_cast<u32>(base) + seg_total;
+  if (m == 0) {
+    for (u64 b = first_b + tid; b < end_b; b += kSegThreads)
+      dir[b] = static_cast<u32>(base);
+  } else {
+    const u64 k0b = u64(sk[0]) >> shift;  // bounds in [lo_key, sk[0]]
+    for (u64 b = first_b + tid; b <= k0b; b += kSegThreads)
+      dir[b] = static_cast<u32>(base);
+    const u64 klb = (u64(sk[m - 1]) >> shift) + 1;  // bounds after the last
+    for (u64 b = klb + tid; b < end_b; b += kSegThreads) dir[b] = seg_end_row;
+  }
+
+  if (tid == 0) {
+    if (seg == 255) meta->count = seg_end_row;
+    const u64 tb = T::kTomb;
+    if ((lo_key <= tb && tb < hi_key) || (seg == 0 && tb < lo_key) ||
+        (seg == 255 && hi_key <= tb))
+      meta->split = static_cast<u32>(base) + sh.misc[1];
+    if (sh.misc[2]) meta->tombs = 1;
+    if (seg == 0) meta->dir_bits = dir_bits;
+    if (m && start == 0) {  // the first keys of the run
+      meta->lo_bucket = sk[0] >> shift;
+      dir[sk[0] >> shift] = 0;
+    }
+    if (m && end == n) {  // the last keys of the run
+      const u32 hb = sk[m - 1] >> shift;
+      meta->hi_bucket = hb;
+      dir[hb + 1] = seg_end_row;
+    }
+  }
+}
+
+}  // namespace lsm
+}  // namespace gdict

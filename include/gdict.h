@@ -157,7 +157,10 @@ typedef struct gd_lsm_tuning {
                                       this many keys, <= 4096 (4096)       */
   uint32_t max_runs;               /* runs kept before the smallest adjacent
                                       pair merges, 2..47 (10)              */
-  uint32_t reserved[2];
+  uint32_t segmented_admission;    /* 32-bit batches above 16384 keys: 1 =
+                                      partition + per-segment LDS sort, 2 =
+                                      radix passes + finalize; 0 = keep    */
+  uint32_t reserved;
 } gd_lsm_tuning;
 
 gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning);

@@ -389,6 +389,7 @@ def test_config2_scaled(gd, orc, io):
     {"dir_rows_log": 5, "grouped_lookup_min_log": 8, "small_batch_max": 1},
     {"max_runs": 2, "small_batch_max": 1024, "sort_block": 512},
     {"max_runs": 40, "grouped_lookup_min_log": 63, "merge_tile_log": 12},
+    {"segmented_admission": 1, "dir_rows_log": 2, "small_batch_max": 1024},
 ])
 def test_tuning_knobs_keep_results(gd, orc, io, knobs):
     """Launch-shape knobs change speed only: a mixed stream with small and
@@ -419,7 +420,7 @@ def test_tuning_knobs_keep_results(gd, orc, io, knobs):
 
 def test_invalid_tuning_rejected(gd):
     for bad in ({"sort_block": 384}, {"merge_tile_log": 13},
-                {"no_such_knob": 1}):
+                {"segmented_admission": 3}, {"no_such_knob": 1}):
         with pytest.raises(gd.InvalidArgument):
             gd.GPULSMOpt(max_elements=1 << 16, batch_capacity=1 << 10,
                          tuning=bad)
@@ -502,3 +503,44 @@ def test_snapshot_save_restore(gd, orc, io, tmp_path):
     small = gd.GPULSMOpt(max_elements=1 << 10, batch_capacity=16)
     with pytest.raises(gd.OutOfMemory):
         small.restore(path)
+
+
+@pytest.mark.parametrize("span,dup", [(1 << 12, False), (1 << 20, False),
+                                      (1 << 32, False), (1 << 32, True),
+                                      (1 << 31, True)])
+def test_segmented_admission(gd, orc, io, span, dup):
+    """The partition + per-segment LDS sort admission (32-bit batches above
+    16384 keys) against the oracle: narrow key spreads (few varying bits),
+    the full key space, and hot keys whose segment exceeds the LDS tile
+    (sorted in global memory by its workgroup)."""
+    rng = np.random.default_rng(span % 1000 + dup)
+    p = Pair(gd, orc, io, tuning={"segmented_admission": 1},
+             max_elements=1 << 21, batch_capacity=1 << 17)
+    assert p.d.get_tuning()["segmented_admission"] == 1
+    for b in range(10):
+        n = int(rng.integers(16385, 1 << 17))
+        keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+        if dup:  # ~40% of the batch on a handful of hot keys
+            hot = rng.integers(0, span, 5, dtype=np.uint64).astype(np.uint32)
+            sel = rng.random(n) < 0.4
+            keys[sel] = hot[rng.integers(0, 5, int(sel.sum()))]
+        vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        kind = b % 3
+        if kind == 0:
+            p.insert(keys, vals)
+        elif kind == 1:
+            live = p.o.dump()[0]
+            p.erase(np.concatenate([live[rng.integers(0, len(live), n // 2)],
+                                    keys[: n - n // 2]]))
+        else:
+            erase = (rng.random(n) < 0.2).astype(np.uint8)
+            p.mixed(keys, vals, erase)
+        if b % 3 == 2 or b == 9:
+            p.check_contents()
+            q = _queries(rng, p.o.dump()[0], 1 << 14, span)
+            p.check_lookup(q, True)
+            lo = rng.integers(0, span, 1000, dtype=np.uint64)
+            hi = np.minimum(lo + rng.integers(0, max(span // 256, 2), 1000,
+                                              dtype=np.uint64), 0xFFFFFFFF)
+            p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
+            p.check_successor(q[:4096])
