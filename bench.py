@@ -74,8 +74,13 @@ def parse():
     return ap.parse_args()
 
 
-PMC_KERNELS = ("merge_kernel", "lookup_grouped_kernel", "onesweep_kernel",
-               "finalize_batch_kernel", "build_directory_kernel")
+# (summary name, substring of the kernel name that selects its dispatches)
+PMC_KERNELS = (("merge_kernel", "merge_kernel<"),
+               ("lookup_grouped_kernel", "lookup_grouped_kernel<"),
+               ("lookup_gather_kernel", "lookup_gather_kernel<"),
+               ("seg_sort_kernel", "seg_sort_kernel<"),
+               ("seg_partition_onesweep_kernel", "SegPayload"),
+               ("range_kernel", "range_kernel<"))
 COPY_PROBE_KERNEL = "neg_kernel"
 
 
@@ -121,8 +126,9 @@ def _pmc_per_kernel(directory, counter):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row.get("Kernel_Name", "")
-                for k in PMC_KERNELS + (COPY_PROBE_KERNEL,):
-                    if k in name:
+                for k, sub in PMC_KERNELS + ((COPY_PROBE_KERNEL,
+                                              COPY_PROBE_KERNEL),):
+                    if sub in name:
                         key = (k, row.get("Dispatch_Id"))
                         per[key] = per.get(key, 0.0) + \
                             float(row["Counter_Value"])
@@ -174,7 +180,7 @@ def pmc_report(args):
     # (the probe's own, not a smaller negation elsewhere)
     f_cal = max(fetch[COPY_PROBE_KERNEL]) / probe_kib
     w_cal = max(write[COPY_PROBE_KERNEL]) / probe_kib
-    for k in PMC_KERNELS:
+    for k, _ in PMC_KERNELS:
         if k not in fetch or k not in write:
             continue
         fk = float(np.mean(fetch[k]))

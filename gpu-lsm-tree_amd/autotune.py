@@ -4,8 +4,8 @@ It measures, on the GPU at hand, the knobs that change speed and never a
 result:
 
 * LSM (gd_lsm_tuning): directory density, merge tile, onesweep block shape,
-  the batch size from which lookups are grouped, and the fused small-batch
-  limit;
+  the batch size from which lookups are grouped, the fused small-batch
+  limit, the run cap and the admission path (segmented or radix);
 * serving: keys coalesced per step and steps in flight;
 * FliX: slots per node and initial fill.
 
@@ -34,10 +34,11 @@ LSM_SPACE = {
     "grouped_lookup_min_log": [16, 18, 20, 63],
     "small_batch_max": [1024, 4096],
     "max_runs": [6, 10, 16],
+    "segmented_admission": [1, 2],
 }
 LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 512,
                "grouped_lookup_min_log": 18, "small_batch_max": 4096,
-               "max_runs": 10}
+               "max_runs": 10, "segmented_admission": 1}
 SERVE_SPACE = {"max_step_keys": [1 << 20, 1 << 22, 1 << 24],
                "max_inflight_steps": [1, 2, 3]}
 FLIX_SPACE = {"node_slots": [16, 32, 64], "fill_percent": [50, 75]}

@@ -144,6 +144,31 @@ __device__ __forceinline__ bool dir_window(const RunView<T> &r,
   return true;
 }
 
+// First row in [lo, hi) whose key's low bits are >= target (the rows of
+// one directory bucket): bisection down to kFindWindow rows, then one batch
+// of loads.
+constexpr u32 kFindWindow = 8;
+
+template <class T>
+__device__ __forceinline__ u32 window_lower_bound(const typename T::Row *rows,
+                                                  u32 lo, u32 hi,
+                                                  typename T::K target) {
+  while (hi - lo > kFindWindow) {
+    const u32 mid = (lo + hi) >> 1;
+    if ((T::kw(rows[mid]) & T::kLow) < target) lo = mid + 1;
+    else hi = mid;
+  }
+  typename T::Row w[kFindWindow];
+#pragma unroll
+  for (u32 i = 0; i < kFindWindow; ++i)
+    if (lo + i < hi) w[i] = rows[lo + i];
+  u32 below = 0;
+#pragma unroll
+  for (u32 i = 0; i < kFindWindow; ++i)
+    below += (lo + i < hi && (T::kw(w[i]) & T::kLow) < target) ? 1u : 0u;
+  return lo + below;
+}
+
 // First row whose key is >= x.
 template <class T>
 __device__ __forceinline__ u32 run_lower_bound(const RunView<T> &r,
@@ -151,13 +176,7 @@ __device__ __forceinline__ u32 run_lower_bound(const RunView<T> &r,
                                                typename T::K x) {
   u32 lo, hi;
   if (!dir_window(r, m, x, lo, hi)) return lo;
-  const typename T::K target = x & T::kLow;
-  while (lo < hi) {
-    const u32 mid = (lo + hi) >> 1;
-    if ((T::kw(r.rows[mid]) & T::kLow) < target) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
+  return window_lower_bound<T>(r.rows, lo, hi, x & T::kLow);
 }
 
 // First row whose key is > x.
@@ -169,27 +188,42 @@ __device__ __forceinline__ u32 run_upper_bound(const RunView<T> &r,
 }
 
 // Point search; returns the row position or 0xffffffff (row in `row`).
+// Keys are unique inside a run, so a binary search that meets the key
+// stops there; once the directory window is down to kFindWindow rows, its
+// loads are issued together (one memory round trip instead of
+// log2(window) dependent ones).
+
 template <class T>
 __device__ __forceinline__ u32 run_find(const RunView<T> &r, const RunMeta &m,
                                         typename T::K key,
                                         typename T::Row &row) {
-  u32 lo, end;
-  if (!dir_window(r, m, key, lo, end)) return 0xffffffffu;
-  u32 hi = end;
+  u32 lo, hi;
+  if (!dir_window(r, m, key, lo, hi)) return 0xffffffffu;
   const typename T::K target = key & T::kLow;
-  while (lo < hi) {
+  while (hi - lo > kFindWindow) {
     const u32 mid = (lo + hi) >> 1;
-    if ((T::kw(r.rows[mid]) & T::kLow) < target) lo = mid + 1;
+    const typename T::Row w = r.rows[mid];
+    const typename T::K km = T::kw(w) & T::kLow;
+    if (km == target) {
+      row = w;
+      return mid;
+    }
+    if (km < target) lo = mid + 1;
     else hi = mid;
   }
-  if (lo < end) {
-    const typename T::Row w = r.rows[lo];
-    if ((T::kw(w) & T::kLow) == target) {
-      row = w;
-      return lo;
+  typename T::Row w[kFindWindow];
+#pragma unroll
+  for (u32 i = 0; i < kFindWindow; ++i)
+    if (lo + i < hi) w[i] = r.rows[lo + i];
+  u32 at = 0xffffffffu;
+#pragma unroll
+  for (u32 i = 0; i < kFindWindow; ++i) {
+    if (lo + i < hi && (T::kw(w[i]) & T::kLow) == target) {
+      at = lo + i;
+      row = w[i];
     }
   }
-  return 0xffffffffu;
+  return at;
 }
 
 // ------------------------------------------------------------ batch runs
@@ -703,28 +737,42 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   const u32 na = i1 - i0, nb = j1 - j0;
 
   // LDS layout: [0] = A[i0-1] (previous A row, for the equal-key test),
-  // [1, na] = A[i0, i1), [na+1, na+nb] = B[j0, j1).
-  for (u32 t = threadIdx.x; t < na + 1; t += kMergeThreads) {
-    if (t == 0 && i0 == 0) {
-      s_key[0] = 0;
-      s_tomb[0] = 2;  // "no previous row"
-      continue;
+  // [1, na] = A[i0, i1), [na+1, na+nb] = B[j0, j1).  Slot t of the
+  // na + 1 + nb <= kMergeTile + 1 slots is loaded by thread t % 256; every
+  // load of a thread is issued before the first one is used.
+  constexpr int kLoads = kMergeItems + 1;
+  const u32 slots = na + 1 + nb;
+  Row rr[kLoads];
+#pragma unroll
+  for (int r = 0; r < kLoads; ++r) {
+    const u32 t = threadIdx.x + r * kMergeThreads;
+    if (t < slots) {
+      if (t > na) rr[r] = b.rows[j0 + (t - na - 1)];
+      else if (t > 0 || i0 > 0) rr[r] = a.rows[i0 + t - 1];
     }
-    const u32 pos = i0 + t - 1;
-    const Row r = a.rows[pos];
-    s_key[t] = decode_key<T>(T::kw(r), pos, sa);
-    s_tomb[t] = static_cast<u8>(T::kw(r) >> (T::kBits - 1));
-    s_val[t] = T::val(r);
   }
-  for (u32 t = threadIdx.x; t < nb; t += kMergeThreads) {
-    const u32 pos = j0 + t;
-    const Row r = b.rows[pos];
-    s_key[na + 1 + t] = decode_key<T>(T::kw(r), pos, sb);
-    s_tomb[na + 1 + t] = static_cast<u8>(T::kw(r) >> (T::kBits - 1));
-    s_val[na + 1 + t] = T::val(r);
+  Row prev_b_row{};
+  const bool load_prev_b = threadIdx.x == kMergeThreads - 1 && j0 > 0;
+  if (load_prev_b) prev_b_row = b.rows[j0 - 1];
+#pragma unroll
+  for (int r = 0; r < kLoads; ++r) {
+    const u32 t = threadIdx.x + r * kMergeThreads;
+    if (t < slots) {
+      if (t == 0 && i0 == 0) {
+        s_key[0] = 0;
+        s_tomb[0] = 2;  // "no previous row"
+      } else {
+        const bool from_a = t <= na;
+        const u32 pos = from_a ? i0 + t - 1 : j0 + (t - na - 1);
+        s_key[t] = decode_key<T>(T::kw(rr[r]), pos, from_a ? sa : sb);
+        s_tomb[t] = static_cast<u8>(T::kw(rr[r]) >> (T::kBits - 1));
+        s_val[t] = T::val(rr[r]);
+      }
+    }
   }
   if (threadIdx.x == kMergeThreads - 1)
-    s_prev_b = j0 ? decode_key<T>(T::kw(b.rows[j0 - 1]), j0 - 1, sb) : K(0);
+    s_prev_b = load_prev_b ? decode_key<T>(T::kw(prev_b_row), j0 - 1, sb)
+                           : K(0);
   __syncthreads();
   // Header fields without atomics: the tile reaching the end writes the
   // count; the first tile whose inputs reach the top key bit (or the last
@@ -772,41 +820,63 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   K ok[kMergeItems];
   V ov[kMergeItems];
   u8 ot[kMergeItems];
+  u32 osrc[kMergeItems];  // LDS slot of each merged element
   bool okeep[kMergeItems];
   u32 kept = 0;
   const u32 steps =
       dk < tile_len ? (tile_len - dk < kMergeItems ? tile_len - dk
                                                    : u32(kMergeItems))
                     : 0u;
+  // Serial merge.  Registers carry the next key of each side and the last
+  // A key taken (a B row equal to it is shadowed by the newer A row), so
+  // the only LDS read on the serial path is the next key of the side that
+  // advanced; values and tombstone flags are read afterwards, all at once.
+  {
+    bool a_left = ia < na, b_left = ib < nb;
+    K ka = (steps && a_left) ? s_key[1 + ia] : K(0);
+    K kb = (steps && b_left) ? s_key[na + 1 + ib] : K(0);
+    // LDS slot ia holds A[ia - 1] (A[i0 - 1] at 0, flag 2 when none)
+    bool have_last_a = steps && s_tomb[ia] != 2;
+    K last_a = steps ? s_key[ia] : K(0);
+#pragma unroll
+    for (int s = 0; s < kMergeItems; ++s) {
+      okeep[s] = false;
+      ok[s] = 0;
+      osrc[s] = 0;
+      if (static_cast<u32>(s) < steps) {
+        const bool take_a = a_left && (!b_left || ka <= kb);
+        const K k = take_a ? ka : kb;
+        ok[s] = k;
+        osrc[s] = take_a ? 1 + ia : na + 1 + ib;
+        okeep[s] = take_a || !(have_last_a && last_a == k);
+        if (take_a) {
+          last_a = k;
+          have_last_a = true;
+        }
+        ia += take_a;
+        ib += !take_a;
+        a_left = ia < na;
+        b_left = ib < nb;
+        const bool more = take_a ? a_left : b_left;
+        const K next = more ? s_key[take_a ? 1 + ia : na + 1 + ib] : K(0);
+        if (take_a) ka = next;
+        else kb = next;
+      }
+    }
+  }
 #pragma unroll
   for (int s = 0; s < kMergeItems; ++s) {
-    okeep[s] = false;
-    ok[s] = 0;
     ov[s] = 0;
     ot[s] = 0;
     if (static_cast<u32>(s) < steps) {
-      bool take_a;
-      if (ia >= na) take_a = false;
-      else if (ib >= nb) take_a = true;
-      else take_a = s_key[1 + ia] <= s_key[na + 1 + ib];
-      bool keep = true;
-      if (take_a) {
-        ok[s] = s_key[1 + ia];
-        ov[s] = s_val[1 + ia];
-        ot[s] = s_tomb[1 + ia];
-        ++ia;
-      } else {
-        ok[s] = s_key[na + 1 + ib];
-        ov[s] = s_val[na + 1 + ib];
-        ot[s] = s_tomb[na + 1 + ib];
-        ++ib;
-        // shadowed by the newer A row immediately before it?
-        if (s_tomb[ia] != 2 && s_key[ia] == ok[s]) keep = false;
-      }
-      if (keep && ot[s] && drop_tombstones) keep = false;
-      okeep[s] = keep;
-      kept += keep;
+      ot[s] = s_tomb[osrc[s]];
+      ov[s] = s_val[osrc[s]];
     }
+  }
+#pragma unroll
+  for (int s = 0; s < kMergeItems; ++s) {
+    if (okeep[s] && ot[s] && drop_tombstones) okeep[s] = false;
+    kept += okeep[s];
   }
   const u32 local = block_exclusive_scan<kMergeThreads>(kept, s_scan);
   const u32 tile_total = s_scan[kMergeThreads / kWave];
@@ -934,8 +1004,16 @@ __global__ void lookup_kernel(IO io, u64 n, RunSet<T> runs) {
   }
 }
 
-// Lookups over queries partitioned by their top key byte (qk = grouped
-// keys).  Results are written in the grouped order (res_values[p],
+// Result of a grouped lookup: value and found flag side by side, so the
+// gather back to query order reads each result with one access.
+template <class T>
+struct alignas(2 * sizeof(typename T::V)) GroupedResult {
+  typename T::V value;
+  typename T::V found;
+};
+
+// Lookups over queries partitioned by their top varying key byte (qk =
+// grouped keys).  Results are written in the grouped order (res_values[p],
 // res_found[p]), so every store is coalesced; lookup_gather_kernel then
 // brings them back to query order.  Scattering each result straight to its
 // query position would dirty one cache line per query and evict the run
@@ -943,41 +1021,164 @@ __global__ void lookup_kernel(IO io, u64 n, RunSet<T> runs) {
 // 8 XCDs, so block b serves the contiguous chunk (b % 8) * (G / 8) + b / 8:
 // every XCD walks its own eighth of the grouped queries, i.e. its own eighth
 // of the key space, and the run rows it needs stay in that XCD's L2.
+// A thread carries kQ queries through each run in lock step (directory
+// loads of all of them, then their row windows), so their memory round
+// trips overlap; run headers are read once per workgroup into LDS.
 template <class T>
-__global__ void lookup_grouped_kernel(const typename T::K *__restrict__ qk,
-                                      u64 n,
-                                      typename T::V *__restrict__ res_values,
-                                      u8 *__restrict__ res_found,
-                                      RunSet<T> runs) {
+__global__ __launch_bounds__(256) void lookup_grouped_kernel(
+    const typename T::K *__restrict__ qk, u64 n,
+    GroupedResult<T> *__restrict__ res, RunSet<T> runs) {
+  using K = typename T::K;
+  using V = typename T::V;
+  using Row = typename T::Row;
+  constexpr int kQ = 2;
+  __shared__ RunMeta s_meta[kMaxRuns];
+  if (threadIdx.x < runs.count) s_meta[threadIdx.x] = *runs.run[threadIdx.x].meta;
+  __syncthreads();
   const u32 G = gridDim.x;  // a multiple of 8
   const u32 logical = (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
   const u64 chunk = (n + G - 1) / G;
   const u64 begin = u64(logical) * chunk;
   const u64 end = begin + chunk < n ? begin + chunk : n;
-  for (u64 p = begin + threadIdx.x; p < end; p += blockDim.x) {
-    typename T::V value = 0;
-    const bool live = lookup_one(runs, qk[p], value);
-    res_values[p] = live ? value : typename T::V(0);
-    res_found[p] = live ? 1 : 0;
+  for (u64 p0 = begin + threadIdx.x; p0 < end; p0 += u64(kQ) * blockDim.x) {
+    K key[kQ];
+    V val[kQ];
+    bool open[kQ], live[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const u64 p = p0 + u64(q) * blockDim.x;
+      open[q] = p < end;
+      key[q] = open[q] ? qk[p] : K(0);
+      live[q] = false;
+      val[q] = 0;
+    }
+    for (u32 r = 0; r < runs.count; ++r) {
+      const RunMeta m = s_meta[r];
+      const RunView<T> &rv = runs.run[r];
+      u32 lo[kQ], hi[kQ];
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        lo[q] = hi[q] = 0;
+        if (open[q]) (void)dir_window(rv, m, key[q], lo[q], hi[q]);
+      }
+      bool hit[kQ];
+      Row row[kQ];
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        hit[q] = false;
+        // wide directory buckets (skewed keys): bisect down to the window
+        const K target = key[q] & T::kLow;
+        while (hi[q] - lo[q] > kFindWindow) {
+          const u32 mid = (lo[q] + hi[q]) >> 1;
+          const Row w = rv.rows[mid];
+          const K km = T::kw(w) & T::kLow;
+          if (km == target) {
+            hit[q] = true;
+            row[q] = w;
+            lo[q] = hi[q] = mid;
+          } else if (km < target) {
+            lo[q] = mid + 1;
+          } else {
+            hi[q] = mid;
+          }
+        }
+      }
+      // the row windows: 8-byte rows are loaded in aligned pairs (one
+      // 16-byte access covers two rows; row arrays carry one spare row, so
+      // the pair of a window's last row stays inside the allocation)
+      constexpr u32 kPair = sizeof(Row) == 8 ? 2 : 1;
+      constexpr u32 kSlots = kPair == 2 ? kFindWindow + 2 : kFindWindow;
+      Row w[kQ][kSlots];
+      u32 first[kQ];
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        first[q] = kPair == 2 ? (lo[q] & ~1u) : lo[q];
+#pragma unroll
+        for (u32 i = 0; i < kSlots; i += kPair) {
+          const u32 at = first[q] + i;
+          if (at < hi[q]) {
+            if constexpr (kPair == 2) {
+              const uint4 v = *reinterpret_cast<const uint4 *>(rv.rows + at);
+              w[q][i] = T::make(v.x, v.y);
+              w[q][i + 1] = T::make(v.z, v.w);
+            } else {
+              w[q][i] = rv.rows[at];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        const K target = key[q] & T::kLow;
+#pragma unroll
+        for (u32 i = 0; i < kSlots; ++i) {
+          const u32 at = first[q] + i;
+          if (at >= lo[q] && at < hi[q] &&
+              (T::kw(w[q][i]) & T::kLow) == target) {
+            hit[q] = true;
+            row[q] = w[q][i];
+          }
+        }
+        if (open[q] && hit[q]) {
+          open[q] = false;
+          live[q] = !(T::kw(row[q]) & T::kTomb);
+          val[q] = T::val(row[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const u64 p = p0 + u64(q) * blockDim.x;
+      if (p < end) {
+        GroupedResult<T> g;
+        g.value = live[q] ? val[q] : V(0);
+        g.found = live[q] ? 1 : 0;
+        res[p] = g;
+      }
+    }
   }
 }
 
 // Result i = grouped result dest[i], written through the IO policy: dest
 // (the partition pass's destination of query i) is read in query order and
-// the grouped results come from 256 streams (one per key byte) that each
+// the grouped results come from 256 streams (one per key group) that each
 // advance through their own segment, so reads and stores both stream.
+// Blocks are dealt to the XCDs like lookup_grouped_kernel's: XCD x answers
+// the x-th eighth of the queries, whose results are the x-th eighth of
+// every group's segment (the partition is stable), so no result line is
+// fetched into more than one XCD's L2.
 template <class T, class IO>
-__global__ void lookup_gather_kernel(IO io, const u32 *__restrict__ dest,
-                                     u64 n,
-                                     const typename T::V *__restrict__ res_values,
-                                     const u8 *__restrict__ res_found) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += stride) {
-    typename IO::Slot slot;
-    (void)io.load(i, slot);
-    const u32 d = dest[i];
-    io.put(slot, res_found[d] != 0, res_values[d]);
+__global__ __launch_bounds__(256) void lookup_gather_kernel(
+    IO io, const u32 *__restrict__ dest, u64 n,
+    const GroupedResult<T> *__restrict__ res) {
+  constexpr int kQ = 4;
+  const u32 G = gridDim.x;  // a multiple of 8
+  const u32 logical = (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
+  const u64 chunk = (n + G - 1) / G;
+  const u64 begin = u64(logical) * chunk;
+  const u64 end = begin + chunk < n ? begin + chunk : n;
+  for (u64 p0 = begin + threadIdx.x; p0 < end; p0 += u64(kQ) * blockDim.x) {
+    u32 d[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const u64 i = p0 + u64(q) * blockDim.x;
+      d[q] = i < end ? dest[i] : 0u;
+    }
+    GroupedResult<T> g[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const u64 i = p0 + u64(q) * blockDim.x;
+      if (i < end) g[q] = res[d[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const u64 i = p0 + u64(q) * blockDim.x;
+      if (i < end) {
+        typename IO::Slot slot;
+        (void)io.load(i, slot);
+        io.put(slot, g[q].found != 0, g[q].value);
+      }
+    }
   }
 }
 
@@ -1016,76 +1217,105 @@ struct PlainRange {
   }
 };
 
+// Queries whose bounds a wave resolves together: one lane per (query,
+// run, side), at least one query.
+__host__ __device__ __forceinline__ u32 range_queries_per_wave(u32 runs) {
+  const u32 per = 2 * (runs ? runs : 1);
+  return per >= static_cast<u32>(kWave) ? 1u : static_cast<u32>(kWave) / per;
+}
+
 template <class T, class IO>
 __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io, u64 n,
                                                               RunSet<T> runs) {
   using K = typename T::K;
-  __shared__ u32 s_lo[kRangeWaves][kMaxRuns];
-  __shared__ u32 s_hi[kRangeWaves][kMaxRuns];
+  constexpr u32 kSlots =
+      2 * kMaxRuns > kWave ? 2 * kMaxRuns : static_cast<u32>(kWave);
+  // s_b[wave][2 * (qi * runs + r) + side]: lower (side 0) and upper (side
+  // 1) bound of query qi of the round in run r
+  __shared__ u32 s_b[kRangeWaves][kSlots];
   const u32 lane = lane_id();
   const u32 wave = threadIdx.x / kWave;
-  const u64 wave_stride = u64(gridDim.x) * kRangeWaves;
   const u32 nr = runs.count;
-  for (u64 q = u64(blockIdx.x) * kRangeWaves + wave; q < n; q += wave_stride) {
-    typename IO::Slot slot;
-    K a, z;
-    bool count_only;
-    io.load(q, slot, a, z, count_only);
-    if (a > z) {
-      if (lane == 0) io.put(slot, typename T::V(0));
-      continue;
-    }
-    // segment bounds of every run, one lane per run
-    for (u32 r = lane; r < nr; r += kWave) {
-      const RunView<T> &rv = runs.run[r];
-      const RunMeta m = *rv.meta;
-      s_lo[wave][r] = run_lower_bound(rv, m, a);
-      s_hi[wave][r] = run_upper_bound(rv, m, z);
+  const u32 per = 2 * nr;
+  const u32 qw = range_queries_per_wave(nr);
+  const u64 wave_id = u64(blockIdx.x) * kRangeWaves + wave;
+  const u64 wave_stride = u64(gridDim.x) * kRangeWaves;
+  for (u64 q0 = wave_id * qw; q0 < n; q0 += wave_stride * qw) {
+    // every bound of the round at once, one lane each
+    for (u32 t = lane; t < qw * per; t += kWave) {
+      const u64 q = q0 + t / per;
+      const u32 r = (t % per) >> 1;
+      u32 b = 0;
+      if (q < n) {
+        typename IO::Slot slot;
+        K a, z;
+        bool count_only;
+        io.load(q, slot, a, z, count_only);
+        if (a <= z) {
+          const RunView<T> &rv = runs.run[r];
+          const RunMeta m = *rv.meta;
+          b = (t & 1) ? run_upper_bound(rv, m, z) : run_lower_bound(rv, m, a);
+        }
+      }
+      s_b[wave][t] = b;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    u64 acc = 0;
-    bool newer_rows = false;
-    for (u32 r = 0; r < nr; ++r) {
-      const RunView<T> &rv = runs.run[r];
-      const u32 L = s_lo[wave][r], H = s_hi[wave][r];
-      if (L >= H) continue;
-      const u32 tombs = rv.meta->tombs;
-      if (count_only && !newer_rows && tombs == 0) {
-        if (lane == 0) acc += H - L;
-        newer_rows = true;
+    for (u32 qi = 0; qi < qw; ++qi) {
+      const u64 q = q0 + qi;
+      if (q >= n) break;
+      typename IO::Slot slot;
+      K a, z;
+      bool count_only;
+      io.load(q, slot, a, z, count_only);
+      if (a > z) {
+        if (lane == 0) io.put(slot, typename T::V(0));
         continue;
       }
-      const u32 split = rv.meta->split;
-      for (u32 idx = L + lane; idx < H; idx += kWave) {
-        const typename T::Row w = rv.rows[idx];
-        if (T::kw(w) & T::kTomb) continue;
-        bool shadowed = false;
-        if (newer_rows) {
-          const K key = decode_key<T>(T::kw(w), idx, split);
-          for (u32 s = 0; s < r && !shadowed; ++s) {
-            u32 l2 = s_lo[wave][s], h2 = s_hi[wave][s];
-            if (l2 >= h2) continue;
-            const RunView<T> &nv = runs.run[s];
-            const u32 ns = nv.meta->split;
-            while (l2 < h2) {
-              const u32 mid = (l2 + h2) >> 1;
-              if (decode_key<T>(T::kw(nv.rows[mid]), mid, ns) < key)
-                l2 = mid + 1;
-              else
-                h2 = mid;
-            }
-            if (l2 < s_hi[wave][s] &&
-                decode_key<T>(T::kw(nv.rows[l2]), l2, ns) == key)
-              shadowed = true;
-          }
+      const u32 *bnd = &s_b[wave][qi * per];  // [2r] lower, [2r + 1] upper
+      u64 acc = 0;
+      bool newer_rows = false;
+      for (u32 r = 0; r < nr; ++r) {
+        const RunView<T> &rv = runs.run[r];
+        const u32 L = bnd[2 * r], H = bnd[2 * r + 1];
+        if (L >= H) continue;
+        const u32 tombs = rv.meta->tombs;
+        if (count_only && !newer_rows && tombs == 0) {
+          if (lane == 0) acc += H - L;
+          newer_rows = true;
+          continue;
         }
-        if (!shadowed) acc += count_only ? u64(1) : u64(T::val(w));
+        const u32 split = rv.meta->split;
+        for (u32 idx = L + lane; idx < H; idx += kWave) {
+          const typename T::Row w = rv.rows[idx];
+          if (T::kw(w) & T::kTomb) continue;
+          bool shadowed = false;
+          if (newer_rows) {
+            const K key = decode_key<T>(T::kw(w), idx, split);
+            for (u32 s = 0; s < r && !shadowed; ++s) {
+              u32 l2 = bnd[2 * s], h2 = bnd[2 * s + 1];
+              if (l2 >= h2) continue;
+              const RunView<T> &nv = runs.run[s];
+              const u32 ns = nv.meta->split;
+              while (l2 < h2) {
+                const u32 mid = (l2 + h2) >> 1;
+                if (decode_key<T>(T::kw(nv.rows[mid]), mid, ns) < key)
+                  l2 = mid + 1;
+                else
+                  h2 = mid;
+              }
+              if (l2 < bnd[2 * s + 1] &&
+                  decode_key<T>(T::kw(nv.rows[l2]), l2, ns) == key)
+                shadowed = true;
+            }
+          }
+          if (!shadowed) acc += count_only ? u64(1) : u64(T::val(w));
+        }
+        newer_rows = true;
       }
-      newer_rows = true;
+      acc = wave_sum_u64(acc);
+      if (lane == 0) io.put(slot, static_cast<typename T::V>(acc));
     }
-    acc = wave_sum_u64(acc);
-    if (lane == 0) io.put(slot, static_cast<typename T::V>(acc));
     __builtin_amdgcn_wave_barrier();
   }
 }

@@ -195,27 +195,26 @@ class LsmT : public HandleBase {
     try {
       const RunSet<T> set = run_set();
       if (n >= grouped_lookup_min() && n <= 0xffffffffull && !runs_.empty()) {
-        // group the queries by their top key byte first (one radix pass
-        // that also records where each query went), look up in grouped
-        // order, then gather the results back to query order
+        // group the queries by their top varying key byte first (one radix
+        // pass that also records where each query went), look up in
+        // grouped order, then gather the results back to query order
         const u32 nn = static_cast<u32>(n);
         Lease<K> qk(pool_, n);
         Lease<u32> dest(pool_, n);
-        Lease<V> rv(pool_, n);
-        Lease<u8> rf(pool_, n);
-        Lease<u32> ctrl(pool_, rs::control_words(n, 1));
-        GD_HIP(rs::partition_pairs<K>(q, nn, T::kBits - 8, qk.get(), nullptr,
-                                        ctrl.get(), s, tuning_.sort_block,
-                                        dest.get()));
+        Lease<GroupedResult<T>> res(pool_, n);
+        Lease<u32> ctrl(pool_, SpreadLayout<K>(n).words);
+        partition_by_spread<K>(q, nn, qk.get(), nullptr, dest.get(),
+                               rs::PayloadPos{}, ctrl.get(), s,
+                               tuning_.sort_block);
         ProfRec rec{};
         if (profiling_) rec = prof_begin(1, s, n);
         lookup_grouped_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
-            qk.get(), n, rv.get(), rf.get(), set);
+            qk.get(), n, res.get(), set);
         if (profiling_) prof_end(rec, s);
         GD_LAUNCH_CHECK();
-        lookup_gather_kernel<T><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
-            PlainLookup<T>{q, out_values, out_found}, dest.get(), n, rv.get(),
-            rf.get());
+        lookup_gather_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
+            PlainLookup<T>{q, out_values, out_found}, dest.get(), n,
+            res.get());
       } else {
         ProfRec rec{};
         if (profiling_) rec = prof_begin(1, s, n);
@@ -238,7 +237,9 @@ class LsmT : public HandleBase {
       throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt range");
     order_.begin(s);
     const RunSet<T> set = run_set();
-    range_kernel<T><<<grid_for(n, kRangeWaves, 1u << 16), kRangeThreads, 0,
+    const u64 waves = (n + range_queries_per_wave(set.count) - 1) /
+                      range_queries_per_wave(set.count);
+    range_kernel<T><<<grid_for(waves, kRangeWaves, 1u << 16), kRangeThreads, 0,
                       s>>>(PlainRange<T>{lo, hi, out, count_only ? 1 : 0}, n,
                            set);
     GD_LAUNCH_CHECK();
@@ -706,7 +707,8 @@ class LsmT : public HandleBase {
     Run r;
     r.rows_ub = rows_ub;
     r.max_bits = dir_bits_for(rows_ub, kMaxDirBits, tuning_.dir_rows_log);
-    r.rows = pool.get_n<Row>(std::max<u64>(rows_ub, 1));
+    // one spare row: pair loads of a window's last row stay in bounds
+    r.rows = pool.get_n<Row>(rows_ub + 1);
     r.dir = static_cast<u32 *>(
         pool.get(kMetaBytes + ((u64(1) << r.max_bits) + 1) * sizeof(u32)));
     return r;
@@ -778,31 +780,34 @@ class LsmT : public HandleBase {
   // workgroup per segment sorts, dedupes, encodes and indexes it.
   Run segmented_run(const K *keys, const V *values, const u8 *flags, u32 n,
                     int mode, hipStream_t s, Run r) {
-    const u64 part_words = rs::control_words(n, 1);
-    // [partition control | span (64) | shift, pad | status (256 x u64) |
-    //  ticket, pad]
-    const u64 span_at = part_words + (part_words & 1);
-    const u64 shift_at = span_at + kSpanBlocks;
-    const u64 status_at = shift_at + 2;
+    // [spread partition control | status (256 x u64) | ticket, pad]
+    const SpreadLayout<K> L(n);
+    const u64 status_at = L.words + (L.words & 1);
     const u64 ticket_at = status_at + 2 * 256;
     const u64 words = ticket_at + 2;
     Lease<u32> ctrl(pool_, words);
     Lease<u32> ka(pool_, n), va(pool_, n), kb(pool_, n), vb(pool_, n);
     u32 *c = ctrl.get();
     const rs::Control pc = rs::control_layout(c, 1);
+    // the partition's clearing launch also clears this kernel's state
+    // behind its own and the run header
     rs::clear_control(c, words, s, reinterpret_cast<u32 *>(r.meta()),
                       kMetaWords);
-    keyspan_kernel<<<kSpanBlocks, 256, 0, s>>>(keys, n, c + span_at);
-    seg_hist_kernel<<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
-        keys, n, c + span_at, pc.rows, c + shift_at);
-    rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
-    rs::launch_pass<u32, true>(
-        tuning_.sort_block ? tuning_.sort_block : rs::block_for(n),
-        rs::tiles_for(n), keys, nullptr, ka.get(), va.get(), n, 0,
-        pc.offsets, pc.status, pc.tickets, s, nullptr, c + shift_at);
+    {
+      K *span = reinterpret_cast<K *>(c + L.span_at);
+      keyspan_kernel<K><<<kSpanBlocks, 256, 0, s>>>(keys, n, span);
+      seg_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
+          keys, n, span, pc.rows, c + L.shift_at);
+      rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
+      rs::launch_pass<u32>(
+          tuning_.sort_block ? tuning_.sort_block : rs::block_for(n),
+          rs::tiles_for(n), keys, SegPayload{values, flags, mode}, ka.get(),
+          va.get(), n, 0, pc.offsets, pc.status, pc.tickets, s, nullptr,
+          c + L.shift_at);
+    }
     seg_sort_kernel<T><<<256, kSegThreads, 0, s>>>(
         keys, ka.get(), va.get(), kb.get(), vb.get(), n, pc.offsets,
-        c + shift_at, values, flags, mode, r.rows, r.meta(), r.entries(),
+        c + L.shift_at, values, flags, mode, r.rows, r.meta(), r.entries(),
         r.max_bits, reinterpret_cast<u64 *>(c + status_at), c + ticket_at);
     GD_LAUNCH_CHECK();
     return r;
@@ -993,8 +998,10 @@ class LsmT : public HandleBase {
 
   u64 batch_capacity_ = 1;
   u64 capacity_ = 0;
+  // segmented admission (1) for 32-bit batches; 64-bit batches always take
+  // the radix passes
   gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(T::kSmallTile), 10,
-                        2, 0};
+                        1, 0};
 
   struct ProfRec {
     int kind;  // 0 merge, 1 lookup

@@ -94,13 +94,30 @@ static __global__ __launch_bounds__(kRadix) void scan_kernel(
   offsets[p * kRadix + threadIdx.x] = block_exclusive_scan<kRadix>(v, scratch);
 }
 
+// Payload policies of a digit pass: payload(idx, valid, key, shift)
+// returns the 32-bit payload of input element idx (and may rewrite key
+// bits that no digit of the sort reads).
+struct PayloadIn {  // the payload array of the previous pass
+  const u32 *vals;
+  template <class K>
+  __device__ __forceinline__ u32 operator()(u64 idx, bool valid, K &,
+                                            u32) const {
+    return valid ? vals[idx] : 0u;
+  }
+};
+struct PayloadPos {  // the input position
+  template <class K>
+  __device__ __forceinline__ u32 operator()(u64 idx, bool, K &, u32) const {
+    return static_cast<u32>(idx);
+  }
+};
+
 // One digit pass.  status has tiles*kRadix words and must be zero;
-// *ticket must be zero.  kFirst: the payload is generated (= position).
-// kBlock threads (256 or 512) x kItems keys per tile; threads below 256
-// own one digit each for the per-digit bookkeeping.
-template <class K, bool kFirst, int kItems, int kBlock>
+// *ticket must be zero.  kBlock threads (256 or 512) x kItems keys per
+// tile; threads below 256 own one digit each for the per-digit bookkeeping.
+template <class K, class Payload, int kItems, int kBlock>
 __global__ __launch_bounds__(kBlock) void onesweep_kernel(
-    const K *__restrict__ keys_in, const u32 *__restrict__ vals_in,
+    const K *__restrict__ keys_in, Payload payload,
     K *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
     u32 shift_arg, const u32 *__restrict__ digit_offsets, u64 *status,
     u32 *ticket, u32 *__restrict__ dest_out, const u32 *shift_dev) {
@@ -135,11 +152,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const u64 idx = wave_base + u64(j) * kWave + lane;
     const bool valid = idx < n;
     key[j] = valid ? keys_in[idx] : ~K(0);
-    if constexpr (kFirst) {
-      val[j] = static_cast<u32>(idx);
-    } else {
-      val[j] = valid ? vals_in[idx] : 0u;
-    }
+    val[j] = payload(idx, valid, key[j], shift);
   }
 
   // Wave-local stable ranking (item-major, then lane order == input order).
@@ -293,21 +306,21 @@ inline void clear_control(u32 *control, u64 words, hipStream_t stream,
                     256, 0, stream>>>(control, words, extra, extra_words);
 }
 
-template <class K, bool kFirst>
+template <class K, class Payload>
 inline void launch_pass(int block, u32 tiles, const K *src_k,
-                        const u32 *src_v, K *dst_k, u32 *dst_v, u32 n,
+                        Payload payload, K *dst_k, u32 *dst_v, u32 n,
                         u32 shift, const u32 *offsets, u64 *status,
                         u32 *ticket, hipStream_t stream,
                         u32 *dest = nullptr,
                         const u32 *shift_dev = nullptr) {
   if (block == 512)
-    onesweep_kernel<K, kFirst, 8, 512><<<tiles, 512, 0, stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest,
-        shift_dev);
+    onesweep_kernel<K, Payload, 8, 512><<<tiles, 512, 0, stream>>>(
+        src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
+        dest, shift_dev);
   else
-    onesweep_kernel<K, kFirst, 16, 256><<<tiles, 256, 0, stream>>>(
-        src_k, src_v, dst_k, dst_v, n, shift, offsets, status, ticket, dest,
-        shift_dev);
+    onesweep_kernel<K, Payload, 16, 256><<<tiles, 256, 0, stream>>>(
+        src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
+        dest, shift_dev);
 }
 
 // Sorts n (key, position) pairs stably.  keys_in is read only; the sorted
@@ -343,12 +356,12 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   for (int p = 0; p < kPasses; ++p) {
     u64 *pass_status = status + u64(p) * tiles * kRadix;
     if (p == 0) {
-      launch_pass<K, true>(shape, tiles, src_k, src_v, dst_k, dst_v, n, 0,
-                           offsets, pass_status, tickets, stream);
+      launch_pass<K>(shape, tiles, src_k, PayloadPos{}, dst_k, dst_v, n, 0,
+                     offsets, pass_status, tickets, stream);
     } else {
-      launch_pass<K, false>(shape, tiles, src_k, src_v, dst_k, dst_v, n,
-                            8u * p, offsets + p * kRadix, pass_status,
-                            tickets + p, stream);
+      launch_pass<K>(shape, tiles, src_k, PayloadIn{src_v}, dst_k, dst_v, n,
+                     8u * p, offsets + p * kRadix, pass_status, tickets + p,
+                     stream);
     }
     src_k = dst_k;
     src_v = dst_v;
@@ -384,8 +397,8 @@ inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
   const u32 *offsets = c.offsets;
   u32 *tickets = c.tickets;
   u64 *status = c.status;
-  launch_pass<K, true>(shape, tiles, keys_in, nullptr, keys_out, pos_out, n,
-                       shift, offsets, status, tickets, stream, dest);
+  launch_pass<K>(shape, tiles, keys_in, PayloadPos{}, keys_out, pos_out, n,
+                 shift, offsets, status, tickets, stream, dest);
   return hipGetLastError();
 }
 

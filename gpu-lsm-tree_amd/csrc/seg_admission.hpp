@@ -6,10 +6,11 @@
 //                     digit (key >> s0) & 0xff, so the 256 segments split
 //                     the batch's actual key spread, not the key space
 //   scan_kernel       segment starts
-//   onesweep pass     stable partition by that digit (positions as payload)
+//   onesweep pass     stable partition by that digit, carrying the values
+//                     (SegPayload), so the sort needs no random gathers
 //   seg_sort_kernel   one workgroup per segment, in key order (ticket =
 //                     segment): an LDS LSD sort of the bits below s0, the
-//                     newest-wins dedupe, value / tombstone gather, row
+//                     newest-wins dedupe, tombstone decoding, row
 //                     encoding, the chained scan for the output offset, the
 //                     radix-directory entries of the segment's key range
 //                     and the run header.
@@ -33,14 +34,15 @@ constexpr int kSpanBlocks = 256;
 
 // span[blockIdx.x] = OR over the block's keys of (key ^ keys[0]); 16 keys
 // per thread per round, loaded together.
+template <class K>
 static __global__ __launch_bounds__(256) void keyspan_kernel(
-    const u32 *__restrict__ keys, u32 n, u32 *__restrict__ span) {
-  __shared__ u32 s_or[256 / kWave];
-  const u32 k0 = keys[0];
-  u32 acc = 0;
+    const K *__restrict__ keys, u32 n, K *__restrict__ span) {
+  __shared__ K s_or[256 / kWave];
+  const K k0 = keys[0];
+  K acc = 0;
   const u64 stride = u64(gridDim.x) * 256 * 16;
   for (u64 base = u64(blockIdx.x) * 256 * 16; base < n; base += stride) {
-    u32 k[16];
+    K k[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const u64 i = base + u64(j) * 256 + threadIdx.x;
@@ -55,29 +57,35 @@ static __global__ __launch_bounds__(256) void keyspan_kernel(
   if (lane_id() == 0) s_or[threadIdx.x / kWave] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    u32 v = 0;
+    K v = 0;
     for (int w = 0; w < 256 / kWave; ++w) v |= s_or[w];
     span[blockIdx.x] = v;
   }
 }
 
 // Digit shift from the key spread: the top 8 varying bits.
-__device__ __forceinline__ u32 seg_shift_from(const u32 *span) {
-  // every thread folds the block results (one wave-wide OR per 64)
-  u32 diff = 0;
+template <class K>
+__device__ __forceinline__ u32 seg_shift_from(const K *span) {
+  // every wave folds the block results (one wave-wide OR per 64)
+  K diff = 0;
   for (int b = static_cast<int>(lane_id()); b < kSpanBlocks; b += kWave)
     diff |= span[b];
 #pragma unroll
   for (int offset = kWave / 2; offset > 0; offset >>= 1)
     diff |= __shfl_xor(diff, offset, kWave);
-  const u32 h = diff ? 31u - static_cast<u32>(__clz(diff)) : 0u;
+  u32 h = 0;
+  if (diff) {
+    if constexpr (sizeof(K) == 8) h = 63u - static_cast<u32>(__clzll(diff));
+    else h = 31u - static_cast<u32>(__clz(diff));
+  }
   return h >= 7 ? h - 7 : 0u;
 }
 
 // Histogram of the segment digit into kHistRows partial rows (zeroed);
 // block 0 also stores the digit shift for the later launches.
+template <class K>
 static __global__ __launch_bounds__(rs::kThreads) void seg_hist_kernel(
-    const u32 *__restrict__ keys, u32 n, const u32 *__restrict__ span,
+    const K *__restrict__ keys, u32 n, const K *__restrict__ span,
     u32 *__restrict__ rows, u32 *shift_out) {
   __shared__ u32 s_hist[rs::kRadix];
   const u32 s0 = seg_shift_from(span);
@@ -87,16 +95,16 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_hist_kernel(
   const u64 round = u64(rs::kThreads) * rs::kHistItems;
   for (u64 base = u64(blockIdx.x) * round; base < n;
        base += u64(gridDim.x) * round) {
-    u32 k[rs::kHistItems];
+    K k[rs::kHistItems];
 #pragma unroll
     for (int j = 0; j < rs::kHistItems; ++j) {
       const u64 i = base + u64(j) * rs::kThreads + threadIdx.x;
-      k[j] = i < n ? keys[i] : 0u;
+      k[j] = i < n ? keys[i] : K(0);
     }
 #pragma unroll
     for (int j = 0; j < rs::kHistItems; ++j)
       if (base + u64(j) * rs::kThreads + threadIdx.x < n)
-        atomicAdd(&s_hist[(k[j] >> s0) & 0xffu], 1u);
+        atomicAdd(&s_hist[static_cast<u32>(k[j] >> s0) & 0xffu], 1u);
   }
   __syncthreads();
   u32 *row = rows + u64(blockIdx.x % rs::kHistRows) * rs::kRadix;
@@ -106,11 +114,78 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_hist_kernel(
   }
 }
 
+// Control words of a spread partition of n keys: the partition's own, the
+// per-block spans and the device-side shift (all cleared by one launch).
+template <class K>
+struct SpreadLayout {
+  u64 span_at, shift_at, words;
+  explicit SpreadLayout(u64 n) {
+    const u64 part = rs::control_words(n, 1);
+    span_at = part + (part & 1);  // 8-byte aligned for 64-bit spans
+    shift_at = span_at + u64(kSpanBlocks) * (sizeof(K) / 4);
+    words = shift_at + 2;
+  }
+};
+
+// Stable partition of n keys by their top 8 varying bits (the 256 groups
+// split the keys' actual spread, not the key space): keys_out receives the
+// grouped keys, pos_out (nullable) each one's input position, dest
+// (nullable) each input key's grouped position, in input order; *shift_out
+// (inside `control`) the digit shift.  `control` holds
+// SpreadLayout<K>(n).words words; payload selects what pos_out carries.
+template <class K, class Payload>
+inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
+                                u32 *pos_out, u32 *dest, Payload payload,
+                                u32 *control, hipStream_t s, int block,
+                                u32 *extra = nullptr, u64 extra_words = 0) {
+  const SpreadLayout<K> L(n);
+  const rs::Control pc = rs::control_layout(control, 1);
+  rs::clear_control(control, L.words, s, extra, extra_words);
+  K *span = reinterpret_cast<K *>(control + L.span_at);
+  keyspan_kernel<K><<<kSpanBlocks, 256, 0, s>>>(keys, n, span);
+  seg_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
+      keys, n, span, pc.rows, control + L.shift_at);
+  rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
+  rs::launch_pass<K>(block ? block : rs::block_for(n), rs::tiles_for(n), keys,
+                     payload, keys_out, pos_out, n, 0, pc.offsets, pc.status,
+                     pc.tickets, s, dest, control + L.shift_at);
+}
+
+// Payload of the partition pass.  When s0 <= 23 the top varying key bit
+// is at most bit 30, so every key of the batch shares bit 31 and the pass
+// may carry the erase flag there:
+//   insert            the update's value
+//   erase             nothing (every row is a tombstone)
+//   mixed, s0 <= 23   the value; the erase flag rides in key bit 31
+//   mixed, s0 = 24    the input position (value and flag are gathered
+//                     after the sort)
+constexpr u32 kStashBit = 0x80000000u;
+
+__device__ __forceinline__ bool seg_stashed(int mode, u32 s0) {
+  return mode == kModeMixed && s0 <= 23;
+}
+
+struct SegPayload {
+  const u32 *values;
+  const u8 *flags;
+  int mode;
+  template <class K>
+  __device__ __forceinline__ u32 operator()(u64 idx, bool valid, K &key,
+                                            u32 shift) const {
+    if (!valid || mode == kModeErase) return 0u;
+    if (mode == kModeMixed) {
+      if (shift > 23) return static_cast<u32>(idx);
+      key = (key & ~kStashBit) | (flags[idx] ? kStashBit : 0u);
+    }
+    return values[idx];
+  }
+};
+
 // Shared state of seg_sort_kernel.
 struct SegShared {
   u32 k[2][kSegTile];
   u32 p[2][kSegTile];
-  u32 wave_hist[kSegWaves][256];
+  u32 wave_hist[2][kSegWaves][256];  // alternate between ranking rounds
   u32 digit_start[256];  // chunk-local start of each digit
   u32 run[256];          // running global offsets (global-memory sort)
   u32 scan[kSegWaves + 1];
@@ -119,63 +194,83 @@ struct SegShared {
   u64 base;
 };
 
-// Stable ranking of one chunk of up to kSegTile elements (wave w holds
-// chunk elements [w*1024, (w+1)*1024), item j of lane l is w*1024 + j*64 +
-// l): on return dst_local[j] is each item's position in the chunk sorted
-// by its digit, and sh.digit_start / sh.wave_hist hold the digit layout
-// (count of digit d = digit_start[d+1] - digit_start[d]).
-__device__ __forceinline__ void seg_rank(SegShared &sh, const u32 (&key)[kSegItems],
-                                         u32 count, u32 shift,
+// Stable ranking of one chunk of `count` elements, ipt per lane: wave w
+// holds chunk elements [w*64*ipt, (w+1)*64*ipt), item j of lane l is
+// w*64*ipt + j*64 + l, so all waves share the work of a short chunk.
+// hist (zero on entry) takes the per-wave digit counts; other (the
+// histogram of the previous round) is cleared for the next round.  On
+// return dst_local[j] is each item's position in the chunk sorted by its
+// digit and sh.digit_start the chunk-local start of each digit.
+__device__ __forceinline__ void seg_rank(SegShared &sh, u32 (*hist)[256],
+                                         u32 (*other)[256],
+                                         const u32 (&key)[kSegItems],
+                                         u32 count, u32 ipt, u32 shift,
                                          u32 (&dst_local)[kSegItems]) {
   const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  for (u32 i = tid; i < kSegWaves * 256; i += kSegThreads)
-    (&sh.wave_hist[0][0])[i] = 0;
-  __syncthreads();
-  volatile u32 *my_hist = sh.wave_hist[wave];
+  const u32 wave_base = wave * kWave * ipt;
+  volatile u32 *my_hist = hist[wave];
   u32 rank[kSegItems];
 #pragma unroll
   for (int j = 0; j < kSegItems; ++j) {
-    const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
-    const bool valid = idx < count;
-    const u32 d = (key[j] >> shift) & 0xffu;
-    u64 match = ballot(valid);
+    if (static_cast<u32>(j) < ipt) {
+      const bool valid = wave_base + j * kWave + lane < count;
+      const u32 d = (key[j] >> shift) & 0xffu;
+      u64 match = ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const u64 vote = ballot(bit);
-      match &= bit ? vote : ~vote;
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const u64 vote = ballot(bit);
+        match &= bit ? vote : ~vote;
+      }
+      u32 prior = 0;
+      if (valid) prior = my_hist[d];
+      rank[j] = prior + popc64(match & lanemask_lt());
+      if (valid && (match >> lane) == 1ull) my_hist[d] = prior + popc64(match);
     }
-    u32 prior = 0;
-    if (valid) prior = my_hist[d];
-    rank[j] = prior + popc64(match & lanemask_lt());
-    if (valid && (match >> lane) == 1ull) my_hist[d] = prior + popc64(match);
   }
   __syncthreads();
-  // per digit: wave prefixes, then digit starts (threads 0..255)
+  // per digit (threads 0..255): counts of the waves, digit start from a
+  // scan over digits, then hist[w][d] = start of digit d in wave w
+  u32 c[kSegWaves];
   u32 total = 0;
   if (tid < 256) {
+#pragma unroll
     for (int w = 0; w < kSegWaves; ++w) {
-      const u32 c = sh.wave_hist[w][tid];
-      sh.wave_hist[w][tid] = total;
-      total += c;
+      c[w] = hist[w][tid];
+      total += c[w];
     }
   }
-  const u32 start = block_exclusive_scan<kSegThreads>(tid < 256 ? total : 0u,
-                                                      sh.scan);
-  if (tid < 256) sh.digit_start[tid] = start;
+  const u32 inclusive = wave_inclusive_scan(total);
+  if (tid < 256 && lane == kWave - 1) sh.scan[wave] = inclusive;
+  __syncthreads();
+  if (tid < 256) {
+    u32 start = inclusive - total;
+    for (u32 w = 0; w < wave; ++w) start += sh.scan[w];
+    sh.digit_start[tid] = start;
+#pragma unroll
+    for (int w = 0; w < kSegWaves; ++w) {
+      hist[w][tid] = start;
+      start += c[w];
+    }
+  }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kSegItems; ++j) {
-    const u32 d = (key[j] >> shift) & 0xffu;
-    dst_local[j] = sh.digit_start[d] + sh.wave_hist[wave][d] + rank[j];
+    if (static_cast<u32>(j) < ipt) {
+      const u32 d = (key[j] >> shift) & 0xffu;
+      dst_local[j] = hist[wave][d] + rank[j];
+    }
   }
+  // the previous round's histogram is free: clear it for the next round
+  for (u32 i = tid; i < kSegWaves * 256; i += kSegThreads)
+    (&other[0][0])[i] = 0;
 }
 
 // One workgroup per segment (ticket order = key order).  keys_in: the
 // batch (only its first key is read, for the shared high bits); pk/pp: the
-// partitioned keys and input positions; tk/tp: scratch of n entries for
-// segments too large for LDS.  offsets: segment starts (exclusive scan of
-// the segment histogram).  Writes the run (rows, header, directory).
+// partitioned keys and SegPayload payloads; tk/tp: scratch of n entries
+// for segments too large for LDS.  offsets: segment starts (exclusive scan
+// of the segment histogram).  Writes the run (rows, header, directory).
 template <class T>
 __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     const u32 *__restrict__ keys_in, u32 *pk, u32 *pp, u32 *tk, u32 *tp,
@@ -189,6 +284,8 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   __shared__ u32 s_ticket;
   const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
+  for (u32 i = tid; i < 2 * kSegWaves * 256; i += kSegThreads)
+    (&sh.wave_hist[0][0][0])[i] = 0;
   __syncthreads();
   const u32 seg = s_ticket;  // grid = 256: one ticket per segment
   const u32 s0 = *shift_in;
@@ -196,32 +293,74 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   const u32 end = seg == 255 ? n : offsets[seg + 1];
   const u32 m = end - start;
   const u32 passes = (s0 + 7) / 8;
+  // key bit 31 of the sorted keys may carry the erase flag; restore()
+  // gives the key itself
+  const bool stashed = seg_stashed(mode, s0);
+  const bool gathered = mode == kModeMixed && !stashed;
+  const u32 key_mask = stashed ? ~kStashBit : ~0u;
+  const u32 high_bit = keys_in[0] & kStashBit;
+  auto restore = [&](u32 x) { return stashed ? (x & ~kStashBit) | high_bit : x; };
+  int round = 0;  // ranking rounds: selects the histogram buffer
 
   // ---------------------------------------------------------------- sort
-  const u32 *sk;  // sorted keys / positions of the segment (LDS or global)
+  const u32 *sk;  // sorted keys / payloads of the segment (LDS or global)
   const u32 *sp;
   if (m <= static_cast<u32>(kSegTile)) {
-    for (u32 i = tid; i < m; i += kSegThreads) {
-      sh.k[0][i] = pk[start + i];
-      sh.p[0][i] = pp[start + i];
+    const u32 ipt = m ? (m + kSegThreads - 1) / kSegThreads : 1u;
+    const u32 wave_base = wave * kWave * ipt;
+    if (passes == 0) {
+      // nothing to sort (each segment holds one key): copy it to LDS, all
+      // loads of a thread issued before the first store
+      u32 lk[kSegItems], lp[kSegItems];
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 i = tid + j * kSegThreads;
+        if (i < m) {
+          lk[j] = pk[start + i];
+          lp[j] = pp[start + i];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 i = tid + j * kSegThreads;
+        if (i < m) {
+          sh.k[0][i] = lk[j];
+          sh.p[0][i] = lp[j];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
     int cur = 0;
     for (u32 pass = 0; pass < passes; ++pass) {
-      u32 key[kSegItems], pos[kSegItems], dst[kSegItems];
+      // the first pass reads the partitioned segment from global memory,
+      // the later ones the previous pass's LDS arrays
+      u32 key[kSegItems], pay[kSegItems], dst[kSegItems];
+      if (pass == 0) {
 #pragma unroll
-      for (int j = 0; j < kSegItems; ++j) {
-        const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
-        key[j] = idx < m ? sh.k[cur][idx] : 0xffffffffu;
-        pos[j] = idx < m ? sh.p[cur][idx] : 0u;
+        for (int j = 0; j < kSegItems; ++j) {
+          const u32 idx = wave_base + j * kWave + lane;
+          const bool valid = static_cast<u32>(j) < ipt && idx < m;
+          key[j] = valid ? pk[start + idx] : 0xffffffffu;
+          pay[j] = valid ? pp[start + idx] : 0u;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < kSegItems; ++j) {
+          const u32 idx = wave_base + j * kWave + lane;
+          const bool valid = static_cast<u32>(j) < ipt && idx < m;
+          key[j] = valid ? sh.k[cur][idx] : 0xffffffffu;
+          pay[j] = valid ? sh.p[cur][idx] : 0u;
+        }
       }
-      seg_rank(sh, key, m, 8 * pass, dst);
+      seg_rank(sh, sh.wave_hist[round & 1], sh.wave_hist[(round + 1) & 1],
+               key, m, ipt, 8 * pass, dst);
+      ++round;
 #pragma unroll
       for (int j = 0; j < kSegItems; ++j) {
-        const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
-        if (idx < m) {
+        const u32 idx = wave_base + j * kWave + lane;
+        if (static_cast<u32>(j) < ipt && idx < m) {
           sh.k[cur ^ 1][dst[j]] = key[j];
-          sh.p[cur ^ 1][dst[j]] = pos[j];
+          sh.p[cur ^ 1][dst[j]] = pay[j];
         }
       }
       __syncthreads();
@@ -248,22 +387,27 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       }
       for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
         const u32 cnt = m - c0 < u32(kSegTile) ? m - c0 : u32(kSegTile);
-        u32 key[kSegItems], pos[kSegItems], dst[kSegItems];
+        const u32 ipt = (cnt + kSegThreads - 1) / kSegThreads;
+        const u32 wave_base = wave * kWave * ipt;
+        u32 key[kSegItems], pay[kSegItems], dst[kSegItems];
 #pragma unroll
         for (int j = 0; j < kSegItems; ++j) {
-          const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
-          key[j] = idx < cnt ? src_k[c0 + idx] : 0xffffffffu;
-          pos[j] = idx < cnt ? src_p[c0 + idx] : 0u;
+          const u32 idx = wave_base + j * kWave + lane;
+          const bool valid = static_cast<u32>(j) < ipt && idx < cnt;
+          key[j] = valid ? src_k[c0 + idx] : 0xffffffffu;
+          pay[j] = valid ? src_p[c0 + idx] : 0u;
         }
-        seg_rank(sh, key, cnt, shift, dst);
+        seg_rank(sh, sh.wave_hist[round & 1], sh.wave_hist[(round + 1) & 1],
+                 key, cnt, ipt, shift, dst);
+        ++round;
 #pragma unroll
         for (int j = 0; j < kSegItems; ++j) {
-          const u32 idx = wave * (kWave * kSegItems) + j * kWave + lane;
-          if (idx < cnt) {
+          const u32 idx = wave_base + j * kWave + lane;
+          if (static_cast<u32>(j) < ipt && idx < cnt) {
             const u32 d = (key[j] >> shift) & 0xffu;
             const u32 o = sh.run[d] + dst[j] - sh.digit_start[d];
             dst_k[o] = key[j];
-            dst_p[o] = pos[j];
+            dst_p[o] = pay[j];
           }
         }
         __syncthreads();
@@ -288,14 +432,14 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
 
   // ------------------------------------------- survivors: count, offset
   // keep the last occurrence of each key (stable sort: the newest update)
+  auto keep_at = [&](u32 i) {
+    return i < m && (i + 1 == m || ((sk[i + 1] ^ sk[i]) & key_mask) != 0);
+  };
   u32 total = 0;
   for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
     u32 cnt = 0;
 #pragma unroll
-    for (int j = 0; j < kSegItems; ++j) {
-      const u32 i = c0 + j * kSegThreads + tid;
-      cnt += i < m && (i + 1 == m || sk[i + 1] != sk[i]);
-    }
+    for (int j = 0; j < kSegItems; ++j) cnt += keep_at(c0 + j * kSegThreads + tid);
     total += wave_sum_u32(cnt);
   }
   if (lane == 0) sh.scan[wave] = total;
@@ -326,40 +470,46 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     u64 mask[kSegItems];
 #pragma unroll
     for (int j = 0; j < kSegItems; ++j) {
-      const u32 i = c0 + j * kSegThreads + tid;
-      const bool keep = i < m && (i + 1 == m || sk[i + 1] != sk[i]);
-      mask[j] = ballot(keep);
+      mask[j] = ballot(keep_at(c0 + j * kSegThreads + tid));
       if (lane == 0) sh.off[j][wave] = popc64(mask[j]);
     }
     __syncthreads();
-    if (tid == 0) {
-      u32 r = running;
-      for (int j = 0; j < kSegItems; ++j)
-        for (int w = 0; w < kSegWaves; ++w) {
-          const u32 c = sh.off[j][w];
-          sh.off[j][w] = r;
-          r += c;
-        }
-      sh.misc[3] = r;
+    // exclusive scan of the (item, wave) survivor counts in row order, by
+    // wave 0: lane l owns entries 2l and 2l + 1
+    if (wave == 0) {
+      u32 *flat = &sh.off[0][0];
+      const u32 a = flat[2 * lane], b = flat[2 * lane + 1];
+      const u32 inc = wave_inclusive_scan(a + b);
+      const u32 ex = running + inc - (a + b);
+      flat[2 * lane] = ex;
+      flat[2 * lane + 1] = ex + a;
+      if (lane == kWave - 1) sh.misc[3] = running + inc;
     }
     __syncthreads();
-    // the gathers of all items are issued before any row is written, so
-    // their latencies overlap
-    u32 k[kSegItems], prev[kSegItems], pos[kSegItems];
+    u32 k[kSegItems], prev[kSegItems], pay[kSegItems];
+    bool tomb[kSegItems];
 #pragma unroll
     for (int j = 0; j < kSegItems; ++j) {
       const u32 i = c0 + j * kSegThreads + tid;
-      k[j] = i < m ? sk[i] : 0u;
-      prev[j] = (i < m && i > 0) ? sk[i - 1] : k[j];
-      pos[j] = (i < m && ((mask[j] >> lane) & 1ull)) ? sp[i] : 0u;
+      const u32 raw = i < m ? sk[i] : 0u;
+      k[j] = restore(raw);
+      prev[j] = (i < m && i > 0) ? restore(sk[i - 1]) : k[j];
+      pay[j] = (i < m && ((mask[j] >> lane) & 1ull)) ? sp[i] : 0u;
+      tomb[j] = mode == kModeErase || (stashed && (raw & kStashBit));
     }
+    // a position payload: gather value and flag (all items issued before
+    // any row is written, so their latencies overlap)
     u32 val[kSegItems];
-    u8 fl[kSegItems];
 #pragma unroll
     for (int j = 0; j < kSegItems; ++j) {
-      const bool kept = (mask[j] >> lane) & 1ull;
-      fl[j] = (kept && mode == kModeMixed) ? erase_flags[pos[j]] : u8(0);
-      val[j] = (kept && mode != kModeErase) ? values[pos[j]] : 0u;
+      const u32 i = c0 + j * kSegThreads + tid;
+      const bool kept = i < m && ((mask[j] >> lane) & 1ull);
+      if (gathered) {
+        tomb[j] = kept && erase_flags[pay[j]] != 0;
+        val[j] = kept ? values[pay[j]] : 0u;
+      } else {
+        val[j] = pay[j];
+      }
     }
 #pragma unroll
     for (int j = 0; j < kSegItems; ++j) {
@@ -373,11 +523,10 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
           dir[b] = before;
       }
       if ((mask[j] >> lane) & 1ull) {
-        const bool tomb = mode == kModeErase || fl[j] != 0;
-        out[before] = T::make((k[j] & T::kLow) | (tomb ? T::kTomb : 0u),
-                              tomb ? 0u : val[j]);
+        out[before] = T::make((k[j] & T::kLow) | (tomb[j] ? T::kTomb : 0u),
+                              tomb[j] ? 0u : val[j]);
         low += k[j] < T::kTomb;
-        tombs += tomb;
+        tombs += tomb[j];
       }
     }
     running = sh.misc[3];
@@ -404,14 +553,16 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   const u64 first_b = (lo_key + ub - 1) >> shift;  // first bound >= lo_key
   const u64 end_b = (hi_key + ub - 1) >> shift;    // first bound >= hi_key
   const u32 seg_end_row = static_cast<u32>(base) + seg_total;
+  const u32 first_key = m ? restore(sk[0]) : 0u;
+  const u32 last_key = m ? restore(sk[m - 1]) : 0u;
   if (m == 0) {
     for (u64 b = first_b + tid; b < end_b; b += kSegThreads)
       dir[b] = static_cast<u32>(base);
   } else {
-    const u64 k0b = u64(sk[0]) >> shift;  // bounds in [lo_key, sk[0]]
+    const u64 k0b = u64(first_key) >> shift;  // bounds in [lo_key, first]
     for (u64 b = first_b + tid; b <= k0b; b += kSegThreads)
       dir[b] = static_cast<u32>(base);
-    const u64 klb = (u64(sk[m - 1]) >> shift) + 1;  // bounds after the last
+    const u64 klb = (u64(last_key) >> shift) + 1;  // bounds after the last
     for (u64 b = klb + tid; b < end_b; b += kSegThreads) dir[b] = seg_end_row;
   }
 
@@ -424,11 +575,11 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     if (sh.misc[2]) meta->tombs = 1;
     if (seg == 0) meta->dir_bits = dir_bits;
     if (m && start == 0) {  // the first keys of the run
-      meta->lo_bucket = sk[0] >> shift;
-      dir[sk[0] >> shift] = 0;
+      meta->lo_bucket = first_key >> shift;
+      dir[first_key >> shift] = 0;
     }
     if (m && end == n) {  // the last keys of the run
-      const u32 hb = sk[m - 1] >> shift;
+      const u32 hb = last_key >> shift;
       meta->hi_bucket = hb;
       dir[hb + 1] = seg_end_row;
     }

@@ -370,13 +370,13 @@ class Server : public ServerBase {
   static u64 request_bytes(gd_op op, u64 count) {
     // staging of the coalesced batch: keys + values + flags, or queries +
     // results (+ found), plus the grouped-lookup scratch (queries, grouped
-    // keys, destinations, grouped results and found flags)
+    // keys, destinations, grouped results with their found flags)
     constexpr u64 kb = sizeof(K), vb = sizeof(V);
     switch (op) {
       case GD_OP_INSERT:
       case GD_OP_ERASE:  // + sort scratch (two key and two position arrays)
         return count * (kb + vb + 1) + count * (2 * kb + 8) + count * 8;
-      case GD_OP_LOOKUP: return count * (kb + vb + 1 + 2 * kb + 4 + vb + 1);
+      case GD_OP_LOOKUP: return count * (kb + vb + 1 + 2 * kb + 4 + 2 * vb);
       default: return count * (2 * kb + vb);
     }
   }
@@ -522,21 +522,19 @@ class Server : public ServerBase {
         K *q = scratch<K>(st, n_look);
         K *qk = scratch<K>(st, n_look);
         u32 *dest = scratch<u32>(st, n_look);
-        V *rv = scratch<V>(st, n_look);
-        u8 *rf = scratch<u8>(st, n_look);
-        u32 *ctrl = scratch<u32>(st, rs::control_words(n_look, 1));
+        auto *res = scratch<lsm::GroupedResult<T>>(st, n_look);
+        u32 *ctrl = scratch<u32>(st, lsm::SpreadLayout<K>(n_look).words);
         query_gather_kernel<K><<<grid_for(n_look, 256, 1u << 14), 256, 0,
                                   qs_>>>(
             dev_look, static_cast<u32>(look.size()), n_look, q);
-        GD_HIP(rs::partition_pairs<K>(q, static_cast<u32>(n_look),
-                                      T::kBits - 8, qk,
-                                        nullptr, ctrl, qs_,
-                                        lsm_.tuning().sort_block, dest));
+        lsm::partition_by_spread<K>(q, static_cast<u32>(n_look), qk, nullptr,
+                                    dest, rs::PayloadPos{}, ctrl, qs_,
+                                    lsm_.tuning().sort_block);
         lsm::lookup_grouped_kernel<T><<<lsm::grouped_lookup_blocks(n_look), 256,
-                                     0, qs_>>>(qk, n_look, rv, rf, snap);
+                                     0, qs_>>>(qk, n_look, res, snap);
         GD_LAUNCH_CHECK();
-        lsm::lookup_gather_kernel<T><<<grid_for(n_look, 256, 1u << 16), 256, 0,
-                                    qs_>>>(io, dest, n_look, rv, rf);
+        lsm::lookup_gather_kernel<T><<<lsm::grouped_lookup_blocks(n_look), 256, 0,
+                                    qs_>>>(io, dest, n_look, res);
       } else {
         lsm::lookup_kernel<T><<<grid_for(n_look, 256, 1u << 16), 256, 0, qs_>>>(
             io, n_look, snap);
@@ -545,7 +543,9 @@ class Server : public ServerBase {
     }
     if (n_rng) {
       // counts and sums share the kernel: each request chooses its own
-      lsm::range_kernel<T><<<grid_for(n_rng, lsm::kRangeWaves, 1u << 16),
+      const u64 waves = (n_rng + lsm::range_queries_per_wave(snap.count) - 1) /
+                        lsm::range_queries_per_wave(snap.count);
+      lsm::range_kernel<T><<<grid_for(waves, lsm::kRangeWaves, 1u << 16),
                              lsm::kRangeThreads, 0, qs_>>>(
           PieceRange<T>{dev_rng, static_cast<u32>(rng.size())}, n_rng, snap);
       GD_LAUNCH_CHECK();

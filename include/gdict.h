@@ -158,8 +158,8 @@ typedef struct gd_lsm_tuning {
   uint32_t max_runs;               /* runs kept before the smallest adjacent
                                       pair merges, 2..47 (10)              */
   uint32_t segmented_admission;    /* 32-bit batches above 16384 keys: 1 =
-                                      partition + per-segment LDS sort, 2 =
-                                      radix passes + finalize; 0 = keep    */
+                                      partition + per-segment LDS sort (1),
+                                      2 = radix passes + finalize; 0 = keep */
   uint32_t reserved;
 } gd_lsm_tuning;
 

@@ -390,6 +390,7 @@ def test_config2_scaled(gd, orc, io):
     {"max_runs": 2, "small_batch_max": 1024, "sort_block": 512},
     {"max_runs": 40, "grouped_lookup_min_log": 63, "merge_tile_log": 12},
     {"segmented_admission": 1, "dir_rows_log": 2, "small_batch_max": 1024},
+    {"segmented_admission": 2, "dir_rows_log": 4, "small_batch_max": 1024},
 ])
 def test_tuning_knobs_keep_results(gd, orc, io, knobs):
     """Launch-shape knobs change speed only: a mixed stream with small and
@@ -505,23 +506,31 @@ def test_snapshot_save_restore(gd, orc, io, tmp_path):
         small.restore(path)
 
 
-@pytest.mark.parametrize("span,dup", [(1 << 12, False), (1 << 20, False),
-                                      (1 << 32, False), (1 << 32, True),
-                                      (1 << 31, True)])
-def test_segmented_admission(gd, orc, io, span, dup):
+@pytest.mark.parametrize("span,dup,base", [
+    (1 << 12, False, 0), (1 << 20, False, 0), (1 << 32, False, 0),
+    (1 << 32, True, 0), (1 << 31, True, 0), (1 << 20, False, 0xFFF00000),
+    (1 << 31, True, 1 << 31), (1 << 8, True, 0x9000_0000)])
+def test_segmented_admission(gd, orc, io, span, dup, base):
     """The partition + per-segment LDS sort admission (32-bit batches above
     16384 keys) against the oracle: narrow key spreads (few varying bits),
-    the full key space, and hot keys whose segment exceeds the LDS tile
-    (sorted in global memory by its workgroup)."""
-    rng = np.random.default_rng(span % 1000 + dup)
+    the full key space (erase flags travel as positions), keys that all
+    have the top bit set (the flag rides in that bit during the sort), and
+    hot keys whose segment exceeds the LDS tile (sorted in global memory by
+    its workgroup)."""
+    rng = np.random.default_rng(span % 1000 + dup + base % 997)
     p = Pair(gd, orc, io, tuning={"segmented_admission": 1},
              max_elements=1 << 21, batch_capacity=1 << 17)
     assert p.d.get_tuning()["segmented_admission"] == 1
+
+    def draw(size):
+        return (base + rng.integers(0, span, size, dtype=np.uint64)) \
+            .astype(np.uint32)
+
     for b in range(10):
         n = int(rng.integers(16385, 1 << 17))
-        keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+        keys = draw(n)
         if dup:  # ~40% of the batch on a handful of hot keys
-            hot = rng.integers(0, span, 5, dtype=np.uint64).astype(np.uint32)
+            hot = draw(5)
             sel = rng.random(n) < 0.4
             keys[sel] = hot[rng.integers(0, 5, int(sel.sum()))]
         vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
@@ -537,9 +546,10 @@ def test_segmented_admission(gd, orc, io, span, dup):
             p.mixed(keys, vals, erase)
         if b % 3 == 2 or b == 9:
             p.check_contents()
-            q = _queries(rng, p.o.dump()[0], 1 << 14, span)
+            q = _queries(rng, p.o.dump()[0], 1 << 14, span) + np.uint32(base)
+            q[: 1 << 13] -= np.uint32(base)  # the live keys drawn above
             p.check_lookup(q, True)
-            lo = rng.integers(0, span, 1000, dtype=np.uint64)
+            lo = base + rng.integers(0, span, 1000, dtype=np.uint64)
             hi = np.minimum(lo + rng.integers(0, max(span // 256, 2), 1000,
                                               dtype=np.uint64), 0xFFFFFFFF)
             p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
