@@ -956,6 +956,10 @@ struct PlainLookup {
   const typename T::K *q;
   typename T::V *out;
   u8 *found;
+  // IO policies may copy a table they search per query into LDS:
+  // stage(lds) returns the policy to use after the block's next barrier
+  static constexpr u32 kStageBytes = 0;
+  __device__ __forceinline__ PlainLookup stage(void *) const { return *this; }
   struct Slot {
     u64 i;
   };
@@ -992,7 +996,10 @@ __device__ __forceinline__ bool lookup_one(const RunSet<T> &runs,
 }
 
 template <class T, class IO>
-__global__ void lookup_kernel(IO io, u64 n, RunSet<T> runs) {
+__global__ void lookup_kernel(IO io_arg, u64 n, RunSet<T> runs) {
+  __shared__ __align__(16) char s_io[IO::kStageBytes ? IO::kStageBytes : 16];
+  const IO io = io_arg.stage(s_io);
+  __syncthreads();
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride) {
@@ -1149,8 +1156,11 @@ __global__ __launch_bounds__(256) void lookup_grouped_kernel(
 // fetched into more than one XCD's L2.
 template <class T, class IO>
 __global__ __launch_bounds__(256) void lookup_gather_kernel(
-    IO io, const u32 *__restrict__ dest, u64 n,
+    IO io_arg, const u32 *__restrict__ dest, u64 n,
     const GroupedResult<T> *__restrict__ res) {
+  __shared__ __align__(16) char s_io[IO::kStageBytes ? IO::kStageBytes : 16];
+  const IO io = io_arg.stage(s_io);
+  __syncthreads();
   constexpr int kQ = 4;
   const u32 G = gridDim.x;  // a multiple of 8
   const u32 logical = (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
@@ -1201,6 +1211,8 @@ struct PlainRange {
   const typename T::K *lo, *hi;
   typename T::V *out;
   int count_only;
+  static constexpr u32 kStageBytes = 0;
+  __device__ __forceinline__ PlainRange stage(void *) const { return *this; }
   struct Slot {
     u64 q;
   };
@@ -1224,8 +1236,37 @@ __host__ __device__ __forceinline__ u32 range_queries_per_wave(u32 runs) {
   return per >= static_cast<u32>(kWave) ? 1u : static_cast<u32>(kWave) / per;
 }
 
+// One wave per round of qw queries.  The lanes first resolve the lower and
+// upper bounds of the round's queries in every run at once (one lane per
+// query, run and side).  A row counts when it is live and no newer run
+// holds its key.  Then, for the whole round at once:
+//   * the rows of every run newer than a query's oldest run with rows are
+//     staged in LDS (decoded keys, tombstone bits) and set bits of a round
+//     filter;
+//   * a count over an oldest run without tombstones takes that run's row
+//     count, minus one for every key whose newest version lies in a newer
+//     run and that the oldest run also holds (one point search per newer
+//     key), plus the newer keys' own newest versions that are live; only
+//     the newer rows are visited;
+//   * otherwise every row of the query's segments is visited, and a row of
+//     a run other than the newest searches the staged keys of the newer
+//     runs when its filter bit is set;
+// with the work items of all queries of the round flattened over the
+// lanes, so a round costs a few memory round trips however many queries
+// it holds.  When the staged keys of a round exceed kRangeCache, its
+// queries are answered one at a time by the whole wave instead.  Sums
+// accumulate in 64 bits and are truncated to the value width.
+constexpr u32 kRangeCache = 1024;        // staged keys per wave
+constexpr u32 kRangeFilterLog = 13;      // filter bits per wave (log2)
+
+template <class K>
+__device__ __forceinline__ u32 range_filter_bit(K key) {
+  u32 x = static_cast<u32>(key);
+  if constexpr (sizeof(K) == 8) x ^= static_cast<u32>(key >> 32);
+  return (x * 0x9E3779B1u) >> (32 - kRangeFilterLog);
+}
 template <class T, class IO>
-__global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io, u64 n,
+__global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
                                                               RunSet<T> runs) {
   using K = typename T::K;
   constexpr u32 kSlots =
@@ -1233,10 +1274,42 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io, u64 n,
   // s_b[wave][2 * (qi * runs + r) + side]: lower (side 0) and upper (side
   // 1) bound of query qi of the round in run r
   __shared__ u32 s_b[kRangeWaves][kSlots];
+  __shared__ RunMeta s_meta[kMaxRuns];
+  __shared__ K s_cache[kRangeWaves][kRangeCache];
+  __shared__ u32 s_filter[kRangeWaves][(1u << kRangeFilterLog) / 32];
+  __shared__ u32 s_tomb[kRangeWaves][kRangeCache / 32];
+  constexpr u32 kQMax = kWave / 2;  // queries per round (two bounds each)
+  __shared__ typename IO::Slot s_slot[kRangeWaves][kQMax];
+  // per query of the round: flags (exists, a <= z, count), plan (first |
+  // oldest << 8 | fast << 16), work and staged-key offsets, accumulator
+  __shared__ u32 s_flags[kRangeWaves][kQMax];
+  __shared__ u32 s_plan[kRangeWaves][kQMax];
+  __shared__ u32 s_woff[kRangeWaves][kQMax];
+  __shared__ u32 s_coff[kRangeWaves][kQMax];
+  __shared__ u32 s_stg[kRangeWaves][kQMax];  // newer rows (staged)
+  __shared__ u32 s_wrk[kRangeWaves][kQMax];  // work items
+  __shared__ unsigned long long s_acc[kRangeWaves][kQMax];
+  __shared__ __align__(16) char s_io[IO::kStageBytes ? IO::kStageBytes : 16];
+  const IO io = io_arg.stage(s_io);
+  const u32 nr = runs.count;
+  if (threadIdx.x < nr) s_meta[threadIdx.x] = *runs.run[threadIdx.x].meta;
+  __syncthreads();
+  if (nr == 0) {  // no runs: every range is empty
+    const u64 stride = u64(gridDim.x) * blockDim.x;
+    for (u64 q = u64(blockIdx.x) * blockDim.x + threadIdx.x; q < n;
+         q += stride) {
+      typename IO::Slot slot;
+      K a, z;
+      bool count_only;
+      io.load(q, slot, a, z, count_only);
+      io.put(slot, typename T::V(0));
+    }
+    return;
+  }
   const u32 lane = lane_id();
   const u32 wave = threadIdx.x / kWave;
-  const u32 nr = runs.count;
-  const u32 per = 2 * nr;
+  const u32 per = 2 * nr;  // >= 2: the bounds pass writes every query's
+                           // flags and slot
   const u32 qw = range_queries_per_wave(nr);
   const u64 wave_id = u64(blockIdx.x) * kRangeWaves + wave;
   const u64 wave_stride = u64(gridDim.x) * kRangeWaves;
@@ -1246,75 +1319,311 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io, u64 n,
       const u64 q = q0 + t / per;
       const u32 r = (t % per) >> 1;
       u32 b = 0;
+      u32 flags = 0;
+      typename IO::Slot slot{};
       if (q < n) {
-        typename IO::Slot slot;
         K a, z;
         bool count_only;
         io.load(q, slot, a, z, count_only);
+        flags = 1u | (a <= z ? 2u : 0u) | (count_only ? 4u : 0u);
         if (a <= z) {
           const RunView<T> &rv = runs.run[r];
-          const RunMeta m = *rv.meta;
+          const RunMeta m = s_meta[r];
           b = (t & 1) ? run_upper_bound(rv, m, z) : run_lower_bound(rv, m, a);
         }
       }
       s_b[wave][t] = b;
+      if (t % per == 0) {
+        s_slot[wave][t / per] = slot;
+        s_flags[wave][t / per] = flags;
+      }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (u32 qi = 0; qi < qw; ++qi) {
+
+    // ---- plan of query q0 + lane (lanes below qw)
+    u32 work = 0, staged = 0;
+    if (lane < qw) {
+      const u32 flags = s_flags[wave][lane];
+      u32 plan = 0;
+      u64 base = 0;
+      if ((flags & 3u) == 3u) {
+        const u32 *bnd = &s_b[wave][lane * per];
+        u32 first = nr, oldest = 0;
+        for (u32 r = 0; r < nr; ++r) {
+          if (bnd[2 * r] < bnd[2 * r + 1]) {
+            if (first == nr) first = r;
+            oldest = r;
+          }
+        }
+        if (first < nr) {
+          u32 newer = 0;
+          for (u32 r = first; r < oldest; ++r) newer += bnd[2 * r + 1] - bnd[2 * r];
+          const u32 old_rows = bnd[2 * oldest + 1] - bnd[2 * oldest];
+          const bool fast = (flags & 4u) && s_meta[oldest].tombs == 0;
+          staged = newer;
+          work = fast ? newer : newer + old_rows;
+          if (fast) base = old_rows;
+          plan = first | (oldest << 8) | (fast ? 1u << 16 : 0u);
+        }
+      }
+      s_plan[wave][lane] = plan;
+      s_acc[wave][lane] = base;
+    }
+    const u32 work_inc = wave_inclusive_scan(work);
+    const u32 staged_inc = wave_inclusive_scan(staged);
+    if (lane < qw) {
+      s_woff[wave][lane] = work_inc - work;
+      s_coff[wave][lane] = staged_inc - staged;
+      s_stg[wave][lane] = staged;
+      s_wrk[wave][lane] = work;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+    // one query by the whole wave: every row of its segments, the newer
+    // runs searched in place (its newer rows exceed the stage)
+    auto alone = [&](u32 qi) {
       const u64 q = q0 + qi;
-      if (q >= n) break;
+      if (q >= n) return;
       typename IO::Slot slot;
       K a, z;
       bool count_only;
       io.load(q, slot, a, z, count_only);
       if (a > z) {
         if (lane == 0) io.put(slot, typename T::V(0));
-        continue;
+        return;
       }
       const u32 *bnd = &s_b[wave][qi * per];  // [2r] lower, [2r + 1] upper
-      u64 acc = 0;
-      bool newer_rows = false;
+      // the newest run with rows in the range
+      u32 first = nr;
       for (u32 r = 0; r < nr; ++r) {
-        const RunView<T> &rv = runs.run[r];
-        const u32 L = bnd[2 * r], H = bnd[2 * r + 1];
-        if (L >= H) continue;
-        const u32 tombs = rv.meta->tombs;
-        if (count_only && !newer_rows && tombs == 0) {
-          if (lane == 0) acc += H - L;
-          newer_rows = true;
-          continue;
+        if (bnd[2 * r] < bnd[2 * r + 1]) {
+          first = r;
+          break;
         }
-        const u32 split = rv.meta->split;
-        for (u32 idx = L + lane; idx < H; idx += kWave) {
-          const typename T::Row w = rv.rows[idx];
-          if (T::kw(w) & T::kTomb) continue;
-          bool shadowed = false;
-          if (newer_rows) {
-            const K key = decode_key<T>(T::kw(w), idx, split);
-            for (u32 s = 0; s < r && !shadowed; ++s) {
-              u32 l2 = bnd[2 * s], h2 = bnd[2 * s + 1];
-              if (l2 >= h2) continue;
-              const RunView<T> &nv = runs.run[s];
-              const u32 ns = nv.meta->split;
-              while (l2 < h2) {
-                const u32 mid = (l2 + h2) >> 1;
-                if (decode_key<T>(T::kw(nv.rows[mid]), mid, ns) < key)
-                  l2 = mid + 1;
-                else
-                  h2 = mid;
-              }
-              if (l2 < bnd[2 * s + 1] &&
-                  decode_key<T>(T::kw(nv.rows[l2]), l2, ns) == key)
-                shadowed = true;
-            }
+      }
+      u64 acc = 0;
+      u32 skip = nr;  // a run counted without a scan
+      if (first < nr && count_only && s_meta[first].tombs == 0) {
+        if (lane == 0) acc = bnd[2 * first + 1] - bnd[2 * first];
+        skip = first;
+      }
+      u32 total = 0, oldest = first;
+      for (u32 r = first; r < nr; ++r) {
+        const u32 cnt = bnd[2 * r + 1] - bnd[2 * r];
+        if (r != skip) total += cnt;
+        if (cnt) oldest = r;
+      }
+      // stage the keys of runs first .. oldest - 1, run after run
+      u32 cached = 0;
+      for (u32 r = first; r < oldest; ++r) cached += bnd[2 * r + 1] - bnd[2 * r];
+      const bool use_cache = cached <= kRangeCache;
+      u32 *filter = s_filter[wave];
+      if (cached) {
+        // (the previous query's reads of s_cache and s_filter are done)
+        __builtin_amdgcn_wave_barrier();
+        for (u32 i = lane; i < (1u << kRangeFilterLog) / 32; i += kWave)
+          filter[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (u32 f = lane; f < cached; f += kWave) {
+          u32 r = first, off = f;
+          while (off >= bnd[2 * r + 1] - bnd[2 * r]) {
+            off -= bnd[2 * r + 1] - bnd[2 * r];
+            ++r;
           }
-          if (!shadowed) acc += count_only ? u64(1) : u64(T::val(w));
+          const u32 idx = bnd[2 * r] + off;
+          const K key = decode_key<T>(T::kw(runs.run[r].rows[idx]), idx,
+                                      s_meta[r].split);
+          if (use_cache) s_cache[wave][f] = key;
+          const u32 bit = range_filter_bit(key);
+          atomicOr(&filter[bit >> 5], 1u << (bit & 31));
         }
-        newer_rows = true;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+      for (u32 f = lane; f < total; f += kWave) {
+        // run r and row idx of flattened row f
+        u32 r = first, off = f;
+        while (true) {
+          if (r != skip) {
+            const u32 cnt = bnd[2 * r + 1] - bnd[2 * r];
+            if (off < cnt) break;
+            off -= cnt;
+          }
+          ++r;
+        }
+        const u32 idx = bnd[2 * r] + off;
+        const RunView<T> &rv = runs.run[r];
+        const typename T::Row w = rv.rows[idx];
+        if (T::kw(w) & T::kTomb) continue;
+        bool shadowed = false;
+        const K key = decode_key<T>(T::kw(w), idx, s_meta[r].split);
+        const u32 bit = range_filter_bit(key);
+        if (r > first && ((filter[bit >> 5] >> (bit & 31)) & 1u)) {
+          u32 base = 0;  // staged keys of runs first .. s - 1
+          for (u32 s = first; s < r && !shadowed; ++s) {
+            u32 l2 = bnd[2 * s], h2 = bnd[2 * s + 1];
+            const u32 cnt = h2 - l2;
+            if (!cnt) continue;
+            if (use_cache) {
+              const K *c = &s_cache[wave][base];
+              u32 lo = 0, hi = cnt;
+              while (lo < hi) {
+                const u32 mid = (lo + hi) >> 1;
+                if (c[mid] < key) lo = mid + 1;
+                else hi = mid;
+              }
+              shadowed = lo < cnt && c[lo] == key;
+              base += cnt;
+              continue;
+            }
+            const RunView<T> &nv = runs.run[s];
+            const u32 ns = s_meta[s].split;
+            const u32 end = h2;
+            while (l2 < h2) {
+              const u32 mid = (l2 + h2) >> 1;
+              if (decode_key<T>(T::kw(nv.rows[mid]), mid, ns) < key)
+                l2 = mid + 1;
+              else
+                h2 = mid;
+            }
+            if (l2 < end && decode_key<T>(T::kw(nv.rows[l2]), l2, ns) == key)
+              shadowed = true;
+          }
+        }
+        if (!shadowed) acc += count_only ? u64(1) : u64(T::val(w));
       }
       acc = wave_sum_u64(acc);
       if (lane == 0) io.put(slot, static_cast<typename T::V>(acc));
+        };
+
+    // query of a flattened index f among queries [g0, g1): the last query
+    // whose offset is <= f
+    auto query_in = [&](const u32 *off, u32 g0, u32 g1, u32 f) {
+      u32 lo = g0, hi = g1;
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (off[mid] <= f) lo = mid;
+        else hi = mid;
+      }
+      return lo;
+    };
+    u32 *filter = s_filter[wave];
+    for (u32 g0 = 0; g0 < qw;) {
+      // the group [g0, g1): consecutive queries whose newer rows fit the
+      // stage together
+      const u32 cbase = s_coff[wave][g0];
+      u32 g1 = g0;
+      while (g1 < qw &&
+             s_coff[wave][g1] + s_stg[wave][g1] - cbase <= kRangeCache)
+        ++g1;
+      if (g1 == g0) {
+        alone(g0);
+        ++g0;
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
+      const u32 wbase = s_woff[wave][g0];
+      const u32 wend = s_woff[wave][g1 - 1] + s_wrk[wave][g1 - 1];
+      const u32 ncache = s_coff[wave][g1 - 1] + s_stg[wave][g1 - 1] - cbase;
+      if (ncache) {
+        // (the previous group's reads of the stage and filter are done)
+        __builtin_amdgcn_wave_barrier();
+        for (u32 i = lane; i < (1u << kRangeFilterLog) / 32; i += kWave)
+          filter[i] = 0;
+        for (u32 i = lane; i < kRangeCache / 32; i += kWave)
+          s_tomb[wave][i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // stage the newer rows: query after query, run after run
+        for (u32 f = cbase + lane; f < cbase + ncache; f += kWave) {
+          const u32 qi = query_in(s_coff[wave], g0, g1, f);
+          const u32 *bnd = &s_b[wave][qi * per];
+          u32 r = s_plan[wave][qi] & 0xffu, off = f - s_coff[wave][qi];
+          while (off >= bnd[2 * r + 1] - bnd[2 * r]) {
+            off -= bnd[2 * r + 1] - bnd[2 * r];
+            ++r;
+          }
+          const u32 idx = bnd[2 * r] + off;
+          const typename T::K w = T::kw(runs.run[r].rows[idx]);
+          const K key = decode_key<T>(w, idx, s_meta[r].split);
+          const u32 ci = f - cbase;
+          s_cache[wave][ci] = key;
+          if (w & T::kTomb) atomicOr(&s_tomb[wave][ci >> 5], 1u << (ci & 31));
+          const u32 bit = range_filter_bit(key);
+          atomicOr(&filter[bit >> 5], 1u << (bit & 31));
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+      // whether a run newer than r (from `first` on) of query qi holds key
+      auto newer_holds = [&](u32 qi, u32 first, u32 r, K key) {
+        const u32 *bnd = &s_b[wave][qi * per];
+        u32 base = s_coff[wave][qi] - cbase;
+        for (u32 s2 = first; s2 < r; ++s2) {
+          const u32 cnt = bnd[2 * s2 + 1] - bnd[2 * s2];
+          if (!cnt) continue;
+          const K *c = &s_cache[wave][base];
+          u32 lo = 0, hi = cnt;
+          while (lo < hi) {
+            const u32 mid = (lo + hi) >> 1;
+            if (c[mid] < key) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo < cnt && c[lo] == key) return true;
+          base += cnt;
+        }
+        return false;
+      };
+      for (u32 f = wbase + lane; f < wend; f += kWave) {
+        const u32 qi = query_in(s_woff[wave], g0, g1, f);
+        const u32 plan = s_plan[wave][qi];
+        const u32 first = plan & 0xffu, oldest = (plan >> 8) & 0xffu;
+        const bool fast = (plan >> 16) & 1u;
+        const bool count = (s_flags[wave][qi] & 4u) != 0;
+        const u32 *bnd = &s_b[wave][qi * per];
+        u32 r = first, off = f - s_woff[wave][qi];
+        while (off >= bnd[2 * r + 1] - bnd[2 * r]) {
+          off -= bnd[2 * r + 1] - bnd[2 * r];
+          ++r;
+        }
+        const u32 idx = bnd[2 * r] + off;
+        u64 contrib = 0;
+        if (fast) {
+          // a newer row: staged at the same position within its query
+          const u32 ci = s_coff[wave][qi] - cbase + (f - s_woff[wave][qi]);
+          const K key = s_cache[wave][ci];
+          if (!newer_holds(qi, first, r, key)) {
+            const bool tomb = (s_tomb[wave][ci >> 5] >> (ci & 31)) & 1u;
+            typename T::Row row;
+            const bool in_oldest =
+                run_find(runs.run[oldest], s_meta[oldest], key, row) !=
+                0xffffffffu;
+            contrib = u64(tomb ? 0 : 1) - u64(in_oldest ? 1 : 0);
+          }
+        } else {
+          const typename T::Row w = runs.run[r].rows[idx];
+          if (!(T::kw(w) & T::kTomb)) {
+            const K key = decode_key<T>(T::kw(w), idx, s_meta[r].split);
+            const u32 bit = range_filter_bit(key);
+            const bool maybe =
+                r > first && ((filter[bit >> 5] >> (bit & 31)) & 1u);
+            if (!(maybe && newer_holds(qi, first, r, key)))
+              contrib = count ? u64(1) : u64(T::val(w));
+          }
+        }
+        if (contrib)
+          atomicAdd(&s_acc[wave][qi], static_cast<unsigned long long>(contrib));
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (lane >= g0 && lane < g1 && (s_flags[wave][lane] & 1u))
+        io.put(s_slot[wave][lane],
+               static_cast<typename T::V>(s_acc[wave][lane]));
+      __builtin_amdgcn_wave_barrier();
+      g0 = g1;
     }
     __builtin_amdgcn_wave_barrier();
   }

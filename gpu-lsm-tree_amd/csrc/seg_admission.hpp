@@ -266,6 +266,122 @@ __device__ __forceinline__ void seg_rank(SegShared &sh, u32 (*hist)[256],
     (&other[0][0])[i] = 0;
 }
 
+// Distinct keys of a segment beyond the LDS tile (hot keys: many updates
+// of a few keys), through an LDS hash table of key -> last index in the
+// segment; the stable partition keeps the segment in input order, so the
+// last index is the newest update.  On success the distinct keys, each
+// with the key word and payload of its last occurrence, are in sh.k[0] /
+// sh.p[0] (in table order) and `count` holds their number.  Returns false
+// when the segment has more than kHashMax distinct keys (it is then sorted
+// in global memory).  The lanes of a wave holding its first lane's key
+// fold into one insert, so a hot key costs one table update per wave.
+constexpr u32 kHashSlots = kSegTile;  // 2^13 (u64 slots over sh.k)
+constexpr u32 kHashMax = kHashSlots / 4 * 3;
+
+__device__ __forceinline__ bool seg_dedupe(SegShared &sh,
+                                           const u32 *__restrict__ keys,
+                                           const u32 *__restrict__ pays, u32 m,
+                                           u32 key_mask, u32 &count) {
+  static_assert(sizeof(sh.k) == kHashSlots * sizeof(u64), "table over sh.k");
+  const u32 tid = threadIdx.x, lane = lane_id();
+  u64 *slot = reinterpret_cast<u64 *>(&sh.k[0][0]);
+  u32 *last = sh.p[0];
+  for (u32 i = tid; i < kHashSlots; i += kSegThreads) {
+    slot[i] = 0;
+    last[i] = 0;
+  }
+  if (tid == 0) {
+    sh.misc[4] = 0;  // distinct keys
+    sh.misc[5] = 0;  // overflow
+  }
+  __syncthreads();
+  for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
+    u32 k[kSegItems];
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const u32 i = c0 + j * kSegThreads + tid;
+      k[j] = i < m ? keys[i] & key_mask : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSegItems; ++j) {
+      const u32 i = c0 + j * kSegThreads + tid;
+      const bool valid = i < m;  // a prefix of the wave's lanes
+      const u32 lead = __builtin_amdgcn_readfirstlane(k[j]);
+      const bool same = valid && k[j] == lead;
+      const u64 sm = ballot(same);
+      u32 idx = i;
+      bool skip = false;
+      if (popc64(sm) > 1) {
+        u32 v = same ? i : 0u;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+          const u32 o = __shfl_xor(v, off, kWave);
+          v = o > v ? o : v;
+        }
+        if (same) idx = v;
+        skip = same && lane != ffs64(sm) - 1;
+      }
+      if (!valid || skip || sh.misc[5]) continue;
+      const u64 tag = (u64(1) << 32) | k[j];
+      u32 h = (k[j] * 0x9E3779B1u) >> 19;  // 13 bits
+      bool placed = false;
+      for (u32 probe = 0; probe < kHashSlots && !placed; ++probe) {
+        u64 cur = __hip_atomic_load(&slot[h], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == 0) {
+          cur = atomicCAS(reinterpret_cast<unsigned long long *>(&slot[h]),
+                          0ull, static_cast<unsigned long long>(tag));
+          if (cur == 0) {
+            if (atomicAdd(&sh.misc[4], 1u) >= kHashMax) sh.misc[5] = 1;
+            cur = tag;
+          }
+        }
+        if (cur == tag) {
+          atomicMax(&last[h], idx);
+          placed = true;
+        } else {
+          h = (h + 1) & (kHashSlots - 1);
+        }
+      }
+      if (!placed) sh.misc[5] = 1;
+    }
+    __syncthreads();
+    if (sh.misc[5]) return false;  // uniform: read after the barrier
+  }
+  // compact the occupied slots: raw key word and payload of each key's
+  // last occurrence
+  bool occ[kSegItems];
+  u32 at[kSegItems];
+  u32 mine = 0;
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 s = j * kSegThreads + tid;
+    occ[j] = slot[s] != 0;
+    at[j] = last[s];
+    mine += occ[j];
+  }
+  u32 pos = block_exclusive_scan<kSegThreads>(mine, sh.scan);
+  count = sh.scan[kSegWaves];
+  u32 rk[kSegItems], rp[kSegItems];
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    if (occ[j]) {
+      rk[j] = keys[at[j]];
+      rp[j] = pays[at[j]];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    if (occ[j]) {
+      sh.k[0][pos] = rk[j];
+      sh.p[0][pos] = rp[j];
+      ++pos;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
 // One workgroup per segment (ticket order = key order).  keys_in: the
 // batch (only its first key is read, for the shared high bits); pk/pp: the
 // partitioned keys and SegPayload payloads; tk/tp: scratch of n entries
@@ -291,7 +407,9 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   const u32 s0 = *shift_in;
   const u32 start = offsets[seg];
   const u32 end = seg == 255 ? n : offsets[seg + 1];
-  const u32 m = end - start;
+  // elements sorted below: the segment, or its distinct keys when a
+  // segment beyond the LDS tile is deduplicated first
+  u32 m = end - start;
   const u32 passes = (s0 + 7) / 8;
   // key bit 31 of the sorted keys may carry the erase flag; restore()
   // gives the key itself
@@ -305,10 +423,19 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   // ---------------------------------------------------------------- sort
   const u32 *sk;  // sorted keys / payloads of the segment (LDS or global)
   const u32 *sp;
-  if (m <= static_cast<u32>(kSegTile)) {
+  bool in_lds = m <= static_cast<u32>(kSegTile);
+  bool staged = false;  // the elements are in sh.k[0] / sh.p[0] already
+  if (!in_lds) {
+    u32 distinct = 0;
+    if (seg_dedupe(sh, pk + start, pp + start, m, key_mask, distinct)) {
+      m = distinct;
+      in_lds = staged = true;
+    }
+  }
+  if (in_lds) {
     const u32 ipt = m ? (m + kSegThreads - 1) / kSegThreads : 1u;
     const u32 wave_base = wave * kWave * ipt;
-    if (passes == 0) {
+    if (passes == 0 && !staged) {
       // nothing to sort (each segment holds one key): copy it to LDS, all
       // loads of a thread issued before the first store
       u32 lk[kSegItems], lp[kSegItems];
@@ -335,7 +462,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       // the first pass reads the partitioned segment from global memory,
       // the later ones the previous pass's LDS arrays
       u32 key[kSegItems], pay[kSegItems], dst[kSegItems];
-      if (pass == 0) {
+      if (pass == 0 && !staged) {
 #pragma unroll
         for (int j = 0; j < kSegItems; ++j) {
           const u32 idx = wave_base + j * kWave + lane;

@@ -65,8 +65,17 @@ __device__ __forceinline__ u32 piece_of(const Piece *pieces, u32 np, u64 i) {
 }
 
 template <class K, class V>
-__global__ void gather_kernel(const Piece *pieces, u32 np, u64 total,
+__global__ void gather_kernel(const Piece *pieces_arg, u32 np, u64 total,
                               K *out_a, V *out_b, u8 *out_flag) {
+  // tables of up to 256 pieces are searched in LDS
+  __shared__ Piece s_pieces[256];
+  const Piece *pieces = pieces_arg;
+  if (np <= 256) {
+    for (u32 i = threadIdx.x; i < np; i += blockDim.x)
+      s_pieces[i] = pieces_arg[i];
+    pieces = s_pieces;
+  }
+  __syncthreads();
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += stride) {
@@ -89,6 +98,18 @@ struct QueryPiece {
   u32 pad;
 };
 
+// Piece tables of up to kStagePieces entries are copied into LDS by the
+// kernels that search them once per query.
+constexpr u32 kStagePieces = 256;
+
+__device__ __forceinline__ const QueryPiece *stage_pieces(
+    const QueryPiece *pieces, u32 np, void *lds) {
+  if (np > kStagePieces) return pieces;
+  QueryPiece *dst = static_cast<QueryPiece *>(lds);
+  for (u32 i = threadIdx.x; i < np; i += blockDim.x) dst[i] = pieces[i];
+  return dst;
+}
+
 __device__ __forceinline__ u32 qpiece_of(const QueryPiece *pieces, u32 np,
                                          u64 i) {
   u32 lo = 0, hi = np;  // last piece with offset <= i
@@ -109,6 +130,12 @@ struct PieceLookup {
   using V = typename T::V;
   const QueryPiece *pieces;
   u32 np;
+  static constexpr u32 kStageBytes = kStagePieces * sizeof(QueryPiece);
+  __device__ __forceinline__ PieceLookup stage(void *lds) const {
+    PieceLookup r = *this;
+    r.pieces = stage_pieces(pieces, np, lds);
+    return r;
+  }
   struct Slot {
     V *out;
     u8 *found;
@@ -138,6 +165,12 @@ struct PieceRange {
   using V = typename T::V;
   const QueryPiece *pieces;
   u32 np;
+  static constexpr u32 kStageBytes = kStagePieces * sizeof(QueryPiece);
+  __device__ __forceinline__ PieceRange stage(void *lds) const {
+    PieceRange r = *this;
+    r.pieces = stage_pieces(pieces, np, lds);
+    return r;
+  }
   struct Slot {
     V *out;
     u64 j;
@@ -158,8 +191,11 @@ struct PieceRange {
 
 // Queries of a step's lookups as one contiguous array (grouped path).
 template <class K>
-__global__ void query_gather_kernel(const QueryPiece *pieces, u32 np,
+__global__ void query_gather_kernel(const QueryPiece *pieces_arg, u32 np,
                                     u64 total, K *out) {
+  __shared__ __align__(16) char s_io[kStagePieces * sizeof(QueryPiece)];
+  const QueryPiece *pieces = stage_pieces(pieces_arg, np, s_io);
+  __syncthreads();
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += stride) {
@@ -491,6 +527,13 @@ class Server : public ServerBase {
     // ------------------------------------------------------------ updates
     // Admission only: the merges it triggers run on the merge stream and
     // neither this step's queries nor later admissions wait for them.
+    // Admissions do wait for the previous step's queries: a sorting launch
+    // whose workgroups chain their offsets (and take a whole CU's LDS)
+    // next to a query launch of tens of thousands of workgroups gets CUs
+    // only as the query workgroups drain, and both then run many times
+    // slower than back to back.
+    if (n_upd && !inflight_.empty())
+      GD_HIP(hipStreamWaitEvent(us_, inflight_.back().queries_done, 0));
     if (n_upd) {
       K *keys = scratch<K>(st, n_upd);
       V *vals = scratch<V>(st, n_upd);

@@ -506,6 +506,69 @@ def test_snapshot_save_restore(gd, orc, io, tmp_path):
         small.restore(path)
 
 
+@pytest.mark.parametrize("seed", [5, 6])
+def test_ranges_over_many_runs(gd, orc, io, seed):
+    """Range counts and sums with up to 12 runs holding shadowed versions
+    and tombstones: narrow ranges (the round's newer rows are staged in
+    LDS), ranges over the whole key space (staged rows beyond the LDS
+    budget: one query at a time), empty and inverted ranges."""
+    rng = np.random.default_rng(seed)
+    p = Pair(gd, orc, io, tuning={"max_runs": 12, "small_batch_max": 1024},
+             max_elements=1 << 20, batch_capacity=1 << 15)
+    span = 1 << 24
+    for b in range(11):
+        # each batch smaller than the run before it: no merges
+        n = 1 << (15 - b)
+        keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+        vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        if b % 3 == 2:
+            p.mixed(keys, vals, (rng.random(n) < 0.4).astype(np.uint8))
+        else:
+            p.insert(keys, vals)
+    assert p.d.shape()[0] >= 8
+    lo = rng.integers(0, span, 6000, dtype=np.uint64)
+    width = np.where(rng.random(6000) < 0.5,
+                     rng.integers(0, 1 << 12, 6000, dtype=np.uint64),
+                     rng.integers(0, span, 6000, dtype=np.uint64))
+    hi = np.minimum(lo + width, 0xFFFFFFFF)
+    lo[:50], hi[:50] = 0, 0xFFFFFFFF      # whole key space
+    lo[50:60], hi[50:60] = 1000, 10       # inverted: empty
+    p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
+
+
+@pytest.mark.parametrize("hot_keys", [3, 4000, 20000])
+def test_segmented_admission_oversized_segments(gd, orc, io, hot_keys):
+    """Segments beyond the LDS tile: 90% of each batch falls into one
+    segment, drawn from `hot_keys` distinct keys (3 and 4000: deduplicated
+    in LDS first; 20000: more distinct keys than the LDS table takes, so
+    the segment is sorted in global memory), in insert, erase and mixed
+    batches, with erase flags carried as positions or in the free top key
+    bit (4000)."""
+    rng = np.random.default_rng(hot_keys)
+    p = Pair(gd, orc, io, tuning={"segmented_admission": 1},
+             max_elements=1 << 20, batch_capacity=1 << 17)
+    hot = rng.integers(0, 1 << 20, hot_keys, dtype=np.uint64) \
+        .astype(np.uint32)
+    for b in range(6):
+        n = int(rng.integers(1 << 16, 1 << 17))
+        # keys below 2^31 leave the top key bit free for the erase flag
+        span = 1 << 31 if hot_keys == 4000 else 1 << 32
+        keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+        sel = rng.random(n) < 0.9
+        keys[sel] = hot[rng.integers(0, hot_keys, int(sel.sum()))]
+        vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        kind = b % 3
+        if kind == 0:
+            p.insert(keys, vals)
+        elif kind == 1:
+            p.erase(keys[: n // 3])
+        else:
+            p.mixed(keys, vals, (rng.random(n) < 0.3).astype(np.uint8))
+        p.check_contents()
+    q = _queries(rng, p.o.dump()[0], 1 << 14, 1 << 32)
+    p.check_lookup(q, True)
+
+
 @pytest.mark.parametrize("span,dup,base", [
     (1 << 12, False, 0), (1 << 20, False, 0), (1 << 32, False, 0),
     (1 << 32, True, 0), (1 << 31, True, 0), (1 << 20, False, 0xFFF00000),
