@@ -104,13 +104,13 @@ class _Tuning(ctypes.Structure):
         ("small_batch_max", ctypes.c_uint32),
         ("max_runs", ctypes.c_uint32),
         ("segmented_admission", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("merge_block", ctypes.c_uint32),
     ]
 
 
 TUNING_FIELDS = ("dir_rows_log", "merge_tile_log", "sort_block",
                  "grouped_lookup_min_log", "small_batch_max", "max_runs",
-                 "segmented_admission")
+                 "segmented_admission", "merge_block")
 
 
 def _tuning(d):

@@ -35,10 +35,11 @@ LSM_SPACE = {
     "small_batch_max": [1024, 4096],
     "max_runs": [6, 10, 16],
     "segmented_admission": [1, 2],
+    "merge_block": [256, 512],
 }
 LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 512,
                "grouped_lookup_min_log": 18, "small_batch_max": 4096,
-               "max_runs": 10, "segmented_admission": 1}
+               "max_runs": 10, "segmented_admission": 1, "merge_block": 512}
 SERVE_SPACE = {"max_step_keys": [1 << 20, 1 << 22, 1 << 24],
                "max_inflight_steps": [1, 2, 3]}
 FLIX_SPACE = {"node_slots": [16, 32, 64], "fill_percent": [50, 75]}

@@ -596,7 +596,7 @@ __global__ void build_directory_kernel(const typename T::Row *__restrict__ rows,
 }
 
 // --------------------------------------------------------------- merging
-constexpr int kMergeThreads = 256;
+// merge workgroups: 256 or 512 threads (tile = threads x items per thread)
 
 template <class T>
 struct MergeSide {
@@ -693,7 +693,7 @@ __global__ void merge_partition_kernel(MergeSide<T> a, MergeSide<T> b,
 // owns the entries of the buckets in (bucket(previous input key),
 // bucket(its key)] in merged order and writes them the number of survivors
 // before it, which is the output row of the first key >= those bounds.
-template <class T, int kMergeItems>
+template <class T, int kMergeItems, int kMergeThreads>
 __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     MergeSide<T> a, MergeSide<T> b, const u32 *__restrict__ parts,
     typename T::Row *__restrict__ out, RunMeta *out_meta,

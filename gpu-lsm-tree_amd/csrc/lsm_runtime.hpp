@@ -548,6 +548,11 @@ class LsmT : public HandleBase {
         throw Error(GD_INVALID_ARGUMENT, "segmented_admission must be 1 or 2");
       v.segmented_admission = t.segmented_admission;
     }
+    if (t.merge_block) {
+      if (t.merge_block != 256 && t.merge_block != 512)
+        throw Error(GD_INVALID_ARGUMENT, "merge_block must be 256 or 512");
+      v.merge_block = t.merge_block;
+    }
     tuning_ = v;
   }
   const gd_lsm_tuning &tuning() const { return tuning_; }
@@ -789,13 +794,15 @@ class LsmT : public HandleBase {
     Lease<u32> ka(pool_, n), va(pool_, n), kb(pool_, n), vb(pool_, n);
     u32 *c = ctrl.get();
     const rs::Control pc = rs::control_layout(c, 1);
-    // the partition's clearing launch also clears this kernel's state
-    // behind its own and the run header
-    rs::clear_control(c, words, s, reinterpret_cast<u32 *>(r.meta()),
-                      kMetaWords);
+    // one launch computes the key spread and clears the partition's state,
+    // this kernel's state behind it (all but the spans) and the run header
     {
       K *span = reinterpret_cast<K *>(c + L.span_at);
-      keyspan_kernel<K><<<kSpanBlocks, 256, 0, s>>>(keys, n, span);
+      span_clear_kernel<K><<<span_clear_blocks(words + kMetaWords), 256, 0,
+                             s>>>(keys, n, span, c, L.span_at, c + L.shift_at,
+                                  words - L.shift_at,
+                                  reinterpret_cast<u32 *>(r.meta()),
+                                  kMetaWords);
       seg_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
           keys, n, span, pc.rows, c + L.shift_at);
       rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
@@ -837,20 +844,23 @@ class LsmT : public HandleBase {
     GD_LAUNCH_CHECK();
     ProfRec rec{};
     if (profiling_) rec = prof_begin(0, s, 0);
-    if constexpr (T::kMergeItemsMax >= 16) {
-      if (tile_rows == 4096) {
-        merge_kernel<T, 16><<<tiles, kMergeThreads, 0, s>>>(
-            sa, sb, parts.get(), out.rows, out.meta(), out.entries(),
-            out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
-      } else {
-        merge_kernel<T, 8><<<tiles, kMergeThreads, 0, s>>>(
-            sa, sb, parts.get(), out.rows, out.meta(), out.entries(),
-            out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
+    // tile = threads x items: 4096 rows as 256 x 16 or 512 x 8, 2048 rows
+    // as 256 x 8 or 512 x 4
+    const bool wide = tuning_.merge_block == 512;
+    auto launch = [&](auto kernel, int threads) {
+      kernel<<<tiles, threads, 0, s>>>(sa, sb, parts.get(), out.rows,
+                                       out.meta(), out.entries(), out.max_bits,
+                                       drop_tombstones ? 1 : 0, status,
+                                       ctrl.get());
+    };
+    if (tile_rows == 4096) {
+      if constexpr (T::kMergeItemsMax >= 16) {
+        if (wide) launch(merge_kernel<T, 8, 512>, 512);
+        else launch(merge_kernel<T, 16, 256>, 256);
       }
     } else {
-      merge_kernel<T, 8><<<tiles, kMergeThreads, 0, s>>>(
-          sa, sb, parts.get(), out.rows, out.meta(), out.entries(),
-          out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
+      if (wide) launch(merge_kernel<T, 4, 512>, 512);
+      else launch(merge_kernel<T, 8, 256>, 256);
     }
     GD_LAUNCH_CHECK();
     if (profiling_) {
@@ -1001,7 +1011,7 @@ class LsmT : public HandleBase {
   // segmented admission (1) for 32-bit batches; 64-bit batches always take
   // the radix passes
   gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(T::kSmallTile), 10,
-                        1, 0};
+                        1, 512};
 
   struct ProfRec {
     int kind;  // 0 merge, 1 lookup

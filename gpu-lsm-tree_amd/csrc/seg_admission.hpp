@@ -1,7 +1,8 @@
 // Segmented admission of a 32-bit batch: the batch becomes a run in two
 // sorting launches instead of four radix passes and a finalize launch.
 //
-//   keyspan_kernel    per-block OR of (key ^ keys[0]): the bits that vary
+//   span_clear_kernel per-block OR of (key ^ keys[0]): the bits that vary
+//                     (the same launch clears the sort's control state)
 //   seg_hist_kernel   s0 = (highest varying bit) - 7; histogram of the
 //                     digit (key >> s0) & 0xff, so the 256 segments split
 //                     the batch's actual key spread, not the key space
@@ -16,8 +17,10 @@
 //                     and the run header.
 //
 // A segment holds up to kSegTile keys in LDS; a larger one (heavy
-// duplication: a hot key) is sorted by its workgroup in global memory
-// (stable LSD, chunk by chunk), which is slower but exact.
+// duplication: hot keys) is first deduplicated through an LDS hash table
+// and then sorted in LDS; one with more distinct keys than the table takes
+// is sorted by its workgroup in global memory (stable LSD, chunk by
+// chunk), which is slower but exact.
 #pragma once
 
 #include "lsm_kernels.hpp"
@@ -32,15 +35,26 @@ constexpr int kSegTile = kSegThreads * kSegItems;  // 8192
 constexpr int kSegWaves = kSegThreads / kWave;
 constexpr int kSpanBlocks = 256;
 
-// span[blockIdx.x] = OR over the block's keys of (key ^ keys[0]); 16 keys
+// The key spread and the clearing of three regions a[0..na), b[0..nb),
+// c[0..nc) in one launch: every block clears its share, and block b below
+// kSpanBlocks writes span[b] = OR over its keys of (key ^ keys[0]), 16 keys
 // per thread per round, loaded together.
 template <class K>
-static __global__ __launch_bounds__(256) void keyspan_kernel(
-    const K *__restrict__ keys, u32 n, K *__restrict__ span) {
+static __global__ __launch_bounds__(256) void span_clear_kernel(
+    const K *__restrict__ keys, u32 n, K *__restrict__ span, u32 *a, u64 na,
+    u32 *b, u64 nb, u32 *c, u64 nc) {
+  const u64 total = na + nb + nc;
+  for (u64 i = u64(blockIdx.x) * 256 + threadIdx.x; i < total;
+       i += u64(gridDim.x) * 256) {
+    if (i < na) a[i] = 0;
+    else if (i < na + nb) b[i - na] = 0;
+    else c[i - na - nb] = 0;
+  }
+  if (blockIdx.x >= static_cast<u32>(kSpanBlocks)) return;
   __shared__ K s_or[256 / kWave];
   const K k0 = keys[0];
   K acc = 0;
-  const u64 stride = u64(gridDim.x) * 256 * 16;
+  const u64 stride = u64(kSpanBlocks) * 256 * 16;
   for (u64 base = u64(blockIdx.x) * 256 * 16; base < n; base += stride) {
     K k[16];
 #pragma unroll
@@ -61,6 +75,12 @@ static __global__ __launch_bounds__(256) void keyspan_kernel(
     for (int w = 0; w < 256 / kWave; ++w) v |= s_or[w];
     span[blockIdx.x] = v;
   }
+}
+
+inline u32 span_clear_blocks(u64 words) {
+  const u64 b = (words + 255) / 256;
+  return static_cast<u32>(b < u64(kSpanBlocks) ? u64(kSpanBlocks)
+                                                : (b > 4096 ? 4096 : b));
 }
 
 // Digit shift from the key spread: the top 8 varying bits.
@@ -140,9 +160,12 @@ inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
                                 u32 *extra = nullptr, u64 extra_words = 0) {
   const SpreadLayout<K> L(n);
   const rs::Control pc = rs::control_layout(control, 1);
-  rs::clear_control(control, L.words, s, extra, extra_words);
   K *span = reinterpret_cast<K *>(control + L.span_at);
-  keyspan_kernel<K><<<kSpanBlocks, 256, 0, s>>>(keys, n, span);
+  // clear everything but the spans (written whole by the same launch)
+  span_clear_kernel<K><<<span_clear_blocks(L.words + extra_words), 256, 0,
+                         s>>>(keys, n, span, control, L.span_at,
+                              control + L.shift_at, L.words - L.shift_at,
+                              extra, extra_words);
   seg_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
       keys, n, span, pc.rows, control + L.shift_at);
   rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);

@@ -160,7 +160,8 @@ typedef struct gd_lsm_tuning {
   uint32_t segmented_admission;    /* 32-bit batches above 16384 keys: 1 =
                                       partition + per-segment LDS sort (1),
                                       2 = radix passes + finalize; 0 = keep */
-  uint32_t reserved;
+  uint32_t merge_block;            /* merge workgroup: 256 or 512 threads
+                                      per tile (512)                       */
 } gd_lsm_tuning;
 
 gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning);

@@ -391,6 +391,8 @@ def test_config2_scaled(gd, orc, io):
     {"max_runs": 40, "grouped_lookup_min_log": 63, "merge_tile_log": 12},
     {"segmented_admission": 1, "dir_rows_log": 2, "small_batch_max": 1024},
     {"segmented_admission": 2, "dir_rows_log": 4, "small_batch_max": 1024},
+    {"merge_block": 512, "merge_tile_log": 12, "max_runs": 3},
+    {"merge_block": 512, "merge_tile_log": 11, "dir_rows_log": 2},
 ])
 def test_tuning_knobs_keep_results(gd, orc, io, knobs):
     """Launch-shape knobs change speed only: a mixed stream with small and
@@ -421,7 +423,8 @@ def test_tuning_knobs_keep_results(gd, orc, io, knobs):
 
 def test_invalid_tuning_rejected(gd):
     for bad in ({"sort_block": 384}, {"merge_tile_log": 13},
-                {"segmented_admission": 3}, {"no_such_knob": 1}):
+                {"segmented_admission": 3}, {"merge_block": 384},
+                {"no_such_knob": 1}):
         with pytest.raises(gd.InvalidArgument):
             gd.GPULSMOpt(max_elements=1 << 16, batch_capacity=1 << 10,
                          tuning=bad)
