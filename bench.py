@@ -548,6 +548,25 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # achievable streaming bandwidth on this box: a plain 2 GiB device copy
+    # (reads and writes 2 GiB each), best of 5, timed with events
+    src = torch.ones(1 << 29, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = None
+    for _ in range(5):
+        c0 = torch.cuda.Event(enable_timing=True)
+        c1 = torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        dst.copy_(src)
+        c1.record(stream)
+        c1.synchronize()
+        ms = c0.elapsed_time(c1)
+        best = ms if best is None else min(best, ms)
+    copy_gbs = 2.0 * src.numel() * 4 / (best * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.synchronize()
+
     # ---------------------------------------------------------- workload
     t0 = time.time()
     w = wl.config2(args.scale_log)
@@ -657,6 +676,8 @@ def main():
         return {"bound": "hbm", "kernel": name,
                 "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "copy_gbs": round(copy_gbs, 1),
+                "frac_of_copy": round(achieved / copy_gbs, 5),
                 "traffic": pmc_traffic(name),
                 "algorithmic_bytes_per_launch": int(nbytes / launches),
                 "avg_launch_ms": round(ms / launches, 4),
