@@ -880,11 +880,13 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   }
   const u32 local = block_exclusive_scan<kMergeThreads>(kept, s_scan);
   const u32 tile_total = s_scan[kMergeThreads / kWave];
+  // Wave 0 waits on the look-back while the other waves stage their
+  // survivors at tile-local positions (the scan's barriers ordered every
+  // LDS input read before these writes).
   if (threadIdx.x < kWave) {
     const u64 ex = wave_lookback(status, tile, tile_total);
     if (threadIdx.x == 0) s_base = ex;
   }
-  __syncthreads();  // also: every LDS input read is complete
   Row *stage_lo = reinterpret_cast<Row *>(s_key);
   Row *stage_hi = reinterpret_cast<Row *>(s_val);
   u32 low = 0, tombs = 0, at = local;

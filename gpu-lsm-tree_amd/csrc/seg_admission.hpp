@@ -33,6 +33,7 @@ constexpr int kSegThreads = 512;
 constexpr int kSegItems = 16;
 constexpr int kSegTile = kSegThreads * kSegItems;  // 8192
 constexpr int kSegWaves = kSegThreads / kWave;
+static_assert(kSegWaves >= 2, "the look-back and the offset scan use two waves");
 constexpr int kSpanBlocks = 256;
 
 // The key spread and the clearing of three regions a[0..na), b[0..nb),
@@ -601,16 +602,24 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   }
   __syncthreads();
   const u32 seg_total = sh.misc[0];
-  if (tid < kWave) {
-    const u64 ex = wave_lookback(status, seg, seg_total);
-    if (tid == 0) sh.base = ex;
-  }
   if (tid == 0) {
     sh.misc[1] = 0;  // survivors below the top key bit
     sh.misc[2] = 0;  // tombstones
   }
-  __syncthreads();
-  const u64 base = sh.base;
+  // the chained scan over segments (wave 0) runs beside the first chunk's
+  // survivor offsets (wave 1); an empty segment scans on its own
+  auto lookback = [&]() {
+    if (wave == 0) {
+      const u64 ex = wave_lookback(status, seg, seg_total);
+      if (tid == 0) sh.base = ex;
+    }
+  };
+  u64 base = 0;
+  if (m == 0) {
+    lookback();
+    __syncthreads();
+    base = sh.base;
+  }
 
   // ------------------------------------------- rows and directory entries
   const u32 shift = 32u - dir_bits;
@@ -624,9 +633,10 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       if (lane == 0) sh.off[j][wave] = popc64(mask[j]);
     }
     __syncthreads();
+    if (c0 == 0) lookback();
     // exclusive scan of the (item, wave) survivor counts in row order, by
-    // wave 0: lane l owns entries 2l and 2l + 1
-    if (wave == 0) {
+    // wave 1: lane l owns entries 2l and 2l + 1
+    if (wave == 1) {
       u32 *flat = &sh.off[0][0];
       const u32 a = flat[2 * lane], b = flat[2 * lane + 1];
       const u32 inc = wave_inclusive_scan(a + b);
@@ -636,6 +646,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       if (lane == kWave - 1) sh.misc[3] = running + inc;
     }
     __syncthreads();
+    base = sh.base;
     u32 k[kSegItems], prev[kSegItems], pay[kSegItems];
     bool tomb[kSegItems];
 #pragma unroll
