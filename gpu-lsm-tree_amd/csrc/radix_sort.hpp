@@ -376,31 +376,5 @@ inline hipError_t sort_pairs(const K *keys_in, u32 n, K *keys_a, u32 *vals_a,
   return hipGetLastError();
 }
 
-// Stable partition of n keys by one 8-bit digit (key >> shift) & 0xff: one
-// histogram, one scan and one onesweep pass.  The partitioned keys land in
-// keys_out; pos_out (nullable) receives each one's input position and dest
-// (nullable) each input element's partitioned position, in input order.
-// `control` holds control_words(n, 1) words.
-template <class K>
-inline hipError_t partition_pairs(const K *keys_in, u32 n, u32 shift,
-                                  K *keys_out, u32 *pos_out, u32 *control,
-                                  hipStream_t stream, int block = 0,
-                                  u32 *dest = nullptr) {
-  if (n == 0) return hipSuccess;
-  const int shape = block ? block : block_for(n);
-  const u32 tiles = tiles_for(n);
-  const Control c = control_layout(control, 1);
-  clear_control(control, control_words(n, 1), stream);
-  histogram_kernel<K, 1><<<hist_blocks_for(n), kThreads, 0, stream>>>(
-      keys_in, n, shift / 8, c.rows);
-  scan_kernel<<<1, kRadix, 0, stream>>>(c.rows, 1, c.offsets);
-  const u32 *offsets = c.offsets;
-  u32 *tickets = c.tickets;
-  u64 *status = c.status;
-  launch_pass<K>(shape, tiles, keys_in, PayloadPos{}, keys_out, pos_out, n,
-                 shift, offsets, status, tickets, stream, dest);
-  return hipGetLastError();
-}
-
 }  // namespace rs
 }  // namespace gdict
