@@ -276,6 +276,8 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
                          max_inflight_steps=3)
         srv.wait(srv.submit(pkg.Server.INSERT, live_k, live_v, count=live))
         srv.drain()
+        # the load's merges finish before the measured stream starts
+        torch.cuda.synchronize()
         srv.stats(reset=True)
         return srv
 
@@ -373,6 +375,7 @@ def c5_measurement(args, pkg, wl, torch, dev):
         srv.wait(srv.submit(S.INSERT, k, v, count=n))
         del k, v
     srv.drain()
+    torch.cuda.synchronize()  # including the growth's merges
     grow_s = time.perf_counter() - t0
     progress("config-5 index grown to %d keys in %.1f s" % (universe, grow_s))
 
@@ -395,6 +398,7 @@ def c5_measurement(args, pkg, wl, torch, dev):
     torch.cuda.synchronize()
 
     def run(rate):
+        torch.cuda.synchronize()  # the previous run's merges are done
         srv.stats(reset=True)
         tickets = []
         t0 = time.perf_counter()

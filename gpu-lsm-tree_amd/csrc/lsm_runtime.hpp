@@ -421,7 +421,34 @@ class LsmT : public HandleBase {
     defer_ = true;
   }
 
+  // Queries wait for the merges enqueued so far when their inputs hold at
+  // most this many rows (a wait of well under a millisecond), and read the
+  // pending merges' inputs otherwise.  Measured on MI355X: a query launch
+  // over the 9-12 runs of expanded pending merges cost more than the wait
+  // (config-4 stream: 2.1x the saturated throughput with waiting).
+  static constexpr u64 kQueryWaitRows = u64(1) << 26;
+
   RunSet<T> snapshot_for(hipStream_t qs) {
+    u64 backlog = 0;
+    for (auto it = pending_.begin(); it != pending_.end();) {
+      const hipError_t q = hipEventQuery(it->second.done);
+      if (q == hipSuccess) {
+        (void)hipEventDestroy(it->second.done);
+        it = pending_.erase(it);
+        continue;
+      }
+      if (q != hipErrorNotReady) GD_HIP(q);
+      backlog += it->second.a.rows_ub + it->second.b.rows_ub;
+      ++it;
+    }
+    if (backlog && backlog <= kQueryWaitRows) {
+      hipEvent_t merged;
+      GD_HIP(hipEventCreateWithFlags(&merged, hipEventDisableTiming));
+      GD_HIP(hipEventRecord(merged, ms_));
+      GD_HIP(hipStreamWaitEvent(qs, merged, 0));
+      (void)hipEventDestroy(merged);
+      return run_set();
+    }
     RunSet<T> set{};
     bool too_many = false;
     // newest first: a pending merge of (newer a, older b) reads as a, b
