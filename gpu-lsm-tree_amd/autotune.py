@@ -5,7 +5,8 @@ result:
 
 * LSM (gd_lsm_tuning): directory density, merge tile, onesweep block shape,
   the batch size from which lookups are grouped, the fused small-batch
-  limit, the run cap and the admission path (segmented or radix);
+  limit, the run cap, the admission path (segmented or radix) and the most
+  runs one merge pass takes;
 * serving: keys coalesced per step and steps in flight;
 * FliX: slots per node and initial fill.
 
@@ -36,10 +37,12 @@ LSM_SPACE = {
     "max_runs": [6, 10, 16],
     "segmented_admission": [1, 2],
     "merge_block": [256, 512],
+    "merge_ways": [2, 4, 8],
 }
 LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 512,
                "grouped_lookup_min_log": 18, "small_batch_max": 4096,
-               "max_runs": 10, "segmented_admission": 1, "merge_block": 512}
+               "max_runs": 10, "segmented_admission": 1, "merge_block": 512,
+               "merge_ways": 8}
 SERVE_SPACE = {"max_step_keys": [1 << 20, 1 << 22, 1 << 24],
                "max_inflight_steps": [1, 2, 3]}
 FLIX_SPACE = {"node_slots": [16, 32, 64], "fill_percent": [50, 75]}

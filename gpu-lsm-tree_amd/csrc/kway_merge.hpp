@@ -1,0 +1,521 @@
+// Merging up to kMaxWay runs in one pass.
+//
+// The binary-counter run policy merges a cascade of runs whenever a batch
+// arrives on top of runs of equal size (runs of 1, 1, 2, 4, ... batches):
+// done pairwise, the rows of the newest runs are rewritten once per level
+// of the cascade; merged at once, every row is read and written once.  Over
+// 64 equal batches that is 224 instead of 384 batch-sized rewrites.
+//
+// Tiles are key ranges, not rank ranges.  Tile t holds, from every input
+// run, the rows with keys in [x_t, x_{t+1}), so every version of a key
+// lands in one tile (the newest-wins rule needs no look across tiles) and a
+// tile's split points are plain lower bounds.  The partition picks x_t so
+// that the rows below it number about t * target (kway_partition_kernel),
+// which bounds every tile by the LDS capacity; each tile then merges its
+// inputs in LDS with log2(ways) rounds of pairwise merge paths and writes
+// the survivors like the two-way merge (chained scan over tiles).
+#pragma once
+
+#include "lsm_kernels.hpp"
+
+namespace gdict {
+namespace lsm {
+
+constexpr int kMaxWay = 8;
+
+template <class T>
+struct KWayIn {
+  RunView<T> run[kMaxWay];  // newest first; entries >= ways are unused
+  u32 ways;
+};
+
+// Rows per merge tile (LDS capacity), the partition's target rows per tile
+// and the largest directory bucket a split may fall into unrefined: a tile
+// holds fewer than target + max(bucket, ways) rows.
+template <class T>
+struct KWayShape {
+  static constexpr u32 kThreads = sizeof(typename T::K) == 4 ? 512 : 256;
+  static constexpr u32 kItems = 8;
+  static constexpr u32 kTile = kThreads * kItems;
+  static constexpr u32 kTarget = kTile / 4 * 3;
+  static constexpr u32 kLightMax = kTile - kTarget;
+};
+
+// Lower bound of key c << (B - g) in a run whose directory has
+// m.dir_bits >= g bits: exact, from one directory entry.
+template <class T>
+__device__ __forceinline__ u32 boundary_lb(const RunView<T> &r,
+                                           const RunMeta &m, u32 c, u32 g) {
+  if (m.count == 0) return 0;
+  const u64 j = u64(c) << (m.dir_bits - g);
+  if (j <= m.lo_bucket) return 0;       // dir[lo_bucket] = 0
+  if (j > m.hi_bucket) return m.count;  // dir[hi_bucket + 1] = count
+  return r.dir[j];
+}
+
+// Split points of the k-way merge, one wave per tile boundary t in
+// [0, tiles]: parts[t * kMaxWay + i] = rows of run i with keys below x_t.
+// Phase 1: a 64-ary search over the directory buckets at the coarsest
+// directory resolution g of the inputs finds the bucket c whose start has
+// at most R = t * target rows below it (summed over the runs) and whose end
+// has more.  When that bucket holds at most light_max rows, x_t is its
+// start.  Otherwise (dense keys) phase 2 searches the keys inside the
+// bucket for the largest x with at most R rows below it.  Also clears the
+// merge's chained-scan state and the output run's header.
+template <class T>
+__global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
+                                      u32 light_max, u32 *__restrict__ parts,
+                                      u64 *status, u32 *ticket,
+                                      RunMeta *out_meta, u32 dir_bits) {
+  using K = typename T::K;
+  const u32 lane = lane_id();
+  const u64 t = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  if (t > tiles) return;
+  if (lane == 0 && t < tiles) status[t] = 0;
+  if (t == 0 && lane == 0) {
+    *ticket = 0;
+    out_meta->count = 0;
+    out_meta->split = 0;
+    out_meta->dir_bits = dir_bits;
+    out_meta->tombs = 0;
+    out_meta->lo_bucket = 1;  // empty until the first and last input
+    out_meta->hi_bucket = 0;  // element write their buckets
+  }
+  const u32 ways = in.ways;
+  RunMeta m[kMaxWay];
+  u64 total = 0;
+  u32 g = kMaxDirBits;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    m[i] = RunMeta{0, 0, kMaxDirBits, 0, 1, 0};
+    if (static_cast<u32>(i) < ways) {
+      m[i] = *in.run[i].meta;
+      total += m[i].count;
+      g = m[i].dir_bits < g ? m[i].dir_bits : g;
+    }
+  }
+  u32 *out = parts + t * kMaxWay;
+  const u64 R = t * u64(target);
+  if (t == 0 || R >= total) {  // below every row / above every row
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i)
+      if (lane == static_cast<u32>(i) && static_cast<u32>(i) < ways)
+        out[i] = t == 0 ? 0u : m[i].count;
+    return;
+  }
+  // rows below the start of bucket c, over all runs
+  auto below = [&](u32 c) -> u64 {
+    u32 v[kMaxWay];
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i)
+      v[i] = static_cast<u32>(i) < ways ? boundary_lb(in.run[i], m[i], c, g)
+                                        : 0u;
+    u64 s = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i) s += v[i];
+    return s;
+  };
+  // phase 1: below(lo) <= R < below(hi)
+  u32 lo = 0, hi = 1u << g;
+  while (hi - lo > 1) {
+    const u32 span = hi - lo;
+    const bool fine = span <= static_cast<u32>(kWave);
+    const u32 c = fine ? lo + lane : lo + static_cast<u32>((u64(span) * lane) >> 6);
+    bool p = lane == 0;  // below(lo) <= R is known
+    if (lane != 0 && c < hi) p = below(c) <= R;
+    const u32 L = popc64(ballot(p)) - 1;  // the true lanes form a prefix
+    const u32 cl = fine ? lo + L : lo + static_cast<u32>((u64(span) * L) >> 6);
+    const u32 cn = fine ? lo + L + 1
+                        : (L + 1 < static_cast<u32>(kWave)
+                               ? lo + static_cast<u32>((u64(span) * (L + 1)) >> 6)
+                               : hi);
+    lo = cl;
+    hi = cn;
+  }
+  u32 a[kMaxWay], b[kMaxWay];
+  u64 below_lo = 0, rows_in = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    a[i] = b[i] = 0;
+    if (static_cast<u32>(i) < ways) {
+      a[i] = boundary_lb(in.run[i], m[i], lo, g);
+      b[i] = boundary_lb(in.run[i], m[i], lo + 1, g);
+    }
+    below_lo += a[i];
+    rows_in += b[i] - a[i];
+  }
+  if (rows_in > light_max) {
+    // phase 2: largest x in the bucket with at most `need` bucket rows
+    // below it; inside one bucket every key shares the top bit, so rows
+    // compare on the low bits
+    const u64 need = R - below_lo;
+    const u32 s = T::kBits - g;  // 1 <= g <= 24: s <= B - 1
+    K x = K(lo) << s;
+    u64 span = u64(1) << s;
+    auto rank_in = [&](K key) -> u64 {
+      u64 r = 0;
+#pragma unroll
+      for (int i = 0; i < kMaxWay; ++i)
+        if (static_cast<u32>(i) < ways)
+          r += window_lower_bound<T>(in.run[i].rows, a[i], b[i],
+                                     key & T::kLow) -
+               a[i];
+      return r;
+    };
+    while (span > 1) {
+      const bool fine = span <= static_cast<u64>(kWave);
+      const u64 step = fine ? 1 : span >> 6;
+      const u64 off = u64(lane) * step;
+      bool p = lane == 0;
+      if (lane != 0 && off < span) p = rank_in(x + K(off)) <= need;
+      const u32 L = popc64(ballot(p)) - 1;
+      x += K(u64(L) * step);
+      span = fine ? 1 : (L + 1 < static_cast<u32>(kWave) ? step
+                                                         : span - u64(L) * step);
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i)
+      if (static_cast<u32>(i) < ways)
+        a[i] = window_lower_bound<T>(in.run[i].rows, a[i], b[i], x & T::kLow);
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i)
+    if (lane == static_cast<u32>(i) && static_cast<u32>(i) < ways) out[i] = a[i];
+}
+
+// Merges the tile's rows of every input run (newest first) into `out`: on
+// equal keys the newest row survives, tombstones drop when
+// `drop_tombstones`; header and directory as in merge_kernel.  Rows are
+// staged in LDS in run order; round r merges adjacent groups of 2^(r-1)
+// runs (newer group first on equal keys) with one merge path per thread,
+// carrying (key, LDS slot) pairs, so after ceil(log2(ways)) rounds the tile
+// is in (key, age) order and each row's value is read once from its slot.
+template <class T, int kItems, int kThreads>
+__global__ __launch_bounds__(kThreads) void kway_merge_kernel(
+    KWayIn<T> in, const u32 *__restrict__ parts,
+    typename T::Row *__restrict__ out, u64 out_rows, RunMeta *out_meta,
+    u32 *__restrict__ dir, u32 dir_bits, int drop_tombstones, u64 *status,
+    u32 *ticket) {
+  using K = typename T::K;
+  using V = typename T::V;
+  using Row = typename T::Row;
+  constexpr u32 kTile = kThreads * kItems;
+  constexpr u32 kStageHalf = kTile / 2;  // rows over the key array
+  static_assert(sizeof(Row) == 2 * sizeof(K) && sizeof(K) == sizeof(V),
+                "row = key word + value");
+  static_assert(kTile <= 65536, "slots are 16-bit");
+  __shared__ u32 s_tile;
+  __shared__ __align__(16) K s_key[kTile];
+  __shared__ __align__(16) V s_val[kTile];
+  __shared__ unsigned short s_slot[kTile];
+  __shared__ u8 s_tomb[kTile];
+  __shared__ u32 s_scan[kThreads / kWave + 1];
+  __shared__ u64 s_base;
+  __shared__ u32 s_low, s_tombs;
+  __shared__ K s_prev;  // largest key before the tile
+  __shared__ u32 s_has_prev, s_before_low;
+
+  if (threadIdx.x == 0) {
+    s_tile = atomicAdd(ticket, 1u);
+    s_low = 0;
+    s_tombs = 0;
+  }
+  __syncthreads();
+  const u32 tile = s_tile;
+  const u32 ways = in.ways;
+  RunMeta m[kMaxWay];
+  u32 beg[kMaxWay], off[kMaxWay + 1];
+  u64 total = 0, ends = 0;
+  bool trailing = true;  // every row lies before this tile
+  off[0] = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    m[i] = RunMeta{0, 0, 0, 0, 1, 0};
+    beg[i] = 0;
+    u32 len = 0;
+    if (static_cast<u32>(i) < ways) {
+      m[i] = *in.run[i].meta;
+      // split points are nondecreasing lower bounds; the clamps only keep
+      // every access inside the run and the tile whatever they hold
+      const u32 cnt = m[i].count;
+      const u32 b0 = parts[u64(tile) * kMaxWay + i];
+      const u32 e0 = parts[(u64(tile) + 1) * kMaxWay + i];
+      beg[i] = b0 < cnt ? b0 : cnt;
+      const u32 end = e0 < beg[i] ? beg[i] : (e0 < cnt ? e0 : cnt);
+      len = end - beg[i];
+      if (len > kTile - off[i]) len = kTile - off[i];
+      total += cnt;
+      ends += end;
+      trailing = trailing && beg[i] == cnt;
+    }
+    off[i + 1] = off[i] + len;
+  }
+  if (tile > 0 && trailing) {
+    // nothing here or after; an empty aggregate keeps every look-back
+    // finite whatever the split points hold
+    if (threadIdx.x == 0) store_relaxed(&status[tile], kFlagAggregate);
+    return;
+  }
+  const u32 n = off[kMaxWay];
+  const bool ends_all = ends == total;
+
+  // Load: slot s of [0, n) holds row beg[i] + (s - off[i]) of run i; every
+  // load of a thread is issued before the first one is used.
+  Row rr[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const u32 s = threadIdx.x + r * kThreads;
+    if (s < n) {
+      // the segment holding s is the last one starting at or before it
+      const Row *src = in.run[0].rows;
+      u32 at = beg[0] + s;
+#pragma unroll
+      for (int i = 1; i < kMaxWay; ++i)
+        if (s >= off[i]) {
+          src = in.run[i].rows;
+          at = beg[i] + (s - off[i]);
+        }
+      rr[r] = src[at];
+    }
+  }
+  // the rows just before the tile (the largest key below it and whether
+  // every row below it has the top key bit clear)
+  if (threadIdx.x < kWave) {
+    const u32 lane = lane_id();
+    K pk = 0;
+    bool has = false;
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i) {
+      if (lane == static_cast<u32>(i) && static_cast<u32>(i) < ways &&
+          beg[i] > 0) {
+        pk = decode_key<T>(T::kw(in.run[i].rows[beg[i] - 1]), beg[i] - 1,
+                           m[i].split);
+        has = true;
+      }
+    }
+    const u64 any = ballot(has);
+    const u64 high = ballot(has && pk >= T::kTomb);
+    K best = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i) {
+      const K v = __shfl(pk, i, kWave);
+      if (((any >> i) & 1ull) && v > best) best = v;
+    }
+    if (lane == 0) {
+      s_prev = best;
+      s_has_prev = any != 0;
+      s_before_low = high == 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const u32 s = threadIdx.x + r * kThreads;
+    if (s < n) {
+      u32 split = m[0].split, pos = beg[0] + s;
+#pragma unroll
+      for (int i = 1; i < kMaxWay; ++i)
+        if (s >= off[i]) {
+          split = m[i].split;
+          pos = beg[i] + (s - off[i]);
+        }
+      s_key[s] = decode_key<T>(T::kw(rr[r]), pos, split);
+      s_tomb[s] = static_cast<u8>(T::kw(rr[r]) >> (T::kBits - 1));
+      s_val[s] = T::val(rr[r]);
+    }
+  }
+  __syncthreads();
+
+  // Merge rounds.  Thread t produces merged positions [t * kItems, +kItems)
+  // of the round; a group is [off[u * w], off[(u + 1) * w]) with w = 2^r
+  // runs, its newer half ending at off[u * w + w / 2].
+  const u32 p0 = threadIdx.x * kItems;
+  const u32 levels = ways <= 2 ? 1u : (ways <= 4 ? 2u : 3u);
+  K ok[kItems];
+  u32 os[kItems];
+#pragma unroll
+  for (int s = 0; s < kItems; ++s) {
+    ok[s] = 0;
+    os[s] = 0;
+  }
+#pragma unroll
+  for (int r = 1; r <= 3; ++r) {
+    if (static_cast<u32>(r) > levels) break;
+    const int w = 1 << r;
+    // group of position q: bounds [glo, gmid) newer, [gmid, ghi) older
+    auto group_of = [&](u32 q, u32 &glo, u32 &gmid, u32 &ghi) {
+      glo = gmid = ghi = 0;
+#pragma unroll
+      for (int u = 0; u < kMaxWay; u += 2) {
+        if (u % w == 0) {
+          const u32 lo_ = off[u];
+          const u32 mid_ = off[u + w / 2];
+          const u32 hi_ = off[u + w];
+          if (q >= lo_ && q < hi_) {
+            glo = lo_;
+            gmid = mid_;
+            ghi = hi_;
+          }
+        }
+      }
+    };
+    auto slot_at = [&](u32 q) -> u32 {
+      return r == 1 ? q : static_cast<u32>(s_slot[q]);
+    };
+    if (p0 < n) {
+      u32 glo, gmid, ghi;
+      group_of(p0, glo, gmid, ghi);
+      const u32 na = gmid - glo, nb = ghi - gmid;
+      const u32 diag = p0 - glo;
+      u32 lo = diag > nb ? diag - nb : 0;
+      u32 hi = diag < na ? diag : na;
+      while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (s_key[glo + mid] <= s_key[gmid + (diag - mid - 1)]) lo = mid + 1;
+        else hi = mid;
+      }
+      u32 ia = lo, ib = diag - lo;
+#pragma unroll
+      for (int s = 0; s < kItems; ++s) {
+        const u32 q = p0 + s;
+        if (q < n) {
+          if (q == ghi) {  // the next group starts here (diagonal 0)
+            group_of(q, glo, gmid, ghi);
+            ia = 0;
+            ib = 0;
+          }
+          const bool a_left = glo + ia < gmid;
+          const bool b_left = gmid + ib < ghi;
+          const K ka = a_left ? s_key[glo + ia] : K(0);
+          const K kb = b_left ? s_key[gmid + ib] : K(0);
+          const bool take_a = a_left && (!b_left || ka <= kb);
+          const u32 from = take_a ? glo + ia : gmid + ib;
+          ok[s] = take_a ? ka : kb;
+          os[s] = slot_at(from);
+          ia += take_a;
+          ib += !take_a;
+        }
+      }
+    }
+    __syncthreads();  // every read of this round precedes the writes
+    if (static_cast<u32>(r) < levels) {
+#pragma unroll
+      for (int s = 0; s < kItems; ++s)
+        if (p0 + s < n) {
+          s_key[p0 + s] = ok[s];
+          s_slot[p0 + s] = static_cast<unsigned short>(os[s]);
+        }
+      __syncthreads();
+    }
+  }
+
+  // ok / os: this thread's merged positions.  The key before a thread's
+  // first position is the last key of the thread before (published in
+  // s_last), or for thread 0 the largest key below the tile.
+  __shared__ K s_last[kThreads];
+  const u32 steps = p0 < n ? (n - p0 < kItems ? n - p0 : kItems) : 0u;
+  if (steps) s_last[threadIdx.x] = ok[steps - 1];
+  __syncthreads();
+  bool has_prev = false;
+  K prev = 0;
+  if (threadIdx.x > 0 && steps) {
+    has_prev = true;
+    prev = s_last[threadIdx.x - 1];
+  } else if (threadIdx.x == 0 && s_has_prev) {
+    has_prev = true;
+    prev = s_prev;
+  }
+  const bool tile_high = n > 0 && s_last[(n - 1) / kItems] >= T::kTomb;
+  V ov[kItems];
+  u8 ot[kItems];
+  bool okeep[kItems];
+  u32 kept = 0;
+#pragma unroll
+  for (int s = 0; s < kItems; ++s) {
+    ov[s] = 0;
+    ot[s] = 0;
+    okeep[s] = false;
+    if (static_cast<u32>(s) < steps) {
+      ov[s] = s_val[os[s]];
+      ot[s] = s_tomb[os[s]];
+      // every version of a key is in this tile: the first (newest) stays
+      const bool dup = s > 0 ? ok[s - 1] == ok[s]
+                             : (threadIdx.x > 0 && prev == ok[s]);
+      okeep[s] = !dup && !(ot[s] && drop_tombstones);
+      kept += okeep[s];
+    }
+  }
+  const u32 local = block_exclusive_scan<kThreads>(kept, s_scan);
+  const u32 tile_total = s_scan[kThreads / kWave];
+  // Wave 0 waits on the look-back while the other waves stage their
+  // survivors at tile-local positions (the scan's barriers ordered every
+  // LDS input read before these writes).
+  if (threadIdx.x < kWave) {
+    const u64 ex = wave_lookback(status, tile, tile_total);
+    if (threadIdx.x == 0) s_base = ex;
+  }
+  Row *stage_lo = reinterpret_cast<Row *>(s_key);
+  Row *stage_hi = reinterpret_cast<Row *>(s_val);
+  u32 low = 0, tombs = 0, at = local;
+#pragma unroll
+  for (int s = 0; s < kItems; ++s) {
+    if (okeep[s]) {
+      const Row row =
+          T::make((ok[s] & T::kLow) | (ot[s] ? T::kTomb : K(0)), ov[s]);
+      if (at < kStageHalf) stage_lo[at] = row;
+      else stage_hi[at - kStageHalf] = row;
+      ++at;
+      low += ok[s] < T::kTomb;
+      tombs += ot[s];
+    }
+  }
+  __syncthreads();
+  const u64 obase = s_base;
+  for (u32 t = threadIdx.x; t < tile_total; t += kThreads)
+    if (obase + t < out_rows)
+      out[obase + t] = t < kStageHalf ? stage_lo[t] : stage_hi[t - kStageHalf];
+  // directory entries inside the bucket window (first to last input key):
+  // every input element owns the buckets in (bucket(previous input key),
+  // bucket(its key)] and writes them the survivors before it
+  {
+    const u32 shift = T::kBits - dir_bits;
+    u32 pbk = static_cast<u32>(prev >> shift);
+    u32 before = static_cast<u32>(obase) + local;
+#pragma unroll
+    for (int s = 0; s < kItems; ++s) {
+      if (static_cast<u32>(s) < steps) {
+        const u32 bk = static_cast<u32>(ok[s] >> shift);
+        if (s == 0 && !has_prev) {  // the first input element of the merge
+          out_meta->lo_bucket = bk;
+          dir[bk] = 0;
+        } else {
+          for (u32 j = pbk + 1; j <= bk; ++j) dir[j] = before;
+        }
+        pbk = bk;
+        before += okeep[s];
+      }
+    }
+    if (ends_all && steps && p0 + steps == n) {
+      // the last input element closes the window at the row count
+      out_meta->hi_bucket = pbk;
+      dir[pbk + 1] = static_cast<u32>(obase + tile_total);
+    }
+  }
+  low = wave_sum_u32(low);
+  tombs = wave_sum_u32(tombs);
+  if (lane_id() == 0) {
+    if (low) atomicAdd(&s_low, low);
+    if (tombs) atomicAdd(&s_tombs, tombs);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // the first tile whose rows reach the top key bit (or the last one)
+    // writes the split, the one holding the last row the count
+    if (ends_all) out_meta->count = static_cast<u32>(obase + tile_total);
+    if (s_before_low && (tile_high || ends_all))
+      out_meta->split = static_cast<u32>(obase + s_low);
+    if (s_tombs) out_meta->tombs = 1;
+  }
+}
+
+}  // namespace lsm
+}  // namespace gdict

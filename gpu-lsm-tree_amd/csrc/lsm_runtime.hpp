@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "host_common.hpp"
+#include "kway_merge.hpp"
 #include "lsm_kernels.hpp"
 #include "radix_sort.hpp"
 #include "seg_admission.hpp"
@@ -438,7 +439,7 @@ class LsmT : public HandleBase {
         continue;
       }
       if (q != hipErrorNotReady) GD_HIP(q);
-      backlog += it->second.a.rows_ub + it->second.b.rows_ub;
+      for (const Run &r : it->second.in) backlog += r.rows_ub;
       ++it;
     }
     if (backlog && backlog <= kQueryWaitRows) {
@@ -451,7 +452,7 @@ class LsmT : public HandleBase {
     }
     RunSet<T> set{};
     bool too_many = false;
-    // newest first: a pending merge of (newer a, older b) reads as a, b
+    // newest first: a pending merge reads as its inputs, newest first
     std::function<void(const Run &)> add = [&](const Run &r) {
       auto it = pending_.find(r.rows);
       if (it != pending_.end()) {
@@ -461,9 +462,8 @@ class LsmT : public HandleBase {
           pending_.erase(it);
         } else {
           if (q != hipErrorNotReady) GD_HIP(q);
-          const Run a = it->second.a, b = it->second.b;
-          add(a);
-          add(b);
+          const std::vector<Run> in = it->second.in;
+          for (const Run &r : in) add(r);
           return;
         }
       }
@@ -580,6 +580,11 @@ class LsmT : public HandleBase {
         throw Error(GD_INVALID_ARGUMENT, "merge_block must be 256 or 512");
       v.merge_block = t.merge_block;
     }
+    if (t.merge_ways) {
+      if (t.merge_ways < 2 || t.merge_ways > static_cast<u32>(kMaxWay))
+        throw Error(GD_INVALID_ARGUMENT, "merge_ways must be 2..8");
+      v.merge_ways = t.merge_ways;
+    }
     tuning_ = v;
   }
   const gd_lsm_tuning &tuning() const { return tuning_; }
@@ -596,7 +601,7 @@ class LsmT : public HandleBase {
   void set_profiling(bool on) {
     if (on && !prof_meta_) {
       GD_HIP(hipHostMalloc(reinterpret_cast<void **>(&prof_meta_),
-                           kProfSlots * 3 * sizeof(RunMeta),
+                           kProfSlots * kProfMetas * sizeof(RunMeta),
                            hipHostMallocDefault));
     }
     profiling_ = on;
@@ -611,11 +616,11 @@ class LsmT : public HandleBase {
       float ms = 0;
       GD_HIP(hipEventElapsedTime(&ms, r.start, r.end));
       if (r.kind == 0) {
-        const RunMeta *m = prof_meta_ + 3 * (r.slot % kProfSlots);
+        const RunMeta *m = prof_meta_ + kProfMetas * (r.slot % kProfSlots);
         out->merge_launches += 1;
         out->merge_ms += ms;
-        out->merge_rows_in += u64(m[0].count) + m[1].count;
-        out->merge_rows_out += m[2].count;
+        for (u64 i = 0; i < r.n; ++i) out->merge_rows_in += m[i].count;
+        out->merge_rows_out += m[kMaxWay].count;
       } else {
         out->lookup_launches += 1;
         out->lookup_ms += ms;
@@ -870,7 +875,7 @@ class LsmT : public HandleBase {
         out.meta(), out.max_bits);
     GD_LAUNCH_CHECK();
     ProfRec rec{};
-    if (profiling_) rec = prof_begin(0, s, 0);
+    if (profiling_) rec = prof_begin(0, s, 2);
     // tile = threads x items: 4096 rows as 256 x 16 or 512 x 8, 2048 rows
     // as 256 x 8 or 512 x 4
     const bool wide = tuning_.merge_block == 512;
@@ -892,13 +897,56 @@ class LsmT : public HandleBase {
     GD_LAUNCH_CHECK();
     if (profiling_) {
       prof_end(rec, s);
-      RunMeta *slot = prof_meta_ + 3 * (rec.slot % kProfSlots);
-      GD_HIP(hipMemcpyAsync(slot, a.meta(), sizeof(RunMeta),
+      const Run in[2] = {a, b};
+      prof_metas(rec.slot, in, 2, out, s);
+    }
+    return out;
+  }
+
+  // Copies the merge's input and output headers behind it (profiling).
+  void prof_metas(u64 rec_slot, const Run *in, u32 ways, const Run &out,
+                  hipStream_t s) {
+    RunMeta *slot = prof_meta_ + kProfMetas * (rec_slot % kProfSlots);
+    for (u32 i = 0; i < ways; ++i)
+      GD_HIP(hipMemcpyAsync(slot + i, in[i].meta(), sizeof(RunMeta),
                             hipMemcpyDeviceToHost, s));
-      GD_HIP(hipMemcpyAsync(slot + 1, b.meta(), sizeof(RunMeta),
-                            hipMemcpyDeviceToHost, s));
-      GD_HIP(hipMemcpyAsync(slot + 2, out.meta(), sizeof(RunMeta),
-                            hipMemcpyDeviceToHost, s));
+    GD_HIP(hipMemcpyAsync(slot + kMaxWay, out.meta(), sizeof(RunMeta),
+                          hipMemcpyDeviceToHost, s));
+  }
+
+  // Merges in[0..ways) (newest first, 3 <= ways <= kMaxWay) into a new run
+  // in one pass; inputs stay owned by the caller.
+  Run merge_many(const Run *in, u32 ways, bool drop_tombstones,
+                 hipStream_t s) {
+    using Shape = KWayShape<T>;
+    u64 ub = 0;
+    KWayIn<T> kin{};
+    for (u32 i = 0; i < ways; ++i) {
+      ub += in[i].rows_ub;
+      kin.run[i] = in[i].view();
+    }
+    kin.ways = ways;
+    DevicePool &pool = pool_for(s);
+    Run out = alloc_run(ub, pool);
+    const u32 tiles = static_cast<u32>(
+        std::max<u64>(1, (ub + Shape::kTarget - 1) / Shape::kTarget));
+    Lease<u32> parts(pool, (u64(tiles) + 1) * kMaxWay);
+    Lease<u32> ctrl(pool, 64 + 2ull * tiles);
+    u64 *status = reinterpret_cast<u64 *>(ctrl.get() + 64);
+    kway_partition_kernel<T><<<(tiles + 4) / 4, 256, 0, s>>>(
+        kin, tiles, Shape::kTarget, Shape::kLightMax, parts.get(), status,
+        ctrl.get(), out.meta(), out.max_bits);
+    GD_LAUNCH_CHECK();
+    ProfRec rec{};
+    if (profiling_) rec = prof_begin(0, s, ways);
+    kway_merge_kernel<T, Shape::kItems, Shape::kThreads>
+        <<<tiles, Shape::kThreads, 0, s>>>(
+            kin, parts.get(), out.rows, ub, out.meta(), out.entries(),
+            out.max_bits, drop_tombstones ? 1 : 0, status, ctrl.get());
+    GD_LAUNCH_CHECK();
+    if (profiling_) {
+      prof_end(rec, s);
+      prof_metas(rec.slot, in, ways, out, s);
     }
     return out;
   }
@@ -937,11 +985,14 @@ class LsmT : public HandleBase {
   }
 
   // Binary-counter policy: merge the two newest runs while the older one is
-  // no larger (equal batches: log2(N/b) merges per row).  Uneven batches can
-  // stack runs of decreasing size that never trigger it, so the number of
-  // runs is also capped: over the cap, the adjacent pair with the smallest
-  // combined size merges (adjacent runs are consecutive in age, so the
-  // newer-wins rule of the merge still holds).  Tombstones drop when the
+  // no larger (equal batches: log2(N/b) merges per row).  The runs such a
+  // cascade would absorb one after another (each no larger than the runs
+  // merged before it) are merged in one pass of up to merge_ways runs, which
+  // reads and writes every row once instead of once per level.  Uneven
+  // batches can stack runs of decreasing size that never trigger it, so the
+  // number of runs is also capped: over the cap, the adjacent pair with the
+  // smallest combined size merges (adjacent runs are consecutive in age, so
+  // the newer-wins rule of the merge still holds).  Tombstones drop when the
   // output becomes the oldest run.
   void rebalance(hipStream_t s) {
     const std::size_t cap =
@@ -949,10 +1000,16 @@ class LsmT : public HandleBase {
                                                            ? tuning_.max_runs
                                                            : 10,
                                                        kMaxRuns - 1));
+    const std::size_t max_ways =
+        tuning_.merge_ways ? tuning_.merge_ways : std::size_t(kMaxWay);
     while (runs_.size() >= 2) {
-      std::size_t i;
+      std::size_t i, ways = 2;
       if (runs_[1].rows_ub <= runs_[0].rows_ub) {
         i = 0;
+        u64 acc = runs_[0].rows_ub + runs_[1].rows_ub;
+        while (ways < runs_.size() && ways < max_ways &&
+               runs_[ways].rows_ub <= acc)
+          acc += runs_[ways++].rows_ub;
       } else if (runs_.size() > cap) {
         i = 0;
         u64 best = ~0ull;
@@ -966,10 +1023,13 @@ class LsmT : public HandleBase {
       } else {
         break;
       }
-      const bool oldest = i + 2 == runs_.size();
-      Run out = merge_two(runs_[i], runs_[i + 1], oldest, s);
+      const bool oldest = i + ways == runs_.size();
+      Run out = ways == 2
+                    ? merge_two(runs_[i], runs_[i + 1], oldest, s)
+                    : merge_many(&runs_[i], static_cast<u32>(ways), oldest, s);
       if (ms_ && s == ms_) {
-        PendingMerge pm{nullptr, runs_[i], runs_[i + 1]};
+        PendingMerge pm{nullptr, std::vector<Run>(runs_.begin() + i,
+                                                  runs_.begin() + i + ways)};
         GD_HIP(hipEventCreateWithFlags(&pm.done, hipEventDisableTiming));
         GD_HIP(hipEventRecord(pm.done, s));
         auto old = pending_.find(out.rows);  // a stale, completed record
@@ -979,9 +1039,8 @@ class LsmT : public HandleBase {
         }
         pending_.emplace(out.rows, pm);
       }
-      free_run(runs_[i]);
-      free_run(runs_[i + 1]);
-      runs_.erase(runs_.begin() + i, runs_.begin() + i + 2);
+      for (std::size_t j = 0; j < ways; ++j) free_run(runs_[i + j]);
+      runs_.erase(runs_.begin() + i, runs_.begin() + i + ways);
       runs_.insert(runs_.begin() + i, out);
     }
   }
@@ -1030,7 +1089,7 @@ class LsmT : public HandleBase {
   };
   struct PendingMerge {
     hipEvent_t done;
-    Run a, b;  // inputs, newer first
+    std::vector<Run> in;  // inputs, newest first
   };
 
   u64 batch_capacity_ = 1;
@@ -1038,17 +1097,18 @@ class LsmT : public HandleBase {
   // segmented admission (1) for 32-bit batches; 64-bit batches always take
   // the radix passes
   gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(T::kSmallTile), 10,
-                        1, 512};
+                        1, 512, kMaxWay};
 
   struct ProfRec {
     int kind;  // 0 merge, 1 lookup
     hipEvent_t start, end;
-    u64 slot, n;
+    u64 slot, n;  // n: merge inputs or lookup queries
   };
   static constexpr u64 kProfSlots = 4096;
+  static constexpr u64 kProfMetas = kMaxWay + 1;  // inputs, then the output
   bool profiling_ = false;
   std::vector<ProfRec> prof_;
-  RunMeta *prof_meta_ = nullptr;  // pinned: [slot][a, b, out]
+  RunMeta *prof_meta_ = nullptr;  // pinned: [slot][inputs..., out]
   u64 prof_next_ = 0;
 
   ProfRec prof_begin(int kind, hipStream_t s, u64 n) {

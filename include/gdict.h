@@ -162,6 +162,8 @@ typedef struct gd_lsm_tuning {
                                       2 = radix passes + finalize; 0 = keep */
   uint32_t merge_block;            /* merge workgroup: 256 or 512 threads
                                       per tile (512)                       */
+  uint32_t merge_ways;             /* most runs one merge pass takes: 2 =
+                                      pairwise only, up to 8 (8)           */
 } gd_lsm_tuning;
 
 gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning);

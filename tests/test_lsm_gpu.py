@@ -393,6 +393,9 @@ def test_config2_scaled(gd, orc, io):
     {"segmented_admission": 2, "dir_rows_log": 4, "small_batch_max": 1024},
     {"merge_block": 512, "merge_tile_log": 12, "max_runs": 3},
     {"merge_block": 512, "merge_tile_log": 11, "dir_rows_log": 2},
+    {"merge_ways": 2, "dir_rows_log": 3},
+    {"merge_ways": 3, "dir_rows_log": 1, "max_runs": 4},
+    {"merge_ways": 8, "dir_rows_log": 6, "small_batch_max": 1},
 ])
 def test_tuning_knobs_keep_results(gd, orc, io, knobs):
     """Launch-shape knobs change speed only: a mixed stream with small and
@@ -424,16 +427,18 @@ def test_tuning_knobs_keep_results(gd, orc, io, knobs):
 def test_invalid_tuning_rejected(gd):
     for bad in ({"sort_block": 384}, {"merge_tile_log": 13},
                 {"segmented_admission": 3}, {"merge_block": 384},
-                {"no_such_knob": 1}):
+                {"merge_ways": 1}, {"merge_ways": 9}, {"no_such_knob": 1}):
         with pytest.raises(gd.InvalidArgument):
             gd.GPULSMOpt(max_elements=1 << 16, batch_capacity=1 << 10,
                          tuning=bad)
 
 
-def test_kernel_profile_counts(gd, orc, io):
+@pytest.mark.parametrize("ways", [2, 8])
+def test_kernel_profile_counts(gd, orc, io, ways):
     """The opt-in kernel profile: merge rows in/out agree with the run
     sizes it merged and lookups count every query; reset clears it."""
-    p = Pair(gd, orc, io, max_elements=1 << 18, batch_capacity=1 << 12)
+    p = Pair(gd, orc, io, max_elements=1 << 18, batch_capacity=1 << 12,
+             tuning={"merge_ways": ways})
     p.d.set_profiling(True)
     rng = np.random.default_rng(5)
     # distinct keys (odd multiplier: a bijection mod 2^32), so merges keep
@@ -448,9 +453,15 @@ def test_kernel_profile_counts(gd, orc, io):
     p.check_lookup(q, False)
     p.check_lookup(q[:3000], True)
     prof = p.d.read_profile(reset=True)
-    # eight equal batches under the binary counter: 4+2+1 merges
-    assert prof["merge_launches"] == 7
-    assert prof["merge_rows_in"] == 4096 * (2 * 4 + 4 * 2 + 8 * 1)
+    if ways == 2:
+        # eight equal batches under the binary counter: 4+2+1 merges
+        assert prof["merge_launches"] == 7
+        assert prof["merge_rows_in"] == 4096 * (2 * 4 + 4 * 2 + 8 * 1)
+    else:
+        # each cascade in one pass: batches 2 and 6 merge 2 runs, batch 4
+        # merges 3 (1+1+2 batches), batch 8 merges 4 (1+1+2+4)
+        assert prof["merge_launches"] == 4
+        assert prof["merge_rows_in"] == 4096 * (2 + 4 + 2 + 8)
     assert prof["merge_rows_out"] == prof["merge_rows_in"]  # distinct keys
     assert prof["merge_ms"] > 0
     assert prof["lookup_launches"] == 2
