@@ -29,16 +29,18 @@ struct KWayIn {
   u32 ways;
 };
 
-// Rows per merge tile (LDS capacity), the partition's target rows per tile
-// and the largest directory bucket a split may fall into unrefined: a tile
-// holds fewer than target + max(bucket, ways) rows.
+// Merge workgroup shape, rows per tile (LDS slots: 64 short of threads x
+// items, which keeps the 32-bit tile's LDS within 40 KB, four workgroups
+// per CU), the partition's target rows per tile and the largest directory
+// bucket a split may fall into unrefined: a tile holds fewer than
+// target + max(bucket, ways) rows.
 template <class T>
 struct KWayShape {
   static constexpr u32 kThreads = sizeof(typename T::K) == 4 ? 512 : 256;
   static constexpr u32 kItems = 8;
-  static constexpr u32 kTile = kThreads * kItems;
-  static constexpr u32 kTarget = kTile / 4 * 3;
-  static constexpr u32 kLightMax = kTile - kTarget;
+  static constexpr u32 kSlots = kThreads * kItems - 64;
+  static constexpr u32 kTarget = kThreads * kItems / 4 * 3;
+  static constexpr u32 kLightMax = kSlots - kTarget;
 };
 
 // Lower bound of key c << (B - g) in a run whose directory has
@@ -188,8 +190,9 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
 // `drop_tombstones`; header and directory as in merge_kernel.  Rows are
 // staged in LDS in run order; round r merges adjacent groups of 2^(r-1)
 // runs (newer group first on equal keys) with one merge path per thread,
-// carrying (key, LDS slot) pairs, so after ceil(log2(ways)) rounds the tile
-// is in (key, age) order and each row's value is read once from its slot.
+// carrying (key, slot) pairs, so after ceil(log2(ways)) rounds the tile is
+// in (key, age) order and each row's value is read once from its slot.  A
+// slot holds the row's LDS position and, in bit 15, its tombstone flag.
 template <class T, int kItems, int kThreads>
 __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
     KWayIn<T> in, const u32 *__restrict__ parts,
@@ -199,30 +202,39 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   using K = typename T::K;
   using V = typename T::V;
   using Row = typename T::Row;
-  constexpr u32 kTile = kThreads * kItems;
+  constexpr u32 kTile = KWayShape<T>::kSlots;  // LDS slots
+  static_assert(kThreads * kItems >= kTile, "every slot has a thread");
+  constexpr u32 kWaves = kThreads / kWave;
   constexpr u32 kStageHalf = kTile / 2;  // rows over the key array
+  constexpr u32 kTombSlot = 0x8000u;
   static_assert(sizeof(Row) == 2 * sizeof(K) && sizeof(K) == sizeof(V),
                 "row = key word + value");
-  static_assert(kTile <= 65536, "slots are 16-bit");
+  static_assert(kTile <= kTombSlot, "slots are 15-bit");
   __shared__ u32 s_tile;
   __shared__ __align__(16) K s_key[kTile];
   __shared__ __align__(16) V s_val[kTile];
   __shared__ unsigned short s_slot[kTile];
-  __shared__ u8 s_tomb[kTile];
-  __shared__ u32 s_scan[kThreads / kWave + 1];
+  __shared__ u32 s_scan[kWaves + 1];
+  __shared__ K s_wave_last[kWaves];
   __shared__ u64 s_base;
-  __shared__ u32 s_low, s_tombs;
+  __shared__ u32 s_low, s_tombs, s_tile_high;
   __shared__ K s_prev;  // largest key before the tile
   __shared__ u32 s_has_prev, s_before_low;
+  // per segment: source rows, row of slot s = s + delta, top-bit split
+  __shared__ const Row *s_seg_rows[kMaxWay];
+  __shared__ u32 s_seg_delta[kMaxWay], s_seg_split[kMaxWay];
 
   if (threadIdx.x == 0) {
     s_tile = atomicAdd(ticket, 1u);
     s_low = 0;
     s_tombs = 0;
+    s_tile_high = 0;
   }
   __syncthreads();
-  const u32 tile = s_tile;
+  const u32 tile = __builtin_amdgcn_readfirstlane(s_tile);
   const u32 ways = in.ways;
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x / kWave;
   RunMeta m[kMaxWay];
   u32 beg[kMaxWay], off[kMaxWay + 1];
   u64 total = 0, ends = 0;
@@ -258,30 +270,35 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   }
   const u32 n = off[kMaxWay];
   const bool ends_all = ends == total;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    if (threadIdx.x == static_cast<u32>(i)) {
+      s_seg_rows[i] = in.run[i].rows;
+      s_seg_delta[i] = beg[i] - off[i];
+      s_seg_split[i] = m[i].split;
+    }
+  }
+  __syncthreads();
 
-  // Load: slot s of [0, n) holds row beg[i] + (s - off[i]) of run i; every
-  // load of a thread is issued before the first one is used.
+  // Load: slot s of [0, n) holds row beg[i] + (s - off[i]) of run i, where
+  // segment i is the last one starting at or before s; every load of a
+  // thread is issued before the first one is used.
   Row rr[kItems];
+  u32 segs = 0;  // 3 bits per item
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const u32 s = threadIdx.x + r * kThreads;
     if (s < n) {
-      // the segment holding s is the last one starting at or before it
-      const Row *src = in.run[0].rows;
-      u32 at = beg[0] + s;
+      u32 i = 0;
 #pragma unroll
-      for (int i = 1; i < kMaxWay; ++i)
-        if (s >= off[i]) {
-          src = in.run[i].rows;
-          at = beg[i] + (s - off[i]);
-        }
-      rr[r] = src[at];
+      for (int j = 1; j < kMaxWay; ++j) i += s >= off[j];
+      segs |= i << (3 * r);
+      rr[r] = s_seg_rows[i][s + s_seg_delta[i]];
     }
   }
   // the rows just before the tile (the largest key below it and whether
   // every row below it has the top key bit clear)
-  if (threadIdx.x < kWave) {
-    const u32 lane = lane_id();
+  if (wave == 0) {
     K pk = 0;
     bool has = false;
 #pragma unroll
@@ -311,23 +328,21 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   for (int r = 0; r < kItems; ++r) {
     const u32 s = threadIdx.x + r * kThreads;
     if (s < n) {
-      u32 split = m[0].split, pos = beg[0] + s;
-#pragma unroll
-      for (int i = 1; i < kMaxWay; ++i)
-        if (s >= off[i]) {
-          split = m[i].split;
-          pos = beg[i] + (s - off[i]);
-        }
-      s_key[s] = decode_key<T>(T::kw(rr[r]), pos, split);
-      s_tomb[s] = static_cast<u8>(T::kw(rr[r]) >> (T::kBits - 1));
+      const u32 i = (segs >> (3 * r)) & 7u;
+      const u32 split = s_seg_split[i], pos = s + s_seg_delta[i];
+      const K w = T::kw(rr[r]);
+      s_key[s] = decode_key<T>(w, pos, split);
       s_val[s] = T::val(rr[r]);
+      s_slot[s] = static_cast<unsigned short>(
+          s | ((w >> (T::kBits - 1)) ? kTombSlot : 0u));
     }
   }
   __syncthreads();
 
   // Merge rounds.  Thread t produces merged positions [t * kItems, +kItems)
   // of the round; a group is [off[u * w], off[(u + 1) * w]) with w = 2^r
-  // runs, its newer half ending at off[u * w + w / 2].
+  // runs, its newer half ending at off[u * w + w / 2].  The next key of
+  // each side rides in a register: a step reads one key and one slot.
   const u32 p0 = threadIdx.x * kItems;
   const u32 levels = ways <= 2 ? 1u : (ways <= 4 ? 2u : 3u);
   K ok[kItems];
@@ -346,20 +361,12 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
       glo = gmid = ghi = 0;
 #pragma unroll
       for (int u = 0; u < kMaxWay; u += 2) {
-        if (u % w == 0) {
-          const u32 lo_ = off[u];
-          const u32 mid_ = off[u + w / 2];
-          const u32 hi_ = off[u + w];
-          if (q >= lo_ && q < hi_) {
-            glo = lo_;
-            gmid = mid_;
-            ghi = hi_;
-          }
+        if (u % w == 0 && q >= off[u] && q < off[u + w]) {
+          glo = off[u];
+          gmid = off[u + w / 2];
+          ghi = off[u + w];
         }
       }
-    };
-    auto slot_at = [&](u32 q) -> u32 {
-      return r == 1 ? q : static_cast<u32>(s_slot[q]);
     };
     if (p0 < n) {
       u32 glo, gmid, ghi;
@@ -373,26 +380,36 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
         if (s_key[glo + mid] <= s_key[gmid + (diag - mid - 1)]) lo = mid + 1;
         else hi = mid;
       }
-      u32 ia = lo, ib = diag - lo;
+      u32 ia = glo + lo, ib = gmid + (diag - lo);  // absolute positions
+      K ka = ia < gmid ? s_key[ia] : K(0);
+      K kb = ib < ghi ? s_key[ib] : K(0);
 #pragma unroll
       for (int s = 0; s < kItems; ++s) {
         const u32 q = p0 + s;
         if (q < n) {
           if (q == ghi) {  // the next group starts here (diagonal 0)
             group_of(q, glo, gmid, ghi);
-            ia = 0;
-            ib = 0;
+            ia = glo;
+            ib = gmid;
+            ka = ia < gmid ? s_key[ia] : K(0);
+            kb = ib < ghi ? s_key[ib] : K(0);
           }
-          const bool a_left = glo + ia < gmid;
-          const bool b_left = gmid + ib < ghi;
-          const K ka = a_left ? s_key[glo + ia] : K(0);
-          const K kb = b_left ? s_key[gmid + ib] : K(0);
+          const bool a_left = ia < gmid;
+          const bool b_left = ib < ghi;
           const bool take_a = a_left && (!b_left || ka <= kb);
-          const u32 from = take_a ? glo + ia : gmid + ib;
+          const u32 from = take_a ? ia : ib;
           ok[s] = take_a ? ka : kb;
-          os[s] = slot_at(from);
-          ia += take_a;
-          ib += !take_a;
+          os[s] = s_slot[from];
+          const u32 nxt = from + 1;
+          const bool more = nxt < (take_a ? gmid : ghi);
+          const K kn = more ? s_key[nxt] : K(0);
+          if (take_a) {
+            ia = nxt;
+            ka = kn;
+          } else {
+            ib = nxt;
+            kb = kn;
+          }
         }
       }
     }
@@ -409,47 +426,46 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   }
 
   // ok / os: this thread's merged positions.  The key before a thread's
-  // first position is the last key of the thread before (published in
-  // s_last), or for thread 0 the largest key below the tile.
-  __shared__ K s_last[kThreads];
+  // first position is the last key of the thread before (every thread
+  // before the last busy one has kItems positions), or for thread 0 the
+  // largest key below the tile.
   const u32 steps = p0 < n ? (n - p0 < kItems ? n - p0 : kItems) : 0u;
-  if (steps) s_last[threadIdx.x] = ok[steps - 1];
+  const K before_me = __shfl_up(ok[kItems - 1], 1, kWave);
+  if (lane == kWave - 1) s_wave_last[wave] = ok[kItems - 1];
+  if (steps && p0 + steps == n) s_tile_high = ok[steps - 1] >= T::kTomb;
   __syncthreads();
   bool has_prev = false;
   K prev = 0;
-  if (threadIdx.x > 0 && steps) {
-    has_prev = true;
-    prev = s_last[threadIdx.x - 1];
-  } else if (threadIdx.x == 0 && s_has_prev) {
-    has_prev = true;
+  if (threadIdx.x == 0) {
+    has_prev = s_has_prev;
     prev = s_prev;
+  } else if (steps) {
+    has_prev = true;
+    prev = lane ? before_me : s_wave_last[wave - 1];
   }
-  const bool tile_high = n > 0 && s_last[(n - 1) / kItems] >= T::kTomb;
+  const bool tile_high = s_tile_high != 0;
   V ov[kItems];
-  u8 ot[kItems];
   bool okeep[kItems];
   u32 kept = 0;
 #pragma unroll
   for (int s = 0; s < kItems; ++s) {
     ov[s] = 0;
-    ot[s] = 0;
     okeep[s] = false;
     if (static_cast<u32>(s) < steps) {
-      ov[s] = s_val[os[s]];
-      ot[s] = s_tomb[os[s]];
+      ov[s] = s_val[os[s] & (kTombSlot - 1)];
       // every version of a key is in this tile: the first (newest) stays
       const bool dup = s > 0 ? ok[s - 1] == ok[s]
                              : (threadIdx.x > 0 && prev == ok[s]);
-      okeep[s] = !dup && !(ot[s] && drop_tombstones);
+      okeep[s] = !dup && !((os[s] & kTombSlot) && drop_tombstones);
       kept += okeep[s];
     }
   }
   const u32 local = block_exclusive_scan<kThreads>(kept, s_scan);
-  const u32 tile_total = s_scan[kThreads / kWave];
+  const u32 tile_total = s_scan[kWaves];
   // Wave 0 waits on the look-back while the other waves stage their
   // survivors at tile-local positions (the scan's barriers ordered every
   // LDS input read before these writes).
-  if (threadIdx.x < kWave) {
+  if (wave == 0) {
     const u64 ex = wave_lookback(status, tile, tile_total);
     if (threadIdx.x == 0) s_base = ex;
   }
@@ -459,13 +475,14 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
 #pragma unroll
   for (int s = 0; s < kItems; ++s) {
     if (okeep[s]) {
+      const bool tomb = (os[s] & kTombSlot) != 0;
       const Row row =
-          T::make((ok[s] & T::kLow) | (ot[s] ? T::kTomb : K(0)), ov[s]);
+          T::make((ok[s] & T::kLow) | (tomb ? T::kTomb : K(0)), ov[s]);
       if (at < kStageHalf) stage_lo[at] = row;
       else stage_hi[at - kStageHalf] = row;
       ++at;
       low += ok[s] < T::kTomb;
-      tombs += ot[s];
+      tombs += tomb;
     }
   }
   __syncthreads();
@@ -502,7 +519,7 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   }
   low = wave_sum_u32(low);
   tombs = wave_sum_u32(tombs);
-  if (lane_id() == 0) {
+  if (lane == 0) {
     if (low) atomicAdd(&s_low, low);
     if (tombs) atomicAdd(&s_tombs, tombs);
   }
