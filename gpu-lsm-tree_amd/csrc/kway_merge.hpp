@@ -39,36 +39,124 @@ struct KWayShape {
   static constexpr u32 kThreads = sizeof(typename T::K) == 4 ? 512 : 256;
   static constexpr u32 kItems = 8;
   static constexpr u32 kSlots = kThreads * kItems - 64;
-  static constexpr u32 kTarget = kThreads * kItems / 4 * 3;
-  static constexpr u32 kLightMax = kSlots - kTarget;
+  static constexpr u32 kLightMax = 128;
+  static constexpr u32 kTarget = kSlots - kLightMax;
 };
 
-// Lower bound of key c << (B - g) in a run whose directory has
-// m.dir_bits >= g bits: exact, from one directory entry.
+// Lower bound of key c << (B - g), c in [0, 2^g]: one directory entry
+// when the run's directory has at least g bits, else a search inside the
+// directory bucket that holds the key.
 template <class T>
 __device__ __forceinline__ u32 boundary_lb(const RunView<T> &r,
                                            const RunMeta &m, u32 c, u32 g) {
+  using K = typename T::K;
   if (m.count == 0) return 0;
-  const u64 j = u64(c) << (m.dir_bits - g);
-  if (j <= m.lo_bucket) return 0;       // dir[lo_bucket] = 0
-  if (j > m.hi_bucket) return m.count;  // dir[hi_bucket + 1] = count
-  return r.dir[j];
+  if (m.dir_bits >= g) {
+    const u64 j = u64(c) << (m.dir_bits - g);
+    if (j <= m.lo_bucket) return 0;       // dir[lo_bucket] = 0
+    if (j > m.hi_bucket) return m.count;  // dir[hi_bucket + 1] = count
+    return r.dir[j];
+  }
+  if ((u64(c) >> g) != 0) return m.count;  // c = 2^g: past every key
+  return run_lower_bound<T>(r, m, K(c) << (T::kBits - g));
+}
+
+// Bounds on the rows below key c << (B - g), c < 2^g, summed over the
+// runs, from directory entries only (every load issued before any use):
+// exact for a run whose directory resolves the key; for a coarser run whose
+// bucket holds the key strictly inside, its rows below the bucket (lo) and
+// below the bucket's end (hi).
+template <class T>
+__device__ __forceinline__ void bounds_at(const KWayIn<T> &in,
+                                          const RunMeta (&m)[kMaxWay],
+                                          u32 ways, u32 c, u32 g, u64 &lo,
+                                          u64 &hi) {
+  u32 v[kMaxWay], e[kMaxWay];
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    v[i] = e[i] = 0;
+    if (static_cast<u32>(i) < ways && m[i].count) {
+      const u32 db = m[i].dir_bits;
+      const u64 j = db >= g ? u64(c) << (db - g) : u64(c) >> (g - db);
+      const bool inner = db < g && (c & ((1u << (g - db)) - 1u)) != 0;
+      if (j < m[i].lo_bucket || (!inner && j == m[i].lo_bucket)) {
+        v[i] = e[i] = 0;
+      } else if (j > m[i].hi_bucket) {
+        v[i] = e[i] = m[i].count;
+      } else {
+        v[i] = in.run[i].dir[j];
+        e[i] = inner ? in.run[i].dir[j + 1] : v[i];
+      }
+    }
+  }
+  lo = hi = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    lo += v[i];
+    hi += e[i];
+  }
+}
+
+// Tile boundaries by buckets, one thread per key bucket c at the coarsest
+// directory resolution g of the inputs (every run's directory resolves it
+// with one entry): with P(c) the rows below the bucket's start summed over
+// the runs, the bucket holds boundary t when P(c) <= t * target <
+// P(c + 1), and records c for it.  Every boundary 0 < t < tiles with
+// t * target below the row total lands in exactly one bucket.  Reads each
+// needed directory entry once instead of searching for every boundary.
+template <class T>
+__global__ void kway_bucket_kernel(KWayIn<T> in, u32 tiles, u32 target,
+                                   u32 *__restrict__ bucket_of) {
+  const u32 ways = in.ways;
+  RunMeta m[kMaxWay];
+  u32 g = kMaxDirBits;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    m[i] = RunMeta{0, 0, kMaxDirBits, 0, 1, 0};
+    if (static_cast<u32>(i) < ways) {
+      m[i] = *in.run[i].meta;
+      g = m[i].dir_bits < g ? m[i].dir_bits : g;
+    }
+  }
+  const u64 c = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >> g) return;
+  u32 v0[kMaxWay], v1[kMaxWay];
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    v0[i] = v1[i] = 0;
+    if (static_cast<u32>(i) < ways) {
+      v0[i] = boundary_lb(in.run[i], m[i], static_cast<u32>(c), g);
+      v1[i] = boundary_lb(in.run[i], m[i], static_cast<u32>(c) + 1, g);
+    }
+  }
+  u64 p0 = 0, p1 = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxWay; ++i) {
+    p0 += v0[i];
+    p1 += v1[i];
+  }
+  u64 t = p0 == 0 ? 1 : (p0 + target - 1) / target;
+  for (; t < tiles && t * target < p1; ++t)
+    bucket_of[t] = static_cast<u32>(c);
 }
 
 // Split points of the k-way merge, one wave per tile boundary t in
 // [0, tiles]: parts[t * kMaxWay + i] = rows of run i with keys below x_t.
-// Phase 1: a 64-ary search over the directory buckets at the coarsest
-// directory resolution g of the inputs finds the bucket c whose start has
-// at most R = t * target rows below it (summed over the runs) and whose end
-// has more.  When that bucket holds at most light_max rows, x_t is its
-// start.  Otherwise (dense keys) phase 2 searches the keys inside the
-// bucket for the largest x with at most R rows below it.  Also clears the
-// merge's chained-scan state and the output run's header.
+// Starting from the boundary's coarse bucket (kway_bucket_kernel), a 64-ary
+// search over its sub-buckets at the finest directory resolution of the
+// inputs finds the bucket c whose start has at most R = t * target rows
+// below it (summed over the runs) and whose end has more.  When that bucket
+// holds at most light_max rows, x_t is its start.  Otherwise (dense keys)
+// phase 2 searches the keys inside the bucket for the largest x with at
+// most R rows below it.  Also clears the merge's chained-scan state and the
+// output run's header.
 template <class T>
 __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
-                                      u32 light_max, u32 *__restrict__ parts,
-                                      u64 *status, u32 *ticket,
-                                      RunMeta *out_meta, u32 dir_bits) {
+                                      u32 light_max,
+                                      const u32 *__restrict__ bucket_of,
+                                      u32 *__restrict__ parts, u64 *status,
+                                      u32 *ticket, RunMeta *out_meta,
+                                      u32 dir_bits) {
   using K = typename T::K;
   const u32 lane = lane_id();
   const u64 t = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
@@ -86,14 +174,15 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
   const u32 ways = in.ways;
   RunMeta m[kMaxWay];
   u64 total = 0;
-  u32 g = kMaxDirBits;
+  u32 g_lo = kMaxDirBits, g = 1;
 #pragma unroll
   for (int i = 0; i < kMaxWay; ++i) {
     m[i] = RunMeta{0, 0, kMaxDirBits, 0, 1, 0};
     if (static_cast<u32>(i) < ways) {
       m[i] = *in.run[i].meta;
       total += m[i].count;
-      g = m[i].dir_bits < g ? m[i].dir_bits : g;
+      g_lo = m[i].dir_bits < g_lo ? m[i].dir_bits : g_lo;
+      g = m[i].dir_bits > g ? m[i].dir_bits : g;
     }
   }
   u32 *out = parts + t * kMaxWay;
@@ -105,55 +194,89 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
         out[i] = t == 0 ? 0u : m[i].count;
     return;
   }
-  // rows below the start of bucket c, over all runs
-  auto below = [&](u32 c) -> u64 {
-    u32 v[kMaxWay];
-#pragma unroll
-    for (int i = 0; i < kMaxWay; ++i)
-      v[i] = static_cast<u32>(i) < ways ? boundary_lb(in.run[i], m[i], c, g)
-                                        : 0u;
-    u64 s = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxWay; ++i) s += v[i];
-    return s;
+  // 64-ary search for the last c in [lo, hi) with below(c) <= R, given
+  // below(lo) <= R < below(hi)
+  auto search = [&](u32 lo, u32 hi, auto below) -> u32 {
+    while (hi - lo > 1) {
+      const u32 span = hi - lo;
+      const bool fine = span <= static_cast<u32>(kWave);
+      const u32 c =
+          fine ? lo + lane : lo + static_cast<u32>((u64(span) * lane) >> 6);
+      bool p = lane == 0;  // below(lo) <= R is known
+      if (lane != 0 && c < hi) p = below(c) <= R;
+      const u32 L = popc64(ballot(p)) - 1;  // the true lanes form a prefix
+      const u32 cl =
+          fine ? lo + L : lo + static_cast<u32>((u64(span) * L) >> 6);
+      const u32 cn =
+          fine ? lo + L + 1
+               : (L + 1 < static_cast<u32>(kWave)
+                      ? lo + static_cast<u32>((u64(span) * (L + 1)) >> 6)
+                      : hi);
+      lo = cl;
+      hi = cn;
+    }
+    return lo;
   };
-  // phase 1: below(lo) <= R < below(hi)
-  u32 lo = 0, hi = 1u << g;
-  while (hi - lo > 1) {
-    const u32 span = hi - lo;
-    const bool fine = span <= static_cast<u32>(kWave);
-    const u32 c = fine ? lo + lane : lo + static_cast<u32>((u64(span) * lane) >> 6);
-    bool p = lane == 0;  // below(lo) <= R is known
-    if (lane != 0 && c < hi) p = below(c) <= R;
-    const u32 L = popc64(ballot(p)) - 1;  // the true lanes form a prefix
-    const u32 cl = fine ? lo + L : lo + static_cast<u32>((u64(span) * L) >> 6);
-    const u32 cn = fine ? lo + L + 1
-                        : (L + 1 < static_cast<u32>(kWave)
-                               ? lo + static_cast<u32>((u64(span) * (L + 1)) >> 6)
-                               : hi);
-    lo = cl;
-    hi = cn;
-  }
-  u32 a[kMaxWay], b[kMaxWay];
-  u64 below_lo = 0, rows_in = 0;
+  // this lane's run (lanes >= ways answer nothing)
+  RunView<T> rv = in.run[0];
+  RunMeta rm = m[0];
+#pragma unroll
+  for (int i = 1; i < kMaxWay; ++i)
+    if (lane == static_cast<u32>(i)) {
+      rv = in.run[i];
+      rm = m[i];
+    }
+  // phase 1: the boundary's coarse bucket (exact), then the last of its
+  // sub-buckets at the finest resolution whose start has at most R rows
+  // below it by the upper bound; the start becomes x_t when the lower
+  // bound leaves fewer than light_max rows short of R, which bounds the
+  // tile, and every run's split is then its lower bound of x_t
+  const u32 c0 = bucket_of[t];
+  u64 below_lo = 0;
+  u32 a_mine = 0, b_mine = 0;
 #pragma unroll
   for (int i = 0; i < kMaxWay; ++i) {
-    a[i] = b[i] = 0;
     if (static_cast<u32>(i) < ways) {
-      a[i] = boundary_lb(in.run[i], m[i], lo, g);
-      b[i] = boundary_lb(in.run[i], m[i], lo + 1, g);
+      const u32 ai = boundary_lb(in.run[i], m[i], c0, g_lo);
+      const u32 bi = boundary_lb(in.run[i], m[i], c0 + 1, g_lo);
+      below_lo += ai;
+      if (lane == static_cast<u32>(i)) {
+        a_mine = ai;
+        b_mine = bi;
+      }
     }
-    below_lo += a[i];
-    rows_in += b[i] - a[i];
   }
-  if (rows_in > light_max) {
-    // phase 2: largest x in the bucket with at most `need` bucket rows
-    // below it; inside one bucket every key shares the top bit, so rows
-    // compare on the low bits
+  K x = K(c0) << (T::kBits - g_lo);
+  bool light = R - below_lo < light_max;
+  if (!light && g > g_lo) {
+    const u32 d = g - g_lo;
+    const u32 c = search(c0 << d, (c0 + 1) << d, [&](u32 cc) -> u64 {
+      u64 plo, phi;
+      bounds_at<T>(in, m, ways, cc, g, plo, phi);
+      return phi;
+    });
+    u64 plo, phi;
+    bounds_at<T>(in, m, ways, c, g, plo, phi);
+    if (R - plo < light_max) {
+      light = true;
+      x = K(c) << (T::kBits - g);
+    }
+  }
+  u32 mine = 0;
+  if (light) {
+    if (lane < ways) mine = run_lower_bound<T>(rv, rm, x);
+  } else {
+    // phase 2 (dense keys): the largest x in the coarse bucket with at
+    // most R rows below it, by a 64-ary search over its keys; inside one
+    // bucket every key shares the top bit, so rows compare on the low bits
+    u32 a[kMaxWay], b[kMaxWay];
+#pragma unroll
+    for (int i = 0; i < kMaxWay; ++i) {
+      a[i] = __shfl(a_mine, i, kWave);
+      b[i] = __shfl(b_mine, i, kWave);
+    }
     const u64 need = R - below_lo;
-    const u32 s = T::kBits - g;  // 1 <= g <= 24: s <= B - 1
-    K x = K(lo) << s;
-    u64 span = u64(1) << s;
+    u64 span = u64(1) << (T::kBits - g_lo);  // 1 <= g_lo <= 24
     auto rank_in = [&](K key) -> u64 {
       u64 r = 0;
 #pragma unroll
@@ -175,14 +298,10 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
       span = fine ? 1 : (L + 1 < static_cast<u32>(kWave) ? step
                                                          : span - u64(L) * step);
     }
-#pragma unroll
-    for (int i = 0; i < kMaxWay; ++i)
-      if (static_cast<u32>(i) < ways)
-        a[i] = window_lower_bound<T>(in.run[i].rows, a[i], b[i], x & T::kLow);
+    if (lane < ways)
+      mine = window_lower_bound<T>(rv.rows, a_mine, b_mine, x & T::kLow);
   }
-#pragma unroll
-  for (int i = 0; i < kMaxWay; ++i)
-    if (lane == static_cast<u32>(i) && static_cast<u32>(i) < ways) out[i] = a[i];
+  if (lane < ways) out[lane] = mine;
 }
 
 // Merges the tile's rows of every input run (newest first) into `out`: on

@@ -931,11 +931,19 @@ class LsmT : public HandleBase {
     const u32 tiles = static_cast<u32>(
         std::max<u64>(1, (ub + Shape::kTarget - 1) / Shape::kTarget));
     Lease<u32> parts(pool, (u64(tiles) + 1) * kMaxWay);
+    Lease<u32> bucket_of(pool, u64(tiles) + 1);
     Lease<u32> ctrl(pool, 64 + 2ull * tiles);
     u64 *status = reinterpret_cast<u64 *>(ctrl.get() + 64);
+    // directory bits never exceed the allocation's: 2^coarse buckets cover
+    // the coarsest directory of the inputs
+    u32 coarse = kMaxDirBits;
+    for (u32 i = 0; i < ways; ++i) coarse = std::min(coarse, in[i].max_bits);
+    const u64 buckets = u64(1) << coarse;
+    kway_bucket_kernel<T><<<static_cast<u32>((buckets + 255) / 256), 256, 0,
+                            s>>>(kin, tiles, Shape::kTarget, bucket_of.get());
     kway_partition_kernel<T><<<(tiles + 4) / 4, 256, 0, s>>>(
-        kin, tiles, Shape::kTarget, Shape::kLightMax, parts.get(), status,
-        ctrl.get(), out.meta(), out.max_bits);
+        kin, tiles, Shape::kTarget, Shape::kLightMax, bucket_of.get(),
+        parts.get(), status, ctrl.get(), out.meta(), out.max_bits);
     GD_LAUNCH_CHECK();
     ProfRec rec{};
     if (profiling_) rec = prof_begin(0, s, ways);
