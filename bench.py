@@ -695,7 +695,9 @@ def main():
             prof["merge_launches"],
             {"rows_in_S": int(prof["merge_rows_in"]),
              "rows_out_M": int(prof["merge_rows_out"]),
-             "bytes_per_row": 8}))
+             "bytes_per_row": 8,
+             "kernels": "two-way merge_kernel and k-way kway_merge_kernel "
+                        "launches (both match merge_kernel< in the traces)"}))
     if prof["lookup_launches"]:
         grouped = Q >= (1 << d.get_tuning()["grouped_lookup_min_log"])
         kernels.append(roof(
