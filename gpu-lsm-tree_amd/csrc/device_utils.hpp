@@ -38,6 +38,22 @@ __device__ __forceinline__ void store_relaxed(u64 *p, u64 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// LDS counters that the lanes of a wave update for each other (one lane
+// writes, all read on the next item): relaxed workgroup-scope accesses, so
+// the compiler neither caches nor forwards them.  The pointer must point
+// into LDS; the address-space cast keeps them ds_read / ds_write (a
+// volatile access through a generic pointer becomes a flat access that
+// also waits for every outstanding global load).
+using lds_u32 = __attribute__((address_space(3))) u32;
+__device__ __forceinline__ u32 lds_load(u32 *p) {
+  return __hip_atomic_load((lds_u32 *)(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_store(u32 *p, u32 v) {
+  __hip_atomic_store((lds_u32 *)(p), v, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Wave-wide inclusive scan of a u32 through DPP-backed shuffles.
 __device__ __forceinline__ u32 wave_inclusive_scan(u32 v) {
   const u32 lane = lane_id();

@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
     __syncthreads();
     K key[kItems];
     u32 pos[kItems], rank[kItems];
-    volatile u32 *my_hist = s_hist + wave * 256;
+    u32 *my_hist = s_hist + wave * 256;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
       const u32 idx = wave * (kWave * kItems) + j * kWave + lane;
@@ -429,9 +429,10 @@ __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
         match &= bit ? vote : ~vote;
       }
       u32 prior = 0;
-      if (valid) prior = my_hist[d];
+      if (valid) prior = lds_load(&my_hist[d]);
       rank[j] = prior + popc64(match & lanemask_lt());
-      if (valid && (match >> lane) == 1ull) my_hist[d] = prior + popc64(match);
+      if (valid && (match >> lane) == 1ull)
+        lds_store(&my_hist[d], prior + popc64(match));
     }
     __syncthreads();
     // wave prefixes per digit (threads 0..255 own one digit each)

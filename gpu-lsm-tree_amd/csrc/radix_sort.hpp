@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   }
 
   // Wave-local stable ranking (item-major, then lane order == input order).
-  volatile u32 *my_hist = s_wave_hist[wave];
+  u32 *my_hist = s_wave_hist[wave];
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const u64 idx = wave_base + u64(j) * kWave + lane;
@@ -170,11 +170,11 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
       match &= bit ? vote : ~vote;
     }
     u32 prior = 0;
-    if (valid) prior = my_hist[d];
+    if (valid) prior = lds_load(&my_hist[d]);
     rank[j] = prior + popc64(match & lanemask_lt());
     // the highest lane of each digit group advances the counter
     if (valid && (match >> lane) == 1ull)
-      my_hist[d] = prior + popc64(match);
+      lds_store(&my_hist[d], prior + popc64(match));
   }
   __syncthreads();
 

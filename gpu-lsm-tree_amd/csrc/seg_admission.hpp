@@ -232,7 +232,7 @@ __device__ __forceinline__ void seg_rank(SegShared &sh, u32 (*hist)[256],
                                          u32 (&dst_local)[kSegItems]) {
   const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const u32 wave_base = wave * kWave * ipt;
-  volatile u32 *my_hist = hist[wave];
+  u32 *my_hist = hist[wave];
   u32 rank[kSegItems];
 #pragma unroll
   for (int j = 0; j < kSegItems; ++j) {
@@ -247,9 +247,10 @@ __device__ __forceinline__ void seg_rank(SegShared &sh, u32 (*hist)[256],
         match &= bit ? vote : ~vote;
       }
       u32 prior = 0;
-      if (valid) prior = my_hist[d];
+      if (valid) prior = lds_load(&my_hist[d]);
       rank[j] = prior + popc64(match & lanemask_lt());
-      if (valid && (match >> lane) == 1ull) my_hist[d] = prior + popc64(match);
+      if (valid && (match >> lane) == 1ull)
+        lds_store(&my_hist[d], prior + popc64(match));
     }
   }
   __syncthreads();
@@ -427,7 +428,10 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   for (u32 i = tid; i < 2 * kSegWaves * 256; i += kSegThreads)
     (&sh.wave_hist[0][0][0])[i] = 0;
   __syncthreads();
-  const u32 seg = s_ticket;  // grid = 256: one ticket per segment
+  // grid = 256: one ticket per segment.  Values derived from it are
+  // wave-uniform; saying so keeps them (and the per-item predicates on
+  // them) in scalar registers instead of per-lane masks
+  const u32 seg = __builtin_amdgcn_readfirstlane(s_ticket);
   const u32 s0 = *shift_in;
   const u32 start = offsets[seg];
   const u32 end = seg == 255 ? n : offsets[seg + 1];
@@ -452,12 +456,13 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   if (!in_lds) {
     u32 distinct = 0;
     if (seg_dedupe(sh, pk + start, pp + start, m, key_mask, distinct)) {
-      m = distinct;
+      m = __builtin_amdgcn_readfirstlane(distinct);
       in_lds = staged = true;
     }
   }
   if (in_lds) {
-    const u32 ipt = m ? (m + kSegThreads - 1) / kSegThreads : 1u;
+    const u32 ipt = __builtin_amdgcn_readfirstlane(
+        m ? (m + kSegThreads - 1) / kSegThreads : 1u);
     const u32 wave_base = wave * kWave * ipt;
     if (passes == 0 && !staged) {
       // nothing to sort (each segment holds one key): copy it to LDS, all
@@ -538,7 +543,8 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       }
       for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
         const u32 cnt = m - c0 < u32(kSegTile) ? m - c0 : u32(kSegTile);
-        const u32 ipt = (cnt + kSegThreads - 1) / kSegThreads;
+        const u32 ipt = __builtin_amdgcn_readfirstlane(
+            (cnt + kSegThreads - 1) / kSegThreads);
         const u32 wave_base = wave * kWave * ipt;
         u32 key[kSegItems], pay[kSegItems], dst[kSegItems];
 #pragma unroll
