@@ -449,8 +449,10 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   int round = 0;  // ranking rounds: selects the histogram buffer
 
   // ---------------------------------------------------------------- sort
-  const u32 *sk;  // sorted keys / payloads of the segment (LDS or global)
-  const u32 *sp;
+  // sorted keys / payloads of the segment: in LDS (cur) or in global
+  // memory (gk, gp)
+  int cur = 0;
+  const u32 *gk = nullptr, *gp = nullptr;
   bool in_lds = m <= static_cast<u32>(kSegTile);
   bool staged = false;  // the elements are in sh.k[0] / sh.p[0] already
   if (!in_lds) {
@@ -486,7 +488,6 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       }
       __syncthreads();
     }
-    int cur = 0;
     for (u32 pass = 0; pass < passes; ++pass) {
       // the first pass reads the partitioned segment from global memory,
       // the later ones the previous pass's LDS arrays
@@ -522,8 +523,6 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       __syncthreads();
       cur ^= 1;
     }
-    sk = sh.k[cur];
-    sp = sh.p[cur];
   } else {
     // stable LSD in global memory, chunk by chunk, ping-pong pk <-> tk
     u32 *src_k = pk + start, *src_p = pp + start;
@@ -583,174 +582,181 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       __threadfence_block();
       __syncthreads();
     }
-    sk = src_k;
-    sp = src_p;
+    gk = src_k;
+    gp = src_p;
   }
 
-  // ------------------------------------------- survivors: count, offset
-  // keep the last occurrence of each key (stable sort: the newest update)
-  auto keep_at = [&](u32 i) {
-    return i < m && (i + 1 == m || ((sk[i + 1] ^ sk[i]) & key_mask) != 0);
+  // The rest runs once per location of the sorted segment, so that each
+  // copy reads it with LDS or global loads rather than generic ones.
+  auto finish = [&](const u32 *sk, const u32 *sp) {
+    // ------------------------------------------- survivors: count, offset
+    // keep the last occurrence of each key (stable sort: the newest update)
+    auto keep_at = [&](u32 i) {
+      return i < m && (i + 1 == m || ((sk[i + 1] ^ sk[i]) & key_mask) != 0);
+    };
+    u32 total = 0;
+    for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
+      u32 cnt = 0;
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j)
+        cnt += keep_at(c0 + j * kSegThreads + tid);
+      total += wave_sum_u32(cnt);
+    }
+    if (lane == 0) sh.scan[wave] = total;
+    __syncthreads();
+    if (tid == 0) {
+      u32 t = 0;
+      for (int w = 0; w < kSegWaves; ++w) t += sh.scan[w];
+      sh.misc[0] = t;
+    }
+    __syncthreads();
+    const u32 seg_total = sh.misc[0];
+    if (tid == 0) {
+      sh.misc[1] = 0;  // survivors below the top key bit
+      sh.misc[2] = 0;  // tombstones
+    }
+    // the chained scan over segments (wave 0) runs beside the first chunk's
+    // survivor offsets (wave 1); an empty segment scans on its own
+    auto lookback = [&]() {
+      if (wave == 0) {
+        const u64 ex = wave_lookback(status, seg, seg_total);
+        if (tid == 0) sh.base = ex;
+      }
+    };
+    u64 base = 0;
+    if (m == 0) {
+      lookback();
+      __syncthreads();
+      base = sh.base;
+    }
+
+    // ------------------------------------------- rows and directory entries
+    const u32 shift = 32u - dir_bits;
+    u32 low = 0, tombs = 0;
+    u32 running = 0;  // survivors of earlier chunks
+    for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
+      u64 mask[kSegItems];
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        mask[j] = ballot(keep_at(c0 + j * kSegThreads + tid));
+        if (lane == 0) sh.off[j][wave] = popc64(mask[j]);
+      }
+      __syncthreads();
+      if (c0 == 0) lookback();
+      // exclusive scan of the (item, wave) survivor counts in row order, by
+      // wave 1: lane l owns entries 2l and 2l + 1
+      if (wave == 1) {
+        u32 *flat = &sh.off[0][0];
+        const u32 a = flat[2 * lane], b = flat[2 * lane + 1];
+        const u32 inc = wave_inclusive_scan(a + b);
+        const u32 ex = running + inc - (a + b);
+        flat[2 * lane] = ex;
+        flat[2 * lane + 1] = ex + a;
+        if (lane == kWave - 1) sh.misc[3] = running + inc;
+      }
+      __syncthreads();
+      base = sh.base;
+      u32 k[kSegItems], prev[kSegItems], pay[kSegItems];
+      bool tomb[kSegItems];
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 i = c0 + j * kSegThreads + tid;
+        const u32 raw = i < m ? sk[i] : 0u;
+        k[j] = restore(raw);
+        prev[j] = (i < m && i > 0) ? restore(sk[i - 1]) : k[j];
+        pay[j] = (i < m && ((mask[j] >> lane) & 1ull)) ? sp[i] : 0u;
+        tomb[j] = mode == kModeErase || (stashed && (raw & kStashBit));
+      }
+      // a position payload: gather value and flag (all items issued before
+      // any row is written, so their latencies overlap)
+      u32 val[kSegItems];
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 i = c0 + j * kSegThreads + tid;
+        const bool kept = i < m && ((mask[j] >> lane) & 1ull);
+        if (gathered) {
+          tomb[j] = kept && erase_flags[pay[j]] != 0;
+          val[j] = kept ? values[pay[j]] : 0u;
+        } else {
+          val[j] = pay[j];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSegItems; ++j) {
+        const u32 i = c0 + j * kSegThreads + tid;
+        if (i >= m) continue;
+        // survivors before position i: the output row of key k
+        const u32 before = static_cast<u32>(base) + sh.off[j][wave] +
+                           popc64(mask[j] & lanemask_lt());
+        if (prev[j] != k[j]) {
+          for (u32 b = (prev[j] >> shift) + 1; b <= (k[j] >> shift); ++b)
+            dir[b] = before;
+        }
+        if ((mask[j] >> lane) & 1ull) {
+          out[before] = T::make((k[j] & T::kLow) | (tomb[j] ? T::kTomb : 0u),
+                                tomb[j] ? 0u : val[j]);
+          low += k[j] < T::kTomb;
+          tombs += tomb[j];
+        }
+      }
+      running = sh.misc[3];
+      __syncthreads();
+    }
+    low = wave_sum_u32(low);
+    tombs = wave_sum_u32(tombs);
+    if (lane == 0) {
+      if (low) atomicAdd(&sh.misc[1], low);
+      if (tombs) atomicAdd(&sh.misc[2], tombs);
+    }
+    __syncthreads();
+
+    // the segment's key range [lo_key, hi_key) and its directory bounds:
+    // bounds below its first key and above its last key (all of them when
+    // it is empty) take the rows before / after the segment
+    const u64 unit = u64(1) << s0;
+    // every key of the batch shares the bits at and above s0 + 8
+    const u64 common =
+        s0 + 8 >= 32 ? 0 : (u64(keys_in[0]) >> (s0 + 8)) << (s0 + 8);
+    const u64 lo_key = common + u64(seg) * unit;
+    const u64 hi_key = lo_key + unit;  // exclusive, may be 2^32
+    const u64 ub = u64(1) << shift;    // bucket width
+    const u64 first_b = (lo_key + ub - 1) >> shift;  // first bound >= lo_key
+    const u64 end_b = (hi_key + ub - 1) >> shift;    // first bound >= hi_key
+    const u32 seg_end_row = static_cast<u32>(base) + seg_total;
+    const u32 first_key = m ? restore(sk[0]) : 0u;
+    const u32 last_key = m ? restore(sk[m - 1]) : 0u;
+    if (m == 0) {
+      for (u64 b = first_b + tid; b < end_b; b += kSegThreads)
+        dir[b] = static_cast<u32>(base);
+    } else {
+      const u64 k0b = u64(first_key) >> shift;  // bounds in [lo_key, first]
+      for (u64 b = first_b + tid; b <= k0b; b += kSegThreads)
+        dir[b] = static_cast<u32>(base);
+      const u64 klb = (u64(last_key) >> shift) + 1;  // bounds after the last
+      for (u64 b = klb + tid; b < end_b; b += kSegThreads) dir[b] = seg_end_row;
+    }
+
+    if (tid == 0) {
+      if (seg == 255) meta->count = seg_end_row;
+      const u64 tb = T::kTomb;
+      if ((lo_key <= tb && tb < hi_key) || (seg == 0 && tb < lo_key) ||
+          (seg == 255 && hi_key <= tb))
+        meta->split = static_cast<u32>(base) + sh.misc[1];
+      if (sh.misc[2]) meta->tombs = 1;
+      if (seg == 0) meta->dir_bits = dir_bits;
+      if (m && start == 0) {  // the first keys of the run
+        meta->lo_bucket = first_key >> shift;
+        dir[first_key >> shift] = 0;
+      }
+      if (m && end == n) {  // the last keys of the run
+        const u32 hb = last_key >> shift;
+        meta->hi_bucket = hb;
+        dir[hb + 1] = seg_end_row;
+      }
+    }
   };
-  u32 total = 0;
-  for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
-    u32 cnt = 0;
-#pragma unroll
-    for (int j = 0; j < kSegItems; ++j) cnt += keep_at(c0 + j * kSegThreads + tid);
-    total += wave_sum_u32(cnt);
-  }
-  if (lane == 0) sh.scan[wave] = total;
-  __syncthreads();
-  if (tid == 0) {
-    u32 t = 0;
-    for (int w = 0; w < kSegWaves; ++w) t += sh.scan[w];
-    sh.misc[0] = t;
-  }
-  __syncthreads();
-  const u32 seg_total = sh.misc[0];
-  if (tid == 0) {
-    sh.misc[1] = 0;  // survivors below the top key bit
-    sh.misc[2] = 0;  // tombstones
-  }
-  // the chained scan over segments (wave 0) runs beside the first chunk's
-  // survivor offsets (wave 1); an empty segment scans on its own
-  auto lookback = [&]() {
-    if (wave == 0) {
-      const u64 ex = wave_lookback(status, seg, seg_total);
-      if (tid == 0) sh.base = ex;
-    }
-  };
-  u64 base = 0;
-  if (m == 0) {
-    lookback();
-    __syncthreads();
-    base = sh.base;
-  }
-
-  // ------------------------------------------- rows and directory entries
-  const u32 shift = 32u - dir_bits;
-  u32 low = 0, tombs = 0;
-  u32 running = 0;  // survivors of earlier chunks
-  for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
-    u64 mask[kSegItems];
-#pragma unroll
-    for (int j = 0; j < kSegItems; ++j) {
-      mask[j] = ballot(keep_at(c0 + j * kSegThreads + tid));
-      if (lane == 0) sh.off[j][wave] = popc64(mask[j]);
-    }
-    __syncthreads();
-    if (c0 == 0) lookback();
-    // exclusive scan of the (item, wave) survivor counts in row order, by
-    // wave 1: lane l owns entries 2l and 2l + 1
-    if (wave == 1) {
-      u32 *flat = &sh.off[0][0];
-      const u32 a = flat[2 * lane], b = flat[2 * lane + 1];
-      const u32 inc = wave_inclusive_scan(a + b);
-      const u32 ex = running + inc - (a + b);
-      flat[2 * lane] = ex;
-      flat[2 * lane + 1] = ex + a;
-      if (lane == kWave - 1) sh.misc[3] = running + inc;
-    }
-    __syncthreads();
-    base = sh.base;
-    u32 k[kSegItems], prev[kSegItems], pay[kSegItems];
-    bool tomb[kSegItems];
-#pragma unroll
-    for (int j = 0; j < kSegItems; ++j) {
-      const u32 i = c0 + j * kSegThreads + tid;
-      const u32 raw = i < m ? sk[i] : 0u;
-      k[j] = restore(raw);
-      prev[j] = (i < m && i > 0) ? restore(sk[i - 1]) : k[j];
-      pay[j] = (i < m && ((mask[j] >> lane) & 1ull)) ? sp[i] : 0u;
-      tomb[j] = mode == kModeErase || (stashed && (raw & kStashBit));
-    }
-    // a position payload: gather value and flag (all items issued before
-    // any row is written, so their latencies overlap)
-    u32 val[kSegItems];
-#pragma unroll
-    for (int j = 0; j < kSegItems; ++j) {
-      const u32 i = c0 + j * kSegThreads + tid;
-      const bool kept = i < m && ((mask[j] >> lane) & 1ull);
-      if (gathered) {
-        tomb[j] = kept && erase_flags[pay[j]] != 0;
-        val[j] = kept ? values[pay[j]] : 0u;
-      } else {
-        val[j] = pay[j];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kSegItems; ++j) {
-      const u32 i = c0 + j * kSegThreads + tid;
-      if (i >= m) continue;
-      // survivors before position i: the output row of key k
-      const u32 before = static_cast<u32>(base) + sh.off[j][wave] +
-                         popc64(mask[j] & lanemask_lt());
-      if (prev[j] != k[j]) {
-        for (u32 b = (prev[j] >> shift) + 1; b <= (k[j] >> shift); ++b)
-          dir[b] = before;
-      }
-      if ((mask[j] >> lane) & 1ull) {
-        out[before] = T::make((k[j] & T::kLow) | (tomb[j] ? T::kTomb : 0u),
-                              tomb[j] ? 0u : val[j]);
-        low += k[j] < T::kTomb;
-        tombs += tomb[j];
-      }
-    }
-    running = sh.misc[3];
-    __syncthreads();
-  }
-  low = wave_sum_u32(low);
-  tombs = wave_sum_u32(tombs);
-  if (lane == 0) {
-    if (low) atomicAdd(&sh.misc[1], low);
-    if (tombs) atomicAdd(&sh.misc[2], tombs);
-  }
-  __syncthreads();
-
-  // the segment's key range [lo_key, hi_key) and its directory bounds:
-  // bounds below its first key and above its last key (all of them when
-  // it is empty) take the rows before / after the segment
-  const u64 unit = u64(1) << s0;
-  // every key of the batch shares the bits at and above s0 + 8
-  const u64 common =
-      s0 + 8 >= 32 ? 0 : (u64(keys_in[0]) >> (s0 + 8)) << (s0 + 8);
-  const u64 lo_key = common + u64(seg) * unit;
-  const u64 hi_key = lo_key + unit;  // exclusive, may be 2^32
-  const u64 ub = u64(1) << shift;    // bucket width
-  const u64 first_b = (lo_key + ub - 1) >> shift;  // first bound >= lo_key
-  const u64 end_b = (hi_key + ub - 1) >> shift;    // first bound >= hi_key
-  const u32 seg_end_row = static_cast<u32>(base) + seg_total;
-  const u32 first_key = m ? restore(sk[0]) : 0u;
-  const u32 last_key = m ? restore(sk[m - 1]) : 0u;
-  if (m == 0) {
-    for (u64 b = first_b + tid; b < end_b; b += kSegThreads)
-      dir[b] = static_cast<u32>(base);
-  } else {
-    const u64 k0b = u64(first_key) >> shift;  // bounds in [lo_key, first]
-    for (u64 b = first_b + tid; b <= k0b; b += kSegThreads)
-      dir[b] = static_cast<u32>(base);
-    const u64 klb = (u64(last_key) >> shift) + 1;  // bounds after the last
-    for (u64 b = klb + tid; b < end_b; b += kSegThreads) dir[b] = seg_end_row;
-  }
-
-  if (tid == 0) {
-    if (seg == 255) meta->count = seg_end_row;
-    const u64 tb = T::kTomb;
-    if ((lo_key <= tb && tb < hi_key) || (seg == 0 && tb < lo_key) ||
-        (seg == 255 && hi_key <= tb))
-      meta->split = static_cast<u32>(base) + sh.misc[1];
-    if (sh.misc[2]) meta->tombs = 1;
-    if (seg == 0) meta->dir_bits = dir_bits;
-    if (m && start == 0) {  // the first keys of the run
-      meta->lo_bucket = first_key >> shift;
-      dir[first_key >> shift] = 0;
-    }
-    if (m && end == n) {  // the last keys of the run
-      const u32 hb = last_key >> shift;
-      meta->hi_bucket = hb;
-      dir[hb + 1] = seg_end_row;
-    }
-  }
+  if (in_lds) finish(sh.k[cur], sh.p[cur]);
+  else finish(gk, gp);
 }
 
 }  // namespace lsm
