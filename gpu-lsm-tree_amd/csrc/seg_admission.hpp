@@ -635,11 +635,15 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     u32 low = 0, tombs = 0;
     u32 running = 0;  // survivors of earlier chunks
     for (u32 c0 = 0; c0 < m; c0 += kSegTile) {
-      u64 mask[kSegItems];
+      // per-lane keep bits (bit j: item j), turned into wave masks where
+      // needed: sixteen live 64-bit ballots would crowd the scalar registers
+      u32 keep = 0;
 #pragma unroll
       for (int j = 0; j < kSegItems; ++j) {
-        mask[j] = ballot(keep_at(c0 + j * kSegThreads + tid));
-        if (lane == 0) sh.off[j][wave] = popc64(mask[j]);
+        const bool kj = keep_at(c0 + j * kSegThreads + tid);
+        keep |= kj ? (1u << j) : 0u;
+        const u64 mj = ballot(kj);
+        if (lane == 0) sh.off[j][wave] = popc64(mj);
       }
       __syncthreads();
       if (c0 == 0) lookback();
@@ -664,7 +668,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
         const u32 raw = i < m ? sk[i] : 0u;
         k[j] = restore(raw);
         prev[j] = (i < m && i > 0) ? restore(sk[i - 1]) : k[j];
-        pay[j] = (i < m && ((mask[j] >> lane) & 1ull)) ? sp[i] : 0u;
+        pay[j] = (i < m && ((keep >> j) & 1u)) ? sp[i] : 0u;
         tomb[j] = mode == kModeErase || (stashed && (raw & kStashBit));
       }
       // a position payload: gather value and flag (all items issued before
@@ -673,7 +677,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
 #pragma unroll
       for (int j = 0; j < kSegItems; ++j) {
         const u32 i = c0 + j * kSegThreads + tid;
-        const bool kept = i < m && ((mask[j] >> lane) & 1ull);
+        const bool kept = i < m && ((keep >> j) & 1u);
         if (gathered) {
           tomb[j] = kept && erase_flags[pay[j]] != 0;
           val[j] = kept ? values[pay[j]] : 0u;
@@ -687,12 +691,12 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
         if (i >= m) continue;
         // survivors before position i: the output row of key k
         const u32 before = static_cast<u32>(base) + sh.off[j][wave] +
-                           popc64(mask[j] & lanemask_lt());
+                           popc64(ballot((keep >> j) & 1u) & lanemask_lt());
         if (prev[j] != k[j]) {
           for (u32 b = (prev[j] >> shift) + 1; b <= (k[j] >> shift); ++b)
             dir[b] = before;
         }
-        if ((mask[j] >> lane) & 1ull) {
+        if ((keep >> j) & 1u) {
           out[before] = T::make((k[j] & T::kLow) | (tomb[j] ? T::kTomb : 0u),
                                 tomb[j] ? 0u : val[j]);
           low += k[j] < T::kTomb;
