@@ -29,8 +29,8 @@
 namespace gdict {
 namespace lsm {
 
-constexpr int kSegThreads = 512;
-constexpr int kSegItems = 16;
+constexpr int kSegThreads = 1024;
+constexpr int kSegItems = 8;
 constexpr int kSegTile = kSegThreads * kSegItems;  // 8192
 constexpr int kSegWaves = kSegThreads / kWave;
 static_assert(kSegWaves >= 2, "the look-back and the offset scan use two waves");
@@ -209,7 +209,7 @@ struct SegPayload {
 struct SegShared {
   u32 k[2][kSegTile];
   u32 p[2][kSegTile];
-  u32 wave_hist[2][kSegWaves][256];  // alternate between ranking rounds
+  u32 wave_hist[kSegWaves][256];
   u32 digit_start[256];  // chunk-local start of each digit
   u32 run[256];          // running global offsets (global-memory sort)
   u32 scan[kSegWaves + 1];
@@ -221,17 +221,19 @@ struct SegShared {
 // Stable ranking of one chunk of `count` elements, ipt per lane: wave w
 // holds chunk elements [w*64*ipt, (w+1)*64*ipt), item j of lane l is
 // w*64*ipt + j*64 + l, so all waves share the work of a short chunk.
-// hist (zero on entry) takes the per-wave digit counts; other (the
-// histogram of the previous round) is cleared for the next round.  On
-// return dst_local[j] is each item's position in the chunk sorted by its
-// digit and sh.digit_start the chunk-local start of each digit.
+// hist takes the per-wave digit counts (cleared on entry: every reader of
+// the previous round has passed a barrier since).  On return dst_local[j]
+// is each item's position in the chunk sorted by its digit and
+// sh.digit_start the chunk-local start of each digit.
 __device__ __forceinline__ void seg_rank(SegShared &sh, u32 (*hist)[256],
-                                         u32 (*other)[256],
                                          const u32 (&key)[kSegItems],
                                          u32 count, u32 ipt, u32 shift,
                                          u32 (&dst_local)[kSegItems]) {
   const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const u32 wave_base = wave * kWave * ipt;
+  for (u32 i = tid; i < kSegWaves * 256; i += kSegThreads)
+    (&hist[0][0])[i] = 0;
+  __syncthreads();
   u32 *my_hist = hist[wave];
   u32 rank[kSegItems];
 #pragma unroll
@@ -286,9 +288,6 @@ __device__ __forceinline__ void seg_rank(SegShared &sh, u32 (*hist)[256],
       dst_local[j] = hist[wave][d] + rank[j];
     }
   }
-  // the previous round's histogram is free: clear it for the next round
-  for (u32 i = tid; i < kSegWaves * 256; i += kSegThreads)
-    (&other[0][0])[i] = 0;
 }
 
 // Distinct keys of a segment beyond the LDS tile (hot keys: many updates
@@ -425,8 +424,6 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   __shared__ u32 s_ticket;
   const u32 tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   if (tid == 0) s_ticket = atomicAdd(ticket, 1u);
-  for (u32 i = tid; i < 2 * kSegWaves * 256; i += kSegThreads)
-    (&sh.wave_hist[0][0][0])[i] = 0;
   __syncthreads();
   // grid = 256: one ticket per segment.  Values derived from it are
   // wave-uniform; saying so keeps them (and the per-item predicates on
@@ -446,7 +443,6 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   const u32 key_mask = stashed ? ~kStashBit : ~0u;
   const u32 high_bit = keys_in[0] & kStashBit;
   auto restore = [&](u32 x) { return stashed ? (x & ~kStashBit) | high_bit : x; };
-  int round = 0;  // ranking rounds: selects the histogram buffer
 
   // ---------------------------------------------------------------- sort
   // sorted keys / payloads of the segment: in LDS (cur) or in global
@@ -509,9 +505,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
           pay[j] = valid ? sh.p[cur][idx] : 0u;
         }
       }
-      seg_rank(sh, sh.wave_hist[round & 1], sh.wave_hist[(round + 1) & 1],
-               key, m, ipt, 8 * pass, dst);
-      ++round;
+      seg_rank(sh, sh.wave_hist, key, m, ipt, 8 * pass, dst);
 #pragma unroll
       for (int j = 0; j < kSegItems; ++j) {
         const u32 idx = wave_base + j * kWave + lane;
@@ -553,9 +547,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
           key[j] = valid ? src_k[c0 + idx] : 0xffffffffu;
           pay[j] = valid ? src_p[c0 + idx] : 0u;
         }
-        seg_rank(sh, sh.wave_hist[round & 1], sh.wave_hist[(round + 1) & 1],
-                 key, cnt, ipt, shift, dst);
-        ++round;
+        seg_rank(sh, sh.wave_hist, key, cnt, ipt, shift, dst);
 #pragma unroll
         for (int j = 0; j < kSegItems; ++j) {
           const u32 idx = wave_base + j * kWave + lane;
