@@ -31,7 +31,7 @@ DEFAULT_OUT = os.path.join(_HERE, "tuned.json")
 LSM_SPACE = {
     "dir_rows_log": [2, 3, 4],
     "merge_tile_log": [11, 12],
-    "sort_block": [256, 512],
+    "sort_block": [256, 512, 1024],
     "grouped_lookup_min_log": [16, 18, 20, 63],
     "small_batch_max": [1024, 4096],
     "max_runs": [6, 10, 16],
@@ -39,7 +39,7 @@ LSM_SPACE = {
     "merge_block": [256, 512],
     "merge_ways": [2, 4, 8],
 }
-LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 512,
+LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 1024,
                "grouped_lookup_min_log": 18, "small_batch_max": 4096,
                "max_runs": 10, "segmented_admission": 1, "merge_block": 512,
                "merge_ways": 8}

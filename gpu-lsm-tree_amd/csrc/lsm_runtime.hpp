@@ -559,8 +559,9 @@ class LsmT : public HandleBase {
       v.merge_tile_log = t.merge_tile_log;
     }
     if (t.sort_block) {
-      if (t.sort_block != 256 && t.sort_block != 512)
-        throw Error(GD_INVALID_ARGUMENT, "sort_block must be 256 or 512");
+      if (t.sort_block != 256 && t.sort_block != 512 && t.sort_block != 1024)
+        throw Error(GD_INVALID_ARGUMENT,
+                    "sort_block must be 256, 512 or 1024");
       v.sort_block = t.sort_block;
     }
     if (t.grouped_lookup_min_log)

@@ -260,10 +260,11 @@ static __global__ void zero_u32_kernel(u32 *p, u64 n) {
 }
 
 
-// Tile = 4096 keys: 512 threads x 8 keys below 2^22 keys (twice the waves
-// per CU for the 256-tile sorts of 2^20-key batches), 256 x 16 above.
-// (2048-key tiles were measured slower on MI355X: more look-back steps.)
-inline int block_for(u64 n) { return n < (u64(1) << 22) ? 512 : 256; }
+// Tile = 4096 keys: 1024 threads x 4 keys below 2^22 keys (the 256-tile
+// passes of 2^20-key batches run one tile per CU, so more waves per tile
+// hide more latency), 256 x 16 above.  (2048-key tiles were measured
+// slower on MI355X: more look-back steps.)
+inline int block_for(u64 n) { return n < (u64(1) << 22) ? 1024 : 256; }
 
 inline u32 tiles_for(u64 n) {
   const u64 tile = 4096;
@@ -313,7 +314,11 @@ inline void launch_pass(int block, u32 tiles, const K *src_k,
                         u32 *ticket, hipStream_t stream,
                         u32 *dest = nullptr,
                         const u32 *shift_dev = nullptr) {
-  if (block == 512)
+  if (block == 1024)
+    onesweep_kernel<K, Payload, 4, 1024><<<tiles, 1024, 0, stream>>>(
+        src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
+        dest, shift_dev);
+  else if (block == 512)
     onesweep_kernel<K, Payload, 8, 512><<<tiles, 512, 0, stream>>>(
         src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
         dest, shift_dev);

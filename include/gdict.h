@@ -150,7 +150,8 @@ gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes);
 typedef struct gd_lsm_tuning {
   uint32_t dir_rows_log;           /* ~2^k rows per directory bucket (3)   */
   uint32_t merge_tile_log;         /* merge tile 2^k rows: 11 or 12 (12)   */
-  uint32_t sort_block;             /* onesweep block: 256 or 512 (by size) */
+  uint32_t sort_block;             /* onesweep block: 256, 512 or 1024
+                                      threads (by size)                    */
   uint32_t grouped_lookup_min_log; /* group lookups from 2^k queries (18);
                                       63 = never                           */
   uint32_t small_batch_max;        /* fused one-workgroup admission up to

@@ -396,6 +396,7 @@ def test_config2_scaled(gd, orc, io):
     {"merge_ways": 2, "dir_rows_log": 3},
     {"merge_ways": 3, "dir_rows_log": 1, "max_runs": 4},
     {"merge_ways": 8, "dir_rows_log": 6, "small_batch_max": 1},
+    {"sort_block": 1024, "segmented_admission": 2, "max_runs": 5},
 ])
 def test_tuning_knobs_keep_results(gd, orc, io, knobs):
     """Launch-shape knobs change speed only: a mixed stream with small and
