@@ -106,12 +106,14 @@ class _Tuning(ctypes.Structure):
         ("segmented_admission", ctypes.c_uint32),
         ("merge_block", ctypes.c_uint32),
         ("merge_ways", ctypes.c_uint32),
+        ("merge_fanout", ctypes.c_uint32),
     ]
 
 
 TUNING_FIELDS = ("dir_rows_log", "merge_tile_log", "sort_block",
                  "grouped_lookup_min_log", "small_batch_max", "max_runs",
-                 "segmented_admission", "merge_block", "merge_ways")
+                 "segmented_admission", "merge_block", "merge_ways",
+                 "merge_fanout")
 
 
 def _tuning(d):

@@ -586,6 +586,11 @@ class LsmT : public HandleBase {
         throw Error(GD_INVALID_ARGUMENT, "merge_ways must be 2..8");
       v.merge_ways = t.merge_ways;
     }
+    if (t.merge_fanout) {
+      if (t.merge_fanout < 2 || t.merge_fanout > static_cast<u32>(kMaxWay))
+        throw Error(GD_INVALID_ARGUMENT, "merge_fanout must be 2..8");
+      v.merge_fanout = t.merge_fanout;
+    }
     tuning_ = v;
   }
   const gd_lsm_tuning &tuning() const { return tuning_; }
@@ -1011,11 +1016,15 @@ class LsmT : public HandleBase {
                                                        kMaxRuns - 1));
     const std::size_t max_ways =
         tuning_.merge_ways ? tuning_.merge_ways : std::size_t(kMaxWay);
+    const std::size_t fan = std::min<std::size_t>(
+        max_ways, tuning_.merge_fanout ? tuning_.merge_fanout : 2);
     while (runs_.size() >= 2) {
       std::size_t i, ways = 2;
-      if (runs_[1].rows_ub <= runs_[0].rows_ub) {
+      if (runs_.size() >= fan && runs_[fan - 1].rows_ub <= runs_[0].rows_ub) {
         i = 0;
-        u64 acc = runs_[0].rows_ub + runs_[1].rows_ub;
+        ways = fan;
+        u64 acc = 0;
+        for (std::size_t j = 0; j < fan; ++j) acc += runs_[j].rows_ub;
         while (ways < runs_.size() && ways < max_ways &&
                runs_[ways].rows_ub <= acc)
           acc += runs_[ways++].rows_ub;
@@ -1106,7 +1115,7 @@ class LsmT : public HandleBase {
   // segmented admission (1) for 32-bit batches; 64-bit batches always take
   // the radix passes
   gd_lsm_tuning tuning_{3, 12, 0, 18, static_cast<u32>(T::kSmallTile), 10,
-                        1, 512, kMaxWay};
+                        1, 512, kMaxWay, 2};
 
   struct ProfRec {
     int kind;  // 0 merge, 1 lookup

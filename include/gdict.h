@@ -165,6 +165,8 @@ typedef struct gd_lsm_tuning {
                                       per tile (512)                       */
   uint32_t merge_ways;             /* most runs one merge pass takes: 2 =
                                       pairwise only, up to 8 (8)           */
+  uint32_t merge_fanout;           /* newest runs of one size that merge: 2
+                                      = binary counter, up to merge_ways (2) */
 } gd_lsm_tuning;
 
 gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning);

@@ -5,8 +5,9 @@ patterns cover uniform keys over both halves of the key space, dense keys
 (directory buckets far above a merge tile, which the partition refines by
 key), disjoint sequential runs, hot keys updated in every run, tombstones
 that drop at the oldest run and erase-everything streams that leave empty
-runs.  Each case is also checked with merge_ways = 2 (pairwise merges) for
-the same answers."""
+runs.  Each case is also checked with merge_ways = 2 (pairwise merges) and
+with merge_fanout = 4 (four newest runs of one size merge) for the same
+answers."""
 
 import numpy as np
 import pytest
@@ -57,13 +58,13 @@ def _run(p, rng, pattern, batches, n, bits, dt):
     return live
 
 
-@pytest.mark.parametrize("ways", [8, 2])
+@pytest.mark.parametrize("ways,fanout", [(8, 2), (2, 2), (8, 4)])
 @pytest.mark.parametrize("pattern", PATTERNS)
-def test_cascades_32(gd, orc, io, pattern, ways):
+def test_cascades_32(gd, orc, io, pattern, ways, fanout):
     rng = np.random.default_rng(PATTERNS.index(pattern) + 11)
     n = 6000
-    p = Pair(gd, orc, io, tuning={"merge_ways": ways}, max_elements=1 << 20,
-             batch_capacity=n)
+    p = Pair(gd, orc, io, tuning={"merge_ways": ways, "merge_fanout": fanout},
+             max_elements=1 << 20, batch_capacity=n)
     live = _run(p, rng, pattern, 32, n, 32, np.uint32)
     q = _queries(rng, live, 1 << 13, 1 << 32)
     p.check_lookup(q, True)

@@ -397,6 +397,8 @@ def test_config2_scaled(gd, orc, io):
     {"merge_ways": 3, "dir_rows_log": 1, "max_runs": 4},
     {"merge_ways": 8, "dir_rows_log": 6, "small_batch_max": 1},
     {"sort_block": 1024, "segmented_admission": 2, "max_runs": 5},
+    {"merge_fanout": 4, "merge_ways": 8},
+    {"merge_fanout": 3, "merge_ways": 3, "max_runs": 6},
 ])
 def test_tuning_knobs_keep_results(gd, orc, io, knobs):
     """Launch-shape knobs change speed only: a mixed stream with small and
@@ -428,7 +430,8 @@ def test_tuning_knobs_keep_results(gd, orc, io, knobs):
 def test_invalid_tuning_rejected(gd):
     for bad in ({"sort_block": 384}, {"merge_tile_log": 13},
                 {"segmented_admission": 3}, {"merge_block": 384},
-                {"merge_ways": 1}, {"merge_ways": 9}, {"no_such_knob": 1}):
+                {"merge_ways": 1}, {"merge_ways": 9}, {"merge_fanout": 1},
+                {"merge_fanout": 9}, {"no_such_knob": 1}):
         with pytest.raises(gd.InvalidArgument):
             gd.GPULSMOpt(max_elements=1 << 16, batch_capacity=1 << 10,
                          tuning=bad)
