@@ -411,6 +411,113 @@ __device__ __forceinline__ bool seg_dedupe(SegShared &sh,
 // partitioned keys and SegPayload payloads; tk/tp: scratch of n entries
 // for segments too large for LDS.  offsets: segment starts (exclusive scan
 // of the segment histogram).  Writes the run (rows, header, directory).
+// Sorts a segment of m <= kSegTile keys by their bits below s0 into
+// sh.k[0] / sh.p[0], stably (equal keys keep their input order), without
+// the LSD passes: one counting pass over kBuckets buckets (the top
+// min(s0, 12) of those bits; about one key per bucket for a 4096-key
+// segment), every key claims a slot of its bucket, then each bucket is put
+// in (key, input index) order by insertion.  Returns false, having changed
+// nothing the LSD passes read, when a bucket holds more than kBucketMax
+// keys (dense or repeated keys).
+constexpr u32 kBuckets = kSegWaves * 256;  // counters over sh.wave_hist
+constexpr u32 kBucketMax = 16;
+
+__device__ __forceinline__ bool seg_bucket_sort(SegShared &sh,
+                                                const u32 *__restrict__ keys,
+                                                const u32 *__restrict__ pays,
+                                                u32 m, u32 s0) {
+  static_assert(kBuckets == 4096 && kBuckets % kSegThreads == 0,
+                "4096 buckets, an equal share per thread");
+  constexpr u32 kPer = kBuckets / kSegThreads;
+  const u32 tid = threadIdx.x, lane = lane_id();
+  u32 *cnt = &sh.wave_hist[0][0];
+  u32 *idx = sh.k[1];  // input index of each placed key
+  const u32 hb = s0 < 12 ? s0 : 12;
+  const u32 lsh = s0 - hb;
+  const u32 low = (1u << s0) - 1u;  // the bits that order a segment
+  for (u32 i = tid; i < kBuckets; i += kSegThreads) cnt[i] = 0;
+  if (tid == 0) sh.misc[6] = 0;  // largest bucket
+  __syncthreads();
+  u32 k[kSegItems], p[kSegItems], b[kSegItems];
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 i = tid + j * kSegThreads;
+    if (i < m) {
+      k[j] = keys[i];
+      p[j] = pays[i];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 i = tid + j * kSegThreads;
+    b[j] = 0;
+    if (i < m) {
+      b[j] = (k[j] & low) >> lsh;
+      atomicAdd(&cnt[b[j]], 1u);
+    }
+  }
+  __syncthreads();
+  // bucket starts: thread t owns buckets [t * kPer, (t + 1) * kPer)
+  u32 c[kPer], sum = 0, most = 0;
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    c[q] = cnt[tid * kPer + q];
+    sum += c[q];
+    most = c[q] > most ? c[q] : most;
+  }
+  const u32 ex = block_exclusive_scan<kSegThreads>(sum, sh.scan);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const u32 o = __shfl_xor(most, off, kWave);
+    most = o > most ? o : most;
+  }
+  if (lane == 0 && most) atomicMax(&sh.misc[6], most);
+  u32 run = ex;
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    cnt[tid * kPer + q] = run;
+    run += c[q];
+  }
+  __syncthreads();
+  if (sh.misc[6] > kBucketMax) return false;  // uniform
+  // every key claims a slot of its bucket (cnt[b] ends at the bucket's end)
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 i = tid + j * kSegThreads;
+    if (i < m) {
+      const u32 slot = atomicAdd(&cnt[b[j]], 1u);
+      sh.k[0][slot] = k[j];
+      sh.p[0][slot] = p[j];
+      idx[slot] = i;
+    }
+  }
+  __syncthreads();
+  // (key, input index) order inside each bucket [end of the previous, end)
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    const u32 bb = tid * kPer + q;
+    const u32 end = cnt[bb];
+    const u32 beg = bb ? cnt[bb - 1] : 0u;
+    for (u32 x = beg + 1; x < end; ++x) {
+      const u32 kx = sh.k[0][x], px = sh.p[0][x], ix = idx[x];
+      u32 y = x;
+      while (y > beg) {
+        const u32 ky = sh.k[0][y - 1] & low, iy = idx[y - 1];
+        if (ky < (kx & low) || (ky == (kx & low) && iy < ix)) break;
+        sh.k[0][y] = sh.k[0][y - 1];
+        sh.p[0][y] = sh.p[0][y - 1];
+        idx[y] = iy;
+        --y;
+      }
+      sh.k[0][y] = kx;
+      sh.p[0][y] = px;
+      idx[y] = ix;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
 template <class T>
 __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     const u32 *__restrict__ keys_in, u32 *pk, u32 *pp, u32 *tk, u32 *tp,
@@ -462,6 +569,11 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     const u32 ipt = __builtin_amdgcn_readfirstlane(
         m ? (m + kSegThreads - 1) / kSegThreads : 1u);
     const u32 wave_base = wave * kWave * ipt;
+    // the bucket sort handles every segment without dense keys; the LSD
+    // passes below take the rest
+    const bool bucketed =
+        passes >= 2 && !staged && m && seg_bucket_sort(sh, pk + start,
+                                                        pp + start, m, s0);
     if (passes == 0 && !staged) {
       // nothing to sort (each segment holds one key): copy it to LDS, all
       // loads of a thread issued before the first store
@@ -484,7 +596,7 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
       }
       __syncthreads();
     }
-    for (u32 pass = 0; pass < passes; ++pass) {
+    for (u32 pass = 0; pass < (bucketed ? 0u : passes); ++pass) {
       // the first pass reads the partitioned segment from global memory,
       // the later ones the previous pass's LDS arrays
       u32 key[kSegItems], pay[kSegItems], dst[kSegItems];
