@@ -415,8 +415,8 @@ __device__ __forceinline__ bool seg_dedupe(SegShared &sh,
 // sh.k[0] / sh.p[0], stably (equal keys keep their input order), without
 // the LSD passes: one counting pass over kBuckets buckets (the top
 // min(s0, 12) of those bits; about one key per bucket for a 4096-key
-// segment), every key claims a slot of its bucket, then each bucket is put
-// in (key, input index) order by insertion.  Returns false, having changed
+// segment), every key claims a slot of its bucket, then takes the place
+// its (key, input index) rank inside the bucket gives it.  Returns false, having changed
 // nothing the LSD passes read, when a bucket holds more than kBucketMax
 // keys (dense or repeated keys).
 constexpr u32 kBuckets = kSegWaves * 256;  // counters over sh.wave_hist
@@ -431,7 +431,6 @@ __device__ __forceinline__ bool seg_bucket_sort(SegShared &sh,
   constexpr u32 kPer = kBuckets / kSegThreads;
   const u32 tid = threadIdx.x, lane = lane_id();
   u32 *cnt = &sh.wave_hist[0][0];
-  u32 *idx = sh.k[1];  // input index of each placed key
   const u32 hb = s0 < 12 ? s0 : 12;
   const u32 lsh = s0 - hb;
   const u32 low = (1u << s0) - 1u;  // the bits that order a segment
@@ -480,38 +479,40 @@ __device__ __forceinline__ bool seg_bucket_sort(SegShared &sh,
   }
   __syncthreads();
   if (sh.misc[6] > kBucketMax) return false;  // uniform
-  // every key claims a slot of its bucket (cnt[b] ends at the bucket's end)
+  // every key claims a slot of its bucket in sh.k[1] / sh.p[1] (key, input
+  // index); cnt[b] then ends at the bucket's end
+  u32 slot[kSegItems];
+#pragma unroll
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 i = tid + j * kSegThreads;
+    slot[j] = 0;
+    if (i < m) slot[j] = atomicAdd(&cnt[b[j]], 1u);
+  }
 #pragma unroll
   for (int j = 0; j < kSegItems; ++j) {
     const u32 i = tid + j * kSegThreads;
     if (i < m) {
-      const u32 slot = atomicAdd(&cnt[b[j]], 1u);
-      sh.k[0][slot] = k[j];
-      sh.p[0][slot] = p[j];
-      idx[slot] = i;
+      sh.k[1][slot[j]] = k[j];
+      sh.p[1][slot[j]] = i;
     }
   }
   __syncthreads();
-  // (key, input index) order inside each bucket [end of the previous, end)
+  // each key's place: its bucket's start plus the keys of the bucket before
+  // it in (key, input index) order (no dependent steps, unlike insertion)
 #pragma unroll
-  for (u32 q = 0; q < kPer; ++q) {
-    const u32 bb = tid * kPer + q;
-    const u32 end = cnt[bb];
-    const u32 beg = bb ? cnt[bb - 1] : 0u;
-    for (u32 x = beg + 1; x < end; ++x) {
-      const u32 kx = sh.k[0][x], px = sh.p[0][x], ix = idx[x];
-      u32 y = x;
-      while (y > beg) {
-        const u32 ky = sh.k[0][y - 1] & low, iy = idx[y - 1];
-        if (ky < (kx & low) || (ky == (kx & low) && iy < ix)) break;
-        sh.k[0][y] = sh.k[0][y - 1];
-        sh.p[0][y] = sh.p[0][y - 1];
-        idx[y] = iy;
-        --y;
+  for (int j = 0; j < kSegItems; ++j) {
+    const u32 i = tid + j * kSegThreads;
+    if (i < m) {
+      const u32 end = cnt[b[j]];
+      const u32 beg = b[j] ? cnt[b[j] - 1] : 0u;
+      const u32 kx = k[j] & low;
+      u32 rank = 0;
+      for (u32 y = beg; y < end; ++y) {
+        const u32 ky = sh.k[1][y] & low;
+        rank += (ky < kx || (ky == kx && sh.p[1][y] < i)) ? 1u : 0u;
       }
-      sh.k[0][y] = kx;
-      sh.p[0][y] = px;
-      idx[y] = ix;
+      sh.k[0][beg + rank] = k[j];
+      sh.p[0][beg + rank] = p[j];
     }
   }
   __syncthreads();
