@@ -841,14 +841,20 @@ class LsmT : public HandleBase {
                                   words - L.shift_at,
                                   reinterpret_cast<u32 *>(r.meta()),
                                   kMetaWords);
-      seg_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
-          keys, n, span, pc.rows, c + L.shift_at);
-      rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
+      // per-tile digit prefixes replace the partition's look-back
+      const u32 tiles = rs::tiles_for(n);
+      Lease<u32> tile_hist(pool_, u64(tiles) * rs::kRadix);
+      seg_tile_hist_kernel<K><<<tiles, rs::kThreads, 0, s>>>(
+          keys, n, span, tile_hist.get(), c + L.shift_at);
+      if (tiles > kTileScanMax)
+        throw Error(GD_INVALID_ARGUMENT, "segmented batch too large");
+      seg_tile_scan_kernel<<<rs::kRadix, rs::kThreads, 0, s>>>(
+          tile_hist.get(), tiles, pc.offsets);
       rs::launch_pass<u32>(
-          tuning_.sort_block ? tuning_.sort_block : rs::block_for(n),
-          rs::tiles_for(n), keys, SegPayload{values, flags, mode}, ka.get(),
-          va.get(), n, 0, pc.offsets, pc.status, pc.tickets, s, nullptr,
-          c + L.shift_at);
+          tuning_.sort_block ? tuning_.sort_block : rs::block_for(n), tiles,
+          keys, SegPayload{values, flags, mode}, ka.get(), va.get(), n, 0,
+          pc.offsets, pc.status, pc.tickets, s, nullptr, c + L.shift_at,
+          tile_hist.get());
     }
     seg_sort_kernel<T><<<256, kSegThreads, 0, s>>>(
         keys, ka.get(), va.get(), kb.get(), vb.get(), n, pc.offsets,

@@ -120,7 +120,8 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     const K *__restrict__ keys_in, Payload payload,
     K *__restrict__ keys_out, u32 *__restrict__ vals_out, u32 n,
     u32 shift_arg, const u32 *__restrict__ digit_offsets, u64 *status,
-    u32 *ticket, u32 *__restrict__ dest_out, const u32 *shift_dev) {
+    u32 *ticket, u32 *__restrict__ dest_out, const u32 *shift_dev,
+    const u32 *__restrict__ tile_prefix) {
   constexpr int kTile = kBlock * kItems;
   // the digit shift: an argument, or computed on the device by an earlier
   // launch
@@ -137,7 +138,10 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
   const u32 tid = threadIdx.x;
   const u32 lane = lane_id();
   const u32 wave = tid / kWave;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  // with precomputed per-tile digit prefixes (tile_prefix[d * tiles + t]:
+  // keys of digit d in tiles before t; digit_offsets then holds the digit
+  // totals) the tiles are independent: no ticket and no look-back
+  if (tid == 0) s_tile = tile_prefix ? blockIdx.x : atomicAdd(ticket, 1u);
   for (int i = tid; i < kBlockWaves * kRadix; i += kBlock)
     (&s_wave_hist[0][0])[i] = 0;
   __syncthreads();
@@ -191,7 +195,9 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     }
     // look-back: every digit owns one column of the status array
     u64 *column = status;  // status[t * kRadix + d]
-    if (tile == 0) {
+    if (tile_prefix) {
+      exclusive = tile_prefix[u64(d) * gridDim.x + tile];
+    } else if (tile == 0) {
       store_relaxed(&column[d], kFlagPrefix | total);
     } else {
       store_relaxed(&column[u64(tile) * kRadix + d], kFlagAggregate | total);
@@ -210,10 +216,18 @@ __global__ __launch_bounds__(kBlock) void onesweep_kernel(
     }
   }
   const u32 block_start = block_exclusive_scan<kBlock>(total, s_scan);
+  // the digit's start: given, or a scan of the digit totals
+  u32 digit_start = 0;
+  if (tile_prefix) {
+    __shared__ u32 s_scan2[kBlockWaves + 1];
+    digit_start = block_exclusive_scan<kBlock>(
+        tid < kRadix ? digit_offsets[tid] : 0u, s_scan2);
+  } else if (tid < kRadix) {
+    digit_start = digit_offsets[d];
+  }
   if (tid < kRadix) {
     s_block_start[d] = block_start;
-    s_global_base[d] =
-        digit_offsets[d] + static_cast<u32>(exclusive) - block_start;
+    s_global_base[d] = digit_start + static_cast<u32>(exclusive) - block_start;
   }
   __syncthreads();
 
@@ -313,19 +327,20 @@ inline void launch_pass(int block, u32 tiles, const K *src_k,
                         u32 shift, const u32 *offsets, u64 *status,
                         u32 *ticket, hipStream_t stream,
                         u32 *dest = nullptr,
-                        const u32 *shift_dev = nullptr) {
+                        const u32 *shift_dev = nullptr,
+                        const u32 *tile_prefix = nullptr) {
   if (block == 1024)
     onesweep_kernel<K, Payload, 4, 1024><<<tiles, 1024, 0, stream>>>(
         src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
-        dest, shift_dev);
+        dest, shift_dev, tile_prefix);
   else if (block == 512)
     onesweep_kernel<K, Payload, 8, 512><<<tiles, 512, 0, stream>>>(
         src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
-        dest, shift_dev);
+        dest, shift_dev, tile_prefix);
   else
     onesweep_kernel<K, Payload, 16, 256><<<tiles, 256, 0, stream>>>(
         src_k, payload, dst_k, dst_v, n, shift, offsets, status, ticket,
-        dest, shift_dev);
+        dest, shift_dev, tile_prefix);
 }
 
 // Sorts n (key, position) pairs stably.  keys_in is read only; the sorted

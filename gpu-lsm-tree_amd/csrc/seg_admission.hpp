@@ -3,12 +3,15 @@
 //
 //   span_clear_kernel per-block OR of (key ^ keys[0]): the bits that vary
 //                     (the same launch clears the sort's control state)
-//   seg_hist_kernel   s0 = (highest varying bit) - 7; histogram of the
-//                     digit (key >> s0) & 0xff, so the 256 segments split
-//                     the batch's actual key spread, not the key space
-//   scan_kernel       segment starts
+//   seg_tile_hist_kernel  s0 = (highest varying bit) - 7; per-tile
+//                     histograms of the digit (key >> s0) & 0xff, so the
+//                     256 segments split the batch's actual key spread,
+//                     not the key space
+//   seg_tile_scan_kernel  per-tile digit prefixes and segment starts
 //   onesweep pass     stable partition by that digit, carrying the values
-//                     (SegPayload), so the sort needs no random gathers
+//                     (SegPayload), so the sort needs no random gathers;
+//                     its tiles read their digit offsets instead of
+//                     chaining them through a look-back
 //   seg_sort_kernel   one workgroup per segment, in key order (ticket =
 //                     segment): an LDS LSD sort of the bits below s0, the
 //                     newest-wins dedupe, tombstone decoding, row
@@ -133,6 +136,65 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_hist_kernel(
     const u32 c = s_hist[i];
     if (c) atomicAdd(&row[i], c);
   }
+}
+
+// Per-tile digit counts of the admission partition, digit-major: block t
+// counts the digit (key >> s0) & 0xff of keys [t * 4096, (t + 1) * 4096),
+// one onesweep tile, into tile_hist[d * tiles + t]; block 0 also stores the
+// digit shift.
+template <class K>
+static __global__ __launch_bounds__(rs::kThreads) void seg_tile_hist_kernel(
+    const K *__restrict__ keys, u32 n, const K *__restrict__ span,
+    u32 *__restrict__ tile_hist, u32 *shift_out) {
+  static_assert(rs::kThreads == rs::kRadix &&
+                    rs::kThreads * rs::kHistItems == 4096,
+                "a thread per digit, one onesweep tile per block");
+  __shared__ u32 s_hist[rs::kRadix];
+  const u32 s0 = seg_shift_from(span);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *shift_out = s0;
+  s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const u64 base = u64(blockIdx.x) * 4096;
+  K k[rs::kHistItems];
+#pragma unroll
+  for (int j = 0; j < rs::kHistItems; ++j) {
+    const u64 i = base + u64(j) * rs::kThreads + threadIdx.x;
+    k[j] = i < n ? keys[i] : K(0);
+  }
+#pragma unroll
+  for (int j = 0; j < rs::kHistItems; ++j)
+    if (base + u64(j) * rs::kThreads + threadIdx.x < n)
+      atomicAdd(&s_hist[static_cast<u32>(k[j] >> s0) & 0xffu], 1u);
+  __syncthreads();
+  tile_hist[u64(threadIdx.x) * gridDim.x + blockIdx.x] = s_hist[threadIdx.x];
+}
+
+// One block per digit d: the exclusive prefix of its counts over the
+// tiles, in place (tiles <= 1024), and its total into totals[d].  The
+// onesweep pass then places tile t's keys of digit d at (the digit's start,
+// a scan of the totals) + tile_hist[d * tiles + t], without a look-back.
+constexpr u32 kTileScanMax = 4 * rs::kThreads;
+static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
+    u32 *__restrict__ tile_hist, u32 tiles, u32 *__restrict__ totals) {
+  __shared__ u32 s_scan[rs::kThreads / kWave + 1];
+  u32 *col = tile_hist + u64(blockIdx.x) * tiles;
+  const u32 per = (tiles + rs::kThreads - 1) / rs::kThreads;  // <= 4
+  const u32 first = threadIdx.x * per;
+  u32 c[4], sum = 0;
+#pragma unroll
+  for (u32 q = 0; q < 4; ++q) {
+    c[q] = q < per && first + q < tiles ? col[first + q] : 0u;
+    sum += c[q];
+  }
+  u32 run = block_exclusive_scan<rs::kThreads>(sum, s_scan);
+#pragma unroll
+  for (u32 q = 0; q < 4; ++q) {
+    if (q < per && first + q < tiles) {
+      col[first + q] = run;
+      run += c[q];
+    }
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = s_scan[rs::kThreads / kWave];
 }
 
 // Control words of a spread partition of n keys: the partition's own, the
@@ -522,7 +584,7 @@ __device__ __forceinline__ bool seg_bucket_sort(SegShared &sh,
 template <class T>
 __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     const u32 *__restrict__ keys_in, u32 *pk, u32 *pp, u32 *tk, u32 *tp,
-    u32 n, const u32 *__restrict__ offsets,
+    u32 n, const u32 *__restrict__ totals,
     const u32 *shift_in, const u32 *__restrict__ values,
     const u8 *__restrict__ erase_flags, int mode,
     typename T::Row *__restrict__ out, RunMeta *meta, u32 *__restrict__ dir,
@@ -538,8 +600,16 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
   // them) in scalar registers instead of per-lane masks
   const u32 seg = __builtin_amdgcn_readfirstlane(s_ticket);
   const u32 s0 = *shift_in;
-  const u32 start = offsets[seg];
-  const u32 end = seg == 255 ? n : offsets[seg + 1];
+  // the segment's start: the keys of the segments before it (a scan of
+  // the per-segment totals)
+  {
+    const u32 mine = tid < 256 ? totals[tid] : 0u;
+    const u32 ex = block_exclusive_scan<kSegThreads>(mine, sh.scan);
+    if (tid == seg) sh.misc[7] = ex;
+    __syncthreads();
+  }
+  const u32 start = __builtin_amdgcn_readfirstlane(sh.misc[7]);
+  const u32 end = start + totals[seg];
   // elements sorted below: the segment, or its distinct keys when a
   // segment beyond the LDS tile is deduplicated first
   u32 m = end - start;
