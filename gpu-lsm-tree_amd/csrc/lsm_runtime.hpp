@@ -846,8 +846,6 @@ class LsmT : public HandleBase {
       Lease<u32> tile_hist(pool_, u64(tiles) * rs::kRadix);
       seg_tile_hist_kernel<K><<<tiles, rs::kThreads, 0, s>>>(
           keys, n, span, tile_hist.get(), c + L.shift_at);
-      if (tiles > kTileScanMax)
-        throw Error(GD_INVALID_ARGUMENT, "segmented batch too large");
       seg_tile_scan_kernel<<<rs::kRadix, rs::kThreads, 0, s>>>(
           tile_hist.get(), tiles, pc.offsets);
       rs::launch_pass<u32>(

@@ -105,39 +105,6 @@ __device__ __forceinline__ u32 seg_shift_from(const K *span) {
   return h >= 7 ? h - 7 : 0u;
 }
 
-// Histogram of the segment digit into kHistRows partial rows (zeroed);
-// block 0 also stores the digit shift for the later launches.
-template <class K>
-static __global__ __launch_bounds__(rs::kThreads) void seg_hist_kernel(
-    const K *__restrict__ keys, u32 n, const K *__restrict__ span,
-    u32 *__restrict__ rows, u32 *shift_out) {
-  __shared__ u32 s_hist[rs::kRadix];
-  const u32 s0 = seg_shift_from(span);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *shift_out = s0;
-  for (int i = threadIdx.x; i < rs::kRadix; i += rs::kThreads) s_hist[i] = 0;
-  __syncthreads();
-  const u64 round = u64(rs::kThreads) * rs::kHistItems;
-  for (u64 base = u64(blockIdx.x) * round; base < n;
-       base += u64(gridDim.x) * round) {
-    K k[rs::kHistItems];
-#pragma unroll
-    for (int j = 0; j < rs::kHistItems; ++j) {
-      const u64 i = base + u64(j) * rs::kThreads + threadIdx.x;
-      k[j] = i < n ? keys[i] : K(0);
-    }
-#pragma unroll
-    for (int j = 0; j < rs::kHistItems; ++j)
-      if (base + u64(j) * rs::kThreads + threadIdx.x < n)
-        atomicAdd(&s_hist[static_cast<u32>(k[j] >> s0) & 0xffu], 1u);
-  }
-  __syncthreads();
-  u32 *row = rows + u64(blockIdx.x % rs::kHistRows) * rs::kRadix;
-  for (int i = threadIdx.x; i < rs::kRadix; i += rs::kThreads) {
-    const u32 c = s_hist[i];
-    if (c) atomicAdd(&row[i], c);
-  }
-}
-
 // Per-tile digit counts of the admission partition, digit-major: block t
 // counts the digit (key >> s0) & 0xff of keys [t * 4096, (t + 1) * 4096),
 // one onesweep tile, into tile_hist[d * tiles + t]; block 0 also stores the
@@ -170,31 +137,37 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_hist_kernel(
 }
 
 // One block per digit d: the exclusive prefix of its counts over the
-// tiles, in place (tiles <= 1024), and its total into totals[d].  The
-// onesweep pass then places tile t's keys of digit d at (the digit's start,
-// a scan of the totals) + tile_hist[d * tiles + t], without a look-back.
-constexpr u32 kTileScanMax = 4 * rs::kThreads;
+// tiles, in place, and its total into totals[d].  The onesweep pass then
+// places tile t's keys of digit d at (the digit's start, a scan of the
+// totals) + tile_hist[d * tiles + t], without a look-back.  Chunks of 4
+// consecutive tiles per thread, 1024 tiles per round.
 static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
     u32 *__restrict__ tile_hist, u32 tiles, u32 *__restrict__ totals) {
+  constexpr u32 kPer = 4;
+  constexpr u32 kRound = kPer * rs::kThreads;
   __shared__ u32 s_scan[rs::kThreads / kWave + 1];
   u32 *col = tile_hist + u64(blockIdx.x) * tiles;
-  const u32 per = (tiles + rs::kThreads - 1) / rs::kThreads;  // <= 4
-  const u32 first = threadIdx.x * per;
-  u32 c[4], sum = 0;
+  u32 carry = 0;
+  for (u32 r0 = 0; r0 < tiles; r0 += kRound) {
+    const u32 first = r0 + threadIdx.x * kPer;
+    u32 c[kPer], sum = 0;
 #pragma unroll
-  for (u32 q = 0; q < 4; ++q) {
-    c[q] = q < per && first + q < tiles ? col[first + q] : 0u;
-    sum += c[q];
-  }
-  u32 run = block_exclusive_scan<rs::kThreads>(sum, s_scan);
-#pragma unroll
-  for (u32 q = 0; q < 4; ++q) {
-    if (q < per && first + q < tiles) {
-      col[first + q] = run;
-      run += c[q];
+    for (u32 q = 0; q < kPer; ++q) {
+      c[q] = first + q < tiles ? col[first + q] : 0u;
+      sum += c[q];
     }
+    u32 run = carry + block_exclusive_scan<rs::kThreads>(sum, s_scan);
+#pragma unroll
+    for (u32 q = 0; q < kPer; ++q) {
+      if (first + q < tiles) {
+        col[first + q] = run;
+        run += c[q];
+      }
+    }
+    carry += s_scan[rs::kThreads / kWave];
+    __syncthreads();  // s_scan is reused by the next round
   }
-  if (threadIdx.x == 0) totals[blockIdx.x] = s_scan[rs::kThreads / kWave];
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // Control words of a spread partition of n keys: the partition's own, the
@@ -224,17 +197,25 @@ inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
   const SpreadLayout<K> L(n);
   const rs::Control pc = rs::control_layout(control, 1);
   K *span = reinterpret_cast<K *>(control + L.span_at);
-  // clear everything but the spans (written whole by the same launch)
-  span_clear_kernel<K><<<span_clear_blocks(L.words + extra_words), 256, 0,
-                         s>>>(keys, n, span, control, L.span_at,
-                              control + L.shift_at, L.words - L.shift_at,
-                              extra, extra_words);
-  seg_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
-      keys, n, span, pc.rows, control + L.shift_at);
-  rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(pc.rows, 1, pc.offsets);
-  rs::launch_pass<K>(block ? block : rs::block_for(n), rs::tiles_for(n), keys,
-                     payload, keys_out, pos_out, n, 0, pc.offsets, pc.status,
-                     pc.tickets, s, dest, control + L.shift_at);
+  // Per-tile digit histograms (tiles x 256 words, in the room of the
+  // look-back state, which this pass does not use) give every tile its
+  // offsets; nothing of the partition's control needs clearing, only the
+  // shift and the caller's extra region (the spans are written whole).
+  const u32 tiles = rs::tiles_for(n);
+  u32 *tile_hist = reinterpret_cast<u32 *>(pc.status);
+  span_clear_kernel<K><<<span_clear_blocks(L.words - L.shift_at +
+                                           extra_words),
+                         256, 0, s>>>(keys, n, span, control, 0,
+                                      control + L.shift_at,
+                                      L.words - L.shift_at, extra,
+                                      extra_words);
+  seg_tile_hist_kernel<K><<<tiles, rs::kThreads, 0, s>>>(
+      keys, n, span, tile_hist, control + L.shift_at);
+  seg_tile_scan_kernel<<<rs::kRadix, rs::kThreads, 0, s>>>(tile_hist, tiles,
+                                                           pc.offsets);
+  rs::launch_pass<K>(block ? block : rs::block_for(n), tiles, keys, payload,
+                     keys_out, pos_out, n, 0, pc.offsets, pc.status,
+                     pc.tickets, s, dest, control + L.shift_at, tile_hist);
 }
 
 // Payload of the partition pass.  When s0 <= 23 the top varying key bit
