@@ -161,6 +161,9 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
   const u32 lane = lane_id();
   const u64 t = (u64(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   if (t > tiles) return;
+  // the boundary's coarse bucket, loaded beside the run headers (unused,
+  // and possibly never written, for t = 0 and t * target >= rows)
+  const u32 c0 = bucket_of[t];
   if (lane == 0 && t < tiles) status[t] = 0;
   if (t == 0 && lane == 0) {
     *ticket = 0;
@@ -194,17 +197,56 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
         out[i] = t == 0 ? 0u : m[i].count;
     return;
   }
-  // 64-ary search for the last c in [lo, hi) with below(c) <= R, given
-  // below(lo) <= R < below(hi)
-  auto search = [&](u32 lo, u32 hi, auto below) -> u32 {
+  // this lane's run (lanes >= ways answer nothing)
+  RunView<T> rv = in.run[0];
+  RunMeta rm = m[0];
+#pragma unroll
+  for (int i = 1; i < kMaxWay; ++i)
+    if (lane == static_cast<u32>(i)) {
+      rv = in.run[i];
+      rm = m[i];
+    }
+  // phase 1: the boundary's coarse bucket c0 (exact), then the last of its
+  // sub-buckets at the finest resolution whose start has at most R rows
+  // below it by the upper bound; a start becomes x_t when its lower bound
+  // leaves fewer than light_max rows short of R, which bounds the tile,
+  // and every run's split is then its lower bound of x_t.  The coarse
+  // start is preferred when it qualifies.
+  u64 below_lo = 0;  // rows below the coarse start, summed over the runs
+  u32 a_mine = 0, b_mine = 0;  // this lane's run's rows below c0, c0 + 1
+  K x = K(c0) << (T::kBits - g_lo);
+  bool light = false;
+  if (g > g_lo) {
+    // 64-ary search for the last c in [lo, hi) with hi_bound(c) <= R,
+    // given that of lo <= R < that of hi.  Lane 0 also evaluates lo in
+    // the first round (exact there: every run's directory resolves the
+    // coarse start), so below_lo comes from the same loads, and each
+    // round keeps the chosen start's lower bound, so none is reloaded.
+    const u32 d = g - g_lo;
+    u32 lo = c0 << d, hi = (c0 + 1) << d;
+    u64 plo_lo = 0;
+    bool first = true;
     while (hi - lo > 1) {
       const u32 span = hi - lo;
       const bool fine = span <= static_cast<u32>(kWave);
       const u32 c =
           fine ? lo + lane : lo + static_cast<u32>((u64(span) * lane) >> 6);
-      bool p = lane == 0;  // below(lo) <= R is known
-      if (lane != 0 && c < hi) p = below(c) <= R;
+      bool p = lane == 0;  // hi_bound(lo) <= R is known
+      u64 plo = 0, phi = 0;
+      if ((lane == 0 && first) || (lane != 0 && c < hi)) {
+        bounds_at<T>(in, m, ways, c, g, plo, phi);
+        if (lane != 0) p = phi <= R;
+      }
+      if (first) {
+        below_lo = static_cast<u64>(
+            __shfl(static_cast<unsigned long long>(plo), 0, kWave));
+        plo_lo = below_lo;
+        first = false;
+      }
       const u32 L = popc64(ballot(p)) - 1;  // the true lanes form a prefix
+      const u64 plo_l = static_cast<u64>(__shfl(
+          static_cast<unsigned long long>(plo), static_cast<int>(L), kWave));
+      if (L != 0) plo_lo = plo_l;
       const u32 cl =
           fine ? lo + L : lo + static_cast<u32>((u64(span) * L) >> 6);
       const u32 cn =
@@ -215,52 +257,26 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
       lo = cl;
       hi = cn;
     }
-    return lo;
-  };
-  // this lane's run (lanes >= ways answer nothing)
-  RunView<T> rv = in.run[0];
-  RunMeta rm = m[0];
-#pragma unroll
-  for (int i = 1; i < kMaxWay; ++i)
-    if (lane == static_cast<u32>(i)) {
-      rv = in.run[i];
-      rm = m[i];
+    light = R - below_lo < light_max;
+    if (!light && R - plo_lo < light_max) {
+      light = true;
+      x = K(lo) << (T::kBits - g);
     }
-  // phase 1: the boundary's coarse bucket (exact), then the last of its
-  // sub-buckets at the finest resolution whose start has at most R rows
-  // below it by the upper bound; the start becomes x_t when the lower
-  // bound leaves fewer than light_max rows short of R, which bounds the
-  // tile, and every run's split is then its lower bound of x_t
-  const u32 c0 = bucket_of[t];
-  u64 below_lo = 0;
-  u32 a_mine = 0, b_mine = 0;
+  } else {
+    // every run's directory resolves g: the coarse bounds are entries
 #pragma unroll
-  for (int i = 0; i < kMaxWay; ++i) {
-    if (static_cast<u32>(i) < ways) {
-      const u32 ai = boundary_lb(in.run[i], m[i], c0, g_lo);
-      const u32 bi = boundary_lb(in.run[i], m[i], c0 + 1, g_lo);
-      below_lo += ai;
-      if (lane == static_cast<u32>(i)) {
-        a_mine = ai;
-        b_mine = bi;
+    for (int i = 0; i < kMaxWay; ++i) {
+      if (static_cast<u32>(i) < ways) {
+        const u32 ai = boundary_lb(in.run[i], m[i], c0, g_lo);
+        const u32 bi = boundary_lb(in.run[i], m[i], c0 + 1, g_lo);
+        below_lo += ai;
+        if (lane == static_cast<u32>(i)) {
+          a_mine = ai;
+          b_mine = bi;
+        }
       }
     }
-  }
-  K x = K(c0) << (T::kBits - g_lo);
-  bool light = R - below_lo < light_max;
-  if (!light && g > g_lo) {
-    const u32 d = g - g_lo;
-    const u32 c = search(c0 << d, (c0 + 1) << d, [&](u32 cc) -> u64 {
-      u64 plo, phi;
-      bounds_at<T>(in, m, ways, cc, g, plo, phi);
-      return phi;
-    });
-    u64 plo, phi;
-    bounds_at<T>(in, m, ways, c, g, plo, phi);
-    if (R - plo < light_max) {
-      light = true;
-      x = K(c) << (T::kBits - g);
-    }
+    light = R - below_lo < light_max;
   }
   u32 mine = 0;
   if (light) {
@@ -269,6 +285,10 @@ __global__ void kway_partition_kernel(KWayIn<T> in, u32 tiles, u32 target,
     // phase 2 (dense keys): the largest x in the coarse bucket with at
     // most R rows below it, by a 64-ary search over its keys; inside one
     // bucket every key shares the top bit, so rows compare on the low bits
+    if (g > g_lo && lane < ways) {
+      a_mine = boundary_lb(rv, rm, c0, g_lo);
+      b_mine = boundary_lb(rv, rm, c0 + 1, g_lo);
+    }
     u32 a[kMaxWay], b[kMaxWay];
 #pragma unroll
     for (int i = 0; i < kMaxWay; ++i) {
