@@ -832,27 +832,26 @@ class LsmT : public HandleBase {
     Lease<u32> ka(pool_, n), va(pool_, n), kb(pool_, n), vb(pool_, n);
     u32 *c = ctrl.get();
     const rs::Control pc = rs::control_layout(c, 1);
-    // one launch computes the key spread and clears the partition's state,
-    // this kernel's state behind it (all but the spans) and the run header
+    // per-tile digit counts and spreads; the same launch clears the
+    // partition's state, this kernel's state behind it (all but the
+    // spans) and the run header.  Per-tile digit prefixes at the batch's
+    // shift replace the partition's look-back.
     {
       K *span = reinterpret_cast<K *>(c + L.span_at);
-      span_clear_kernel<K><<<span_clear_blocks(words + kMetaWords), 256, 0,
-                             s>>>(keys, n, span, c, L.span_at, c + L.shift_at,
-                                  words - L.shift_at,
-                                  reinterpret_cast<u32 *>(r.meta()),
-                                  kMetaWords);
-      // per-tile digit prefixes replace the partition's look-back
       const u32 tiles = rs::tiles_for(n);
-      Lease<u32> tile_hist(pool_, u64(tiles) * rs::kRadix);
+      Lease<u32> tile_hist(pool_, 2 * u64(tiles) * rs::kRadix);
+      u32 *tile_prefix = tile_hist.get() + u64(tiles) * rs::kRadix;
       seg_tile_hist_kernel<K><<<tiles, rs::kThreads, 0, s>>>(
-          keys, n, span, tile_hist.get(), c + L.shift_at);
-      seg_tile_scan_kernel<<<rs::kRadix, rs::kThreads, 0, s>>>(
-          tile_hist.get(), tiles, pc.offsets);
+          keys, n, tile_hist.get(), span, c, L.span_at, c + L.shift_at,
+          words - L.shift_at, reinterpret_cast<u32 *>(r.meta()), kMetaWords);
+      seg_tile_scan_kernel<K><<<rs::kRadix, rs::kThreads, 0, s>>>(
+          keys, n, tile_hist.get(), span, tiles, tile_prefix, pc.offsets,
+          c + L.shift_at);
       rs::launch_pass<u32>(
           tuning_.sort_block ? tuning_.sort_block : rs::block_for(n), tiles,
           keys, SegPayload{values, flags, mode}, ka.get(), va.get(), n, 0,
           pc.offsets, pc.status, pc.tickets, s, nullptr, c + L.shift_at,
-          tile_hist.get());
+          tile_prefix);
     }
     seg_sort_kernel<T><<<256, kSegThreads, 0, s>>>(
         keys, ka.get(), va.get(), kb.get(), vb.get(), n, pc.offsets,

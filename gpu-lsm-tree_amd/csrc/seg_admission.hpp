@@ -1,13 +1,15 @@
 // Segmented admission of a 32-bit batch: the batch becomes a run in two
 // sorting launches instead of four radix passes and a finalize launch.
 //
-//   span_clear_kernel per-block OR of (key ^ keys[0]): the bits that vary
-//                     (the same launch clears the sort's control state)
-//   seg_tile_hist_kernel  s0 = (highest varying bit) - 7; per-tile
-//                     histograms of the digit (key >> s0) & 0xff, so the
-//                     256 segments split the batch's actual key spread,
-//                     not the key space
-//   seg_tile_scan_kernel  per-tile digit prefixes and segment starts
+//   seg_tile_hist_kernel  per tile (4096 keys): the OR of (key ^ keys[0])
+//                     (the bits that vary inside the tile) and counts of
+//                     the digit at the tile's own shift (the same launch
+//                     clears the sort's control state and the run header)
+//   seg_tile_scan_kernel  s0 = (highest varying bit of the batch) - 7;
+//                     every tile's counts of the digit (key >> s0) & 0xff,
+//                     derived from its own, so the 256 segments split the
+//                     batch's actual key spread, not the key space; per-tile
+//                     digit prefixes and segment sizes
 //   onesweep pass     stable partition by that digit, carrying the values
 //                     (SegPayload), so the sort needs no random gathers;
 //                     its tiles read their digit offsets instead of
@@ -37,148 +39,159 @@ constexpr int kSegItems = 8;
 constexpr int kSegTile = kSegThreads * kSegItems;  // 8192
 constexpr int kSegWaves = kSegThreads / kWave;
 static_assert(kSegWaves >= 2, "the look-back and the offset scan use two waves");
-constexpr int kSpanBlocks = 256;
-
-// The key spread and the clearing of three regions a[0..na), b[0..nb),
-// c[0..nc) in one launch: every block clears its share, and block b below
-// kSpanBlocks writes span[b] = OR over its keys of (key ^ keys[0]), 16 keys
-// per thread per round, loaded together.
+// Top set bit of x, 0 when x == 0.
 template <class K>
-static __global__ __launch_bounds__(256) void span_clear_kernel(
-    const K *__restrict__ keys, u32 n, K *__restrict__ span, u32 *a, u64 na,
-    u32 *b, u64 nb, u32 *c, u64 nc) {
-  const u64 total = na + nb + nc;
-  for (u64 i = u64(blockIdx.x) * 256 + threadIdx.x; i < total;
-       i += u64(gridDim.x) * 256) {
-    if (i < na) a[i] = 0;
-    else if (i < na + nb) b[i - na] = 0;
-    else c[i - na - nb] = 0;
-  }
-  if (blockIdx.x >= static_cast<u32>(kSpanBlocks)) return;
-  __shared__ K s_or[256 / kWave];
-  const K k0 = keys[0];
-  K acc = 0;
-  const u64 stride = u64(kSpanBlocks) * 256 * 16;
-  for (u64 base = u64(blockIdx.x) * 256 * 16; base < n; base += stride) {
-    K k[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const u64 i = base + u64(j) * 256 + threadIdx.x;
-      k[j] = i < n ? keys[i] : k0;
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) acc |= k[j] ^ k0;
-  }
-#pragma unroll
-  for (int offset = kWave / 2; offset > 0; offset >>= 1)
-    acc |= __shfl_xor(acc, offset, kWave);
-  if (lane_id() == 0) s_or[threadIdx.x / kWave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    K v = 0;
-    for (int w = 0; w < 256 / kWave; ++w) v |= s_or[w];
-    span[blockIdx.x] = v;
-  }
+__device__ __forceinline__ u32 top_bit(K x) {
+  if (!x) return 0;
+  if constexpr (sizeof(K) == 8) return 63u - static_cast<u32>(__clzll(x));
+  else return 31u - static_cast<u32>(__clz(x));
 }
 
-inline u32 span_clear_blocks(u64 words) {
-  const u64 b = (words + 255) / 256;
-  return static_cast<u32>(b < u64(kSpanBlocks) ? u64(kSpanBlocks)
-                                                : (b > 4096 ? 4096 : b));
-}
-
-// Digit shift from the key spread: the top 8 varying bits.
-template <class K>
-__device__ __forceinline__ u32 seg_shift_from(const K *span) {
-  // every wave folds the block results (one wave-wide OR per 64)
-  K diff = 0;
-  for (int b = static_cast<int>(lane_id()); b < kSpanBlocks; b += kWave)
-    diff |= span[b];
-#pragma unroll
-  for (int offset = kWave / 2; offset > 0; offset >>= 1)
-    diff |= __shfl_xor(diff, offset, kWave);
-  u32 h = 0;
-  if (diff) {
-    if constexpr (sizeof(K) == 8) h = 63u - static_cast<u32>(__clzll(diff));
-    else h = 31u - static_cast<u32>(__clz(diff));
-  }
-  return h >= 7 ? h - 7 : 0u;
-}
-
-// Per-tile digit counts of the admission partition, digit-major: block t
-// counts the digit (key >> s0) & 0xff of keys [t * 4096, (t + 1) * 4096),
-// one onesweep tile, into tile_hist[d * tiles + t]; block 0 also stores the
-// digit shift.
+// Per-tile digit counts of a spread partition, digit-major, each at the
+// tile's own shift: block t (one onesweep tile, keys [t * 4096,
+// (t + 1) * 4096)) writes tile_span[t] = OR over its keys of
+// (key ^ keys[0]) and counts the digit (key >> st) & 0xff into
+// tile_hist[d * tiles + t], st = max(h_t, 7) - 7 for the tile's top
+// varying bit h_t.  Every key of the tile carries keys[0]'s bits above
+// max(h_t, 7), which lets seg_tile_scan_kernel derive the counts at the
+// batch's shift.  The launch also clears a[0..na), b[0..nb) and c[0..nc),
+// every block its share.
 template <class K>
 static __global__ __launch_bounds__(rs::kThreads) void seg_tile_hist_kernel(
-    const K *__restrict__ keys, u32 n, const K *__restrict__ span,
-    u32 *__restrict__ tile_hist, u32 *shift_out) {
+    const K *__restrict__ keys, u32 n, u32 *__restrict__ tile_hist,
+    K *__restrict__ tile_span, u32 *a, u64 na, u32 *b, u64 nb, u32 *c,
+    u64 nc) {
   static_assert(rs::kThreads == rs::kRadix &&
                     rs::kThreads * rs::kHistItems == 4096,
                 "a thread per digit, one onesweep tile per block");
   __shared__ u32 s_hist[rs::kRadix];
-  const u32 s0 = seg_shift_from(span);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *shift_out = s0;
+  __shared__ K s_or[rs::kThreads / kWave];
+  const u64 total = na + nb + nc;
+  for (u64 i = u64(blockIdx.x) * rs::kThreads + threadIdx.x; i < total;
+       i += u64(gridDim.x) * rs::kThreads) {
+    if (i < na) a[i] = 0;
+    else if (i < na + nb) b[i - na] = 0;
+    else c[i - na - nb] = 0;
+  }
   s_hist[threadIdx.x] = 0;
-  __syncthreads();
+  const K k0 = keys[0];
   const u64 base = u64(blockIdx.x) * 4096;
   K k[rs::kHistItems];
+  K acc = 0;
 #pragma unroll
   for (int j = 0; j < rs::kHistItems; ++j) {
     const u64 i = base + u64(j) * rs::kThreads + threadIdx.x;
-    k[j] = i < n ? keys[i] : K(0);
+    k[j] = i < n ? keys[i] : k0;
   }
+#pragma unroll
+  for (int j = 0; j < rs::kHistItems; ++j) acc |= k[j] ^ k0;
+#pragma unroll
+  for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    acc |= __shfl_xor(acc, offset, kWave);
+  if (lane_id() == 0) s_or[threadIdx.x / kWave] = acc;
+  __syncthreads();  // also orders the counter clearing before the counts
+  K o = 0;
+#pragma unroll
+  for (int w = 0; w < rs::kThreads / kWave; ++w) o |= s_or[w];
+  const u32 h = top_bit(o);
+  const u32 st = h >= 7 ? h - 7 : 0u;
 #pragma unroll
   for (int j = 0; j < rs::kHistItems; ++j)
     if (base + u64(j) * rs::kThreads + threadIdx.x < n)
-      atomicAdd(&s_hist[static_cast<u32>(k[j] >> s0) & 0xffu], 1u);
+      atomicAdd(&s_hist[static_cast<u32>(k[j] >> st) & 0xffu], 1u);
   __syncthreads();
   tile_hist[u64(threadIdx.x) * gridDim.x + blockIdx.x] = s_hist[threadIdx.x];
+  if (threadIdx.x == 0) tile_span[blockIdx.x] = o;
 }
 
-// One block per digit d: the exclusive prefix of its counts over the
-// tiles, in place, and its total into totals[d].  The onesweep pass then
-// places tile t's keys of digit d at (the digit's start, a scan of the
-// totals) + tile_hist[d * tiles + t], without a look-back.  Chunks of 4
-// consecutive tiles per thread, 1024 tiles per round.
+// One block per digit d at the batch's shift s0 = hg - 7, hg = max(h, 7)
+// for the top bit h of the OR of the tile spreads.  Tile t's count of d
+// comes from its counts at its own shift (seg_tile_hist_kernel): with
+// ht = max(h_t, 7) and e = hg - ht, its keys carry keys[0]'s bits ht + 1
+// .. hg, so d counts nothing unless its top e bits equal those, and then
+// gathers the tile's digits whose top 8 - e bits are d's low bits (e >= 8:
+// every key of the tile has keys[0]'s digit).  Writes prefix[d * tiles +
+// t] = the exclusive prefix of d's counts over the tiles and totals[d];
+// block 0 also stores the shift.  The onesweep pass then places tile t's
+// keys of digit d at (the digit's start, a scan of the totals) +
+// prefix[d * tiles + t], without a look-back.  Chunks of 4 consecutive
+// tiles per thread, 1024 tiles per round.
+template <class K>
 static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
-    u32 *__restrict__ tile_hist, u32 tiles, u32 *__restrict__ totals) {
+    const K *__restrict__ keys, u32 n, const u32 *__restrict__ tile_hist,
+    const K *__restrict__ tile_span, u32 tiles, u32 *__restrict__ prefix,
+    u32 *__restrict__ totals, u32 *__restrict__ shift_out) {
   constexpr u32 kPer = 4;
   constexpr u32 kRound = kPer * rs::kThreads;
   __shared__ u32 s_scan[rs::kThreads / kWave + 1];
-  u32 *col = tile_hist + u64(blockIdx.x) * tiles;
+  __shared__ K s_or[rs::kThreads / kWave];
+  const u32 d = blockIdx.x;
+  K o = 0;
+  for (u32 t = threadIdx.x; t < tiles; t += rs::kThreads) o |= tile_span[t];
+#pragma unroll
+  for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    o |= __shfl_xor(o, offset, kWave);
+  if (lane_id() == 0) s_or[threadIdx.x / kWave] = o;
+  __syncthreads();
+  o = 0;
+#pragma unroll
+  for (int w = 0; w < rs::kThreads / kWave; ++w) o |= s_or[w];
+  const u32 hg = top_bit(o) > 7 ? top_bit(o) : 7u;
+  const u32 s0 = hg - 7;
+  if (d == 0 && threadIdx.x == 0) *shift_out = s0;
+  const K k0 = keys[0];
+  const u32 own = static_cast<u32>(k0 >> s0) & 0xffu;  // keys[0]'s digit
+  auto count = [&](u32 t) -> u32 {
+    const u32 h_t = top_bit(tile_span[t]);
+    const u32 ht = h_t > 7 ? h_t : 7u;
+    const u32 e = hg - ht;
+    if (e == 0) return tile_hist[u64(d) * tiles + t];
+    if (e >= 8) {
+      const u64 first = u64(t) * 4096;
+      return d == own ? static_cast<u32>(n - first < 4096 ? n - first : 4096)
+                      : 0u;
+    }
+    const u32 top = static_cast<u32>(k0 >> (ht + 1)) & ((1u << e) - 1u);
+    if ((d >> (8 - e)) != top) return 0;
+    const u32 j0 = (d & ((1u << (8 - e)) - 1u)) << e;
+    u32 sum = 0;
+    for (u32 j = j0; j < j0 + (1u << e); ++j)
+      sum += tile_hist[u64(j) * tiles + t];
+    return sum;
+  };
+  u32 *col = prefix + u64(d) * tiles;
   u32 carry = 0;
   for (u32 r0 = 0; r0 < tiles; r0 += kRound) {
     const u32 first = r0 + threadIdx.x * kPer;
-    u32 c[kPer], sum = 0;
+    u32 cnt[kPer], sum = 0;
 #pragma unroll
     for (u32 q = 0; q < kPer; ++q) {
-      c[q] = first + q < tiles ? col[first + q] : 0u;
-      sum += c[q];
+      cnt[q] = first + q < tiles ? count(first + q) : 0u;
+      sum += cnt[q];
     }
     u32 run = carry + block_exclusive_scan<rs::kThreads>(sum, s_scan);
 #pragma unroll
     for (u32 q = 0; q < kPer; ++q) {
       if (first + q < tiles) {
         col[first + q] = run;
-        run += c[q];
+        run += cnt[q];
       }
     }
     carry += s_scan[rs::kThreads / kWave];
     __syncthreads();  // s_scan is reused by the next round
   }
-  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+  if (threadIdx.x == 0) totals[d] = carry;
 }
 
 // Control words of a spread partition of n keys: the partition's own, the
-// per-block spans and the device-side shift (all cleared by one launch).
+// per-tile spans and the device-side shift.
 template <class K>
 struct SpreadLayout {
   u64 span_at, shift_at, words;
   explicit SpreadLayout(u64 n) {
     const u64 part = rs::control_words(n, 1);
     span_at = part + (part & 1);  // 8-byte aligned for 64-bit spans
-    shift_at = span_at + u64(kSpanBlocks) * (sizeof(K) / 4);
+    shift_at = span_at + u64(rs::tiles_for(n)) * (sizeof(K) / 4);
     words = shift_at + 2;
   }
 };
@@ -197,25 +210,24 @@ inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
   const SpreadLayout<K> L(n);
   const rs::Control pc = rs::control_layout(control, 1);
   K *span = reinterpret_cast<K *>(control + L.span_at);
-  // Per-tile digit histograms (tiles x 256 words, in the room of the
-  // look-back state, which this pass does not use) give every tile its
-  // offsets; nothing of the partition's control needs clearing, only the
-  // shift and the caller's extra region (the spans are written whole).
+  // Per-tile digit counts and prefixes (tiles x 256 words each, in the
+  // room of the look-back state, which this pass does not use) give every
+  // tile its offsets; nothing of the partition's control needs clearing,
+  // only the shift and the caller's extra region (the spans are written
+  // whole).
   const u32 tiles = rs::tiles_for(n);
   u32 *tile_hist = reinterpret_cast<u32 *>(pc.status);
-  span_clear_kernel<K><<<span_clear_blocks(L.words - L.shift_at +
-                                           extra_words),
-                         256, 0, s>>>(keys, n, span, control, 0,
-                                      control + L.shift_at,
-                                      L.words - L.shift_at, extra,
-                                      extra_words);
+  u32 *tile_prefix = tile_hist + u64(tiles) * rs::kRadix;
   seg_tile_hist_kernel<K><<<tiles, rs::kThreads, 0, s>>>(
-      keys, n, span, tile_hist, control + L.shift_at);
-  seg_tile_scan_kernel<<<rs::kRadix, rs::kThreads, 0, s>>>(tile_hist, tiles,
-                                                           pc.offsets);
+      keys, n, tile_hist, span, control, 0, control + L.shift_at,
+      L.words - L.shift_at, extra, extra_words);
+  seg_tile_scan_kernel<K><<<rs::kRadix, rs::kThreads, 0, s>>>(
+      keys, n, tile_hist, span, tiles, tile_prefix, pc.offsets,
+      control + L.shift_at);
   rs::launch_pass<K>(block ? block : rs::block_for(n), tiles, keys, payload,
                      keys_out, pos_out, n, 0, pc.offsets, pc.status,
-                     pc.tickets, s, dest, control + L.shift_at, tile_hist);
+                     pc.tickets, s, dest, control + L.shift_at,
+                     tile_prefix);
 }
 
 // Payload of the partition pass.  When s0 <= 23 the top varying key bit
