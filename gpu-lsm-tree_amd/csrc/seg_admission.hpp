@@ -126,8 +126,20 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
   __shared__ u32 s_scan[rs::kThreads / kWave + 1];
   __shared__ K s_or[rs::kThreads / kWave];
   const u32 d = blockIdx.x;
+  // the first round's spreads and counts load with the spread reduction
+  // (one memory round trip when tiles <= 1024)
+  K sp0[kPer];
+  u32 raw0[kPer];
   K o = 0;
-  for (u32 t = threadIdx.x; t < tiles; t += rs::kThreads) o |= tile_span[t];
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    const u32 t = threadIdx.x * kPer + q;
+    sp0[q] = t < tiles ? tile_span[t] : K(0);
+    raw0[q] = t < tiles ? tile_hist[u64(d) * tiles + t] : 0u;
+    o |= sp0[q];
+  }
+  for (u32 t = kRound + threadIdx.x; t < tiles; t += rs::kThreads)
+    o |= tile_span[t];
 #pragma unroll
   for (int offset = kWave / 2; offset > 0; offset >>= 1)
     o |= __shfl_xor(o, offset, kWave);
@@ -141,11 +153,12 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
   if (d == 0 && threadIdx.x == 0) *shift_out = s0;
   const K k0 = keys[0];
   const u32 own = static_cast<u32>(k0 >> s0) & 0xffu;  // keys[0]'s digit
-  auto count = [&](u32 t) -> u32 {
-    const u32 h_t = top_bit(tile_span[t]);
+  // tile t's count of d from its spread and its count of its own digit d
+  auto count = [&](u32 t, K span_t, u32 raw_t) -> u32 {
+    const u32 h_t = top_bit(span_t);
     const u32 ht = h_t > 7 ? h_t : 7u;
     const u32 e = hg - ht;
-    if (e == 0) return tile_hist[u64(d) * tiles + t];
+    if (e == 0) return raw_t;
     if (e >= 8) {
       const u64 first = u64(t) * 4096;
       return d == own ? static_cast<u32>(n - first < 4096 ? n - first : 4096)
@@ -166,7 +179,13 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
     u32 cnt[kPer], sum = 0;
 #pragma unroll
     for (u32 q = 0; q < kPer; ++q) {
-      cnt[q] = first + q < tiles ? count(first + q) : 0u;
+      const u32 t = first + q;
+      cnt[q] = 0;
+      if (t < tiles) {
+        const K sp = r0 == 0 ? sp0[q] : tile_span[t];
+        const u32 raw = r0 == 0 ? raw0[q] : tile_hist[u64(d) * tiles + t];
+        cnt[q] = count(t, sp, raw);
+      }
       sum += cnt[q];
     }
     u32 run = carry + block_exclusive_scan<rs::kThreads>(sum, s_scan);
