@@ -126,8 +126,9 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
   __shared__ u32 s_scan[rs::kThreads / kWave + 1];
   __shared__ K s_or[rs::kThreads / kWave];
   const u32 d = blockIdx.x;
-  // the first round's spreads and counts load with the spread reduction
-  // (one memory round trip when tiles <= 1024)
+  // keys[0] and the first round's spreads and counts load with the spread
+  // reduction (one memory round trip when tiles <= 1024)
+  const K k0 = keys[0];
   K sp0[kPer];
   u32 raw0[kPer];
   K o = 0;
@@ -138,6 +139,8 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
     raw0[q] = t < tiles ? tile_hist[u64(d) * tiles + t] : 0u;
     o |= sp0[q];
   }
+  // the rest (tiles > 1024, large query sets): eight loads in flight
+#pragma unroll 8
   for (u32 t = kRound + threadIdx.x; t < tiles; t += rs::kThreads)
     o |= tile_span[t];
 #pragma unroll
@@ -151,7 +154,6 @@ static __global__ __launch_bounds__(rs::kThreads) void seg_tile_scan_kernel(
   const u32 hg = top_bit(o) > 7 ? top_bit(o) : 7u;
   const u32 s0 = hg - 7;
   if (d == 0 && threadIdx.x == 0) *shift_out = s0;
-  const K k0 = keys[0];
   const u32 own = static_cast<u32>(k0 >> s0) & 0xffu;  // keys[0]'s digit
   // tile t's count of d from its spread and its count of its own digit d
   auto count = [&](u32 t, K span_t, u32 raw_t) -> u32 {
