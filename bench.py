@@ -47,6 +47,9 @@ def parse():
                     help="requests in the serving comparison (0 = skip)")
     ap.add_argument("--flix-batches", type=int, default=4,
                     help="config-3 FliX batches to run (0 = skip)")
+    ap.add_argument("--flix-ref", type=int, default=1,
+                    help="FliX on the shape of the reference's recorded run "
+                    "(1 = run, 0 = skip)")
     ap.add_argument("--serve-rate-x", type=float, default=2.0,
                     help="serving arrival rate as a multiple of the plain "
                          "path's request rate")
@@ -445,6 +448,121 @@ def c5_measurement(args, pkg, wl, torch, dev):
         "saturated": saturated, "poisson_sweep": sweep}
 
 
+# FliX's recorded run (FliX/build/sample_output, 32-bit keys, 32-slot
+# nodes): median bulk insert and delete of 13,421,772 keys and median
+# sorted lookups of 2^26 probes, in Mops/s.
+FLIX_RECORDED = {"insert_mops": 1962.0, "delete_mops": 2526.0,
+                 "sorted_lookup_mops": 9750.0}
+
+
+def flix_reference_shape(torch, pkg, dev):
+    """FliX on the shape of the reference's recorded run
+    (FliX/benchmarks_updates.cuh:1590-1700, keys as in
+    FliX/input_generation.h:201-300 with x = 25, y = 90): a build of 2^25
+    distinct uniform 32-bit keys (values = positions), five insert steps of
+    13,421,772 new keys (90% uniform in the lowest quarter of the key range,
+    10% uniform above it; unsorted, values = positions), then five delete
+    steps of 13,421,772 sorted keys drawn uniformly from the live keys; after
+    the build and after every step, 2^26 probes drawn from the live keys,
+    sorted outside the timed region.  Medians of HIP-event times."""
+    n_build, batch, probes, steps = 1 << 25, 13421772, 1 << 26, 5
+    lo_key, hi_key = 1, (1 << 32) - 2
+    dense_hi = lo_key + ((hi_key - lo_key) * 25 + 99) // 100 - 1
+    g = torch.Generator(device=dev)
+    g.manual_seed(42)
+
+    def draw(n, a, b):
+        return torch.randint(a, b + 1, (n,), generator=g, device=dev,
+                             dtype=torch.int64)
+
+    def u32(x):  # uint32 bit patterns in an int32 tensor
+        return (x - (x >= (1 << 31)).long() * (1 << 32)).to(torch.int32)
+
+    u = torch.unique(draw(int(n_build * 1.02), lo_key, hi_key))
+    build = u[torch.randperm(len(u), generator=g, device=dev)[:n_build]]
+    # new keys: 90% dense, 10% above the dense region; candidates that
+    # repeat a built key or each other are dropped (all keys distinct)
+    need = steps * batch
+    m = int(need * 1.15)
+    n_dense = (m * 90 + 99) // 100
+    cand = torch.cat([draw(n_dense, lo_key, dense_hi),
+                      draw(m - n_dense, dense_hi + 1, hi_key)])
+    cand = cand[torch.randperm(m, generator=g, device=dev)]
+    allk = torch.cat([build, cand])
+    _, inv, cnt = torch.unique(allk, return_inverse=True, return_counts=True)
+    fresh = cand[cnt[inv[n_build:]] == 1][:need]
+    del allk, inv, cnt, cand, u
+    assert len(fresh) == need
+    f = pkg.FlixIndex(key_bytes=4, value_bytes=4, node_slots=32,
+                      fill_percent=50)
+    stream = torch.cuda.current_stream()
+    pos = torch.arange(n_build + need, device=dev, dtype=torch.int64)
+    out = torch.empty(probes, dtype=torch.int32, device=dev)
+    live = build.clone()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    misses = 0
+
+    def probe():
+        nonlocal misses
+        q = u32(torch.sort(live[torch.randint(
+            0, len(live), (probes,), generator=g, device=dev)])[0])
+        ms = timed(lambda: f.lookups_ordered(q, out))
+        misses += int((out == -1).sum().item())
+        return ms
+
+    build_ms = timed(lambda: f.build_bucket_layer_only(
+        u32(build), max_size=n_build + need))
+    probe_ms, ins_ms, del_ms = [probe()], [], []
+    for t in range(steps):
+        k = u32(fresh[t * batch:(t + 1) * batch])
+        v = pos[n_build + t * batch:n_build + (t + 1) * batch].to(torch.int32)
+        ins_ms.append(timed(lambda: f.insert(k, v)))
+        live = torch.cat([live, fresh[t * batch:(t + 1) * batch]])
+        probe_ms.append(probe())
+    for t in range(steps):
+        pick = torch.randperm(len(live), generator=g, device=dev)
+        dk = u32(torch.sort(live[pick[:batch]])[0])
+        del_ms.append(timed(lambda: f.remove(dk)))
+        live = live[pick[batch:]]
+        probe_ms.append(probe())
+    f.close()
+
+    def med(x):
+        return sorted(x)[len(x) // 2]
+
+    res = {
+        "workload": "FliX, 32-bit keys and values, 32-slot nodes filled 50%%: "
+                    "build %d distinct uniform keys, 5 insert steps of %d new "
+                    "keys (90%% in the lowest quarter of the key range), 5 "
+                    "delete steps of %d sorted live keys, %d sorted all-hit "
+                    "probes after the build and every step (the sort is not "
+                    "timed); medians" % (n_build, batch, batch, probes),
+        "build_mkeys_per_s": round(n_build / build_ms / 1e3, 1),
+        "insert_mops": round(batch / med(ins_ms) / 1e3, 1),
+        "delete_mops": round(batch / med(del_ms) / 1e3, 1),
+        "sorted_lookup_mops": round(probes / med(probe_ms) / 1e3, 1),
+        "probe_misses": misses,
+        "ms": {"insert": round(med(ins_ms), 3), "delete": round(med(del_ms), 3),
+               "probe": round(med(probe_ms), 3)},
+        "reference_recorded": FLIX_RECORDED,
+        "source": "FliX/build/sample_output (NVIDIA RTX A6000 / 6000 Ada "
+                  "class GPU)",
+    }
+    res["ratio_vs_recorded"] = {k: round(res[k] / v, 2)
+                                for k, v in FLIX_RECORDED.items()}
+    return res
+
+
 def flix_measurement(args, pkg, wl, torch, dev):
     """Config 3 (FliX, uint64 keys and values): bulk build of 2^27 clustered
     keys, then `flix_batches` batches of 2^22 mixed ops, each followed by
@@ -798,6 +916,10 @@ def main():
         d.close()
         result["flix"] = flix_measurement(args, pkg, wl, torch, dev)
         progress("flix done")
+    if args.flix_ref > 0:
+        d.close()
+        result["flix_reference_shape"] = flix_reference_shape(torch, pkg, dev)
+        progress("flix reference shape done")
     if args.c5_scale_log >= 0:
         d.close()
         try:

@@ -276,8 +276,10 @@ static __global__ __launch_bounds__(256) void insert_kernel(
       p.next[id] = (i + 1 < outs) ? s_out[w][i + 1] + 1 : nxt;
     }
     // make this step's slots, counts and links visible to the next step's
-    // loads (agent-scope fence: waits for the stores, invalidates L1)
-    __threadfence();
+    // loads, which this wave makes on this CU: a workgroup-scope fence waits
+    // for the stores (the CU's L1 serves its own work-groups coherently).
+    // An agent-scope fence here would write back the L2 once per step.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     pos = q;
     // stay on `node`: the loop re-walks the freshly linked nodes
   }
