@@ -209,12 +209,17 @@ static __global__ __launch_bounds__(256) void insert_kernel(
   u32 node = b;
   while (pos < end) {
     const u32 nxt = p.next[node];
-    // batch keys below the first key of the next node belong here
+    // batch keys below the first key of the next node belong here; when
+    // the slice's first key is not below it, none does (one comparison
+    // instead of a search of the slice)
     u64 q = end;
-    if (nxt) q = lower_bound_g(bk, pos, end, p.keys[u64(nxt - 1) * S]);
-    if (q == pos) {
-      node = nxt - 1;  // nxt != 0 here: without it q == end > pos
-      continue;
+    if (nxt) {
+      const K first_next = p.keys[u64(nxt - 1) * S];
+      if (bk[pos] >= first_next) {
+        node = nxt - 1;
+        continue;
+      }
+      q = lower_bound_g(bk, pos, end, first_next);
     }
     if (q - pos > kPieceKeys) q = pos + kPieceKeys;
     const u32 m = static_cast<u32>(q - pos);
@@ -303,8 +308,13 @@ static __global__ __launch_bounds__(256) void delete_kernel(
   u32 prev = ~0u;
   while (true) {
     const u32 nxt = p.next[node];
+    // delete keys below the next node's first key (none when the slice's
+    // first key is not below it)
     u64 q = end;
-    if (nxt) q = lower_bound_g(dk, pos, end, p.keys[u64(nxt - 1) * S]);
+    if (nxt) {
+      const K first_next = p.keys[u64(nxt - 1) * S];
+      q = dk[pos] >= first_next ? pos : lower_bound_g(dk, pos, end, first_next);
+    }
     bool emptied = false;
     if (q > pos) {
       const u32 c = p.count[node];
