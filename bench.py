@@ -520,8 +520,10 @@ def flix_reference_shape(torch, pkg, dev):
         misses += int((out == -1).sum().item())
         return ms
 
+    bk32 = u32(build)
     build_ms = timed(lambda: f.build_bucket_layer_only(
-        u32(build), max_size=n_build + need))
+        bk32, max_size=n_build + need))
+    del bk32
     probe_ms, ins_ms, del_ms = [probe()], [], []
     for t in range(steps):
         k = u32(fresh[t * batch:(t + 1) * batch])
