@@ -377,8 +377,9 @@ class FlixImpl : public FlixBase {
     GD_LAUNCH_CHECK();
     u32 *count = work + std::min<u64>(P_, m);
     rs::zero_u32_kernel<<<1, 64, 0, s>>>(count, 1);
-    work_list_kernel<<<grid_for(P_, 256), 256, 0, s>>>(begin, P_, work,
-                                                        count);
+    work_list_kernel<<<static_cast<u32>(
+                           (u64(P_) + kWorkListBuckets - 1) / kWorkListBuckets),
+                       256, 0, s>>>(begin, P_, work, count);
     GD_LAUNCH_CHECK();
     u32 items = 0;
     GD_HIP(hipMemcpyAsync(&items, count, sizeof(u32), hipMemcpyDeviceToHost,

@@ -589,12 +589,33 @@ static __global__ void mark_first_kernel(const K *sk, u64 n, u32 *keep) {
     keep[i] = (i == 0 || sk[i - 1] != sk[i]) ? 1u : 0u;
 }
 
-// Buckets with a non-empty batch slice -> compact work list.
-static __global__ void work_list_kernel(const u64 *begin, u32 buckets, u32 *work,
-                                 u32 *work_count) {
-  const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= buckets) return;
-  if (begin[b + 1] > begin[b]) work[atomicAdd(work_count, 1u)] = b;
+// Buckets with a non-empty batch slice -> compact work list (in any
+// order).  A block covers kWorkListBuckets consecutive buckets, 16 per
+// thread, and takes its share of the list with one atomic, so the shared
+// counter sees one update per 4096 buckets.
+constexpr u32 kWorkListBuckets = 4096;
+static __global__ __launch_bounds__(256) void work_list_kernel(
+    const u64 *begin, u32 buckets, u32 *work, u32 *work_count) {
+  constexpr u32 kPer = kWorkListBuckets / 256;
+  __shared__ u32 s_scan[256 / kWave + 1];
+  __shared__ u32 s_base;
+  const u64 first = (u64(blockIdx.x) * 256 + threadIdx.x) * kPer;
+  u32 mask = 0, c = 0;
+#pragma unroll
+  for (u32 j = 0; j < kPer; ++j) {
+    const u64 b = first + j;
+    if (b < buckets && begin[b + 1] > begin[b]) {
+      mask |= 1u << j;
+      ++c;
+    }
+  }
+  const u32 ex = block_exclusive_scan<256>(c, s_scan);
+  if (threadIdx.x == 0) s_base = atomicAdd(work_count, s_scan[256 / kWave]);
+  __syncthreads();
+  u32 o = s_base + ex;
+#pragma unroll
+  for (u32 j = 0; j < kPer; ++j)
+    if ((mask >> j) & 1u) work[o++] = static_cast<u32>(first + j);
 }
 
 template <class K, class V>
