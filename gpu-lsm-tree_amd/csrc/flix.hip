@@ -320,7 +320,7 @@ class FlixImpl : public FlixBase {
 
   // out[0..n) = exclusive scan of in[0..n), out[n] = total.
   void exclusive_scan(const u64 *in, u64 n, u64 *out, hipStream_t s) {
-    const u64 tiles = (n + 1 + 255) / 256;
+    const u64 tiles = (n + 1 + kScanTile - 1) / kScanTile;
     Lease<u32> ctrl(pool_, 64 + 2 * tiles);
     rs::zero_u32_kernel<<<grid_for(64 + 2 * tiles, 256, 1024), 256, 0, s>>>(
         ctrl.get(), 64 + 2 * tiles);

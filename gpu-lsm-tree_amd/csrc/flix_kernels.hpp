@@ -522,20 +522,41 @@ static __global__ __launch_bounds__(256) void flatten_kernel(Pool<K, V> p,
   }
 }
 
-// Exclusive scan of u64 values with a single-pass chained scan.
+// Exclusive scan of u64 values with a single-pass chained scan over tiles
+// of kScanTile values, 16 per thread: coalesced loads into LDS, each
+// thread sums 16 consecutive values, a block scan of the thread sums, the
+// tile's look-back, then coalesced stores.  The LDS index is padded by one
+// slot per 16 so a thread's consecutive values fall in distinct banks.
+// out[n] receives the total (the value at n counts as 0).
+constexpr u32 kScanItems = 16;
+constexpr u32 kScanTile = 256 * kScanItems;
+__device__ __forceinline__ u32 scan_pad(u32 i) { return i + i / kScanItems; }
 static __global__ __launch_bounds__(256) void scan_u64_kernel(const u64 *in, u64 n,
                                                        u64 *out, u64 *status,
                                                        u32 *ticket) {
   __shared__ u32 s_tile;
+  __shared__ u64 s_v[kScanTile + kScanTile / kScanItems];
   __shared__ u64 s_part[256 / kWave + 1];
   __shared__ u64 s_base;
   if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const u32 tile = s_tile;
-  const u64 i = u64(tile) * 256 + threadIdx.x;
-  const u64 v = i < n ? in[i] : 0;
-  // wave inclusive scan (u64)
-  u64 inc = v;
+  const u64 base = u64(tile) * kScanTile;
+#pragma unroll
+  for (u32 j = 0; j < kScanItems; ++j) {
+    const u32 e = j * 256 + threadIdx.x;
+    s_v[scan_pad(e)] = base + e < n ? in[base + e] : 0;
+  }
+  __syncthreads();
+  u64 local[kScanItems];
+  u64 sum = 0;
+#pragma unroll
+  for (u32 j = 0; j < kScanItems; ++j) {
+    local[j] = s_v[scan_pad(threadIdx.x * kScanItems + j)];
+    sum += local[j];
+  }
+  // block scan of the thread sums (u64)
+  u64 inc = sum;
   const u32 lane = lane_id();
   for (int off = 1; off < kWave; off <<= 1) {
     const u64 o = __shfl_up(inc, off, kWave);
@@ -559,8 +580,18 @@ static __global__ __launch_bounds__(256) void scan_u64_kernel(const u64 *in, u64
     if (threadIdx.x == 0) s_base = ex;
   }
   __syncthreads();
-  if (i < n) out[i] = s_base + s_part[wv] + inc - v;
-  if (i == n) out[i] = s_base + s_part[wv] + inc - v;  // total at out[n]
+  u64 run = s_base + s_part[wv] + inc - sum;
+#pragma unroll
+  for (u32 j = 0; j < kScanItems; ++j) {
+    s_v[scan_pad(threadIdx.x * kScanItems + j)] = run;
+    run += local[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (u32 j = 0; j < kScanItems; ++j) {
+    const u32 e = j * 256 + threadIdx.x;
+    if (base + e <= n) out[base + e] = s_v[scan_pad(e)];
+  }
 }
 
 // Compacts the marked rows of a sorted batch and gathers their values
