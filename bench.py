@@ -3,7 +3,7 @@
 Workload (one step, config 2: GPULSMOpt at scale): clear, then 2^26 uint32
 keys in 64 mixed batches of 2^20 (10% of the entries of every batch after
 the first are tombstones of earlier keys, values murmur3(key)), then 2^26
-lookups (70% drawn from inserted keys) and 2^20 range counts with widths
+lookups (70% live keys, 30% absent keys) and 2^20 range counts with widths
 log-uniform in [2^4, 2^20].  Plain bulk form: every batch and query set is
 one API call in arrival order, each synchronised, so per-call latency is
 completion - submission.
@@ -699,7 +699,7 @@ def main():
     vals = np.concatenate([b[1] for b in batches])
     erase = np.concatenate([b[2] for b in batches])
     inserted = keys[erase == 0]
-    queries = w.lookups(inserted)
+    queries = w.lookups(w.live_keys(batches))
     lo, hi = w.ranges()
     gen_s = time.time() - t0
     progress("workload generated in %.1f s" % gen_s)

@@ -299,10 +299,54 @@ def _stream(s):
     return s.cuda_stream
 
 
+def _bad(msg):
+    return InvalidArgument(GD_INVALID_ARGUMENT, msg)
+
+
 def _count(x, count):
+    """The element count of a call: `count` when given, else x.numel().
+    An explicit count beyond a tensor's length is rejected."""
     if count is not None:
-        return int(count)
+        n = int(count)
+        if n < 0:
+            raise _bad("negative count")
+        if x is not None and not isinstance(x, int) and n > x.numel():
+            raise _bad("count %d exceeds the %d elements of the buffer"
+                       % (n, x.numel()))
+        return n
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        raise _bad("a raw device address needs an explicit count")
     return int(x.numel())
+
+
+def check_buffer(x, name, width, need, optional=False):
+    """Validates one device-buffer argument before anything is launched and
+    returns its address.
+
+    A tensor must live on a GPU, be contiguous, have `width`-byte elements
+    and hold at least `need` of them; None passes only when `optional` (or
+    when nothing is read or written, need == 0).  Raw integer addresses are
+    the caller's responsibility and pass unchecked.  Violations raise
+    InvalidArgument, so no kernel ever reads or writes past a tensor."""
+    if x is None:
+        if optional or need == 0:
+            return None
+        raise _bad("%s is required" % name)
+    if isinstance(x, int):
+        return x
+    if not getattr(x, "is_cuda", False):
+        raise _bad("%s must be a GPU tensor" % name)
+    if not x.is_contiguous():
+        raise _bad("%s must be contiguous" % name)
+    if x.element_size() != width:
+        raise _bad("%s has %d-byte elements, %d-byte ones are required"
+                   % (name, x.element_size(), width))
+    if x.numel() < need:
+        raise _bad("%s holds %d elements, %d are required"
+                   % (name, x.numel(), need))
+    return x.data_ptr()
 
 
 def _check(status, handle):
@@ -369,50 +413,70 @@ class GPULSMOpt:
     def clear(self, stream=None):
         self._call("clear", _stream(stream))
 
+    def _keys(self, x, name, n):
+        return check_buffer(x, name, self.key_bytes, n)
+
     def bulk_build(self, keys, values, count=None, stream=None):
-        self._call("bulk_build", _ptr(keys), _ptr(values),
-                   _count(keys, count), _stream(stream))
+        n = _count(keys, count)
+        self._call("bulk_build", self._keys(keys, "keys", n),
+                   self._keys(values, "values", n), n, _stream(stream))
 
     def insert(self, keys, values, count=None, stream=None):
-        self._call("insert", _ptr(keys), _ptr(values), _count(keys, count),
-                   _stream(stream))
+        n = _count(keys, count)
+        self._call("insert", self._keys(keys, "keys", n),
+                   self._keys(values, "values", n), n, _stream(stream))
 
     def erase(self, keys, count=None, stream=None):
-        self._call("erase", _ptr(keys), _count(keys, count), _stream(stream))
+        n = _count(keys, count)
+        self._call("erase", self._keys(keys, "keys", n), n, _stream(stream))
 
     def apply_mixed(self, keys, values, is_erase, count=None, stream=None):
-        self._call("apply_mixed", _ptr(keys), _ptr(values), _ptr(is_erase),
-                   _count(keys, count), _stream(stream))
+        n = _count(keys, count)
+        self._call("apply_mixed", self._keys(keys, "keys", n),
+                   self._keys(values, "values", n),
+                   check_buffer(is_erase, "is_erase", 1, n), n,
+                   _stream(stream))
 
     def lookup(self, queries, out_values, out_found=None, count=None,
                stream=None, quotients_grouped=False):
+        n = _count(queries, count)
+        q = self._keys(queries, "queries", n)
+        ov = self._keys(out_values, "out_values", n)
+        of = check_buffer(out_found, "out_found", 1, n, optional=True)
         if self.key_bytes == 4:
-            self._call("lookup", _ptr(queries), _count(queries, count),
-                       _ptr(out_values), _ptr(out_found),
-                       1 if quotients_grouped else 0, _stream(stream))
+            self._call("lookup", q, n, ov, of, 1 if quotients_grouped else 0,
+                       _stream(stream))
         else:
-            self._call("lookup", _ptr(queries), _count(queries, count),
-                       _ptr(out_values), _ptr(out_found), _stream(stream))
+            self._call("lookup", q, n, ov, of, _stream(stream))
 
     def range(self, lo, hi, out_sums, count=None, stream=None):
-        self._call("range_sum", _ptr(lo), _ptr(hi), _count(lo, count),
-                   _ptr(out_sums), _stream(stream))
+        n = _count(lo, count)
+        self._call("range_sum", self._keys(lo, "lo", n),
+                   self._keys(hi, "hi", n), n,
+                   self._keys(out_sums, "out_sums", n), _stream(stream))
 
     def range_count(self, lo, hi, out_counts, count=None, stream=None):
-        self._call("range_count", _ptr(lo), _ptr(hi), _count(lo, count),
-                   _ptr(out_counts), _stream(stream))
+        n = _count(lo, count)
+        self._call("range_count", self._keys(lo, "lo", n),
+                   self._keys(hi, "hi", n), n,
+                   self._keys(out_counts, "out_counts", n), _stream(stream))
 
     def successor(self, queries, out_keys, count=None, stream=None):
-        self._call("successor", _ptr(queries), _count(queries, count),
-                   _ptr(out_keys), _stream(stream))
+        n = _count(queries, count)
+        self._call("successor", self._keys(queries, "queries", n), n,
+                   self._keys(out_keys, "out_keys", n), _stream(stream))
 
     def dump_sorted(self, out_keys=None, out_values=None, capacity=0,
                     stream=None):
         """Returns the live count; fills out_keys/out_values when given."""
+        capacity = int(capacity)
+        ok = check_buffer(out_keys, "out_keys", self.key_bytes, capacity,
+                          optional=True)
+        ov = check_buffer(out_values, "out_values", self.key_bytes, capacity,
+                          optional=True)
         n = _U64(0)
         st = getattr(lib(), self._p + "dump_sorted")(
-            self._h, _ptr(out_keys), _ptr(out_values), int(capacity),
-            ctypes.byref(n), _stream(stream))
+            self._h, ok, ov, capacity, ctypes.byref(n), _stream(stream))
         if st not in (GD_OK, GD_LENGTH):
             _check(st, self._h)
         return int(n.value)
@@ -490,11 +554,23 @@ class Server:
             pass
 
     def submit(self, op, a, b=None, out=None, out_found=None, count=None):
+        n = _count(a, count)
+        w = self.key_bytes
+        op = int(op)
+        pa = check_buffer(a, "a", w, n)
+        if op == self.INSERT:
+            pb = check_buffer(b, "b (values)", w, n)
+        elif op in (self.RANGE_COUNT, self.RANGE_SUM):
+            pb = check_buffer(b, "b (hi)", w, n)
+        else:
+            pb = check_buffer(b, "b", w, n, optional=True)
+        is_query = op in (self.LOOKUP, self.RANGE_COUNT, self.RANGE_SUM)
+        po = check_buffer(out, "out", w, n if is_query else 0,
+                          optional=not is_query)
+        pf = check_buffer(out_found, "out_found", 1, n, optional=True)
         t = _U64(0)
-        _check(lib().gd_serve_submit(self._h, int(op), _ptr(a), _ptr(b),
-                                     _count(a, count), _ptr(out),
-                                     _ptr(out_found), ctypes.byref(t)),
-               self._h)
+        _check(lib().gd_serve_submit(self._h, op, pa, pb, n, po, pf,
+                                     ctypes.byref(t)), self._h)
         self._keep[t.value] = (a, b, out, out_found)
         return int(t.value)
 
@@ -547,43 +623,64 @@ class FlixIndex:
     def _call(self, name, *args):
         _check(getattr(lib(), name)(self._h, *args), self._h)
 
+    def _k(self, x, name, n, optional=False):
+        return check_buffer(x, name, self.key_bytes, n, optional)
+
+    def _v(self, x, name, n):
+        return check_buffer(x, name, self.value_bytes, n)
+
     def build_bucket_layer_only(self, keys, max_size=0, count=None,
                                 stream=None):
-        self._call("gd_flix_build", _ptr(keys), _count(keys, count),
+        n = _count(keys, count)
+        self._call("gd_flix_build", self._k(keys, "keys", n), n,
                    int(max_size), _stream(stream))
 
     def insert(self, keys, values, count=None, stream=None):
-        self._call("gd_flix_insert", _ptr(keys), _ptr(values),
-                   _count(keys, count), _stream(stream))
+        n = _count(keys, count)
+        self._call("gd_flix_insert", self._k(keys, "keys", n),
+                   self._v(values, "values", n), n, _stream(stream))
 
     def remove(self, keys, count=None, stream=None):
-        self._call("gd_flix_remove", _ptr(keys), _count(keys, count),
+        n = _count(keys, count)
+        self._call("gd_flix_remove", self._k(keys, "keys", n), n,
                    _stream(stream))
 
     def lookups_ordered(self, sorted_keys, out_values, count=None,
                         stream=None):
-        self._call("gd_flix_lookup_sorted", _ptr(sorted_keys),
-                   _count(sorted_keys, count), _ptr(out_values),
-                   _stream(stream))
+        n = _count(sorted_keys, count)
+        self._call("gd_flix_lookup_sorted", self._k(sorted_keys, "keys", n),
+                   n, self._v(out_values, "out_values", n), _stream(stream))
 
     def lookup(self, keys, out_values, count=None, stream=None):
-        self._call("gd_flix_lookup", _ptr(keys), _count(keys, count),
-                   _ptr(out_values), _stream(stream))
+        n = _count(keys, count)
+        self._call("gd_flix_lookup", self._k(keys, "keys", n), n,
+                   self._v(out_values, "out_values", n), _stream(stream))
 
     def lookups_successor(self, sorted_keys, out_keys, count=None,
                           stream=None):
-        self._call("gd_flix_successor_sorted", _ptr(sorted_keys),
-                   _count(sorted_keys, count), _ptr(out_keys),
-                   _stream(stream))
+        n = _count(sorted_keys, count)
+        self._call("gd_flix_successor_sorted",
+                   self._k(sorted_keys, "keys", n), n,
+                   self._k(out_keys, "out_keys", n), _stream(stream))
 
     def rebuild_bucket_layer_only(self, stream=None):
         self._call("gd_flix_rebuild", _stream(stream))
 
     def range(self, lo, hi, out_offsets, out_keys=None, out_values=None,
               count=None, stream=None):
-        self._call("gd_flix_range", _ptr(lo), _ptr(hi), _count(lo, count),
-                   _ptr(out_offsets), _ptr(out_keys), _ptr(out_values),
-                   _stream(stream))
+        """Pass 1 (out_keys None): per-query counts, scanned into
+        out_offsets[0..n] (n + 1 uint64).  Pass 2: the pairs of query i at
+        out_keys/out_values[out_offsets[i]:out_offsets[i + 1]]; both
+        buffers must hold out_offsets[n] entries."""
+        n = _count(lo, count)
+        po = check_buffer(out_offsets, "out_offsets", 8, n + 1)
+        pk = self._k(out_keys, "out_keys", 0, optional=True)
+        pv = check_buffer(out_values, "out_values", self.value_bytes, 0,
+                          optional=True)
+        if (pk is None) != (pv is None):
+            raise _bad("out_keys and out_values go together")
+        self._call("gd_flix_range", self._k(lo, "lo", n), self._k(hi, "hi", n),
+                   n, po, pk, pv, _stream(stream))
 
     def compute_total_size(self, stream=None):
         v = _U64(0)

@@ -72,7 +72,7 @@ class LsmBench:
         keys = np.concatenate([x[0] for x in b])
         vals = np.concatenate([x[1] for x in b])
         erase = np.concatenate([x[2] for x in b])
-        q = w.lookups(keys[erase == 0])
+        q = w.lookups(w.live_keys(b))
         lo, hi = w.ranges()
         up = lambda a, dt=np.int32: torch.from_numpy(  # noqa: E731
             np.ascontiguousarray(a).view(dt)).to(dev)

@@ -318,9 +318,10 @@ class LsmT : public HandleBase {
     if (!path) throw Error(GD_INVALID_ARGUMENT, "null snapshot path");
     order_.begin(s);
     u64 count = 0;
+    // the merged copy goes back to the pool on every exit path
+    Run merged{};
+    bool have = false;
     try {
-      Run merged{};
-      bool have = false;
       if (!runs_.empty()) {
         merged = merge_all(runs_, /*consume=*/false, s);
         have = true;
@@ -349,8 +350,12 @@ class LsmT : public HandleBase {
       if (std::fseek(f.p, 0, SEEK_SET) != 0)
         throw Error(GD_INVALID_ARGUMENT, "snapshot seek failed");
       write_all(f.p, &h, sizeof(h));
+      // a failed final flush (a full disk) must not report success
+      f.close_checked();
       if (have) free_run(merged);
+      have = false;
     } catch (...) {
+      if (have) free_run(merged);
       order_.end(s);
       throw;
     }
@@ -659,6 +664,16 @@ class LsmT : public HandleBase {
   struct FileCloser {
     std::FILE *p;
     explicit FileCloser(std::FILE *f) : p(f) {}
+    // Flushes and closes, throwing when either fails (written files).
+    void close_checked() {
+      std::FILE *f = p;
+      p = nullptr;
+      const bool flushed = std::fflush(f) == 0;
+      const bool closed = std::fclose(f) == 0;
+      if (!flushed || !closed)
+        throw Error(GD_INVALID_ARGUMENT,
+                    "snapshot write failed at flush/close");
+    }
     ~FileCloser() {
       if (p) std::fclose(p);
     }

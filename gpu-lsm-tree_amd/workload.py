@@ -88,7 +88,8 @@ class LsmWorkload:
     first, entry i is a tombstone with probability `erase_fraction` (decided
     by its own stream), and a tombstone erases a key drawn uniformly from the
     keys inserted by earlier batches.  Lookups draw `hit_fraction` of their
-    keys from the inserted keys and the rest uniformly from [0, 2^31).
+    keys from the live keys and the rest uniformly from the absent keys of
+    [0, 2^31).
     Range lower bounds are uniform, widths log-uniform in [wmin, wmax],
     hi = min(lo + w - 1, 2^31 - 1).
     """
@@ -135,14 +136,42 @@ class LsmWorkload:
             parts.append(keys[e == 0])
         return np.concatenate(parts) if parts else np.empty(0, np.uint32)
 
-    def lookups(self, inserted):
+    @staticmethod
+    def live_keys(batches):
+        """Sorted keys live after replaying `batches` ((keys, values,
+        is_erase) in order): the last operation on a key decides, so a key
+        erased and inserted again is live."""
+        if not batches:
+            return np.empty(0, np.uint32)
+        keys = np.concatenate([b[0] for b in batches])
+        erase = np.concatenate([b[2] for b in batches])
+        # last occurrence of each key: first occurrence in reversed order
+        rk = keys[::-1]
+        uniq, first = np.unique(rk, return_index=True)
+        return uniq[erase[::-1][first] == 0]
+
+    def lookups(self, live):
+        """`hit_fraction` of the queries are drawn from `live` (sorted live
+        keys, see live_keys); the rest are uniform draws from [0, 2^31)
+        that are not live (present draws are redrawn from the stream's
+        continuation), so the hit rate is exactly the fraction drawn."""
         n = self.n_lookups
+        live = np.ascontiguousarray(live, dtype=np.uint32)
         r = stream(sub_seed(self.seed, 2), n)
         hit = (r >> np.uint64(32)) < np.uint64(int(self.hit_fraction * 2 ** 32))
-        pick = (r & np.uint64(0xFFFFFFFF)) % np.uint64(max(len(inserted), 1))
+        pick = (r & np.uint64(0xFFFFFFFF)) % np.uint64(max(len(live), 1))
         q = uniform31(sub_seed(self.seed, 3), n)
-        if len(inserted):
-            q[hit] = inserted[pick[hit].astype(np.int64)]
+        if len(live):
+            q[hit] = live[pick[hit].astype(np.int64)]
+            miss = np.nonzero(~hit)[0]
+            extra = n
+            while len(miss):
+                pos = np.searchsorted(live, q[miss])
+                present = (pos < len(live)) & \
+                    (live[np.minimum(pos, len(live) - 1)] == q[miss])
+                miss = miss[present]
+                q[miss] = uniform31(sub_seed(self.seed, 3), len(miss), extra)
+                extra += len(miss)
         return q
 
     def ranges(self):
@@ -165,7 +194,7 @@ def config1(scale_log=0):
 
 def config2(scale_log=0):
     """2^26 keys in batches of 2^20, 10% tombstones of earlier keys;
-    2^26 lookups (70% drawn from inserted keys); 2^20 range counts with
+    2^26 lookups (70% live keys, 30% absent keys); 2^20 range counts with
     widths in [2^4, 2^20]."""
     s = scale_log
     return LsmWorkload(SEEDS[2], 1 << (26 - s), 1 << (20 - s), 1 << (26 - s),
@@ -417,9 +446,9 @@ class FlixWorkload:
     """Config 3 (FliX): a bulk build of `n_build` uint64 keys in clustered
     runs of 2^12 consecutive keys at random 64-bit offsets (kept inside
     [2, 2^64 - 4]), then batches of `batch` mixed ops (75% inserts of new
-    uniform keys, 25% deletes of built keys), each followed by `lookups`
-    lookups of built keys and `ranges` range queries whose width targets
-    16..1024 pairs inside one run."""
+    uniform keys, 25% deletes of live built keys), each followed by
+    `lookups` lookups of live keys and `ranges` range queries whose width
+    targets 16..1024 pairs inside one run."""
 
     def __init__(self, seed, n_build, batch, lookups, ranges):
         self.seed, self.n_build, self.batch = seed, n_build, batch
@@ -450,26 +479,52 @@ class FlixWorkload:
         keys = (starts[:, None] + np.arange(4096, dtype=np.uint64)).ravel()
         return np.sort(keys[: self.n_build])
 
+    def _built_order(self, j):
+        """A bijection of [0, n) for n = len(built keys) = a power of two:
+        j -> (j * 0x9E3779B1 + 0x7F4A7C15) mod n (an odd multiplier), the
+        order in which built keys are deleted, so no key is deleted twice
+        and the built keys at order positions >= the deletions so far are
+        exactly the live ones."""
+        n = np.uint64(self.n_build)
+        with np.errstate(over="ignore"):
+            return ((np.asarray(j, np.uint64) * np.uint64(0x9E3779B1) +
+                     np.uint64(0x7F4A7C15)) % n).astype(np.int64)
+
+    def deletes_per_batch(self):
+        return self.batch - (3 * self.batch) // 4
+
     def batch_ops(self, t, built):
-        """75% inserts of uniform keys over the key range (a uniform 64-bit
-        key hits one of the built keys with probability < 2^-36, and an
-        insert of a present key is a no-op in FliX and in the oracle alike),
-        25% deletes of built keys."""
+        """Batch t: 75% inserts of uniform keys over the key range (a
+        uniform 64-bit key hits one of the built keys with probability
+        < 2^-36, and an insert of a present key is a no-op in FliX and in
+        the oracle alike), 25% deletes of live built keys, never deleted
+        before (the next slice of the deletion order)."""
         n = self.batch
-        x = stream(sub_seed(self.seed, 21), 2 * n, t * 2 * n)
         n_ins = (3 * n) // 4
-        ins = x[:n_ins] % np.uint64(self.KEY_SPAN) + np.uint64(self.KEY_LO)
-        pick = (x[n_ins:n] % np.uint64(len(built))).astype(np.int64)
-        dels = built[pick]
+        nd = n - n_ins
+        x = stream(sub_seed(self.seed, 21), n_ins, t * n_ins)
+        ins = x % np.uint64(self.KEY_SPAN) + np.uint64(self.KEY_LO)
+        dels = built[self._built_order(np.arange(t * nd, (t + 1) * nd))]
         vals = splitmix64(ins)
         return ins, vals, dels
 
     def queries(self, t, built):
+        """After batch t: `lookups` lookups of live built keys and `ranges`
+        range queries [lo, lo + w - 1] with lo a live built key and w
+        log-uniform in [16, 1024] (inside a run of consecutive keys, so
+        each returns up to w pairs)."""
         x = stream(sub_seed(self.seed, 22), self.lookups + 2 * self.ranges,
                    t * (self.lookups + 2 * self.ranges))
-        q = built[(x[: self.lookups] % np.uint64(len(built))).astype(np.int64)]
+        gone = np.uint64(min((t + 1) * self.deletes_per_batch(),
+                             self.n_build - 1))
+        left = np.uint64(self.n_build) - gone
+
+        def live(r):
+            return built[self._built_order(gone + r % left)]
+
+        q = live(x[: self.lookups])
         r = x[self.lookups:]
-        lo = built[(r[: self.ranges] % np.uint64(len(built))).astype(np.int64)]
+        lo = live(r[: self.ranges])
         w = log_uniform(r[self.ranges:], 16, 1024)
         room = np.uint64(self.KEY_LO + self.KEY_SPAN) - lo  # no wrap-around
         hi = lo + np.minimum(w - np.uint64(1), room)

@@ -202,3 +202,94 @@ def test_allocation_region_exhaustion(gd, orc, io):
     # after a rebuild with room, inserts succeed again
     f.rebuild_bucket_layer_only()
     f.insert(io.up(ins[:100]), io.up(ins[:100]), count=100)
+
+
+def _check_queries(p, q, lo, hi):
+    """Lookups (any-order and sorted API), successors of the sorted
+    queries, and range enumeration, all against the oracle."""
+    assert np.array_equal(p.lookup(q), p.o.lookup(q))
+    qs = np.sort(q)
+    assert np.array_equal(p.lookup(qs, True), p.o.lookup(qs))
+    assert np.array_equal(p.successor(qs), p.o.successor(qs))
+    go, gk, gv = p.range(lo, hi)
+    eo, ek, ev = p.o.range(lo, hi)
+    assert np.array_equal(go, eo)
+    assert np.array_equal(gk, ek)
+    assert np.array_equal(gv, ev)
+    return int(eo[-1])
+
+
+@pytest.mark.parametrize("scale_log,batches", [(6, 8)])
+def test_config3_scaled(gd, orc, io, scale_log, batches):
+    """Config 3 (FliX, uint64 keys and values) at 1/64 of its size: a build
+    of 2^21 keys in clustered runs of 2^12 (131,072 buckets: the
+    multi-block work list), then batches of 2^16 mixed ops (75% inserts of
+    new keys, 25% deletes of live keys), each followed by 2^18 lookups,
+    successors and 2^12 range enumerations of 16..1024 pairs, all against
+    the oracle; full contents and total size every other batch."""
+    import importlib
+    wl = importlib.import_module("gdict_amd.workload")
+    w = wl.config3(scale_log)
+    built = w.build_keys()
+    p = FlixPair(gd, orc, io, 8, 8)
+    p.build(built, max_size=len(built) + batches * w.batch)
+    assert p.f.compute_total_size() == len(built)
+    pairs = 0
+    for t in range(batches):
+        ins, vals, dels = w.batch_ops(t, built)
+        p.insert(ins, vals)
+        p.remove(dels)
+        q, lo, hi = w.queries(t, built)
+        pairs += _check_queries(p, q, lo, hi)
+        if t % 2 == 1 or t == batches - 1:
+            ek, ev = p.o.dump()
+            assert p.f.compute_total_size() == len(ek)
+            o, k, v = p.range(np.array([0], np.uint64),
+                              np.array([2**64 - 1], np.uint64))
+            assert np.array_equal(k, ek)
+            assert np.array_equal(v, ev)
+    assert pairs > 16 * w.ranges * batches // 2  # k-pair ranges were served
+
+
+def test_reference_shape_scaled(gd, orc, io):
+    """The shape of FliX's recorded run (FliX/build/sample_output: 32-bit
+    keys, 32-slot nodes at 50%, inserts with 90% of the keys in the lowest
+    quarter of the key range, then deletes of live keys) at 1/32 of its
+    size: many buckets split in every batch, so many waves take nodes of
+    the allocation region at once; contents and answers after every step."""
+    rng = np.random.default_rng(25)
+    top = (1 << 32) - 4
+    p = FlixPair(gd, orc, io, 4, 4, node_slots=32, fill_percent=50)
+    build = np.unique(rng.integers(2, top, 1 << 20, dtype=np.uint64)
+                      .astype(np.uint32))
+    n_step = 419430  # 13,421,772 / 32
+    p.build(build, max_size=len(build) + 5 * n_step)
+
+    def check():
+        ek, ev = p.o.dump()
+        o, k, v = p.range(np.array([0], np.uint32),
+                          np.array([0xFFFFFFFF], np.uint32))
+        assert np.array_equal(k, ek) and np.array_equal(v, ev)
+        assert p.f.compute_total_size() == len(ek)
+        q = ek[rng.integers(0, len(ek), 1 << 16)]
+        q[::3] = rng.integers(2, top, len(q[::3]), dtype=np.uint64)
+        lo = np.sort(rng.integers(2, top, 2000, dtype=np.uint64)
+                     .astype(np.uint32))
+        hi = np.minimum(lo.astype(np.uint64) + rng.integers(
+            0, 1 << 16, 2000, dtype=np.uint64), top).astype(np.uint32)
+        _check_queries(p, q, lo, hi)
+
+    for step in range(5):
+        dense = rng.random(n_step) < 0.9
+        keys = np.where(dense,
+                        rng.integers(2, top // 4, n_step, dtype=np.uint64),
+                        rng.integers(top // 4, top, n_step, dtype=np.uint64)
+                        ).astype(np.uint32)
+        vals = rng.integers(0, 1 << 32, n_step, dtype=np.uint64).astype(
+            np.uint32)
+        p.insert(keys, vals)
+        check()
+    for step in range(5):
+        live = p.o.dump()[0]
+        p.remove(np.sort(live[rng.choice(len(live), n_step, replace=False)]))
+        check()

@@ -280,3 +280,27 @@ def test_snapshot_round_trip_64(gd, orc, io, tmp_path):
     assert fresh.o.bulk_build(k, v) == 0
     fresh.check_contents()
     fresh.check_lookup(_queries(rng, k, 4000, 0), True)
+
+
+def test_compact_64(gd, orc, io):
+    """compact() for 64-bit keys: one run of exactly the live keys, same
+    answers before and after."""
+    rng = np.random.default_rng(77)
+    p = Pair64(gd, orc, io, max_elements=1 << 19, batch_capacity=1 << 13)
+    for b in range(19):
+        n = int(rng.integers(100, 1 << 13))
+        keys = _keys(rng, n, 1 << 40)
+        vals = rng.integers(0, 2**64, n, dtype=np.uint64, endpoint=False)
+        erase = (rng.random(n) < 0.3).astype(np.uint8)
+        p.mixed(keys, vals, erase)
+    runs, rows = p.d.shape()
+    live = p.o.dump()[0]
+    assert rows > len(live)  # tombstones / shadowed versions held
+    p.d.compact()
+    assert p.d.shape() == (1, len(live))
+    p.check_contents()
+    q = _queries(rng, live, 6000, 1 << 40)
+    p.check_lookup(q, with_found=True)
+    lo, hi = _ranges(rng, 800, 1 << 40, 1 << 34)
+    p.check_ranges(lo, hi)
+    p.check_successor(q)

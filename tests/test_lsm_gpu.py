@@ -351,12 +351,11 @@ def test_config1_plain_lsm(gd, orc, io):
     wl = importlib.import_module("gdict_amd.workload")
     w = wl.config1()
     p = Pair(gd, orc, io, max_elements=w.n_keys, batch_capacity=w.batch)
-    inserted = []
-    for keys, vals, _ in w.batches():
+    batches = list(w.batches())
+    for keys, vals, _ in batches:
         p.insert(keys, vals)
-        inserted.append(keys)
     p.check_contents()
-    q = w.lookups(np.concatenate(inserted))
+    q = w.lookups(w.live_keys(batches))
     p.check_lookup(q, False)
     p.check_lookup(q[:1 << 16], True)
     lo, hi = w.ranges()
@@ -371,17 +370,39 @@ def test_config2_scaled(gd, orc, io):
     wl = importlib.import_module("gdict_amd.workload")
     w = wl.config2(scale_log=6)
     p = Pair(gd, orc, io, max_elements=w.n_keys, batch_capacity=w.batch)
-    inserted = []
-    for t, (keys, vals, erase) in enumerate(w.batches()):
+    batches = list(w.batches())
+    for t, (keys, vals, erase) in enumerate(batches):
         p.mixed(keys, vals, erase)
-        inserted.append(keys[erase == 0])
         if t % 16 == 15:
             p.check_contents()
-    q = w.lookups(np.concatenate(inserted))
+    q = w.lookups(w.live_keys(batches))
     p.check_lookup(q, True)
     lo, hi = w.ranges()
     p.check_ranges(lo, hi)
     p.check_successor(q[:1 << 14])
+
+
+def test_config2_quarter(gd, orc, io):
+    """Config 2 at 1/4 of its size: 2^24 keys in 64 mixed batches of 2^18
+    (10% tombstones of earlier keys), full sorted contents after every 16
+    batches, then all 2^24 lookups (70% live hits, found flags), all 2^18
+    range counts and sums, and 2^20 successors against the oracle."""
+    import importlib
+    wl = importlib.import_module("gdict_amd.workload")
+    w = wl.config2(scale_log=2)
+    p = Pair(gd, orc, io, max_elements=w.n_keys, batch_capacity=w.batch)
+    batches = list(w.batches())
+    for t, (keys, vals, erase) in enumerate(batches):
+        p.mixed(keys, vals, erase)
+        if t % 16 == 15:
+            p.check_contents()
+    live = w.live_keys(batches)
+    assert p.o.size() == len(live)
+    q = w.lookups(live)
+    p.check_lookup(q, True)
+    lo, hi = w.ranges()
+    p.check_ranges(lo, hi)
+    p.check_successor(q[:1 << 20])
 
 
 @pytest.mark.parametrize("knobs", [
@@ -522,6 +543,86 @@ def test_snapshot_save_restore(gd, orc, io, tmp_path):
     small = gd.GPULSMOpt(max_elements=1 << 10, batch_capacity=16)
     with pytest.raises(gd.OutOfMemory):
         small.restore(path)
+
+
+def test_snapshot_write_failures(gd, orc, io, tmp_path):
+    """A snapshot that cannot be written (no such directory, or a device
+    that is always full, where the final flush fails) raises
+    InvalidArgument; the merged copy it made goes back to the pool, so
+    repeated failures do not grow the resident bytes."""
+    rng = np.random.default_rng(41)
+    p = Pair(gd, orc, io, max_elements=1 << 16, batch_capacity=1 << 12)
+    for _ in range(5):
+        keys = rng.integers(0, 1 << 20, 3000, dtype=np.uint64).astype(
+            np.uint32)
+        p.insert(keys, keys)
+    bad = str(tmp_path / "missing" / "x.snap")
+    with pytest.raises(gd.InvalidArgument):
+        p.d.save(bad)
+    held = p.d.gpu_resident_bytes()
+    for _ in range(3):
+        with pytest.raises(gd.InvalidArgument):
+            p.d.save(bad)
+    assert p.d.gpu_resident_bytes() == held
+    import os
+    if os.path.exists("/dev/full"):
+        with pytest.raises(gd.InvalidArgument):
+            p.d.save("/dev/full")
+        assert p.d.gpu_resident_bytes() == held
+    p.check_contents()  # the dictionary is untouched
+
+
+@pytest.mark.parametrize("tombstone_rate", [0.0, 0.3, 1.0])
+def test_compact(gd, orc, io, tombstone_rate):
+    """compact() (LSMu's cleanup, FliX/impl_lsm_tree.cuh:1255-1262) leaves
+    one run holding exactly the live keys (tombstones and shadowed versions
+    gone), with the same answers before and after, and updates after it
+    behave as before."""
+    rng = np.random.default_rng(int(tombstone_rate * 10) + 50)
+    p = Pair(gd, orc, io, max_elements=1 << 20, batch_capacity=1 << 14)
+    for b in range(23):
+        n = int(rng.integers(100, 1 << 14))
+        keys = rng.integers(0, 1 << 18, n, dtype=np.uint64).astype(np.uint32)
+        erase = (rng.random(n) < tombstone_rate).astype(np.uint8)
+        p.mixed(keys, keys * np.uint32(7), erase)
+    runs_before, rows_before = p.d.shape()
+    assert runs_before >= 1
+    live = p.o.size()
+    if tombstone_rate > 0:
+        assert rows_before > live  # tombstones / shadowed versions held
+    q = _queries(rng, p.o.dump()[0], 20000, 1 << 19)
+    lo = rng.integers(0, 1 << 18, 3000, dtype=np.uint64).astype(np.uint32)
+    hi = (lo + rng.integers(0, 1 << 12, 3000, dtype=np.uint64).astype(
+        np.uint32))
+    p.check_contents()
+    p.d.compact()
+    assert p.d.shape() == (1, live)
+    p.check_contents()
+    p.check_lookup(q, True)
+    p.check_lookup(q, False)
+    p.check_ranges(lo, hi)
+    p.check_successor(q[:5000])
+    p.d.compact()  # idempotent
+    assert p.d.shape() == (1, live)
+    for b in range(5):
+        keys = rng.integers(0, 1 << 18, 5000, dtype=np.uint64).astype(
+            np.uint32)
+        erase = (rng.random(5000) < 0.2).astype(np.uint8)
+        p.mixed(keys, keys + np.uint32(b), erase)
+    p.check_contents()
+    p.check_ranges(lo, hi)
+
+
+def test_compact_empty(gd, io):
+    d = gd.GPULSMOpt(max_elements=1 << 12, batch_capacity=64)
+    d.compact()
+    assert d.dump_sorted() == 0
+    k = io.up(np.arange(10, dtype=np.uint32))
+    d.insert(k, k)
+    d.erase(k)
+    d.compact()
+    assert d.dump_sorted() == 0
+    assert d.shape() == (1, 0)
 
 
 @pytest.mark.parametrize("seed", [5, 6])
