@@ -187,6 +187,7 @@ SIGNATURES = {
     "gd_lsm_erase": [_P, _P, _U64, _P],
     "gd_lsm_apply_mixed": [_P, _P, _P, _P, _U64, _P],
     "gd_lsm_lookup": [_P, _P, _U64, _P, _P, _I, _P],
+    "gd_lsm_lookup_noalloc": [_P, _P, _U64, _P, _P, _P],
     "gd_lsm_range_sum": [_P, _P, _P, _U64, _P, _P],
     "gd_lsm_range_count": [_P, _P, _P, _U64, _P, _P],
     "gd_lsm_successor": [_P, _P, _U64, _P, _P],
@@ -448,6 +449,18 @@ class GPULSMOpt:
                        _stream(stream))
         else:
             self._call("lookup", q, n, ov, of, _stream(stream))
+
+    def lookup_noalloc(self, queries, out_values, out_found=None, count=None,
+                       stream=None):
+        """gd_lsm_lookup_noalloc: lookup() without any device allocation
+        (uint32 dictionaries, at most 2^20 queries)."""
+        if self.key_bytes != 4:
+            raise _bad("lookup_noalloc serves uint32 dictionaries")
+        n = _count(queries, count)
+        q = self._keys(queries, "queries", n)
+        ov = self._keys(out_values, "out_values", n)
+        of = check_buffer(out_found, "out_found", 1, n, optional=True)
+        self._call("lookup_noalloc", q, n, ov, of, _stream(stream))
 
     def range(self, lo, hi, out_sums, count=None, stream=None):
         n = _count(lo, count)

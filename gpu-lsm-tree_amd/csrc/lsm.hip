@@ -1,4 +1,5 @@
 // C ABI of the LSM dictionary (gd_lsm_*); the runtime is lsm_runtime.hpp.
+#include <array>
 #include <string>
 
 #include "lsm_runtime.hpp"
@@ -19,6 +20,9 @@ using gdict::lsm::Lsm64;
 
 struct gd_lsm : Lsm {
   using Lsm::Lsm;
+  // This handle's device copy of `scratch` (gd_lsm_lookup_noalloc), looked
+  // up at creation; null until a lookup succeeds.
+  void *noalloc_scratch = nullptr;
 };
 
 struct gd_lsm64 : Lsm64 {
@@ -45,6 +49,39 @@ gd_status dump_sorted(H *h, typename H::K *out_keys, typename H::V *out_values,
   return GD_OK;
 }
 
+// Scratch of gd_lsm_lookup_noalloc: the grouped lookup path's partition
+// and result buffers for up to kNoAllocMaxQueries queries (16.6 MiB at
+// 2^20).  Every GPU the library runs on holds one copy for the life of the
+// process, at its own address; every allocation-free call on that GPU
+// shares it, so such calls that overlap on one GPU race.
+constexpr uint64_t kNoAllocMaxQueries = uint64_t(1) << 20;
+constexpr std::size_t kNoAllocScratchBytes = std::size_t(17) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` per device id.  In host code the array's name
+// is only a stand-in for the symbol; kernels receive this address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+std::array<void *, kMaxDevices> g_scratch_ptr{};
+
+// The address of the current device's copy of `scratch`, looked up once per
+// device (hipGetSymbolAddress with that device current).
+hipError_t current_device_scratch(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -60,10 +97,18 @@ gd_status gd_lsm_create(const gd_config *config, gd_lsm **out) {
     return GD_INVALID_ARGUMENT;
   }
   *out = nullptr;
-  return guarded(nullptr, [&] {
+  gd_status st = guarded(nullptr, [&] {
     (void)Lsm::capacity_for(*config);  // validate before touching the GPU
     *out = new gd_lsm(*config);
   });
+  if (st == GD_OK) {
+    // set-up of the allocation-free lookup on the creating (current)
+    // device; a failure here is reported by that lookup, which retries
+    void *p = nullptr;
+    if (current_device_scratch(&p) == hipSuccess) (*out)->noalloc_scratch = p;
+    else (void)hipGetLastError();
+  }
+  return st;
 }
 
 gd_status gd_lsm_destroy(gd_lsm *h) {
@@ -111,6 +156,27 @@ gd_status gd_lsm_lookup(gd_lsm *h, const uint32_t *queries, uint64_t count,
                         int quotients_grouped, hipStream_t s) {
   (void)quotients_grouped;
   GD_LSM_CALL(h, h->lookup(queries, count, out_values, out_found, s));
+}
+
+gd_status gd_lsm_lookup_noalloc(gd_lsm *h, const uint32_t *queries,
+                                uint64_t count, uint32_t *out_values,
+                                uint8_t *out_found, hipStream_t s) {
+  GD_LSM_CALL(h, {
+    if (count > kNoAllocMaxQueries)
+      throw gdict::Error(
+          GD_INVALID_ARGUMENT,
+          "gd_lsm_lookup_noalloc takes at most " +
+              std::to_string(kNoAllocMaxQueries) + " queries, got " +
+              std::to_string(count));
+    if (!h->noalloc_scratch) {
+      void *p = nullptr;
+      GD_HIP(current_device_scratch(&p));
+      h->noalloc_scratch = p;
+    }
+    h->lookup_in_scratch(queries, count, out_values, out_found,
+                         static_cast<char *>(h->noalloc_scratch),
+                         kNoAllocScratchBytes, s);
+  });
 }
 
 gd_status gd_lsm_range_sum(gd_lsm *h, const uint32_t *lo, const uint32_t *hi,

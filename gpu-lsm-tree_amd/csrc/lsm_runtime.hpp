@@ -199,29 +199,87 @@ class LsmT : public HandleBase {
         // group the queries by their top varying key byte first (one radix
         // pass that also records where each query went), look up in
         // grouped order, then gather the results back to query order
-        const u32 nn = static_cast<u32>(n);
         Lease<K> qk(pool_, n);
         Lease<u32> dest(pool_, n);
         Lease<GroupedResult<T>> res(pool_, n);
         Lease<u32> ctrl(pool_, SpreadLayout<K>(n).words);
-        partition_by_spread<K>(q, nn, qk.get(), nullptr, dest.get(),
-                               rs::PayloadPos{}, ctrl.get(), s,
-                               tuning_.sort_block);
-        ProfRec rec{};
-        if (profiling_) rec = prof_begin(1, s, n);
-        lookup_grouped_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
-            qk.get(), n, res.get(), set);
-        if (profiling_) prof_end(rec, s);
-        GD_LAUNCH_CHECK();
-        lookup_gather_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
-            PlainLookup<T>{q, out_values, out_found}, dest.get(), n,
-            res.get());
+        grouped_lookup(q, n, out_values, out_found,
+                       LookupScratch{qk.get(), dest.get(), res.get(),
+                                     ctrl.get(), 0},
+                       set, s, profiling_);
       } else {
         ProfRec rec{};
         if (profiling_) rec = prof_begin(1, s, n);
         lookup_kernel<T><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
             PlainLookup<T>{q, out_values, out_found}, n, set);
         if (profiling_) prof_end(rec, s);
+      }
+      GD_LAUNCH_CHECK();
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
+    order_.end(s);
+  }
+
+  // Device scratch of a grouped lookup of n queries: the grouped keys, each
+  // query's grouped position, the results in grouped order and the
+  // partition's control words.  carve() lays them out in one caller-owned
+  // buffer (parts 256-byte aligned); bytes = the buffer size they need.
+  struct LookupScratch {
+    K *qk;
+    u32 *dest;
+    GroupedResult<T> *res;
+    u32 *ctrl;
+    u64 bytes;
+
+    static LookupScratch carve(char *base, u64 n) {
+      auto up = [](u64 b) { return (b + 255) & ~u64(255); };
+      LookupScratch L{};
+      u64 off = 0;
+      L.qk = reinterpret_cast<K *>(base + off);
+      off += up(n * sizeof(K));
+      L.dest = reinterpret_cast<u32 *>(base + off);
+      off += up(n * sizeof(u32));
+      L.res = reinterpret_cast<GroupedResult<T> *>(base + off);
+      off += up(n * sizeof(GroupedResult<T>));
+      L.ctrl = reinterpret_cast<u32 *>(base + off);
+      off += up(SpreadLayout<K>(n).words * sizeof(u32));
+      L.bytes = off;
+      return L;
+    }
+  };
+
+  // lookup() with the grouped path's buffers taken from `scratch`
+  // (scratch_bytes long, 256-byte aligned device memory) instead of the
+  // handle's pool; allocates nothing.  Same kernels and results as lookup().
+  void lookup_in_scratch(const K *q, u64 n, V *out_values, u8 *out_found,
+                         char *scratch, u64 scratch_bytes, hipStream_t s) {
+    if (!n) return;
+    if (!q || !out_values)
+      throw Error(GD_INVALID_ARGUMENT, "invalid GPULSMOpt lookup");
+    const bool grouped =
+        n >= grouped_lookup_min() && n <= 0xffffffffull && !runs_.empty();
+    LookupScratch L{};
+    if (grouped) {
+      if (!scratch)
+        throw Error(GD_INVALID_ARGUMENT, "no lookup scratch buffer");
+      L = LookupScratch::carve(scratch, n);
+      if (L.bytes > scratch_bytes)
+        throw Error(GD_INVALID_ARGUMENT,
+                    "a grouped lookup of " + std::to_string(n) +
+                        " queries needs " + std::to_string(L.bytes) +
+                        " scratch bytes, " + std::to_string(scratch_bytes) +
+                        " are available");
+    }
+    order_.begin(s);
+    try {
+      const RunSet<T> set = run_set();
+      if (grouped) {
+        grouped_lookup(q, n, out_values, out_found, L, set, s);
+      } else {
+        lookup_kernel<T><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            PlainLookup<T>{q, out_values, out_found}, n, set);
       }
       GD_LAUNCH_CHECK();
     } catch (...) {
@@ -661,6 +719,26 @@ class LsmT : public HandleBase {
 
  private:
   // ----------------------------------------------------------- helpers
+  // The grouped lookup path: one spread partition of the queries (which
+  // also records each query's grouped position), the lookups in grouped
+  // order, then the gather back to query order.  The kernel profile, when
+  // on, times the lookup kernel (lookup() only: its records use pinned
+  // host memory).
+  void grouped_lookup(const K *q, u64 n, V *out_values, u8 *out_found,
+                      const LookupScratch &L, const RunSet<T> &set,
+                      hipStream_t s, bool profile = false) {
+    partition_by_spread<K>(q, static_cast<u32>(n), L.qk, nullptr, L.dest,
+                           rs::PayloadPos{}, L.ctrl, s, tuning_.sort_block);
+    ProfRec rec{};
+    if (profile) rec = prof_begin(1, s, n);
+    lookup_grouped_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
+        L.qk, n, L.res, set);
+    if (profile) prof_end(rec, s);
+    GD_LAUNCH_CHECK();
+    lookup_gather_kernel<T><<<grouped_lookup_blocks(n), 256, 0, s>>>(
+        PlainLookup<T>{q, out_values, out_found}, L.dest, n, L.res);
+  }
+
   struct FileCloser {
     std::FILE *p;
     explicit FileCloser(std::FILE *f) : p(f) {}

@@ -103,6 +103,18 @@ gd_status gd_lsm_apply_mixed(gd_lsm *h, const uint32_t *keys,
 gd_status gd_lsm_lookup(gd_lsm *h, const uint32_t *queries, uint64_t count,
                         uint32_t *out_values, uint8_t *out_found,
                         int quotients_grouped, hipStream_t stream);
+/* No reference counterpart: gd_lsm_lookup that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as gd_lsm_lookup; the buffers of its grouped path
+ * (query partition and grouped results) come from a library-owned device
+ * array of 17 MiB per GPU, held until the process ends.  At most 2^20
+ * queries per call (GD_INVALID_ARGUMENT above that, nothing launched).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other handles) race, so the caller orders them. */
+gd_status gd_lsm_lookup_noalloc(gd_lsm *h, const uint32_t *queries,
+                                uint64_t count, uint32_t *out_values,
+                                uint8_t *out_found, hipStream_t stream);
 /* range(const DeviceRangeOutputBatch&, stream)  GPULSMOpt.cuh:5858:
  * out_sums[i] = sum of live values with lo[i] <= key <= hi[i], accumulated in
  * 64 bits and truncated to 32; lo > hi gives 0. */
