@@ -92,6 +92,7 @@ class _FlixConfig(ctypes.Structure):
         ("value_bytes", ctypes.c_uint32),
         ("node_slots", ctypes.c_uint32),
         ("fill_percent", ctypes.c_uint32),
+        ("rebuild_chain_nodes", ctypes.c_uint32),
     ]
 
 
@@ -609,13 +610,16 @@ class FlixIndex:
     """Mirror of the FliX updatable index over gd_flix_*."""
 
     def __init__(self, key_bytes=4, value_bytes=4, node_slots=0,
-                 fill_percent=0, tuning=None):
-        """tuning="auto": node_slots / fill_percent from tuned.json."""
+                 fill_percent=0, tuning=None, rebuild_chain_nodes=0):
+        """tuning="auto": node_slots / fill_percent from tuned.json.
+        rebuild_chain_nodes > 0: an insert leaving a chain longer than that
+        rebuilds the bucket layer afterwards."""
         if tuning == "auto":
             t = tuned_config("flix")
             node_slots = node_slots or t.get("node_slots", 0)
             fill_percent = fill_percent or t.get("fill_percent", 0)
-        cfg = _FlixConfig(key_bytes, value_bytes, node_slots, fill_percent)
+        cfg = _FlixConfig(key_bytes, value_bytes, node_slots, fill_percent,
+                          rebuild_chain_nodes)
         h = _P()
         _check(lib().gd_flix_create(ctypes.byref(cfg), ctypes.byref(h)), None)
         self._h = h

@@ -104,6 +104,35 @@ __device__ __forceinline__ u32 block_exclusive_scan(u32 v, u32 *scratch) {
   return scratch[wave] + inclusive - v;
 }
 
+// block_exclusive_scan for u64 values; `scratch` needs THREADS/64 + 1
+// words and holds the total in scratch[THREADS/64] afterwards.
+template <int THREADS>
+__device__ __forceinline__ u64 block_exclusive_scan_u64(u64 v, u64 *scratch) {
+  constexpr int kWaves = THREADS / kWave;
+  const u32 lane = lane_id();
+  const u32 wave = threadIdx.x / kWave;
+  u64 inclusive = v;
+#pragma unroll
+  for (int offset = 1; offset < kWave; offset <<= 1) {
+    const u64 other = __shfl_up(inclusive, offset, kWave);
+    if (lane >= static_cast<u32>(offset)) inclusive += other;
+  }
+  if (lane == kWave - 1) scratch[wave] = inclusive;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 run = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const u64 t = scratch[w];
+      scratch[w] = run;
+      run += t;
+    }
+    scratch[kWaves] = run;
+  }
+  __syncthreads();
+  return scratch[wave] + inclusive - v;
+}
+
 // --------------------------------------------------------------------------
 // Single-pass chained scan ("decoupled look-back") over tiles that take
 // tickets in launch order.  A status word packs

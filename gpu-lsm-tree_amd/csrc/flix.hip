@@ -18,27 +18,17 @@
 namespace gdict {
 namespace flix {
 
+// Ring cursors and the plan of the current insert, read back together.
 struct RingCounters {
   unsigned long long alloc_cursor;
   unsigned long long free_count;
-  unsigned long long first_fail;
-  u32 fail_flag;
-  u32 pad;
+  InsertPlan plan;
 };
 
 __global__ void ring_counters_init_kernel(RingCounters *c, u64 free_count) {
   c->alloc_cursor = 0;
   c->free_count = free_count;
-  c->first_fail = ~0ull;
-  c->fail_flag = 0;
-}
-
-__global__ void ring_repair_kernel(RingCounters *c) {
-  if (c->fail_flag) {
-    if (c->alloc_cursor > c->first_fail) c->alloc_cursor = c->first_fail;
-    c->first_fail = ~0ull;
-    c->fail_flag = 0;
-  }
+  c->plan = InsertPlan{0u, 0u, 0u, 0u, 0u, 0u, 0ull};
 }
 
 __global__ void widen_kernel(const u32 *in, u64 n, u64 *out) {
@@ -66,7 +56,10 @@ class FlixBase {
 template <class K, class V>
 class FlixImpl : public FlixBase {
  public:
-  FlixImpl(u32 slots, u32 fill_percent) : S_(slots), fill_percent_(fill_percent) {}
+  FlixImpl(u32 slots, u32 fill_percent, u32 rebuild_chain_nodes)
+      : S_(slots),
+        fill_percent_(fill_percent),
+        rebuild_chain_nodes_(rebuild_chain_nodes) {}
 
   ~FlixImpl() override {
     try {
@@ -99,6 +92,13 @@ class FlixImpl : public FlixBase {
   }
 
   // ---------------------------------------------------------- updates
+  // Insert-if-absent of a batch (impl_cg_rtx_index_updates.cuh:2195).  The
+  // batch is sorted and reduced to the keys not yet stored; the plan sizes
+  // every touched chain and reserves its new nodes; when the ring cannot
+  // hold them the index is rebuilt with a larger allocation region first,
+  // so a batch is applied whole or (GD_CAPACITY, GD_OUT_OF_MEMORY) not at
+  // all.  With a chain-length threshold set, an insert whose longest chain
+  // exceeds it rebuilds the bucket layer afterwards.
   void insert(const void *keys_v, const void *vals_v, u64 n,
               hipStream_t s) override {
     require_built();
@@ -109,44 +109,18 @@ class FlixImpl : public FlixBase {
     if (n > 0xffffffffull)
       throw Error(GD_INVALID_ARGUMENT, "batch size exceeds 2^32-1");
     order_.begin(s);
-    bool failed = false;
     try {
       Lease<K> bk(pool_, n);
       Lease<V> bv(pool_, n);
       const Pool<K, V> existing = pool_view();
       const u64 m = sort_unique(keys, vals, n, false, &existing, bk.get(),
                                 bv.get(), s);
-      if (m) {
-        Lease<u64> begin(pool_, u64(P_) + 1);
-        Lease<u32> work(pool_, std::min<u64>(P_, m) + 1);
-        const u32 work_items = slices(bk.get(), m, begin.get(), work.get(), s);
-        if (work_items) {
-          insert_kernel<K, V><<<(work_items + 3) / 4, 256, 0, s>>>(
-              pool_view(), ring_view(), bk.get(), bv.get(), begin.get(),
-              work.get(), work_items);
-          GD_LAUNCH_CHECK();
-        }
-        RingCounters rc{};
-        GD_HIP(hipMemcpyAsync(&rc, counters_, sizeof(rc),
-                              hipMemcpyDeviceToHost, s));
-        GD_HIP(hipStreamSynchronize(s));
-        if (rc.fail_flag) {
-          ring_repair_kernel<<<1, 1, 0, s>>>(counters_);
-          GD_LAUNCH_CHECK();
-          failed = true;
-        }
-      }
+      if (m) apply_insert(bk.get(), bv.get(), m, s);
     } catch (...) {
       order_.end(s);
       throw;
     }
     order_.end(s);
-    if (failed)
-      throw Error(GD_CAPACITY,
-                  "FliX allocation region exhausted (" +
-                      std::to_string(ring_cap_) +
-                      " nodes); some keys were not inserted; rebuild with a "
-                      "larger max_size");
   }
 
   void remove(const void *keys_v, u64 n, hipStream_t s) override {
@@ -237,23 +211,7 @@ class FlixImpl : public FlixBase {
     require_built();
     order_.begin(s);
     try {
-      Lease<u64> sizes(pool_, u64(P_) + 1);
-      Lease<u64> offs(pool_, u64(P_) + 1);
-      chain_sizes_kernel<K, V><<<grid_for(P_, 256), 256, 0, s>>>(
-          pool_view(), sizes.get());
-      GD_LAUNCH_CHECK();
-      exclusive_scan(sizes.get(), P_, offs.get(), s);
-      u64 total = 0;
-      GD_HIP(hipMemcpyAsync(&total, offs.get() + P_, sizeof(u64),
-                            hipMemcpyDeviceToHost, s));
-      GD_HIP(hipStreamSynchronize(s));
-      Lease<K> uk(pool_, std::max<u64>(total, 1));
-      Lease<V> uv(pool_, std::max<u64>(total, 1));
-      flatten_kernel<K, V><<<(P_ + 3) / 4, 256, 0, s>>>(
-          pool_view(), offs.get(), uk.get(), uv.get());
-      GD_LAUNCH_CHECK();
-      max_size_ = std::max(max_size_, total);
-      layout(uk.get(), uv.get(), total, s);
+      rebuild_into(0, s);
     } catch (...) {
       order_.end(s);
       throw;
@@ -292,13 +250,12 @@ class FlixImpl : public FlixBase {
 
   Pool<K, V> pool_view() const {
     return Pool<K, V>{node_keys_, node_vals_, node_count_, node_next_,
-                      maxv_, S_, P_, total_nodes_};
+                      maxv_, bsize_, blen_, S_, P_, total_nodes_};
   }
 
   Ring ring_view() const {
     return Ring{ring_ids_, ring_cap_, &counters_->alloc_cursor,
-                &counters_->free_count, &counters_->first_fail,
-                &counters_->fail_flag};
+                &counters_->free_count};
   }
 
   void release_structure() {
@@ -307,6 +264,8 @@ class FlixImpl : public FlixBase {
     pool_.put(node_count_);
     pool_.put(node_next_);
     pool_.put(maxv_);
+    pool_.put(bsize_);
+    pool_.put(blen_);
     pool_.put(ring_ids_);
     pool_.put(counters_);
     node_keys_ = nullptr;
@@ -314,6 +273,8 @@ class FlixImpl : public FlixBase {
     node_count_ = nullptr;
     node_next_ = nullptr;
     maxv_ = nullptr;
+    bsize_ = nullptr;
+    blen_ = nullptr;
     ring_ids_ = nullptr;
     counters_ = nullptr;
   }
@@ -370,6 +331,92 @@ class FlixImpl : public FlixBase {
     GD_LAUNCH_CHECK();
   }
 
+  // rebuild_bucket_layer_only (impl_cg_rtx_index_updates.cuh:1821-1881):
+  // the chains, flattened in key order, are laid out as a fresh bucket
+  // layer.  grow_for > 0 (an insert of that many keys that the ring cannot
+  // hold) sizes the new allocation region for twice the keys after it.
+  void rebuild_into(u64 grow_for, hipStream_t s) {
+    Lease<u64> sizes(pool_, u64(P_) + 1);
+    Lease<u64> offs(pool_, u64(P_) + 1);
+    chain_sizes_kernel<K, V><<<grid_for(P_, 256), 256, 0, s>>>(pool_view(),
+                                                               sizes.get());
+    GD_LAUNCH_CHECK();
+    exclusive_scan(sizes.get(), P_, offs.get(), s);
+    u64 total = 0;
+    GD_HIP(hipMemcpyAsync(&total, offs.get() + P_, sizeof(u64),
+                          hipMemcpyDeviceToHost, s));
+    GD_HIP(hipStreamSynchronize(s));
+    Lease<K> uk(pool_, std::max<u64>(total, 1));
+    Lease<V> uv(pool_, std::max<u64>(total, 1));
+    flatten_kernel<K, V><<<(P_ + 3) / 4, 256, 0, s>>>(pool_view(), offs.get(),
+                                                      uk.get(), uv.get());
+    GD_LAUNCH_CHECK();
+    max_size_ = std::max(max_size_, total);
+    if (grow_for) max_size_ = std::max(max_size_, 2 * (total + grow_for));
+    layout(uk.get(), uv.get(), total, s);
+  }
+
+  // Applies a sorted batch of keys absent from the index (see insert()).
+  void apply_insert(const K *bk, const V *bv, u64 m, hipStream_t s) {
+    RingCounters rc{};
+    for (int attempt = 0;; ++attempt) {
+      Lease<u64> begin(pool_, u64(P_) + 1);
+      Lease<uint2> fast(pool_, std::min<u64>(m, u64(P_)) + 1);
+      Lease<BigItem> big(pool_, std::min<u64>(m, u64(P_)) + 1);
+      bucket_slices_kernel<K><<<grid_for(u64(P_) + 1, 256), 256, 0, s>>>(
+          maxv_, P_, bk, m, begin.get());
+      plan_reset_kernel<<<1, 1, 0, s>>>(&counters_->plan);
+      insert_plan_kernel<<<static_cast<u32>((u64(P_) + kPlanBuckets - 1) /
+                                            kPlanBuckets),
+                           256, 0, s>>>(begin.get(), bsize_, blen_, P_, S_,
+                                        fast.get(), big.get(),
+                                        &counters_->plan);
+      GD_LAUNCH_CHECK();
+      GD_HIP(hipMemcpyAsync(&rc, counters_, sizeof(rc), hipMemcpyDeviceToHost,
+                            s));
+      GD_HIP(hipStreamSynchronize(s));
+      const u64 avail = rc.free_count - rc.alloc_cursor;
+      if (rc.plan.new_nodes <= avail) {
+        launch_apply(bk, bv, begin.get(), fast.get(), big.get(), rc.plan, s);
+        break;
+      }
+      if (attempt > 0)
+        throw Error(GD_CAPACITY,
+                    "FliX allocation region of " + std::to_string(ring_cap_) +
+                        " nodes cannot hold the batch even after a rebuild; "
+                        "no key was inserted");
+      // grow: rebuild with room for twice the keys after this batch
+      rebuild_into(m, s);
+    }
+    if (rebuild_chain_nodes_ && rc.plan.max_nodes > rebuild_chain_nodes_)
+      rebuild_into(0, s);
+  }
+
+  void launch_apply(const K *bk, const V *bv, const u64 *begin,
+                    const uint2 *fast, const BigItem *big,
+                    const InsertPlan &plan, hipStream_t s) {
+    const Pool<K, V> pv = pool_view();
+    const unsigned long long *cursor = &counters_->alloc_cursor;
+    if (plan.n_fast) {
+      insert_fast_kernel<K, V><<<(plan.n_fast + 3) / 4, 256, 0, s>>>(
+          pv, ring_ids_, ring_cap_, cursor, bk, bv, begin, fast, plan.n_fast);
+      GD_LAUNCH_CHECK();
+    }
+    if (plan.n_big) {
+      Lease<u32> ids(pool_, u64(plan.big_nodes) + 1);
+      Lease<u32> at(pool_, u64(plan.big_nodes) + 1);
+      Lease<K> sk(pool_, plan.big_keys + 1);
+      Lease<V> sv(pool_, plan.big_keys + 1);
+      insert_big_kernel<K, V><<<plan.n_big, 256, 0, s>>>(
+          pv, ring_ids_, ring_cap_, cursor, bk, bv, begin, big, ids.get(),
+          at.get(), sk.get(), sv.get());
+      GD_LAUNCH_CHECK();
+    }
+    ring_advance_kernel<<<1, 1, 0, s>>>(&counters_->alloc_cursor,
+                                        &counters_->plan);
+    GD_LAUNCH_CHECK();
+  }
+
   // Bucket slices of a sorted batch + compact list of touched buckets.
   u32 slices(const K *batch, u64 m, u64 *begin, u32 *work, hipStream_t s) {
     bucket_slices_kernel<K><<<grid_for(u64(P_) + 1, 256), 256, 0, s>>>(
@@ -404,6 +451,8 @@ class FlixImpl : public FlixBase {
     u32 *count = pool_.get_n<u32>(total);
     u32 *next = pool_.get_n<u32>(total);
     K *maxv = pool_.get_n<K>(P);
+    u32 *bsize = pool_.get_n<u32>(P);
+    u32 *blen = pool_.get_n<u32>(P);
     u32 *ring = pool_.get_n<u32>(region);
     RingCounters *ctr = pool_.get_n<RingCounters>(1);
     release_structure();
@@ -412,6 +461,8 @@ class FlixImpl : public FlixBase {
     node_count_ = count;
     node_next_ = next;
     maxv_ = maxv;
+    bsize_ = bsize;
+    blen_ = blen;
     ring_ids_ = ring;
     counters_ = ctr;
     P_ = static_cast<u32>(P);
@@ -435,6 +486,7 @@ class FlixImpl : public FlixBase {
   OperationOrder order_;
   u32 S_;
   u32 fill_percent_;
+  u32 rebuild_chain_nodes_;  // rebuild after an insert leaves a longer chain
   u64 max_size_ = 0;
   u32 P_ = 0;
   u64 total_nodes_ = 0;
@@ -444,6 +496,8 @@ class FlixImpl : public FlixBase {
   u32 *node_count_ = nullptr;
   u32 *node_next_ = nullptr;
   K *maxv_ = nullptr;
+  u32 *bsize_ = nullptr;  // live keys of each bucket's chain
+  u32 *blen_ = nullptr;   // nodes of each bucket's chain
   u32 *ring_ids_ = nullptr;
   RingCounters *counters_ = nullptr;
 };
@@ -482,10 +536,15 @@ gd_status gd_flix_create(const gd_flix_config *config, gd_flix **out) {
     using gdict::u32;
     using gdict::u64;
     using gdict::flix::FlixImpl;
-    if (kb == 4 && vb == 4) h->impl.reset(new FlixImpl<u32, u32>(slots, fill));
-    else if (kb == 4) h->impl.reset(new FlixImpl<u32, u64>(slots, fill));
-    else if (vb == 4) h->impl.reset(new FlixImpl<u64, u32>(slots, fill));
-    else h->impl.reset(new FlixImpl<u64, u64>(slots, fill));
+    const uint32_t chain = config->rebuild_chain_nodes;
+    if (kb == 4 && vb == 4)
+      h->impl.reset(new FlixImpl<u32, u32>(slots, fill, chain));
+    else if (kb == 4)
+      h->impl.reset(new FlixImpl<u32, u64>(slots, fill, chain));
+    else if (vb == 4)
+      h->impl.reset(new FlixImpl<u64, u32>(slots, fill, chain));
+    else
+      h->impl.reset(new FlixImpl<u64, u64>(slots, fill, chain));
     *out = h.release();
   });
 }

@@ -8,6 +8,8 @@
 //                     keys in (maxv[b-1], maxv[b]], the last bucket ends at
 //                     the largest key (FliX/impl_cg_rtx_index_updates.cuh:
 //                     208-240, find_maxvalues_kernel)
+//   bsize[b], blen[b] live keys and nodes of bucket b's chain, kept by every
+//                     update so an insert can size its new nodes up front
 // Nodes [0, P) are the bucket heads; nodes [P, total) form the allocation
 // region, handed out through a ring of free node ids (bump order at build,
 // released nodes of deletes appended).  Inside a chain every key of node i
@@ -25,8 +27,6 @@ namespace gdict {
 namespace flix {
 
 constexpr int kMaxSlots = 64;
-constexpr u32 kPieceKeys = 2048;  // batch keys merged into a node per step
-constexpr u32 kMaxOutNodes = 2 * (kPieceKeys + kMaxSlots) / 8 + 2;
 
 template <class K, class V>
 struct Pool {
@@ -35,6 +35,8 @@ struct Pool {
   u32 *count;
   u32 *next;
   const K *maxv;
+  u32 *bsize;
+  u32 *blen;
   u32 S;
   u32 buckets;
   u64 total_nodes;
@@ -47,8 +49,6 @@ struct Ring {
   u64 cap;
   unsigned long long *alloc_cursor;
   unsigned long long *free_count;
-  unsigned long long *first_fail;  // smallest cursor of a failed allocation
-  u32 *fail_flag;
 };
 
 template <class K>
@@ -106,8 +106,11 @@ static __global__ void layout_kernel(Pool<K, V> p, K *maxv_out, const K *keys,
     }
     if (s == 0) {
       const u64 first = u64(b) * fill;
-      p.count[b] = first < n ? static_cast<u32>(end - first) : 0u;
+      const u32 c = first < n ? static_cast<u32>(end - first) : 0u;
+      p.count[b] = c;
       p.next[b] = 0;
+      p.bsize[b] = c;
+      p.blen[b] = 1;
       maxv_out[b] = (b + 1 == p.buckets) ? ~K(0) : keys[end - 1];
     }
   }
@@ -171,123 +174,360 @@ static __global__ void mark_new_kernel(Pool<K, V> p, const K *batch, u64 n,
   }
 }
 
-// Ring allocation of k nodes for one wave (lane 0 calls).  Returns the first
-// ring position or ~0ull on failure.
-__device__ __forceinline__ unsigned long long ring_take(const Ring &r,
-                                                        u32 k) {
-  const unsigned long long old = atomicAdd(r.alloc_cursor, (unsigned long long)k);
-  const unsigned long long avail = *r.free_count;
-  if (old + k > avail) {
-    atomicMin(r.first_fail, old);
-    atomicExch(r.fail_flag, 1u);
-    return ~0ull;
-  }
-  return old;
+// ------------------------------------------------- planned chain inserts
+// An insert sizes every touched bucket's chain before it writes anything:
+// from the bucket's live keys t0 (bsize) and nodes L (blen) and its slice
+// of m new keys, the chain ends with chain_nodes_after(t0 + m, L) nodes.
+// One scan over the touched buckets gives each its run of new nodes in the
+// ring, so the apply passes take nodes without atomics, and a batch the
+// ring cannot hold is refused before any bucket changes.  Every touched
+// chain is rewritten whole, its t rows spread evenly over its old nodes
+// (in chain order) followed by its new ones.
+//
+// Buckets whose chain and slice fit registers and LDS (at most kFastNodes
+// nodes, kFastKeys keys, kFastOut nodes after) take insert_fast_kernel, a
+// wave each; the others take insert_big_kernel, a workgroup each, which
+// stages the chain's rows in global scratch (sized by the same plan).
+constexpr u32 kFastNodes = 16;
+constexpr u32 kFastKeys = 512;
+constexpr u32 kFastOut = 64;
+
+// Nodes of a chain of L nodes after it holds t keys: unchanged while they
+// fit, else half-full nodes (the reference splits a full node into two
+// halves, FliX/tile_utils.cuh:306-391).
+__host__ __device__ __forceinline__ u32 chain_nodes_after(u64 t, u32 L,
+                                                          u32 S) {
+  if (t <= u64(L) * S) return L;
+  const u64 h = S / 2;
+  const u64 n = (t + h - 1) / h;
+  return n > L ? static_cast<u32>(n) : L;
 }
 
-// One wave per bucket with a non-empty slice of the (deduplicated, sorted)
-// insert batch.  Keys of a node's range are merged with its slots; when the
-// result overflows the node it is spread evenly over half-full nodes taken
-// from the ring and linked in order (a node split,
-// FliX/tile_utils.cuh:306-391, here for any number of keys at once).
+// Device counters of one insert's plan (cleared before it).
+struct InsertPlan {
+  u32 n_fast;          // buckets rewritten by insert_fast_kernel
+  u32 n_big;           // buckets rewritten by insert_big_kernel
+  u32 new_nodes;       // ring nodes the insert takes
+  u32 max_nodes;       // longest chain after the insert
+  u32 big_nodes;       // chain nodes of the big buckets (scratch ids)
+  u32 pad;
+  unsigned long long big_keys;  // chain rows of the big buckets (scratch)
+};
+
+// A big bucket's work: its first new node's offset from the ring cursor
+// and its scratch (node ids and row ranks at id_off, rows at key_off).
+struct BigItem {
+  u32 bucket;
+  u32 node_off;
+  u32 id_off;
+  u32 pad;
+  u64 key_off;
+};
+
+// Touched buckets -> fast items (bucket, first new node's offset from the
+// ring cursor) and big items.  A block covers kPlanBuckets consecutive
+// buckets (16 per thread) and reserves its share of every list, of the new
+// nodes and of the scratch with one atomic per counter.
+constexpr u32 kPlanBuckets = 4096;
+static __global__ __launch_bounds__(256) void insert_plan_kernel(
+    const u64 *begin, const u32 *bsize, const u32 *blen, u32 buckets, u32 S,
+    uint2 *fast_items, BigItem *big_items, InsertPlan *plan) {
+  constexpr u32 kPer = kPlanBuckets / 256;
+  constexpr u32 kW = 256 / kWave;
+  __shared__ u32 s_f[kW + 1], s_n[kW + 1], s_b[kW + 1], s_i[kW + 1];
+  __shared__ u64 s_k[kW + 1];
+  __shared__ u32 s_base[4];
+  __shared__ u64 s_kbase;
+  const u64 first = (u64(blockIdx.x) * 256 + threadIdx.x) * kPer;
+  u32 fast_mask = 0, big_mask = 0, nf = 0, nb = 0, nodes = 0, ids = 0;
+  u32 longest = 0;
+  u64 keys = 0;
+#pragma unroll
+  for (u32 j = 0; j < kPer; ++j) {
+    const u64 b = first + j;
+    if (b >= buckets) break;
+    const u64 m = begin[b + 1] - begin[b];
+    if (!m) continue;
+    const u32 L = blen[b];
+    const u32 t0 = bsize[b];
+    const u32 Lp = chain_nodes_after(t0 + m, L, S);
+    nodes += Lp - L;
+    longest = Lp > longest ? Lp : longest;
+    if (L <= kFastNodes && t0 + m <= kFastKeys && Lp <= kFastOut) {
+      fast_mask |= 1u << j;
+      ++nf;
+    } else {
+      big_mask |= 1u << j;
+      ++nb;
+      ids += L;
+      keys += t0;
+    }
+  }
+  const u32 ex_f = block_exclusive_scan<256>(nf, s_f);
+  const u32 ex_n = block_exclusive_scan<256>(nodes, s_n);
+  const u32 ex_b = block_exclusive_scan<256>(nb, s_b);
+  const u32 ex_i = block_exclusive_scan<256>(ids, s_i);
+  const u64 ex_k = block_exclusive_scan_u64<256>(keys, s_k);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const u32 o = __shfl_xor(longest, off, kWave);
+    longest = o > longest ? o : longest;
+  }
+  if (lane_id() == 0 && longest) atomicMax(&plan->max_nodes, longest);
+  if (threadIdx.x == 0) {
+    s_base[0] = s_f[kW] ? atomicAdd(&plan->n_fast, s_f[kW]) : 0;
+    s_base[1] = s_n[kW] ? atomicAdd(&plan->new_nodes, s_n[kW]) : 0;
+    s_base[2] = s_b[kW] ? atomicAdd(&plan->n_big, s_b[kW]) : 0;
+    s_base[3] = s_i[kW] ? atomicAdd(&plan->big_nodes, s_i[kW]) : 0;
+    s_kbase = s_k[kW] ? atomicAdd(&plan->big_keys,
+                                  (unsigned long long)s_k[kW])
+                      : 0;
+  }
+  __syncthreads();
+  u32 of = s_base[0] + ex_f, on = s_base[1] + ex_n, ob = s_base[2] + ex_b;
+  u32 oi = s_base[3] + ex_i;
+  u64 ok = s_kbase + ex_k;
+#pragma unroll
+  for (u32 j = 0; j < kPer; ++j) {
+    const u64 b = first + j;
+    const bool fast = (fast_mask >> j) & 1u, big = (big_mask >> j) & 1u;
+    if (!fast && !big) continue;
+    const u64 m = begin[b + 1] - begin[b];
+    const u32 L = blen[b];
+    const u32 t0 = bsize[b];
+    const u32 add = chain_nodes_after(t0 + m, L, S) - L;
+    if (fast) {
+      fast_items[of++] = make_uint2(static_cast<u32>(b), on);
+    } else {
+      big_items[ob++] = BigItem{static_cast<u32>(b), on, oi, 0u, ok};
+      oi += L;
+      ok += t0;
+    }
+    on += add;
+  }
+}
+
+// position of the rank-th of t rows spread over n nodes (the first t % n
+// nodes hold one row more)
+__device__ __forceinline__ void spread_slot(u32 rank, u32 q, u32 r, u32 &j,
+                                            u32 &slot) {
+  const u32 big = r * (q + 1);
+  if (rank < big) {
+    j = rank / (q + 1);
+    slot = rank - j * (q + 1);
+  } else {
+    const u32 k = rank - big;
+    j = r + k / q;
+    slot = k - (j - r) * q;
+  }
+}
+
+// One wave per fast item: rewrites the bucket's chain with its slice merged
+// in (see the plan above).  The batch holds only keys absent from the
+// index, sorted and unique, so every rank is distinct.
 template <class K, class V>
-static __global__ __launch_bounds__(256) void insert_kernel(
-    Pool<K, V> p, Ring ring, const K *bk, const V *bv, const u64 *begin,
-    const u32 *work, u32 work_count) {
+static __global__ __launch_bounds__(256) void insert_fast_kernel(
+    Pool<K, V> p, const u32 *ring_ids, u64 ring_cap,
+    const unsigned long long *alloc_cursor, const K *bk, const V *bv,
+    const u64 *begin, const uint2 *items, u32 n_items) {
   constexpr int kWavesPerBlock = 4;
-  __shared__ K s_keys[kWavesPerBlock][kMaxSlots];
-  __shared__ u32 s_out[kWavesPerBlock][kMaxOutNodes];
-  __shared__ unsigned long long s_pos[kWavesPerBlock];
+  __shared__ K s_k[kWavesPerBlock][kFastKeys];
+  __shared__ u32 s_seq[kWavesPerBlock][kFastOut];
   const u32 lane = lane_id();
   const u32 w = threadIdx.x / kWave;
   const u32 item = blockIdx.x * kWavesPerBlock + w;
-  if (item >= work_count) return;
-  const u32 b = work[item];
-  u64 pos = begin[b];
-  const u64 end = begin[b + 1];
+  if (item >= n_items) return;
+  const uint2 it = items[item];
+  const u32 b = it.x;
+  const u64 pos = begin[b];
+  const u32 m = static_cast<u32>(begin[b + 1] - pos);
+  const u32 L = p.blen[b];
+  const u32 t0 = p.bsize[b];
+  const u32 t = t0 + m;
   const u32 S = p.S;
-  u32 node = b;
-  while (pos < end) {
-    const u32 nxt = p.next[node];
-    // batch keys below the first key of the next node belong here; when
-    // the slice's first key is not below it, none does (one comparison
-    // instead of a search of the slice)
-    u64 q = end;
-    if (nxt) {
-      const K first_next = p.keys[u64(nxt - 1) * S];
-      if (bk[pos] >= first_next) {
-        node = nxt - 1;
-        continue;
+  const u32 Lp = chain_nodes_after(t, L, S);
+  K *sk = s_k[w];
+  u32 *seq = s_seq[w];
+  // the slice's keys behind the chain's keys: sk = [chain keys | slice]
+  for (u32 j = lane; j < m; j += kWave) sk[t0 + j] = bk[pos + j];
+  // the chain, a node per round trip: its slots stay in registers
+  K ak[kFastNodes];
+  V av[kFastNodes];
+  u32 at[kFastNodes];  // rank of the node's first key among the chain keys
+  u32 cn[kFastNodes];
+  u32 node = b, a_at = 0;
+#pragma unroll
+  for (u32 i = 0; i < kFastNodes; ++i) {
+    cn[i] = 0;
+    at[i] = a_at;
+    if (i < L && node < p.total_nodes) {
+      const u32 c = p.count[node];
+      const u32 nxt = p.next[node];
+      const u64 base = u64(node) * S;
+      if (lane < c) {
+        ak[i] = p.keys[base + lane];
+        av[i] = p.vals[base + lane];
+        if (a_at + lane < t0) sk[a_at + lane] = ak[i];
       }
-      q = lower_bound_g(bk, pos, end, first_next);
+      if (lane == 0) seq[i] = node;
+      cn[i] = c;
+      a_at += c;
+      node = nxt - 1;
     }
-    if (q - pos > kPieceKeys) q = pos + kPieceKeys;
-    const u32 m = static_cast<u32>(q - pos);
-    const u32 c = p.count[node];
-    const u64 base = u64(node) * S;
-    K ak = K(0);
-    V av = V(0);
-    if (lane < c) {
-      ak = p.keys[base + lane];
-      av = p.vals[base + lane];
-      s_keys[w][lane] = ak;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const u32 t = c + m;
-    u32 outs = 1, h = t;
-    if (t > S) {
-      outs = (2 * t + S - 1) / S;  // half-full nodes after the split
-      h = (t + outs - 1) / outs;
-      outs = (t + h - 1) / h;
-    }
-    // take outs-1 nodes from the ring
-    if (lane == 0) s_pos[w] = outs > 1 ? ring_take(ring, outs - 1) : 0ull;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const unsigned long long rp = s_pos[w];
-    if (rp == ~0ull) return;  // pool exhausted: this bucket keeps its state
-    s_out[w][0] = node;
-    for (u32 i = lane; i + 1 < outs; i += kWave)
-      s_out[w][i + 1] = ring.ids[(rp + i) % ring.cap];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // A rows: rank = own index + number of batch keys below it
-    if (lane < c) {
-      const u64 below = lower_bound_g(bk, pos, q, ak) - pos;
-      const u32 o = lane + static_cast<u32>(below);
-      const u64 dst = u64(s_out[w][o / h]) * S + (o % h);
-      p.keys[dst] = ak;
-      p.vals[dst] = av;
-    }
-    // batch rows: rank = own index + number of node keys below it
-    for (u32 j = lane; j < m; j += kWave) {
-      const K key = bk[pos + j];
-      u32 lo = 0, hi = c;
+  }
+  // new nodes reserved by the plan
+  const unsigned long long first_new = *alloc_cursor + it.y;
+  for (u32 j = lane; j + L < Lp; j += kWave)
+    seq[L + j] = ring_ids[(first_new + j) % ring_cap];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const u32 q = t / Lp, r = t % Lp;
+  // chain rows: rank = own rank + slice keys below
+#pragma unroll
+  for (u32 i = 0; i < kFastNodes; ++i) {
+    if (lane < cn[i]) {
+      const K key = ak[i];
+      u32 lo = 0, hi = m;
       while (lo < hi) {
         const u32 mid = (lo + hi) >> 1;
-        if (s_keys[w][mid] < key) lo = mid + 1;
+        if (sk[t0 + mid] < key) lo = mid + 1;
         else hi = mid;
       }
-      const u32 o = j + lo;
-      const u64 dst = u64(s_out[w][o / h]) * S + (o % h);
-      p.keys[dst] = key;
-      p.vals[dst] = bv[pos + j];
+      const u32 rank = at[i] + lane + lo;
+      if (rank < t) {  // always, while bsize matches the chain
+        u32 j, slot;
+        spread_slot(rank, q, r, j, slot);
+        const u64 dst = u64(seq[j]) * S + slot;
+        p.keys[dst] = key;
+        p.vals[dst] = av[i];
+      }
     }
-    // counts and links: node -> new_1 -> ... -> new_{outs-1} -> nxt
-    for (u32 i = lane; i < outs; i += kWave) {
-      const u32 id = s_out[w][i];
-      p.count[id] = (i + 1 < outs) ? h : t - h * (outs - 1);
-      p.next[id] = (i + 1 < outs) ? s_out[w][i + 1] + 1 : nxt;
-    }
-    // make this step's slots, counts and links visible to the next step's
-    // loads, which this wave makes on this CU: a workgroup-scope fence waits
-    // for the stores (the CU's L1 serves its own work-groups coherently).
-    // An agent-scope fence here would write back the L2 once per step.
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    pos = q;
-    // stay on `node`: the loop re-walks the freshly linked nodes
   }
+  // slice rows: rank = own index + chain keys below
+  for (u32 x = lane; x < m; x += kWave) {
+    const K key = sk[t0 + x];
+    u32 lo = 0, hi = t0;
+    while (lo < hi) {
+      const u32 mid = (lo + hi) >> 1;
+      if (sk[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    u32 j, slot;
+    spread_slot(x + lo, q, r, j, slot);
+    const u64 dst = u64(seq[j]) * S + slot;
+    p.keys[dst] = key;
+    p.vals[dst] = bv[pos + x];
+  }
+  // counts and links: chain nodes in order, then the new ones
+  for (u32 j = lane; j < Lp; j += kWave) {
+    const u32 id = seq[j];
+    p.count[id] = q + (j < r ? 1u : 0u);
+    p.next[id] = j + 1 < Lp ? seq[j + 1] + 1 : 0u;
+  }
+  if (lane == 0) {
+    p.bsize[b] = t;
+    p.blen[b] = Lp;
+  }
+}
+
+// One workgroup per big item (a chain or slice too large for a wave's
+// registers and LDS): thread 0 walks the chain, recording its node ids and
+// each node's first row rank in scratch; the waves copy the chain's rows
+// to scratch; then every row takes its merged rank (a binary search in the
+// other sequence) and its place in the rewritten chain, as in
+// insert_fast_kernel.  The pool is only written after every read of it.
+template <class K, class V>
+static __global__ __launch_bounds__(256) void insert_big_kernel(
+    Pool<K, V> p, const u32 *ring_ids, u64 ring_cap,
+    const unsigned long long *alloc_cursor, const K *bk, const V *bv,
+    const u64 *begin, const BigItem *items, u32 *sc_ids, u32 *sc_at,
+    K *sc_keys, V *sc_vals) {
+  constexpr u32 kWaves = 256 / kWave;
+  const BigItem it = items[blockIdx.x];
+  const u32 b = it.bucket;
+  const u64 pos = begin[b];
+  const u32 m = static_cast<u32>(begin[b + 1] - pos);
+  const u32 L = p.blen[b];
+  const u32 t0 = p.bsize[b];
+  const u32 t = t0 + m;
+  const u32 S = p.S;
+  const u32 Lp = chain_nodes_after(t, L, S);
+  u32 *ids = sc_ids + it.id_off;
+  u32 *at = sc_at + it.id_off;
+  K *ak = sc_keys + it.key_off;
+  V *av = sc_vals + it.key_off;
+  if (threadIdx.x == 0) {
+    u32 node = b, a = 0;
+    for (u32 i = 0; i < L; ++i) {
+      // a chain shorter than blen (never, while blen matches the chain)
+      // repeats its last node rather than leaving the pool
+      if (node >= p.total_nodes) node = ids[i - 1];
+      ids[i] = node;
+      at[i] = a;
+      a += p.count[node];
+      node = p.next[node] - 1;
+    }
+  }
+  __syncthreads();
+  const u32 lane = lane_id();
+  for (u32 i = threadIdx.x / kWave; i < L; i += kWaves) {
+    const u32 node = ids[i];
+    const u32 c = p.count[node];
+    const u32 a = at[i];
+    if (lane < c && a + lane < t0) {
+      ak[a + lane] = p.keys[u64(node) * S + lane];
+      av[a + lane] = p.vals[u64(node) * S + lane];
+    }
+  }
+  __syncthreads();
+  const u32 q = t / Lp, r = t % Lp;
+  const unsigned long long first_new = *alloc_cursor + it.node_off;
+  auto node_at = [&](u32 j) -> u32 {
+    return j < L ? ids[j] : ring_ids[(first_new + (j - L)) % ring_cap];
+  };
+  // chain rows: rank = own rank + slice keys below
+  for (u32 i = threadIdx.x; i < t0; i += 256) {
+    const K key = ak[i];
+    const u32 below = static_cast<u32>(lower_bound_g(bk, pos, pos + m, key) -
+                                       pos);
+    u32 j, slot;
+    if (i + below >= t) continue;  // never, while bsize matches the chain
+    spread_slot(i + below, q, r, j, slot);
+    const u64 dst = u64(node_at(j)) * S + slot;
+    p.keys[dst] = key;
+    p.vals[dst] = av[i];
+  }
+  // slice rows: rank = own index + chain rows below
+  for (u32 x = threadIdx.x; x < m; x += 256) {
+    const K key = bk[pos + x];
+    const u32 below = static_cast<u32>(lower_bound_g(ak, 0, t0, key));
+    u32 j, slot;
+    spread_slot(x + below, q, r, j, slot);
+    const u64 dst = u64(node_at(j)) * S + slot;
+    p.keys[dst] = key;
+    p.vals[dst] = bv[pos + x];
+  }
+  for (u32 j = threadIdx.x; j < Lp; j += 256) {
+    const u32 id = node_at(j);
+    p.count[id] = q + (j < r ? 1u : 0u);
+    p.next[id] = j + 1 < Lp ? node_at(j + 1) + 1 : 0u;
+  }
+  if (threadIdx.x == 0) {
+    p.bsize[b] = t;
+    p.blen[b] = Lp;
+  }
+}
+
+// Moves the ring cursor past the insert's new nodes (after the apply
+// passes read it).
+static __global__ void ring_advance_kernel(unsigned long long *alloc_cursor,
+                                           const InsertPlan *plan) {
+  *alloc_cursor += plan->new_nodes;
+}
+
+// Clears an insert's plan counters.
+static __global__ void plan_reset_kernel(InsertPlan *plan) {
+  *plan = InsertPlan{0u, 0u, 0u, 0u, 0u, 0u, 0ull};
 }
 
 // One wave per bucket with a non-empty slice of the sorted delete batch.
@@ -306,6 +546,7 @@ static __global__ __launch_bounds__(256) void delete_kernel(
   const u32 S = p.S;
   u32 node = b;
   u32 prev = ~0u;
+  u32 removed = 0, unlinked = 0;  // wave-uniform
   while (true) {
     const u32 nxt = p.next[node];
     // delete keys below the next node's first key (none when the slice's
@@ -339,6 +580,7 @@ static __global__ __launch_bounds__(256) void delete_kernel(
         }
         if (lane == 0) p.count[node] = kept;
       }
+      removed += c - kept;
       emptied = kept == 0 && node != b;
     }
     pos = q;
@@ -350,11 +592,16 @@ static __global__ __launch_bounds__(256) void delete_kernel(
         const unsigned long long slot = atomicAdd(ring.free_count, 1ull);
         ring.ids[slot % ring.cap] = node;
       }
+      ++unlinked;
     } else {
       prev = node;
     }
     if (pos >= end || !nxt) break;
     node = nxt - 1;
+  }
+  if (lane == 0 && (removed || unlinked)) {
+    p.bsize[b] -= removed;
+    p.blen[b] -= unlinked;
   }
 }
 

@@ -329,6 +329,12 @@ typedef struct gd_flix_config {
   uint32_t value_bytes;      /* 4 (smallsize offsets, as the reference) or 8 */
   uint32_t node_slots;       /* pairs per node; 0 -> 32 (NODESIZE=5)       */
   uint32_t fill_percent;     /* initial fill; 0 -> 50                      */
+  uint32_t rebuild_chain_nodes; /* rebuild policy: an insert that leaves a
+                                   bucket chain longer than this many nodes
+                                   rebuilds the bucket layer after it (the
+                                   reference's periodic rebuild,
+                                   FliX/benchmarks_updates.cuh:1919-1935);
+                                   0 = only on demand                   */
 } gd_flix_config;
 
 /* Reserved keys: 0 (empty slot) and 1 (tombstone) (FliX/definitions.cuh:83-103,
@@ -343,7 +349,12 @@ gd_status gd_flix_destroy(gd_flix *h);
 gd_status gd_flix_build(gd_flix *h, const void *keys, uint64_t count,
                         uint64_t max_size, hipStream_t stream);
 /* insert(sorted_keys, offsets, n, stream)  impl:2195.  Insert-if-absent;
- * input need not be sorted. */
+ * input need not be sorted.  A batch is applied whole or not at all: when
+ * the allocation region cannot hold its new nodes the index is first
+ * rebuilt with a larger region; GD_CAPACITY (or GD_OUT_OF_MEMORY when the
+ * larger region cannot be allocated) means no key of the batch was
+ * inserted.  Synchronises the stream (the batch's new-key count and the
+ * node plan are read back). */
 gd_status gd_flix_insert(gd_flix *h, const void *keys, const void *values,
                          uint64_t count, hipStream_t stream);
 /* remove(sorted_keys, n, stream)  impl:2253 */

@@ -176,32 +176,60 @@ def test_clustered_runs_and_skewed_inserts(gd, orc, io):
     p.check_all(rng, 2, 1 << 62)
 
 
-def test_allocation_region_exhaustion(gd, orc, io):
-    """A too-small max_size makes an insert fail with CapacityExhausted;
-    the index stays a consistent sorted set of old and new keys."""
+def test_allocation_region_growth(gd, orc, io):
+    """A batch needing more nodes than a too-small max_size left in the
+    allocation region is not refused and not applied in part: the index
+    rebuilds itself with a larger region first, then takes the whole
+    batch (the oracle's contents after it)."""
     rng = np.random.default_rng(9)
-    f = gd.FlixIndex(key_bytes=4, value_bytes=4, node_slots=16)
+    p = FlixPair(gd, orc, io, 4, 4, node_slots=16)
     keys = np.arange(2, 4002, dtype=np.uint32) * 16
-    f.build_bucket_layer_only(io.up(keys), max_size=4000, count=len(keys))
+    p.build(keys, max_size=4000)
+    before = p.f.gpu_resident_bytes()
     ins = rng.integers(2, 70000, 60000, dtype=np.uint64).astype(np.uint32)
-    with pytest.raises(gd.CapacityExhausted):
-        f.insert(io.up(ins), io.up(ins), count=len(ins))
-    import torch
-    offs = torch.zeros(2, dtype=torch.int64, device=io.device)
-    lo, hi = io.up(np.array([0], np.uint32)), io.up(np.array([0xFFFFFFFF],
-                                                             np.uint32))
-    f.range(lo, hi, offs, count=1)
-    total = int(io.down(offs, 2, np.uint64)[1])
-    ok, ov = io.empty(total), io.empty(total)
-    f.range(lo, hi, offs, ok, ov, count=1)
-    k = io.down(ok, total)
-    assert np.all(np.diff(k.astype(np.int64)) > 0)
-    assert set(keys.tolist()) <= set(k.tolist())
-    assert set(k.tolist()) <= set(keys.tolist()) | set(ins.tolist())
-    assert f.compute_total_size() == total
-    # after a rebuild with room, inserts succeed again
-    f.rebuild_bucket_layer_only()
-    f.insert(io.up(ins[:100]), io.up(ins[:100]), count=100)
+    p.insert(ins, ins)
+    assert p.f.gpu_resident_bytes() > before
+    p.check_all(rng, 2, 70000)
+    p.insert(rng.integers(2, 1 << 20, 200000, dtype=np.uint64)
+             .astype(np.uint32), np.arange(200000, dtype=np.uint32))
+    p.check_all(rng, 2, 1 << 20)
+
+
+@pytest.mark.parametrize("slots", [8, 32])
+def test_long_chains(gd, orc, io, slots):
+    """Small batches aimed at a few buckets grow their chains past what one
+    wave rewrites in registers (16 nodes), so later batches take the
+    workgroup path with small slices, and large batches take it too; the
+    contents and answers stay the oracle's after every batch."""
+    rng = np.random.default_rng(slots)
+    p = FlixPair(gd, orc, io, 4, 4, node_slots=slots, fill_percent=50)
+    keys = np.arange(1, 2001, dtype=np.uint32) * 1000
+    p.build(keys, max_size=1 << 12)
+    for t in range(24):
+        # keys inside the gaps of three buckets' ranges
+        lo = np.array([5000, 900000, 1500000], np.uint64)
+        k = (lo[rng.integers(0, 3, 40 + 30 * t)] +
+             rng.integers(1, 20000, 40 + 30 * t, dtype=np.uint64))
+        k = k.astype(np.uint32)
+        p.insert(k, k ^ np.uint32(0x5A5A5A5A))
+        if t % 6 == 5:
+            p.check_all(rng, 2, 2100000)
+    p.remove(p.o.dump()[0][::3])
+    p.check_all(rng, 2, 2100000)
+
+
+def test_rebuild_policy(gd, orc, io):
+    """rebuild_chain_nodes: inserts that grow a chain past the threshold
+    rebuild the bucket layer; the answers stay the oracle's."""
+    rng = np.random.default_rng(41)
+    p = FlixPair(gd, orc, io, 8, 8, node_slots=16, rebuild_chain_nodes=3)
+    keys = np.unique(rng.integers(2, 1 << 40, 20000, dtype=np.uint64))
+    p.build(keys, max_size=len(keys) + 100000)
+    for t in range(6):
+        ins = rng.integers(2, 1 << 38, 15000, dtype=np.uint64)
+        p.insert(ins, ins + np.uint64(7))
+        p.remove(p.o.dump()[0][t::9])
+        p.check_all(rng, 2, 1 << 40)
 
 
 def _check_queries(p, q, lo, hi):
