@@ -40,19 +40,18 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale-log", type=int, default=0,
                     help="shrink every size by 2^s (quick runs)")
-    ap.add_argument("--cpu-threads", type=int,
-                    default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = every core this process "
+                         "may use: its affinity mask, capped by a cgroup CPU "
+                         "quota)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--requests", type=int, default=2000,
+    ap.add_argument("--requests", type=int, default=10000,
                     help="requests in the serving comparison (0 = skip)")
     ap.add_argument("--flix-batches", type=int, default=4,
                     help="config-3 FliX batches to run (0 = skip)")
     ap.add_argument("--flix-ref", type=int, default=1,
                     help="FliX on the shape of the reference's recorded run "
                     "(1 = run, 0 = skip)")
-    ap.add_argument("--serve-rate-x", type=float, default=2.0,
-                    help="serving arrival rate as a multiple of the plain "
-                         "path's request rate")
     ap.add_argument("--lsm-tuning", default="",
                     help="JSON dict of gd_lsm_tuning knobs for the config-2 "
                          "dictionary (default: the built-in defaults)")
@@ -85,6 +84,38 @@ PMC_KERNELS = (("merge_kernel", "merge_kernel<"),
                ("seg_partition_onesweep_kernel", "SegPayload"),
                ("range_kernel", "range_kernel<"))
 COPY_PROBE_KERNEL = "neg_kernel"
+
+
+def host_cpus():
+    """The host's CPU facts for the baseline: nproc (os.cpu_count), the
+    cores in this process's affinity mask, the cgroup CPU quota in cores
+    (None without one), the cores usable = affinity capped by the quota,
+    and the CPU model name."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota_cores": quota,
+            "usable": usable, "model": model}
 
 
 def progress(msg):
@@ -212,68 +243,79 @@ def pmc_report(args):
         print(text)
 
 
+def poisson_arrivals(wl, n, rate):
+    """Arrival offsets (s) of n requests at `rate` per second: unit
+    exponential gaps from one counter stream, scaled by 1/rate, so every
+    rate replays the same arrival pattern, stretched."""
+    u = (wl.stream(wl.SEEDS[5], n) >> np.uint64(11)).astype(np.float64) \
+        * 2.0 ** -53
+    return np.cumsum(-np.log(np.maximum(u, 1e-300))) / rate
+
+
 def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
     """Plain bulk vs serving on the config-4 request stream over the
-    2^26-key index.  Plain: every request is one synchronous call, back to back
-    (latency = call time, no queueing).  Serving: the same requests arrive
-    as an open-loop Poisson stream at serve_rate_x times the plain request
-    rate (latency = completion - submission, queueing included), then all
-    at once (saturated throughput)."""
-    import threading
-    rs = wl.config4(args.requests).requests()
-    ops = int(sum(len(r[1]) for r in rs))
-
-    def up(a):
-        return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)) \
-            .to(dev)
-
+    config-2 index (2^26 keys), both fed the same Poisson arrivals at a
+    sweep of offered rates.  Plain: one server thread issues each request
+    as one synchronous call, in arrival order, when it has arrived and the
+    previous one is done; latency = completion - arrival (queueing
+    included).  Serving: the same requests are submitted to the engine at
+    their arrival times; latency = completion - submission.  A point is
+    sustained when the achieved request rate is at least 95% of the offered
+    one.  The target comparison: for each sustained plain point, its p99 is
+    the latency bound, and the serving engine's best sustained throughput
+    within that bound is set against the plain throughput at that point."""
+    rs = wl.requests_device(wl.config4(args.requests), torch, dev)
+    ops = int(sum(int(r[1].numel()) for r in rs))
     staged = []
-    for kind, a, b in rs:
-        da = up(a)
-        db = up(b) if b is not None else None
-        out = torch.empty(len(a), dtype=torch.int32, device=dev) \
+    for kind, da, db in rs:
+        out = torch.empty(da.numel(), dtype=torch.int32, device=dev) \
             if kind in ("lookup", "range") else None
-        staged.append((kind, da, db, out, len(a)))
+        staged.append((kind, da, db, out, int(da.numel())))
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream().cuda_stream
+    stream = torch.cuda.current_stream()
+    s_ptr = stream.cuda_stream
 
-    # every engine below starts from the dictionary's current contents
+    # every run starts from the dictionary's current contents
     live = d.dump_sorted()
     live_k = torch.empty(max(live, 1), dtype=torch.int32, device=dev)
     live_v = torch.empty(max(live, 1), dtype=torch.int32, device=dev)
     d.dump_sorted(live_k, live_v, live)
-
-    # ---- plain bulk: one synchronous call per request, in arrival order,
-    # on a dictionary holding the live pairs (room for the requests' inserts)
     cap = w.n_keys * 2
     d.close()
-    d = pkg.GPULSMOpt(max_elements=cap, batch_capacity=w.batch)
-    d.insert(live_k, live_v, count=live, stream=stream)
-    torch.cuda.synchronize()
-    lat = []
-    t0 = time.perf_counter()
-    for kind, da, db, out, n in staged:
-        h0 = time.perf_counter()
-        if kind == "insert":
-            d.insert(da, db, count=n, stream=stream)
-        elif kind == "erase":
-            d.erase(da, count=n, stream=stream)
-        elif kind == "lookup":
-            d.lookup(da, out, count=n, stream=stream)
-        else:
-            d.range_count(da, db, out, count=n, stream=stream)
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - h0) * 1e6)
-    plain_s = time.perf_counter() - t0
-    lat = np.array(lat)
-    plain = {"mops": round(ops / plain_s / 1e6, 3),
-             "requests_per_s": round(len(rs) / plain_s, 1),
-             "p50_us": round(float(np.percentile(lat, 50)), 1),
-             "p99_us": round(float(np.percentile(lat, 99)), 1),
-             "seconds": round(plain_s, 4)}
+    ev = torch.cuda.Event()
 
-    # ---- serving engine holding the same contents (the live pairs of the
-    # plain dictionary before its requests, loaded as one insert request)
+    def run_plain(rate):
+        dd = pkg.GPULSMOpt(max_elements=cap, batch_capacity=w.batch)
+        dd.insert(live_k, live_v, count=live, stream=s_ptr)
+        torch.cuda.synchronize()
+        arr = poisson_arrivals(wl, len(staged), rate) if rate else None
+        lat = np.empty(len(staged))
+        t0 = time.perf_counter()
+        for i, (kind, da, db, out, n) in enumerate(staged):
+            if arr is not None:
+                target = t0 + arr[i]
+                while time.perf_counter() < target:
+                    pass
+                start = target
+            else:
+                start = time.perf_counter()
+            if kind == "insert":
+                dd.insert(da, db, count=n, stream=s_ptr)
+            elif kind == "erase":
+                dd.erase(da, count=n, stream=s_ptr)
+            elif kind == "lookup":
+                dd.lookup(da, out, count=n, stream=s_ptr)
+            else:
+                dd.range_count(da, db, out, count=n, stream=s_ptr)
+            ev.record(stream)
+            while not ev.query():
+                pass
+            lat[i] = (time.perf_counter() - start) * 1e6
+        secs = time.perf_counter() - t0
+        dd.close()
+        return point(rate, secs, np.percentile(lat, 50),
+                     np.percentile(lat, 99), np.percentile(lat, 99.9))
+
     def loaded_server():
         srv = pkg.Server(max_elements=cap, batch_capacity=w.batch,
                          max_inflight_steps=3)
@@ -284,74 +326,98 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
         srv.stats(reset=True)
         return srv
 
-    def submit_one(srv, req):
-        kind, da, db, out, n = req
-        op = {"insert": pkg.Server.INSERT, "erase": pkg.Server.ERASE,
-              "lookup": pkg.Server.LOOKUP,
-              "range": pkg.Server.RANGE_COUNT}[kind]
-        return srv.submit(op, da, db, out, None, count=n)
+    opmap = {"insert": pkg.Server.INSERT, "erase": pkg.Server.ERASE,
+             "lookup": pkg.Server.LOOKUP, "range": pkg.Server.RANGE_COUNT}
 
-    # open loop at serve_rate_x times the plain request rate
-    srv = loaded_server()
-    rate = args.serve_rate_x * len(rs) / plain_s
-    gaps = -np.log(np.maximum((wl.stream(wl.SEEDS[5], len(rs)) >>
-                                np.uint64(11)).astype(np.float64) *
-                               2.0 ** -53, 1e-300)) / rate
-    arrivals = np.cumsum(gaps)
-    lag = 0.0
-    t0 = time.perf_counter()
-    tickets = []
-    for i, req in enumerate(staged):
-        target = t0 + arrivals[i]
-        while True:
-            now = time.perf_counter()
-            if now >= target:
-                break
-        lag = max(lag, now - target)
-        tickets.append(submit_one(srv, req))
-    srv.drain()
-    open_s = time.perf_counter() - t0
-    st = srv.stats()
-    serving = {"offered_requests_per_s": round(rate, 1),
-               "mops": round(ops / open_s / 1e6, 3),
-               "p50_us": round(st["latency_p50_us"], 1),
-               "p99_us": round(st["latency_p99_us"], 1),
-               "p999_us": round(st["latency_p999_us"], 1),
-               "steps": int(st["steps"]),
-               "max_submit_lag_us": round(lag * 1e6, 1),
-               "seconds": round(open_s, 4)}
-    for t in tickets:
-        srv.wait(t)
-    srv.close()
+    def run_serving(rate):
+        srv = loaded_server()
+        arr = poisson_arrivals(wl, len(staged), rate) if rate else None
+        lag = 0.0
+        tickets = []
+        t0 = time.perf_counter()
+        for i, (kind, da, db, out, n) in enumerate(staged):
+            if arr is not None:
+                target = t0 + arr[i]
+                while True:
+                    now = time.perf_counter()
+                    if now >= target:
+                        break
+                lag = max(lag, now - target)
+            tickets.append(srv.submit(opmap[kind], da, db, out, None,
+                                      count=n))
+        srv.drain()
+        secs = time.perf_counter() - t0
+        st = srv.stats()
+        for t in tickets:
+            srv.wait(t)
+        srv.close()
+        r = point(rate, secs, st["latency_p50_us"], st["latency_p99_us"],
+                  st["latency_p999_us"])
+        r["steps"] = int(st["steps"])
+        r["max_submit_lag_us"] = round(lag * 1e6, 1)
+        return r
 
-    # saturated: every request available at once
-    srv = loaded_server()
-    t0 = time.perf_counter()
-    tickets = [submit_one(srv, req) for req in staged]
-    srv.drain()
-    sat_s = time.perf_counter() - t0
-    st = srv.stats()
-    saturated = {"mops": round(ops / sat_s / 1e6, 3),
-                 "steps": int(st["steps"]),
-                 "p99_us": round(st["latency_p99_us"], 1),
-                 "seconds": round(sat_s, 4)}
-    for t in tickets:
-        srv.wait(t)
-    srv.close()
-    d.close()
+    def point(rate, secs, p50, p99, p999):
+        achieved = len(staged) / secs
+        return {"offered_requests_per_s": round(rate, 1) if rate else None,
+                "requests_per_s": round(achieved, 1),
+                "mops": round(ops / secs / 1e6, 3),
+                "sustained": bool(rate is None or achieved >= 0.95 * rate),
+                "p50_us": round(float(p50), 1),
+                "p99_us": round(float(p99), 1),
+                "p999_us": round(float(p999), 1),
+                "seconds": round(secs, 4)}
+
+    plain_sat = run_plain(None)
+    base_rate = plain_sat["requests_per_s"]
+    progress("plain saturated: %.1f Mops/s" % plain_sat["mops"])
+    plain_curve = [run_plain(f * base_rate) for f in (0.5, 0.7, 0.8, 0.9)]
+    progress("plain sweep done")
+    serve_curve = [run_serving(x * base_rate)
+                   for x in (1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0)]
+    serve_sat = run_serving(None)
+    progress("serving sweep done")
+
+    # equal-p99 comparison: each sustained plain point sets the bound
+    at_equal_p99 = []
+    for pp in plain_curve:
+        if not pp["sustained"]:
+            continue
+        ok = [sp for sp in serve_curve
+              if sp["sustained"] and sp["p99_us"] <= pp["p99_us"]]
+        best = max(ok, key=lambda sp: sp["mops"]) if ok else None
+        at_equal_p99.append({
+            "plain_offered_fraction": round(pp["offered_requests_per_s"] /
+                                            base_rate, 2),
+            "p99_bound_us": pp["p99_us"],
+            "plain_mops": pp["mops"],
+            "serving_mops": best["mops"] if best else None,
+            "serving_p99_us": best["p99_us"] if best else None,
+            "throughput_ratio": round(best["mops"] / pp["mops"], 3)
+            if best else None})
+    head = at_equal_p99[-1] if at_equal_p99 else None
     return {
-        "workload": "config4 stream (%d of its 10^4 requests): sizes "
-                    "log-uniform [32, 2^18], 40%% insert / 10%% erase / 40%% "
-                    "lookup / 10%% range count, keys Zipf(0.99) over 2^30 "
-                    "scrambled, applied to the config-2 dictionary (2^26 "
-                    "keys); %d ops" % (len(rs), ops),
-        "plain": plain, "serving_open_loop": serving,
-        "serving_saturated": saturated,
-        "throughput_ratio_open_loop": round(serving["mops"] / plain["mops"],
-                                            3),
-        "throughput_ratio_saturated": round(saturated["mops"] / plain["mops"],
-                                            3),
-        "p99_ratio_open_loop": round(serving["p99_us"] / plain["p99_us"], 3),
+        "workload": "config4 stream (%d requests): sizes log-uniform [32, "
+                    "2^18], 40%% insert / 10%% erase / 40%% lookup / 10%% "
+                    "range count, keys Zipf(0.99) over 2^30 scrambled, "
+                    "applied to the config-2 dictionary (2^26 keys); %d ops; "
+                    "Poisson arrivals, the same pattern at every rate"
+                    % (len(staged), ops),
+        "protocol": "plain = one synchronous call per request in arrival "
+                    "order, latency from arrival (queueing included); "
+                    "serving = requests submitted at arrival; sustained = "
+                    "achieved >= 95% of offered; at_equal_p99 = serving's "
+                    "best sustained throughput with p99 <= the plain "
+                    "point's p99",
+        "plain_saturated": plain_sat,
+        "plain": plain_curve,
+        "serving": serve_curve,
+        "serving_saturated": serve_sat,
+        "at_equal_p99": at_equal_p99,
+        "throughput_ratio_at_equal_p99": head["throughput_ratio"]
+        if head else None,
+        "throughput_ratio_saturated": round(serve_sat["mops"] /
+                                            plain_sat["mops"], 3),
     }
 
 
@@ -881,9 +947,11 @@ def main():
     # ------------------------------------------------------ CPU baseline
     if not args.no_cpu:
         import oracle as orc
-        cpu = orc.CpuLsm(args.cpu_threads)
-        frac = 8  # lookups and ranges are timed on a 1/8 prefix
-        qs, rs = Q // frac, R // frac
+        cpus = host_cpus()
+        threads = args.cpu_threads or cpus["usable"]
+        cpu = orc.CpuLsm(threads)
+        frac = 4  # lookups are timed on a 1/4 prefix
+        qs = Q // frac
         c0 = time.perf_counter()
         for t in range(w.n_batches):
             a = t * w.batch
@@ -892,20 +960,29 @@ def main():
         c1 = time.perf_counter()
         c_out, c_found = cpu.lookup(queries[:qs], with_found=True)
         c2 = time.perf_counter()
-        c_cnt = cpu.range(lo[:rs], hi[:rs], count_only=True)
+        c_cnt = cpu.range(lo, hi, count_only=True)
         c3 = time.perf_counter()
-        cpu_s = (c1 - c0) + (c2 - c1) * Q / qs + (c3 - c2) * R / rs
+        phase_s = {"update": c1 - c0, "lookup": (c2 - c1) * Q / qs,
+                   "range": c3 - c2}
+        cpu_s = sum(phase_s.values())
         g_out = d_out.cpu().numpy().view(np.uint32)
         exact = (np.array_equal(c_out, g_out[:qs]) and
                  np.array_equal(c_found, found[:qs]) and
-                 np.array_equal(c_cnt, counts[:rs].astype(np.uint32)))
+                 np.array_equal(c_cnt, counts.astype(np.uint32)))
         result["cpu_baseline"] = {
             "value": round(n_ops / cpu_s / 1e6, 3), "unit": "Mops/s",
-            "cores": args.cpu_threads, "kind": "port",
-            "sample": "same inputs: all %d updates timed in full, the first "
-                      "%d of %d lookups and %d of %d range counts timed and "
-                      "scaled by their op counts; multithreaded sorted-array "
-                      "LSM (oracle/cpu_lsm.cpp)" % (n_upd, qs, Q, rs, R),
+            "cores": threads, "kind": "port",
+            "sample": "same inputs: all %d updates and all %d range counts "
+                      "timed in full, the first %d of %d lookups timed and "
+                      "scaled by the lookup count; multithreaded sorted-array "
+                      "LSM (oracle/cpu_lsm.cpp): parallel sorts and merges, "
+                      "binary-search lookups, range counts by rank difference "
+                      "over the oldest run" % (n_upd, R, qs, Q),
+            "host": cpus,
+            "phase_s": {k: round(v, 3) for k, v in phase_s.items()},
+            "phase_mops": {"update": round(n_upd / phase_s["update"] / 1e6, 2),
+                           "lookup": round(Q / phase_s["lookup"] / 1e6, 2),
+                           "range": round(R / phase_s["range"] / 1e6, 2)},
             "seconds_measured": round(c3 - c0, 3),
             "seconds_full_step_est": round(cpu_s, 3)}
         result["bit_exact_vs_cpu"] = bool(exact)

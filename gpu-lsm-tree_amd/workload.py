@@ -260,6 +260,90 @@ class Zipf:
         return out
 
 
+def _device_uniform01(torch, x):
+    """u64 bit patterns (int64 tensor) -> uniforms in [0, 1): the top 53
+    bits, as (x >> 11) * 2^-53 does for the numpy streams."""
+    return ((x >> 11) & ((1 << 53) - 1)).to(torch.float64) * 2.0 ** -53
+
+
+def zipf_ranks_device(zipf, torch, seed, count, dev, offset=0,
+                      chunk=1 << 25):
+    """Zipf.ranks on a GPU (torch, float64): the same counter streams and
+    rejection-inversion steps, evaluated by the device's float64 log/exp
+    (which may round differently from the host's in the last place, so a
+    rare sample can land on a neighbouring rank).  For streams too large to
+    draw on the host in seconds; returns an int64 tensor of ranks."""
+    s = zipf.s
+
+    def helper1(x):  # log1p(x) / x
+        small = x.abs() <= 1e-8
+        safe = torch.where(small, torch.ones_like(x), x)
+        return torch.where(small, 1.0 - x * (0.5 - x / 3.0),
+                           torch.log1p(safe) / safe)
+
+    def helper2(x):  # expm1(x) / x
+        small = x.abs() <= 1e-8
+        safe = torch.where(small, torch.ones_like(x), x)
+        return torch.where(small, 1.0 + x * 0.5 * (1.0 + x / 3.0),
+                           torch.expm1(safe) / safe)
+
+    def H(x):
+        lx = torch.log(x)
+        return helper2((1.0 - s) * lx) * lx
+
+    def H_inv(x):
+        t = torch.clamp(x * (1.0 - s), min=-1.0)
+        return torch.exp(helper1(t) * x)
+
+    def h(x):
+        return torch.exp(-s * torch.log(x))
+
+    out = torch.empty(count, dtype=torch.int64, device=dev)
+    for a in range(0, count, chunk):
+        n = min(chunk, count - a)
+        idx = torch.arange(offset + a, offset + a + n, dtype=torch.int64,
+                           device=dev)
+        base = device_splitmix64(idx + _i64(int(seed) % (1 << 64)))
+        res = torch.zeros(n, dtype=torch.int64, device=dev)
+        todo = torch.arange(n, dtype=torch.int64, device=dev)
+        j = 0
+        while todo.numel():
+            u01 = _device_uniform01(torch, device_splitmix64(base[todo] + j))
+            u = zipf.h_n + u01 * (zipf.h_x1 - zipf.h_n)
+            x = H_inv(u)
+            k = torch.clamp(torch.floor(x + 0.5), 1.0, zipf.n)
+            ok = (k - x <= zipf.sv) | (u >= H(k + 0.5) - h(k))
+            res[todo[ok]] = k[ok].to(torch.int64)
+            todo = todo[~ok]
+            j += 1
+        out[a:a + n] = res
+    return out
+
+
+def murmur3_u32_device(torch, k):
+    """murmur3_u32 of uint32 keys held in an int64 tensor (values in
+    [0, 2^32)); returns int64 values in [0, 2^32)."""
+    m = (1 << 32) - 1
+
+    def mul(a, c):
+        return (a * c) & m
+
+    def rotl(x, r):
+        return ((x << r) | (x >> (32 - r))) & m
+    k = mul(k, 0xCC9E2D51)
+    k = rotl(k, 15)
+    k = mul(k, 0x1B873593)
+    h = k  # seed 0
+    h = rotl(h, 13)
+    h = (h * 5 + 0xE6546B64) & m
+    h = h ^ 4
+    h = h ^ (h >> 16)
+    h = mul(h, 0x85EBCA6B)
+    h = h ^ (h >> 13)
+    h = mul(h, 0xC2B2AE35)
+    return h ^ (h >> 16)
+
+
 def scrambled_keys(ranks, bits, c=0x5BD1E995):
     """Rank r -> (r * 0x9E3779B1 + c) mod 2^bits: an odd multiplier makes
     it a bijection of the bits-wide key space (the paper-sweep key map,
@@ -310,6 +394,54 @@ class ZipfRequestStream:
                 out.append(("range", k, hi))
             off += n
         return out
+
+
+def _as_i32(torch, x):
+    """[0, 2^32) values in an int64 tensor -> int32 with the same bits."""
+    return (x - (x >= (1 << 31)).to(torch.int64) * (1 << 32)).to(torch.int32)
+
+
+def requests_device(stream_obj, torch, dev):
+    """ZipfRequestStream.requests generated on a GPU (the keys' Zipf ranks,
+    the range widths and the insert values are drawn there; the request
+    kinds and sizes on the host, identically): a list of (kind, a, b) with
+    int32 device tensors holding the uint32 bits.  Same distributions and
+    the same counter streams as requests(); a rare key can differ from the
+    host draw by one Zipf rank (float64 rounding of the device's log/exp)."""
+    w = stream_obj
+    r = stream(sub_seed(w.seed, 30), w.n)
+    sizes = log_uniform(stream(sub_seed(w.seed, 31), w.n), w.smin,
+                        w.smax).astype(np.int64)
+    u = (r >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    total = int(sizes.sum())
+    ranks = zipf_ranks_device(w.zipf, torch, sub_seed(w.seed, 32), total, dev)
+    mask = (1 << w.bits) - 1
+    keys = (ranks * 0x9E3779B1 + 0x5BD1E995) & mask
+    del ranks
+    idx = torch.arange(total, dtype=torch.int64, device=dev)
+    wu = _device_uniform01(torch, device_splitmix64(
+        idx + _i64(sub_seed(w.seed, 33))))
+    del idx
+    widths = torch.clamp(torch.floor(torch.exp2(4.0 + wu * 12.0)), 16,
+                         1 << 16).to(torch.int64)
+    del wu
+    out = []
+    off = 0
+    for i in range(w.n):
+        n = int(sizes[i])
+        k = keys[off:off + n]
+        if u[i] < 0.4:
+            out.append(("insert", _as_i32(torch, k),
+                        _as_i32(torch, murmur3_u32_device(torch, k))))
+        elif u[i] < 0.5:
+            out.append(("erase", _as_i32(torch, k), None))
+        elif u[i] < 0.9:
+            out.append(("lookup", _as_i32(torch, k), None))
+        else:
+            hi = torch.clamp(k + widths[off:off + n] - 1, max=(1 << 32) - 1)
+            out.append(("range", _as_i32(torch, k), _as_i32(torch, hi)))
+        off += n
+    return out
 
 
 def config4(n_requests=10000, scale_log=0):

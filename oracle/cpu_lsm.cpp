@@ -6,9 +6,11 @@
 // Structure: every batch is sorted (std::sort of (key, position) chunks in
 // parallel, then a parallel merge tree), deduplicated keeping the newest
 // occurrence, and pushed as a run; the two newest runs merge while the older
-// is no larger (a binary counter for equal batches).  Merges split both runs
-// at key boundaries so each thread merges an independent slice.  Lookups and
-// range queries run in parallel over queries and binary-search each run.
+// is no larger in batch rows (a binary counter for equal batches).  Merges
+// split both runs at key boundaries so each thread merges an independent
+// slice.  Lookups and range queries run in parallel over queries and
+// binary-search each run; range counts and sums over a tombstone-free
+// oldest run are rank and prefix-sum differences.
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -26,8 +28,16 @@ struct Run {
   std::vector<u32> keys;
   std::vector<u32> vals;
   std::vector<u8> tomb;
+  // rows of the batches merged into the run (an upper bound on its size):
+  // the merge policy compares these, as the GPU library's does, so equal
+  // batches make an exact binary counter
+  u64 ub = 0;
   std::size_t size() const { return keys.size(); }
 };
+
+bool any_tombstone(const Run &r) {
+  return std::find(r.tomb.begin(), r.tomb.end(), u8(1)) != r.tomb.end();
+}
 
 void parallel_for(unsigned threads, std::size_t n,
                   const std::function<void(std::size_t, std::size_t)> &body) {
@@ -48,6 +58,12 @@ void parallel_for(unsigned threads, std::size_t n,
 struct CpuLsm {
   unsigned threads = 1;
   std::vector<Run> runs;  // newest first
+  // Of the oldest run, when it holds no tombstone: prefix sums of its
+  // values (prefix[i] = sum of vals[0..i)), so a range over it is a rank
+  // difference.  Rebuilt by prepare_ranges() after the oldest run changes.
+  std::vector<u64> prefix;
+  bool prefix_valid = false;
+  bool oldest_plain = false;
 
   // Sort one batch into a run.
   Run make_run(const u32 *keys, const u32 *vals, const u8 *flags, u64 n,
@@ -85,6 +101,7 @@ struct CpuLsm {
       items.swap(tmp);
     }
     Run r;
+    r.ub = n;
     r.keys.reserve(n);
     r.vals.reserve(n);
     r.tomb.reserve(n);
@@ -143,6 +160,7 @@ struct CpuLsm {
       });
     for (auto &t : pool) t.join();
     Run out;
+    out.ub = a.ub + b.ub;
     std::size_t total = 0;
     for (auto &p : pieces) total += p.size();
     out.keys.resize(total);
@@ -160,7 +178,9 @@ struct CpuLsm {
 
   void push(Run r) {
     runs.insert(runs.begin(), std::move(r));
-    while (runs.size() >= 2 && runs[1].size() <= runs[0].size()) {
+    if (runs.size() == 1) prefix_valid = false;
+    while (runs.size() >= 2 && runs[1].ub <= runs[0].ub) {
+      if (runs.size() == 2) prefix_valid = false;  // the oldest is rewritten
       const bool oldest = runs.size() == 2;
       Run m = merge(runs[0], runs[1], oldest);
       runs.erase(runs.begin(), runs.begin() + 2);
@@ -182,8 +202,67 @@ struct CpuLsm {
     return -1;
   }
 
+  // Called before a set of range queries (single-threaded).
+  void prepare_ranges() {
+    if (prefix_valid) return;
+    prefix.clear();
+    oldest_plain = !runs.empty() && !any_tombstone(runs.back());
+    if (oldest_plain) {
+      const Run &o = runs.back();
+      prefix.resize(o.size() + 1);
+      prefix[0] = 0;
+      for (std::size_t i = 0; i < o.size(); ++i)
+        prefix[i + 1] = prefix[i] + o.vals[i];
+    }
+    prefix_valid = true;
+  }
+
+  // Live keys (count) or the sum of their values in [lo, hi].  With a
+  // tombstone-free oldest run the oldest run's part is a rank difference
+  // (count) or a prefix-sum difference (sum), corrected by the newest
+  // versions that newer runs hold of its keys; only the newer runs' rows in
+  // the range are visited.
   u64 range(u32 lo, u32 hi, bool count_only) const {
     if (lo > hi) return 0;
+    if (runs.empty()) return 0;
+    if (!oldest_plain || !prefix_valid) return range_visit(lo, hi, count_only);
+    const std::size_t R = runs.size();
+    std::vector<std::pair<std::size_t, std::size_t>> seg(R);
+    for (std::size_t r = 0; r < R; ++r) {
+      const auto &k = runs[r].keys;
+      seg[r].first = std::size_t(std::lower_bound(k.begin(), k.end(), lo) -
+                                 k.begin());
+      seg[r].second = std::size_t(std::upper_bound(k.begin(), k.end(), hi) -
+                                  k.begin());
+    }
+    const Run &old = runs[R - 1];
+    u64 acc = count_only ? u64(seg[R - 1].second - seg[R - 1].first)
+                         : prefix[seg[R - 1].second] - prefix[seg[R - 1].first];
+    for (std::size_t r = 0; r + 1 < R; ++r) {
+      const Run &run = runs[r];
+      for (std::size_t x = seg[r].first; x < seg[r].second; ++x) {
+        const u32 key = run.keys[x];
+        bool shadowed = false;
+        for (std::size_t s = 0; s < r && !shadowed; ++s) {
+          const auto &k = runs[s].keys;
+          shadowed = std::binary_search(k.begin() + seg[s].first,
+                                        k.begin() + seg[s].second, key);
+        }
+        if (shadowed) continue;  // a newer run holds the newest version
+        auto it = std::lower_bound(old.keys.begin() + seg[R - 1].first,
+                                   old.keys.begin() + seg[R - 1].second, key);
+        if (it != old.keys.begin() + seg[R - 1].second && *it == key)
+          acc -= count_only ? 1u
+                            : old.vals[std::size_t(it - old.keys.begin())];
+        if (!run.tomb[x]) acc += count_only ? 1u : run.vals[x];
+      }
+    }
+    return acc;
+  }
+
+  // Every row in the range of every run, each checked against the newer
+  // runs (any oldest run).
+  u64 range_visit(u32 lo, u32 hi, bool count_only) const {
     std::vector<std::pair<std::size_t, std::size_t>> seg(runs.size());
     for (std::size_t r = 0; r < runs.size(); ++r) {
       const auto &k = runs[r].keys;
@@ -223,7 +302,11 @@ void *cpu_lsm_create(unsigned threads) {
 
 void cpu_lsm_destroy(void *h) { delete static_cast<CpuLsm *>(h); }
 
-void cpu_lsm_clear(void *h) { static_cast<CpuLsm *>(h)->runs.clear(); }
+void cpu_lsm_clear(void *h) {
+  auto *c = static_cast<CpuLsm *>(h);
+  c->runs.clear();
+  c->prefix_valid = false;
+}
 
 // mode: 0 insert, 1 erase, 2 mixed (flags[i] != 0 -> erase)
 void cpu_lsm_apply(void *h, const u32 *keys, const u32 *vals, const u8 *flags,
@@ -252,6 +335,7 @@ void cpu_lsm_lookup(void *h, const u32 *q, u64 n, u32 *out, u8 *found) {
 void cpu_lsm_range(void *h, const u32 *lo, const u32 *hi, u64 n, u32 *out,
                    int count_only) {
   auto *c = static_cast<CpuLsm *>(h);
+  c->prepare_ranges();
   // dynamic chunks: range widths vary by orders of magnitude
   std::atomic<u64> next{0};
   const u64 grain = 256;

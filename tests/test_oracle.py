@@ -364,9 +364,23 @@ def test_flix_oracle_against_dict(orc, kb, vb):
 
 # ------------------------------------------------------- CPU LSM baseline
 def test_cpu_baseline_matches_oracle(orc):
+    """The CPU baseline against the oracle; its ranges are checked after
+    several batches, so both range paths run: the rank/prefix difference
+    over a tombstone-free oldest run, and the row visit when the oldest run
+    still holds tombstones (a single run made from a batch with erases)."""
     rng = np.random.default_rng(77)
     c = orc.CpuLsm(4)
     o = orc.LsmOracle(max_elements=1 << 22, batch_capacity=1 << 14)
+
+    def check_ranges():
+        lo = rng.integers(0, 1 << 18, 1500, dtype=np.uint64).astype(np.uint32)
+        hi = lo + rng.integers(0, 9000, 1500, dtype=np.uint64).astype(
+            np.uint32)
+        lo[:4] = [0, 5, 0, 1 << 17]
+        hi[:4] = [0xFFFFFFFF, 4, 0, 1 << 17]
+        assert np.array_equal(c.range(lo, hi), o.range_sum(lo, hi))
+        assert np.array_equal(c.range(lo, hi, True), o.range_count(lo, hi))
+
     for b in range(24):
         n = int(rng.integers(1, 20000))
         k = rng.integers(0, 1 << 18, n, dtype=np.uint64).astype(np.uint32)
@@ -374,6 +388,8 @@ def test_cpu_baseline_matches_oracle(orc):
         e = (rng.random(n) < 0.2).astype(np.uint8)
         c.apply(k, v, e)
         o.apply_mixed(k, v, e)
+        if b in (0, 1, 2, 5, 11, 17):
+            check_ranges()
     q = rng.integers(0, 1 << 18, 20000, dtype=np.uint64).astype(np.uint32)
     a, fa = c.lookup(q, True)
     b, fb = o.lookup(q, True)
