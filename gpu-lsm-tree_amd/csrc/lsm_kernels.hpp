@@ -1256,10 +1256,16 @@ __host__ __device__ __forceinline__ u32 range_queries_per_wave(u32 runs) {
 //     runs when its filter bit is set;
 // with the work items of all queries of the round flattened over the
 // lanes, so a round costs a few memory round trips however many queries
-// it holds.  When the staged keys of a round exceed kRangeCache, its
-// queries are answered one at a time by the whole wave instead.  Sums
-// accumulate in 64 bits and are truncated to the value width.
-constexpr u32 kRangeCache = 1024;        // staged keys per wave
+// it holds.  The count shortcut replaces the visit of the oldest run's
+// rows (one coalesced load per 64 rows) by a point search per newer row
+// (about three dependent round trips), so a query takes it only when its
+// oldest run holds at least kRangeFastRatio times as many rows in the
+// range as the newer runs.  When the staged keys of a round exceed the
+// stage (kRangeStageBytes per wave), its queries are answered one at a
+// time by the whole wave instead.  Sums accumulate in 64 bits and are
+// truncated to the value width.
+constexpr u32 kRangeStageBytes = 8192;   // staged keys per wave, in bytes
+constexpr u32 kRangeFastRatio = 2;
 constexpr u32 kRangeFilterLog = 13;      // filter bits per wave (log2)
 
 template <class K>
@@ -1272,6 +1278,7 @@ template <class T, class IO>
 __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
                                                               RunSet<T> runs) {
   using K = typename T::K;
+  constexpr u32 kRangeCache = kRangeStageBytes / sizeof(K);
   constexpr u32 kSlots =
       2 * kMaxRuns > kWave ? 2 * kMaxRuns : static_cast<u32>(kWave);
   // s_b[wave][2 * (qi * runs + r) + side]: lower (side 0) and upper (side
@@ -1363,7 +1370,8 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
           u32 newer = 0;
           for (u32 r = first; r < oldest; ++r) newer += bnd[2 * r + 1] - bnd[2 * r];
           const u32 old_rows = bnd[2 * oldest + 1] - bnd[2 * oldest];
-          const bool fast = (flags & 4u) && s_meta[oldest].tombs == 0;
+          const bool fast = (flags & 4u) && s_meta[oldest].tombs == 0 &&
+                            old_rows >= kRangeFastRatio * newer;
           staged = newer;
           work = fast ? newer : newer + old_rows;
           if (fast) base = old_rows;

@@ -557,8 +557,12 @@ class Server : public ServerBase {
 
     // ------------------------------------------------------------ queries
     // the runs after this step's admissions; merges still in flight read
-    // as their inputs
-    const RunSet<T> snap = lsm_.snapshot_for(qs_);
+    // as their inputs.  A large range batch over a stack with a large share
+    // of newer rows first compacts the stack (and waits for it), as the
+    // plain range call does.
+    const bool compact = n_rng && lsm_.range_compaction_pays(n_rng);
+    if (compact) lsm_.compact_async(us_);
+    const RunSet<T> snap = lsm_.snapshot_for(qs_, compact);
     if (n_look) {
       const PieceLookup<T> io{dev_look, static_cast<u32>(look.size())};
       if (n_look >= lsm_.grouped_lookup_min() && snap.count) {

@@ -655,6 +655,36 @@ def test_ranges_over_many_runs(gd, orc, io, seed):
     p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
 
 
+def test_large_range_batch_compacts_first(gd, orc, io):
+    """2^15 ranges over a stack whose newer runs hold more than 1/8 of the
+    oldest run's rows: the range call merges the stack into one run first
+    (tombstones drop); sums, counts, lookups and the contents stay the
+    oracle's, and the stack is one run afterwards."""
+    rng = np.random.default_rng(808)
+    p = Pair(gd, orc, io, tuning={"max_runs": 12},
+             max_elements=1 << 21, batch_capacity=1 << 16)
+    span = 1 << 26
+    for b in range(7):
+        n = 1 << (16 - b)
+        keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+        vals = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        if b % 2 == 1:
+            p.mixed(keys, vals, (rng.random(n) < 0.3).astype(np.uint8))
+        else:
+            p.insert(keys, vals)
+    assert p.d.shape()[0] >= 5
+    n = 1 << 15
+    lo = rng.integers(0, span, n, dtype=np.uint64)
+    hi = np.minimum(lo + rng.integers(0, 1 << 16, n, dtype=np.uint64),
+                    0xFFFFFFFF)
+    lo[:8], hi[:8] = 0, 0xFFFFFFFF
+    lo[8:12], hi[8:12] = 1000, 10
+    p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
+    assert p.d.shape()[0] == 1
+    p.check_lookup(_queries(rng, p.o.dump()[0], 20000, span), True)
+    p.check_contents()
+
+
 @pytest.mark.parametrize("hot_keys", [3, 4000, 20000])
 def test_segmented_admission_oversized_segments(gd, orc, io, hot_keys):
     """Segments beyond the LDS tile: 90% of each batch falls into one

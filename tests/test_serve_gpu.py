@@ -137,6 +137,39 @@ def test_served_results_equal_step_replay(gd, orc, io, seed, span, max_size,
     srv.close()
 
 
+def test_large_range_steps_compact(gd, orc, io):
+    """Steps with at least 2^14 ranges over a stack whose newer runs hold
+    more than 1/8 of the oldest run's rows merge the stack on the merge
+    stream first and the step's queries wait for it; every result still
+    equals the step-order replay and the same request run alone."""
+    rng = np.random.default_rng(77)
+    span = 1 << 26
+    cfg = dict(max_elements=1 << 22, batch_capacity=1 << 16)
+    srv = gd.Server(**cfg, max_inflight_steps=2, max_step_keys=1 << 17)
+
+    def keys(n):
+        return rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+
+    def ranges(n, kind):
+        lo = rng.integers(0, span, n, dtype=np.uint64)
+        hi = np.minimum(lo + rng.integers(0, 1 << 16, n, dtype=np.uint64), NF)
+        return (kind, lo.astype(np.uint32), hi.astype(np.uint32))
+
+    reqs = [("insert", keys(1 << 18), keys(1 << 18))]
+    for t in range(6):
+        reqs.append(("insert", keys(1 << (15 - t)), keys(1 << (15 - t))))
+        reqs.append(("erase", reqs[0][1][rng.integers(0, 1 << 18, 3000)],
+                     None))
+        reqs.append(ranges(1 << 15, "rcount" if t % 2 else "rsum"))
+        reqs.append(("lookup", keys(5000), None))
+    handles = _submit_all(gd, io, srv, reqs)
+    served = []
+    for order, (idx, t, out, found) in enumerate(handles):
+        served.append((idx, order, srv.wait(t), out, found))
+    _check_against_replay(gd, orc, io, reqs, served, cfg)
+    srv.close()
+
+
 def test_concurrent_submitters(gd, orc, io):
     rng = np.random.default_rng(21)
     cfg = dict(max_elements=1 << 20, batch_capacity=1 << 14)
