@@ -144,6 +144,7 @@ class _ServeConfig(ctypes.Structure):
         ("max_inflight_steps", ctypes.c_uint32),
         ("key_bytes", ctypes.c_uint32),
         ("tuning", ctypes.POINTER(_Tuning)),
+        ("step_launch", ctypes.c_uint32),
     ]
 
 
@@ -159,6 +160,7 @@ class _ServeStats(ctypes.Structure):
         ("latency_max_us", ctypes.c_double),
         ("peak_staged_bytes", ctypes.c_uint64),
         ("resident_bytes", ctypes.c_uint64),
+        ("graph_steps", ctypes.c_uint64),
     ]
 
 
@@ -540,7 +542,14 @@ class Server:
 
     def __init__(self, max_elements=0, batch_capacity=0,
                  level_zero_capacity=0, hbm_budget_bytes=0, max_step_keys=0,
-                 max_inflight_steps=0, tuning=None, key_bytes=4):
+                 max_inflight_steps=0, tuning=None, key_bytes=4,
+                 step_launch="kernels"):
+        """step_launch: "kernels" (every step kernel by kernel), "direct"
+        (small steps are fused kernels launched directly) or "graph" (each
+        small step is one captured graph launch)."""
+        modes = {"kernels": 0, "direct": 1, "graph": 2}
+        if step_launch not in modes:
+            raise _bad("step_launch must be one of %s" % sorted(modes))
         if tuning == "auto":
             tuning = tuned_config("lsm")
         self._tuning = _tuning(tuning) if tuning else None
@@ -548,7 +557,8 @@ class Server:
         cfg = _ServeConfig(
             _Config(max_elements, batch_capacity, level_zero_capacity, 0, 0),
             hbm_budget_bytes, max_step_keys, max_inflight_steps, key_bytes,
-            ctypes.pointer(self._tuning) if self._tuning else None)
+            ctypes.pointer(self._tuning) if self._tuning else None,
+            modes[step_launch])
         h = _P()
         _check(lib().gd_serve_create(ctypes.byref(cfg), ctypes.byref(h)),
                None)

@@ -374,13 +374,29 @@ __global__ __launch_bounds__(kFinThreads) void finalize_batch_kernel(
 constexpr int kSmallThreads = 512;
 constexpr int kSmallWaves = kSmallThreads / kWave;
 
+// Where a fused admission reads its batch: entry i's key, value and erase
+// flag.  ArrayBatch reads three arrays; the serving engine's step kernels
+// read the step's request buffers through its piece table instead.
 template <class T>
-__global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
-    const typename T::K *__restrict__ keys, u32 n,
-    const typename T::V *__restrict__ values,
-    const u8 *__restrict__ erase_flags, int mode,
-    typename T::Row *__restrict__ out, RunMeta *meta, u32 *__restrict__ dir,
-    u32 max_bits, u32 rows_log) {
+struct ArrayBatch {
+  const typename T::K *keys;
+  const typename T::V *values;
+  const u8 *erase_flags;
+  __device__ __forceinline__ typename T::K key(u32 i) const { return keys[i]; }
+  __device__ __forceinline__ typename T::V value(u32 i) const {
+    return values[i];
+  }
+  __device__ __forceinline__ bool erased(u32 i) const {
+    return erase_flags[i] != 0;
+  }
+};
+
+// (the body of small_batch_run_kernel; also run by the serving engine's
+// step kernels, which read the arguments from a step descriptor)
+template <class T, class Batch>
+__device__ __forceinline__ void small_batch_run(
+    const Batch &batch, u32 n, int mode, typename T::Row *__restrict__ out,
+    RunMeta *meta, u32 *__restrict__ dir, u32 max_bits, u32 rows_log) {
   using K = typename T::K;
   using V = typename T::V;
   constexpr int kTile = T::kSmallTile;
@@ -401,7 +417,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
   const u32 wave = tid / kWave;
 
   for (u32 i = tid; i < n; i += kSmallThreads) {
-    s_k0[i] = keys[i];
+    s_k0[i] = batch.key(i);
     s_p0[i] = static_cast<unsigned short>(i);
   }
   __syncthreads();
@@ -517,10 +533,10 @@ __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
     const u32 p = src_p[i];
     bool tomb;
     if (mode == kModeErase) tomb = true;
-    else if (mode == kModeMixed) tomb = erase_flags[p] != 0;
+    else if (mode == kModeMixed) tomb = batch.erased(p);
     else tomb = false;
     out[o] = T::make((k & T::kLow) | (tomb ? T::kTomb : K(0)),
-                     tomb ? V(0) : values[p]);
+                     tomb ? V(0) : batch.value(p));
     dst_k[o] = k;
     low += k < T::kTomb;
     tombs += tomb;
@@ -553,6 +569,17 @@ __global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
          ++j)
       dir[j] = t;
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(kSmallThreads) void small_batch_run_kernel(
+    const typename T::K *__restrict__ keys, u32 n,
+    const typename T::V *__restrict__ values,
+    const u8 *__restrict__ erase_flags, int mode,
+    typename T::Row *__restrict__ out, RunMeta *meta, u32 *__restrict__ dir,
+    u32 max_bits, u32 rows_log) {
+  small_batch_run<T>(ArrayBatch<T>{keys, values, erase_flags}, n, mode, out,
+                     meta, dir, max_bits, rows_log);
 }
 
 // Builds the radix directory of a run from its final count/split (one
@@ -1274,9 +1301,15 @@ __device__ __forceinline__ u32 range_filter_bit(K key) {
   if constexpr (sizeof(K) == 8) x ^= static_cast<u32>(key >> 32);
   return (x * 0x9E3779B1u) >> (32 - kRangeFilterLog);
 }
+// The range kernel's work on queries [q_begin, q_end) by the calling
+// workgroup, whose waves are wave_id .. of a grid of wave_stride waves
+// (the range kernel's grid, or the serving engine's step kernel taking
+// chunks of queries).  `io` is already staged; every thread of the
+// workgroup calls it.
 template <class T, class IO>
-__global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
-                                                              RunSet<T> runs) {
+__device__ __forceinline__ void range_block(const IO &io, u64 q_begin,
+                                            u64 q_end, const RunSet<T> &runs,
+                                            u64 wave_id, u64 wave_stride) {
   using K = typename T::K;
   constexpr u32 kRangeCache = kRangeStageBytes / sizeof(K);
   constexpr u32 kSlots =
@@ -1299,15 +1332,15 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
   __shared__ u32 s_stg[kRangeWaves][kQMax];  // newer rows (staged)
   __shared__ u32 s_wrk[kRangeWaves][kQMax];  // work items
   __shared__ unsigned long long s_acc[kRangeWaves][kQMax];
-  __shared__ __align__(16) char s_io[IO::kStageBytes ? IO::kStageBytes : 16];
-  const IO io = io_arg.stage(s_io);
   const u32 nr = runs.count;
+  __syncthreads();  // a previous call's readers of s_meta are done
   if (threadIdx.x < nr) s_meta[threadIdx.x] = *runs.run[threadIdx.x].meta;
   __syncthreads();
   if (nr == 0) {  // no runs: every range is empty
-    const u64 stride = u64(gridDim.x) * blockDim.x;
-    for (u64 q = u64(blockIdx.x) * blockDim.x + threadIdx.x; q < n;
-         q += stride) {
+    const u64 stride = wave_stride * kWave;
+    for (u64 q = q_begin + (wave_id - wave_id % kRangeWaves) * kWave +
+                 threadIdx.x;
+         q < q_end; q += stride) {
       typename IO::Slot slot;
       K a, z;
       bool count_only;
@@ -1321,9 +1354,8 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
   const u32 per = 2 * nr;  // >= 2: the bounds pass writes every query's
                            // flags and slot
   const u32 qw = range_queries_per_wave(nr);
-  const u64 wave_id = u64(blockIdx.x) * kRangeWaves + wave;
-  const u64 wave_stride = u64(gridDim.x) * kRangeWaves;
-  for (u64 q0 = wave_id * qw; q0 < n; q0 += wave_stride * qw) {
+  const u64 n = q_end;
+  for (u64 q0 = q_begin + wave_id * qw; q0 < n; q0 += wave_stride * qw) {
     // every bound of the round at once, one lane each
     for (u32 t = lane; t < qw * per; t += kWave) {
       const u64 q = q0 + t / per;
@@ -1638,6 +1670,16 @@ __global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
     }
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+template <class T, class IO>
+__global__ __launch_bounds__(kRangeThreads) void range_kernel(IO io_arg, u64 n,
+                                                              RunSet<T> runs) {
+  __shared__ __align__(16) char s_io[IO::kStageBytes ? IO::kStageBytes : 16];
+  const IO io = io_arg.stage(s_io);  // range_block's first barrier orders it
+  range_block<T, IO>(io, 0, n, runs,
+                     u64(blockIdx.x) * kRangeWaves + threadIdx.x / kWave,
+                     u64(gridDim.x) * kRangeWaves);
 }
 
 // Smallest live key >= query, all-ones when there is none.

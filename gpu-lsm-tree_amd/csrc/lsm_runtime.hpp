@@ -526,6 +526,62 @@ class LsmT : public HandleBase {
   // (config-4 stream: 2.1x the saturated throughput with waiting).
   static constexpr u64 kQueryWaitRows = u64(1) << 26;
 
+  // ------------------------------------------ serving-engine step graphs
+  // A step whose updates fit one fused small-batch admission runs as a
+  // captured graph: the engine takes the run's buffers here, the graph's
+  // admission kernel fills them, and adopt_run then places the run on top
+  // of the stack exactly as admit_tiles does after sorted_run.
+  u32 small_batch_limit() const { return tuning_.small_batch_max; }
+  u32 dir_rows_log() const { return tuning_.dir_rows_log; }
+  Run alloc_small_run(u32 n) { return alloc_run(n); }
+  void release_run(Run &r) { free_run(r); }
+
+  void adopt_run(const Run &r, hipStream_t s) {
+    runs_.insert(runs_.begin(), r);
+    hipEvent_t admitted;
+    GD_HIP(hipEventCreateWithFlags(&admitted, hipEventDisableTiming));
+    GD_HIP(hipEventRecord(admitted, s));
+    GD_HIP(hipStreamWaitEvent(ms_, admitted, 0));
+    (void)hipEventDestroy(admitted);
+    rebalance(ms_);
+    if (++pending_batches_ == kBatchesPerEpoch) {
+      pending_batches_ = 0;
+      epoch_capacity_check(ms_);
+    }
+  }
+
+  // The runs a step's queries read, without any wait: `newest` (a run the
+  // step's own graph admits, nullable) first, then the stack with pending
+  // merges read as their inputs.  False when they exceed a snapshot.
+  bool snapshot_nowait(RunSet<T> &set, const Run *newest) {
+    set = RunSet<T>{};
+    bool fits = true;
+    auto push = [&](const Run &r) {
+      if (set.count < static_cast<u32>(kMaxRuns)) set.run[set.count] = r.view();
+      else fits = false;
+      ++set.count;
+    };
+    if (newest) push(*newest);
+    std::function<void(const Run &)> add = [&](const Run &r) {
+      auto it = pending_.find(r.rows);
+      if (it != pending_.end()) {
+        const hipError_t q = hipEventQuery(it->second.done);
+        if (q == hipSuccess) {
+          (void)hipEventDestroy(it->second.done);
+          pending_.erase(it);
+        } else {
+          if (q != hipErrorNotReady) GD_HIP(q);
+          const std::vector<Run> in = it->second.in;
+          for (const Run &x : in) add(x);
+          return;
+        }
+      }
+      push(r);
+    };
+    for (const Run &r : runs_) add(r);
+    return fits;
+  }
+
   // Serving mode: merges the whole stack into one run on the merge stream,
   // after the work enqueued on `after` so far (a step's admission), as a
   // merge of the stack's own (readers of older snapshots keep its inputs).

@@ -204,6 +204,139 @@ __global__ void query_gather_kernel(const QueryPiece *pieces_arg, u32 np,
   }
 }
 
+// ------------------------------------------------------- step graphs
+// A small step (updates that fit one fused small-batch admission, lookups
+// below the grouped threshold, ranges that need no compaction, at most
+// kGraphPieces requests of each kind) runs as ONE graph launch: a copy of
+// the step's descriptor (pinned host -> device), the gather of the update
+// pieces, the fused admission, and one persistent query kernel whose
+// workgroups take lookup and range chunks from a work queue (an atomic
+// ticket in the descriptor, zero in every copy).  The graph is captured
+// once per descriptor slot with every kernel reading its sizes and buffers
+// from the slot, so it is replayed as is; no node clears memory.
+constexpr u32 kGraphPieces = 64;
+constexpr u32 kGraphSlots = 8;
+constexpr u32 kGraphQueryBlocks = 256;
+constexpr u32 kStepLookupChunk = 256;   // lookups per work item
+constexpr u32 kStepRangeChunk = 32;     // ranges per work item
+
+template <class T>
+struct StepDesc {
+  using K = typename T::K;
+  using V = typename T::V;
+  using Row = typename T::Row;
+  // updates: pieces -> (keys, vals, flags) -> one run
+  u64 n_upd;
+  u32 np_upd;
+  int mode;
+  Row *run_rows;
+  lsm::RunMeta *run_meta;
+  u32 *run_dir;
+  u32 run_max_bits;
+  u32 rows_log;
+  // queries over `snap`
+  u64 n_look;
+  u64 n_rng;
+  u32 np_look;
+  u32 np_rng;
+  unsigned int ticket;
+  u32 pad;
+  RunSet<T> snap;
+  Piece upd[kGraphPieces];
+  QueryPiece look[kGraphPieces];
+  QueryPiece rng[kGraphPieces];
+};
+
+// A step's updates read through its piece table (entry i of the step's
+// batch is entry i - offset of the piece holding i).
+template <class T>
+struct PieceBatch {
+  const Piece *pieces;
+  u32 np;
+  __device__ __forceinline__ const Piece &at(u32 i, u32 &j) const {
+    const Piece &p = pieces[piece_of(pieces, np, i)];
+    j = static_cast<u32>(i - p.offset);
+    return p;
+  }
+  __device__ __forceinline__ typename T::K key(u32 i) const {
+    u32 j;
+    const Piece &p = at(i, j);
+    return static_cast<const typename T::K *>(p.a)[j];
+  }
+  __device__ __forceinline__ typename T::V value(u32 i) const {
+    u32 j;
+    const Piece &p = at(i, j);
+    return p.b ? static_cast<const typename T::V *>(p.b)[j]
+               : typename T::V(0);
+  }
+  __device__ __forceinline__ bool erased(u32 i) const {
+    u32 j;
+    return at(i, j).flag != 0;
+  }
+};
+
+// The step's fused admission, reading the update requests in place.
+template <class T>
+__global__ __launch_bounds__(lsm::kSmallThreads) void step_admit_kernel(
+    const StepDesc<T> *d) {
+  const u32 n = static_cast<u32>(d->n_upd);
+  if (!n) return;
+  __shared__ Piece s_p[kGraphPieces];
+  const u32 np = d->np_upd;
+  for (u32 i = threadIdx.x; i < np; i += blockDim.x) s_p[i] = d->upd[i];
+  __syncthreads();
+  lsm::small_batch_run<T>(PieceBatch<T>{s_p, np}, n, d->mode, d->run_rows,
+                          d->run_meta, d->run_dir, d->run_max_bits,
+                          d->rows_log);
+}
+
+// The step's lookups and ranges in one launch: a workgroup takes work items
+// (kStepLookupChunk lookups, a thread per lookup, or kStepRangeChunk ranges
+// for the range kernel's wave rounds) from the descriptor's ticket until
+// none is left; workgroups beyond the item count leave at once.
+template <class T>
+__global__ __launch_bounds__(lsm::kRangeThreads) void step_query_kernel(
+    StepDesc<T> *d) {
+  const u64 n_look = d->n_look, n_rng = d->n_rng;
+  const u32 lc = static_cast<u32>((n_look + kStepLookupChunk - 1) /
+                                  kStepLookupChunk);
+  const u32 rc =
+      static_cast<u32>((n_rng + kStepRangeChunk - 1) / kStepRangeChunk);
+  if (blockIdx.x >= lc + rc) return;
+  __shared__ QueryPiece s_look[kGraphPieces];
+  __shared__ QueryPiece s_rng[kGraphPieces];
+  __shared__ u32 s_item;
+  const u32 npl = d->np_look, npr = d->np_rng;
+  for (u32 i = threadIdx.x; i < npl; i += blockDim.x) s_look[i] = d->look[i];
+  for (u32 i = threadIdx.x; i < npr; i += blockDim.x) s_rng[i] = d->rng[i];
+  const RunSet<T> &runs = d->snap;
+  const PieceLookup<T> lio{s_look, npl};
+  const PieceRange<T> rio{s_rng, npr};
+  while (true) {
+    __syncthreads();  // the pieces are staged; s_item is free again
+    if (threadIdx.x == 0) s_item = atomicAdd(&d->ticket, 1u);
+    __syncthreads();
+    const u32 item = s_item;
+    if (item >= lc + rc) break;
+    if (item < lc) {
+      const u64 end = std::min<u64>(n_look, u64(item + 1) * kStepLookupChunk);
+      for (u64 i = u64(item) * kStepLookupChunk + threadIdx.x; i < end;
+           i += blockDim.x) {
+        typename PieceLookup<T>::Slot slot;
+        const typename T::K key = lio.load(i, slot);
+        typename T::V value = 0;
+        const bool live = lsm::lookup_one(runs, key, value);
+        lio.put(slot, live, value);
+      }
+    } else {
+      const u64 r0 = u64(item - lc) * kStepRangeChunk;
+      lsm::range_block<T, PieceRange<T>>(
+          rio, r0, std::min<u64>(n_rng, r0 + kStepRangeChunk), runs,
+          threadIdx.x / kWave, lsm::kRangeWaves);
+    }
+  }
+}
+
 // Pinned host buffers for the per-step piece tables, recycled by size.
 class PinnedCache {
  public:
@@ -290,6 +423,9 @@ class Server : public ServerBase {
     max_step_updates_ = c.max_step_keys ? c.max_step_keys : (u64(1) << 22);
     max_step_queries_ = c.max_step_keys ? c.max_step_keys : (u64(1) << 24);
     budget_ = c.hbm_budget_bytes;
+    if (c.step_launch > 2)
+      throw Error(GD_INVALID_ARGUMENT, "step_launch must be 0, 1 or 2");
+    if (c.step_launch != 0) init_slots(/*capture=*/c.step_launch == 2);
     worker_ = std::thread([this] { run(); });
   }
 
@@ -309,6 +445,7 @@ class Server : public ServerBase {
     }
     for (auto &st : inflight_) release_step(st);
     inflight_.clear();
+    release_graphs();
     (void)hipStreamDestroy(us_);
     (void)hipStreamDestroy(ms_);
     (void)hipStreamDestroy(qs_);
@@ -390,6 +527,7 @@ class Server : public ServerBase {
     out->latency_max_us = l.empty() ? 0.0 : l.back();
     out->peak_staged_bytes = peak_staged_;
     out->resident_bytes = resident_bytes_.load();
+    out->graph_steps = graph_steps_;
   }
 
   void reset_stats() override {
@@ -400,6 +538,7 @@ class Server : public ServerBase {
     coalesced_updates_ = 0;
     coalesced_queries_ = 0;
     peak_staged_ = 0;
+    graph_steps_ = 0;
   }
 
  private:
@@ -475,6 +614,142 @@ class Server : public ServerBase {
     return p;
   }
 
+  // ------------------------------------------------------- step graphs
+  struct GraphSlot {
+    StepDesc<T> *host = nullptr;  // pinned: the step's descriptor
+    StepDesc<T> *dev = nullptr;   // the kernels' copy of it
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;    // the last step using the slot
+    bool used = false;
+  };
+
+  // The three operations of a fused step, on `s`: the descriptor copy, the
+  // admission and the query kernel.
+  void enqueue_fused(const GraphSlot &sl, hipStream_t s) {
+    GD_HIP(hipMemcpyAsync(sl.dev, sl.host, sizeof(StepDesc<T>),
+                          hipMemcpyHostToDevice, s));
+    step_admit_kernel<T><<<1, lsm::kSmallThreads, 0, s>>>(sl.dev);
+    step_query_kernel<T><<<kGraphQueryBlocks, lsm::kRangeThreads, 0, s>>>(
+        sl.dev);
+  }
+
+  // Descriptor slots; in graph mode one captured graph per slot.
+  void init_slots(bool capture) {
+    hipStream_t cs = nullptr;
+    GD_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    try {
+      slots_.resize(kGraphSlots);
+      for (GraphSlot &sl : slots_) {
+        GD_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.host),
+                             sizeof(StepDesc<T>), hipHostMallocDefault));
+        std::memset(static_cast<void *>(sl.host), 0, sizeof(StepDesc<T>));
+        GD_HIP(hipMalloc(reinterpret_cast<void **>(&sl.dev),
+                         sizeof(StepDesc<T>)));
+        GD_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        if (!capture) continue;
+        GD_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        try {
+          enqueue_fused(sl, cs);
+          e = hipGetLastError();
+        } catch (const Error &) {
+          e = hipErrorUnknown;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(cs, &g);
+        GD_HIP(e);
+        GD_HIP(end);
+        e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GD_HIP(e);
+      }
+    } catch (...) {
+      (void)hipStreamDestroy(cs);
+      release_graphs();
+      throw;
+    }
+    (void)hipStreamDestroy(cs);
+  }
+
+  void release_graphs() {
+    for (GraphSlot &sl : slots_) {
+      if (sl.done) {
+        (void)hipEventSynchronize(sl.done);
+        (void)hipEventDestroy(sl.done);
+      }
+      if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+      if (sl.host) (void)hipHostFree(sl.host);
+      (void)hipFree(sl.dev);
+    }
+    slots_.clear();
+  }
+
+  // Runs a small step as a fused step (descriptor copy, admission, query
+  // kernel; one graph launch in graph mode); false (nothing enqueued) when
+  // the step does not fit a slot or the next slot is still in use.
+  bool launch_graph_step(Step &st, const std::vector<Piece> &upd,
+                         const std::vector<QueryPiece> &look,
+                         const std::vector<QueryPiece> &rng, u64 n_upd,
+                         u64 n_look, u64 n_rng, int mode) {
+    if (slots_.empty() || upd.size() > kGraphPieces ||
+        look.size() > kGraphPieces || rng.size() > kGraphPieces)
+      return false;
+    if (n_upd > std::min<u64>(lsm_.small_batch_limit(), T::kSmallTile))
+      return false;
+    if (n_look >= lsm_.grouped_lookup_min()) return false;
+    if (n_upd && lsm_.failed()) return false;  // the regular path reports
+    GraphSlot &sl = slots_[next_slot_];
+    if (sl.used) {
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) return false;
+      GD_HIP(q);
+    }
+    StepDesc<T> &d = *sl.host;
+    typename LsmT<T>::Run run{};
+    if (n_upd) run = lsm_.alloc_small_run(static_cast<u32>(n_upd));
+    if (!lsm_.snapshot_nowait(d.snap, n_upd ? &run : nullptr)) {
+      if (n_upd) lsm_.release_run(run);
+      return false;
+    }
+    d.n_upd = n_upd;
+    d.np_upd = static_cast<u32>(upd.size());
+    d.mode = mode;
+    d.run_rows = run.rows;
+    d.run_meta = n_upd ? run.meta() : nullptr;
+    d.run_dir = n_upd ? run.entries() : nullptr;
+    d.run_max_bits = run.max_bits;
+    d.rows_log = lsm_.dir_rows_log();
+    d.n_look = n_look;
+    d.n_rng = n_rng;
+    d.np_look = static_cast<u32>(look.size());
+    d.np_rng = static_cast<u32>(rng.size());
+    d.ticket = 0;
+    std::copy(upd.begin(), upd.end(), d.upd);
+    std::copy(look.begin(), look.end(), d.look);
+    std::copy(rng.begin(), rng.end(), d.rng);
+    // the step's admission waits for the previous step's queries, as in
+    // the regular path
+    if (!inflight_.empty())
+      GD_HIP(hipStreamWaitEvent(us_, inflight_.back().queries_done, 0));
+    if (sl.exec) GD_HIP(hipGraphLaunch(sl.exec, us_));
+    else enqueue_fused(sl, us_);
+    GD_LAUNCH_CHECK();
+    GD_HIP(hipEventRecord(sl.done, us_));
+    sl.used = true;
+    next_slot_ = (next_slot_ + 1) % kGraphSlots;
+    if (n_upd) lsm_.adopt_run(run, us_);
+    GD_HIP(hipEventCreateWithFlags(&st.updates_done, hipEventDisableTiming));
+    GD_HIP(hipEventRecord(st.updates_done, us_));
+    // query-stream order covers the graph's reads (released runs are kept
+    // until the query stream passes them)
+    GD_HIP(hipStreamWaitEvent(qs_, st.updates_done, 0));
+    GD_HIP(hipEventCreateWithFlags(&st.queries_done, hipEventDisableTiming));
+    GD_HIP(hipEventRecord(st.queries_done, qs_));
+    lsm_.retire_pending();
+    ++graph_steps_;
+    return true;
+  }
+
   void launch_step(std::vector<std::shared_ptr<Request>> reqs, u64 staged) {
     Step st;
     st.id = ++step_counter_;
@@ -499,6 +774,21 @@ class Server : public ServerBase {
                        r->op == GD_OP_RANGE_COUNT ? 1u : 0u, 0});
         n_rng += r->count;
       }
+    }
+    const int step_mode = any_erase ? (any_insert ? lsm::kModeMixed
+                                                  : lsm::kModeErase)
+                                    : lsm::kModeInsert;
+    if (!(n_rng && lsm_.range_compaction_pays(n_rng)) &&
+        launch_graph_step(st, upd, look, rng, n_upd, n_look, n_rng,
+                          step_mode)) {
+      st.reqs = std::move(reqs);
+      std::lock_guard<std::mutex> lk(mu_);
+      coalesced_updates_ += upd.size();
+      coalesced_queries_ += look.size() + rng.size();
+      in_flight_bytes_ += st.staged_bytes;
+      peak_staged_ = std::max(peak_staged_, in_flight_bytes_);
+      inflight_.push_back(std::move(st));
+      return;
     }
     // one pinned table per step: [update pieces | lookups | ranges]
     static_assert(sizeof(Piece) % 8 == 0 && sizeof(QueryPiece) % 8 == 0,
@@ -712,6 +1002,9 @@ class Server : public ServerBase {
   u64 completed_ = 0, steps_done_ = 0;
   u64 coalesced_updates_ = 0, coalesced_queries_ = 0;
   u64 in_flight_bytes_ = 0, peak_staged_ = 0;
+  std::vector<GraphSlot> slots_;
+  u32 next_slot_ = 0;
+  u64 graph_steps_ = 0;
   std::thread worker_;
 };
 

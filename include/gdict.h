@@ -290,6 +290,15 @@ typedef struct gd_serve_config {
   uint32_t key_bytes;          /* 4 (0) or 8: uint32 or uint64 keys and
                                   values (the gd_lsm / gd_lsm64 dictionary) */
   const gd_lsm_tuning *tuning; /* nullable                                 */
+  uint32_t step_launch;         /* 0 = every step kernel by kernel
+                                  (default); 1 = a small step (updates
+                                  that fit one fused admission, lookups
+                                  below the grouped threshold, up to 64
+                                  requests of each kind) is a descriptor
+                                  copy, the fused admission reading the
+                                  requests in place and one persistent
+                                  query kernel; 2 = the same as one
+                                  captured graph launch per step          */
 } gd_serve_config;
 
 typedef struct gd_serve_stats {
@@ -303,6 +312,7 @@ typedef struct gd_serve_stats {
   double latency_max_us;
   uint64_t peak_staged_bytes;
   uint64_t resident_bytes;
+  uint64_t graph_steps;        /* small steps run fused (step_launch 1, 2) */
 } gd_serve_stats;
 
 gd_status gd_serve_create(const gd_serve_config *config, gd_server **out);

@@ -114,17 +114,20 @@ def _check_against_replay(gd, orc, io, reqs, served, cfg):
                 assert np.array_equal(io.down(alone, len(a)), got)
 
 
-@pytest.mark.parametrize("seed,span,max_size,inflight,n_req", [
-    (1, 1 << 12, 300, 2, 300), (2, 1 << 20, 5000, 3, 300),
-    (3, 1 << 32, 20000, 1, 300),
+@pytest.mark.parametrize("seed,span,max_size,inflight,n_req,launch", [
+    (1, 1 << 12, 300, 2, 300, "direct"), (2, 1 << 20, 5000, 3, 300, "direct"),
+    (3, 1 << 32, 20000, 1, 300, "direct"),
     # large inserts: merges of several 2^16-row runs are still running on
     # the merge stream while later steps query their inputs
-    (4, 1 << 32, 1 << 16, 3, 120)])
+    (4, 1 << 32, 1 << 16, 3, 120, "direct"),
+    # small steps as captured graphs, and every step kernel by kernel
+    (5, 1 << 20, 5000, 3, 300, "graph"), (6, 1 << 20, 5000, 3, 300,
+                                          "kernels")])
 def test_served_results_equal_step_replay(gd, orc, io, seed, span, max_size,
-                                          inflight, n_req):
+                                          inflight, n_req, launch):
     rng = np.random.default_rng(seed)
     cfg = dict(max_elements=1 << 22, batch_capacity=1 << 16)
-    srv = gd.Server(**cfg, max_inflight_steps=inflight)
+    srv = gd.Server(**cfg, max_inflight_steps=inflight, step_launch=launch)
     reqs = _make_requests(rng, n_req, span, max_size)
     handles = _submit_all(gd, io, srv, reqs)
     served = []
@@ -133,6 +136,33 @@ def test_served_results_equal_step_replay(gd, orc, io, seed, span, max_size,
     st = srv.stats()
     assert st["requests"] == len(reqs)
     assert st["steps"] < len(reqs)  # requests were coalesced
+    _check_against_replay(gd, orc, io, reqs, served, cfg)
+    srv.close()
+
+
+@pytest.mark.parametrize("span,inflight,launch", [
+    (1 << 12, 3, "graph"), (1 << 32, 2, "graph"), (1 << 32, 3, "direct")])
+def test_small_steps_run_fused(gd, orc, io, span, inflight, launch):
+    """Small requests submitted as they come: most steps are fused steps
+    (descriptor copy, admission reading the requests in place, persistent
+    query kernel), in graph mode one captured graph replayed step after
+    step with the engine's own launches and waits in between; every result
+    equals the step-order replay and the same request run alone."""
+    rng = np.random.default_rng(span % 1000 + inflight)
+    cfg = dict(max_elements=1 << 20, batch_capacity=1 << 14)
+    srv = gd.Server(**cfg, max_inflight_steps=inflight, step_launch=launch)
+    reqs = _make_requests(rng, 600, span, 200)
+    served, pending = [], []
+    for idx in range(len(reqs)):
+        # submit in small groups, waiting in between, so steps stay small
+        (i2, t, out, found), = _submit_all(gd, io, srv, reqs, order=[idx])
+        pending.append((i2, idx, t, out, found))
+        if len(pending) == 7 or idx == len(reqs) - 1:
+            for i3, order, t3, out3, found3 in pending:
+                served.append((i3, order, srv.wait(t3), out3, found3))
+            pending = []
+    st = srv.stats()
+    assert st["graph_steps"] > st["steps"] // 2
     _check_against_replay(gd, orc, io, reqs, served, cfg)
     srv.close()
 
