@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--requests", type=int, default=10000,
                     help="requests in the serving comparison (0 = skip)")
-    ap.add_argument("--flix-batches", type=int, default=4,
+    ap.add_argument("--flix-batches", type=int, default=32,
                     help="config-3 FliX batches to run (0 = skip)")
     ap.add_argument("--flix-ref", type=int, default=1,
                     help="FliX on the shape of the reference's recorded run "
@@ -73,6 +73,20 @@ def parse():
                          "OUT and exit")
     ap.add_argument("--pmc-command", default="",
                     help="the rocprofv3 command lines, quoted in the report")
+    ap.add_argument("--counters-report", nargs=2, metavar=("SQ_DIR",
+                                                             "TCC_DIR"),
+                    help="no GPU: summarise a rocprofv3 --pmc run of SQ "
+                         "counters (SQ_WAVES SQ_INSTS_LDS "
+                         "SQ_LDS_BANK_CONFLICT SQ_WAVE_CYCLES SQ_BUSY_CYCLES)"
+                         " and one of TCC_HIT_sum TCC_MISS_sum into "
+                         "profiles/<kernel>_counters.txt and exit")
+    ap.add_argument("--api-report", nargs=2, metavar=("TRACE_DIR", "OUT"),
+                    help="no GPU: host time per HIP API call and per-kernel "
+                         "durations of a rocprofv3 --hip-trace --kernel-trace "
+                         "--output-format csv run, written to OUT")
+    ap.add_argument("--serve-launch", default="kernels",
+                    choices=("kernels", "direct", "graph"),
+                    help="step launch mode of the serving engine")
     return ap.parse_args()
 
 
@@ -82,8 +96,17 @@ PMC_KERNELS = (("merge_kernel", "merge_kernel<"),
                ("lookup_gather_kernel", "lookup_gather_kernel<"),
                ("seg_sort_kernel", "seg_sort_kernel<"),
                ("seg_partition_onesweep_kernel", "SegPayload"),
-               ("range_kernel", "range_kernel<"))
+               ("seg_tile_hist_kernel", "seg_tile_hist_kernel<"),
+               ("seg_tile_scan_kernel", "seg_tile_scan_kernel<"),
+               ("range_kernel", "range_kernel<"),
+               ("flix_insert_fast_kernel", "insert_fast_kernel<"),
+               ("flix_insert_plan_kernel", "insert_plan_kernel"),
+               ("flix_delete_kernel", "flix::delete_kernel<"),
+               ("flix_lookup_kernel", "flix::lookup_kernel<"))
 COPY_PROBE_KERNEL = "neg_kernel"
+# the kernels of one segmented admission (their traffic adds up)
+ADMISSION_KERNELS = ("seg_tile_hist_kernel", "seg_tile_scan_kernel",
+                     "seg_partition_onesweep_kernel", "seg_sort_kernel")
 
 
 def host_cpus():
@@ -204,6 +227,101 @@ def trace_report(args):
     print("\n".join(out))
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+               "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES")
+TCC_COUNTERS = ("TCC_HIT_sum", "TCC_MISS_sum")
+
+
+def counters_report(args):
+    """Per-dispatch means of the SQ and TCC counters of each kernel in
+    PMC_KERNELS, with the L2 hit rate, LDS bank conflicts per LDS
+    instruction and the mean resident waves (wave-cycles / busy cycles)."""
+    vals = {}
+    for d, names in ((args.counters_report[0], SQ_COUNTERS),
+                     (args.counters_report[1], TCC_COUNTERS)):
+        for c in names:
+            for k, v in _pmc_per_kernel(d, c).items():
+                vals.setdefault(k, {})[c] = float(np.mean(v)), len(v)
+    for k, _ in PMC_KERNELS:
+        if k not in vals:
+            continue
+        c = {n: v[0] for n, v in vals[k].items()}
+        disp = max(v[1] for v in vals[k].values())
+        lines = ["# rocprofv3 --pmc counters of %s" % k, "",
+                 "Commands (one pass per counter block; summary by bench.py "
+                 "--counters-report):", "", args.pmc_command or "(not given)",
+                 "", "DISPATCHES = %d" % disp]
+        for n in SQ_COUNTERS + TCC_COUNTERS:
+            if n in c:
+                lines.append("%s_PER_DISPATCH = %.1f" % (n, c[n]))
+        if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
+            tot = c["TCC_HIT_sum"] + c["TCC_MISS_sum"]
+            lines.append("L2_HIT_RATE = %.4f" % (c["TCC_HIT_sum"] / tot
+                                                 if tot else 0.0))
+        if "SQ_INSTS_LDS" in c and "SQ_LDS_BANK_CONFLICT" in c:
+            lines.append("LDS_BANK_CONFLICT_CYCLES_PER_LDS_INST = %.4f" % (
+                c["SQ_LDS_BANK_CONFLICT"] / c["SQ_INSTS_LDS"]
+                if c["SQ_INSTS_LDS"] else 0.0))
+        if "SQ_WAVE_CYCLES" in c and "SQ_BUSY_CYCLES" in c:
+            lines.append("MEAN_RESIDENT_WAVES = %.1f  (SQ_WAVE_CYCLES / "
+                         "SQ_BUSY_CYCLES, both summed over the chip's "
+                         "sequencers)" % (c["SQ_WAVE_CYCLES"] /
+                                          c["SQ_BUSY_CYCLES"]
+                                          if c["SQ_BUSY_CYCLES"] else 0.0))
+        text = "\n".join(lines) + "\n"
+        path = os.path.join(ROOT, "profiles", k + "_counters.txt")
+        with open(path, "w") as f:
+            f.write(text)
+        print(text)
+
+
+def api_report(args):
+    """Host time per HIP API function and kernel durations from a rocprofv3
+    --hip-trace --kernel-trace --output-format csv run."""
+    import csv
+    import glob
+
+    def rows(pat):
+        files = glob.glob(os.path.join(args.api_report[0], "**", pat),
+                          recursive=True)
+        out = []
+        for path in files:
+            with open(path, newline="") as f:
+                out += list(csv.DictReader(f))
+        return out
+    out = ["command: " + (args.pmc_command or "(not given)"),
+           "source: rocprofv3 hip_api_trace.csv and kernel_trace.csv; "
+           "summary by bench.py --api-report", "",
+           "host time per HIP API call (us):",
+           "%-34s %8s %9s %9s %9s %10s" % ("function", "calls", "mean",
+                                            "p50", "p99", "total_ms")]
+    per = {}
+    for r in rows("*hip_api_trace.csv"):
+        per.setdefault(r["Function"], []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:20]:
+        v.sort()
+        out.append("%-34s %8d %9.1f %9.1f %9.1f %10.2f" % (
+            name, len(v), sum(v) / len(v), v[len(v) // 2],
+            v[int(0.99 * (len(v) - 1))], sum(v) / 1e3))
+    out += ["", "kernel durations (us):",
+            "%-64s %8s %9s %9s %9s" % ("kernel", "calls", "mean", "p50",
+                                       "p99")]
+    per = {}
+    for r in rows("*kernel_trace.csv"):
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")[:64]
+        per.setdefault(name, []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:25]:
+        v.sort()
+        out.append("%-64s %8d %9.1f %9.1f %9.1f" % (
+            name, len(v), sum(v) / len(v), v[len(v) // 2],
+            v[int(0.99 * (len(v) - 1))]))
+    with open(args.api_report[1], "w") as f:
+        f.write("\n".join(out) + "\n")
+    print("\n".join(out))
+
+
 def pmc_report(args):
     fetch = _pmc_per_kernel(args.pmc_report[0], "FETCH_SIZE")
     write = _pmc_per_kernel(args.pmc_report[1], "WRITE_SIZE")
@@ -318,7 +436,7 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
 
     def loaded_server():
         srv = pkg.Server(max_elements=cap, batch_capacity=w.batch,
-                         max_inflight_steps=3)
+                         max_inflight_steps=3, step_launch=args.serve_launch)
         srv.wait(srv.submit(pkg.Server.INSERT, live_k, live_v, count=live))
         srv.drain()
         # the load's merges finish before the measured stream starts
@@ -409,6 +527,7 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
                     "achieved >= 95% of offered; at_equal_p99 = serving's "
                     "best sustained throughput with p99 <= the plain "
                     "point's p99",
+        "step_launch": args.serve_launch,
         "plain_saturated": plain_sat,
         "plain": plain_curve,
         "serving": serve_curve,
@@ -717,6 +836,12 @@ def main():
     if args.trace_report:
         trace_report(args)
         return
+    if args.counters_report:
+        counters_report(args)
+        return
+    if args.api_report:
+        api_report(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the dictionary lives on a single "
               "device and the multi-GPU (sharded) layout is not built. Use "
@@ -908,6 +1033,20 @@ def main():
             prof["lookup_launches"],
             {"Q": Q, "R_rows": int(rows), "L_runs": int(runs),
              "formula": "Q*(K+V+1) + min(R*(K+V), Q*L*64)"}))
+    if prof["admission_batches"]:
+        adm = roof(
+            "admission", prof["admission_ms"], prof["admission_keys"] * 16,
+            prof["admission_batches"],
+            {"keys_n": int(prof["admission_keys"]),
+             "formula": "2*n*(K+V) per tile (n keys of 4 bytes with 4-byte "
+                        "values in, n rows of 8 bytes out)",
+             "kernels": "seg_tile_hist_kernel, seg_tile_scan_kernel, the "
+                        "SegPayload onesweep pass and seg_sort_kernel of each "
+                        "2^20-key tile (one event pair around them)"})
+        parts = [pmc_traffic(k) for k in ADMISSION_KERNELS]
+        adm["traffic"] = sum(parts) if all(p is not None for p in parts) \
+            else None
+        kernels.append(adm)
     kernels.sort(key=lambda k: -k["avg_launch_ms"] * k["launches"])
     roofline = dict(kernels[0]) if kernels else None
 

@@ -173,7 +173,25 @@ class _Profile(ctypes.Structure):
         ("lookup_launches", ctypes.c_uint64),
         ("lookup_ms", ctypes.c_double),
         ("lookup_queries", ctypes.c_uint64),
+        ("admission_batches", ctypes.c_uint64),
+        ("admission_ms", ctypes.c_double),
+        ("admission_keys", ctypes.c_uint64),
     ]
+
+
+class _Memory(ctypes.Structure):
+    _fields_ = [
+        ("held_bytes", ctypes.c_uint64),
+        ("live_bytes", ctypes.c_uint64),
+        ("cached_bytes", ctypes.c_uint64),
+        ("peak_held_bytes", ctypes.c_uint64),
+        ("trimmed_blocks", ctypes.c_uint64),
+        ("row_bytes_ub", ctypes.c_uint64),
+    ]
+
+
+def _memory_dict(m):
+    return {name: int(getattr(m, name)) for name, _ in m._fields_}
 
 
 _P = ctypes.c_void_p
@@ -198,6 +216,8 @@ SIGNATURES = {
     "gd_lsm_compact": [_P, _P],
     "gd_lsm_carry_status": [_P, ctypes.POINTER(ctypes.c_uint32)],
     "gd_lsm_resident_bytes": [_P, ctypes.POINTER(_U64)],
+    "gd_lsm_memory": [_P, ctypes.POINTER(_Memory)],
+    "gd_lsm_trim": [_P],
     "gd_lsm_shape": [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_U64)],
     "gd_lsm_set_tuning": [_P, ctypes.POINTER(_Tuning)],
     "gd_lsm_set_profiling": [_P, _I],
@@ -225,6 +245,8 @@ SIGNATURES = {
     "gd_lsm64_compact": [_P, _P],
     "gd_lsm64_carry_status": [_P, ctypes.POINTER(ctypes.c_uint32)],
     "gd_lsm64_resident_bytes": [_P, ctypes.POINTER(_U64)],
+    "gd_lsm64_memory": [_P, ctypes.POINTER(_Memory)],
+    "gd_lsm64_trim": [_P],
     "gd_lsm64_shape": [_P, ctypes.POINTER(ctypes.c_uint32),
                        ctypes.POINTER(_U64)],
     "gd_lsm64_set_tuning": [_P, ctypes.POINTER(_Tuning)],
@@ -248,6 +270,8 @@ SIGNATURES = {
     "gd_flix_range": [_P, _P, _P, _U64, _P, _P, _P, _P],
     "gd_flix_total_keys": [_P, ctypes.POINTER(_U64), _P],
     "gd_flix_resident_bytes": [_P, ctypes.POINTER(_U64)],
+    "gd_flix_memory": [_P, ctypes.POINTER(_Memory)],
+    "gd_flix_trim": [_P],
 }
 
 _lib = None
@@ -524,6 +548,17 @@ class GPULSMOpt:
         self._call("resident_bytes", ctypes.byref(v))
         return int(v.value)
 
+    def memory(self):
+        """Held, live, cached and peak device bytes of this handle, the
+        blocks trimmed so far and the rows' bytes (upper bound)."""
+        m = _Memory()
+        self._call("memory", ctypes.byref(m))
+        return _memory_dict(m)
+
+    def trim(self):
+        """Returns every cached block to HIP (waits for outstanding work)."""
+        self._call("trim")
+
     def shape(self):
         runs = ctypes.c_uint32(0)
         rows = _U64(0)
@@ -718,3 +753,12 @@ class FlixIndex:
         v = _U64(0)
         self._call("gd_flix_resident_bytes", ctypes.byref(v))
         return int(v.value)
+
+    def memory(self):
+        """As GPULSMOpt.memory(); row_bytes_ub is the node layer's bytes."""
+        m = _Memory()
+        self._call("gd_flix_memory", ctypes.byref(m))
+        return _memory_dict(m)
+
+    def trim(self):
+        self._call("gd_flix_trim")

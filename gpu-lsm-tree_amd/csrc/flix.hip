@@ -51,6 +51,8 @@ class FlixBase {
   virtual void rebuild(hipStream_t s) = 0;
   virtual u64 total_keys(hipStream_t s) = 0;
   virtual u64 resident_bytes() const = 0;
+  virtual void memory(gd_memory_stats *out) const = 0;
+  virtual void trim() = 0;
 };
 
 template <class K, class V>
@@ -241,6 +243,19 @@ class FlixImpl : public FlixBase {
   }
 
   u64 resident_bytes() const override { return pool_.held_bytes(); }
+  void memory(gd_memory_stats *out) const override {
+    out->held_bytes = pool_.held_bytes();
+    out->cached_bytes = pool_.cached_bytes();
+    out->live_bytes = pool_.live_bytes();
+    out->peak_held_bytes = pool_.peak_bytes();
+    out->trimmed_blocks = pool_.trimmed_blocks();
+    out->row_bytes_ub =
+        total_nodes_ * (u64(S_) * (sizeof(K) + sizeof(V)) + 2 * sizeof(u32));
+  }
+  void trim() override {
+    order_.wait_host();
+    pool_.trim();
+  }
 
  private:
   void require_built() const {
@@ -614,5 +629,12 @@ gd_status gd_flix_resident_bytes(gd_flix *h, uint64_t *out_bytes) {
   if (!out_bytes) return GD_INVALID_ARGUMENT;
   GD_FLIX_CALL(h, *out_bytes = f->resident_bytes());
 }
+
+gd_status gd_flix_memory(gd_flix *h, gd_memory_stats *out) {
+  if (!out) return GD_INVALID_ARGUMENT;
+  GD_FLIX_CALL(h, f->memory(out));
+}
+
+gd_status gd_flix_trim(gd_flix *h) { GD_FLIX_CALL(h, f->trim()); }
 
 }  // extern "C"

@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -34,13 +36,27 @@ inline void hip_check(hipError_t e, const char *what, const char *file,
 #define GD_HIP(call) ::gdict::hip_check((call), #call, __FILE__, __LINE__)
 #define GD_LAUNCH_CHECK() GD_HIP(hipGetLastError())
 
-// Caching device allocator.  Buffers are recycled by size class and only
-// returned to HIP on destruction.  All work of a handle is totally ordered
-// on the device (event chaining), so a buffer released by host bookkeeping
-// may be handed to work enqueued later without further synchronisation.
+// Caching device allocator.  Buffers are recycled by size class.  All work
+// of a handle is totally ordered on the device (event chaining), so a buffer
+// released by host bookkeeping may be handed to work enqueued later without
+// further synchronisation.
+//
+// The cache is bounded: when the cached (released, unused) bytes exceed
+// max(kCacheFloor, live / 2), the largest cached blocks go back to HIP until
+// the cache is at most max(kCacheFloor, live / 4).  hipFree synchronises the
+// device before it frees, so a trim never frees memory that enqueued work
+// still reads; it only costs that wait, and it happens when a large
+// structure has been replaced (a merge of the biggest runs), not per batch.
+// Bytes held by a group of pools of one handle, with their common peak.
+struct PoolUsage {
+  std::size_t held = 0, peak = 0;
+};
+
 class DevicePool {
  public:
-  DevicePool() = default;
+  static constexpr std::size_t kCacheFloor = std::size_t(4) << 30;
+
+  explicit DevicePool(PoolUsage *group = nullptr) : group_(group) {}
   DevicePool(const DevicePool &) = delete;
   DevicePool &operator=(const DevicePool &) = delete;
   ~DevicePool() { release_all(); }
@@ -62,6 +78,7 @@ class DevicePool {
     if (it != free_.end()) {
       void *p = it->second;
       free_.erase(it);
+      cached_ -= cls;
       return p;
     }
     void *p = nullptr;
@@ -80,6 +97,11 @@ class DevicePool {
     }
     size_of_[p] = cls;
     held_ += cls;
+    if (held_ > peak_) peak_ = held_;
+    if (group_) {
+      group_->held += cls;
+      group_->peak = std::max(group_->peak, group_->held);
+    }
     return p;
   }
 
@@ -93,32 +115,48 @@ class DevicePool {
     auto it = size_of_.find(p);
     if (it == size_of_.end()) return;
     free_.emplace(it->second, p);
+    cached_ += it->second;
+    const std::size_t live = held_ - cached_;
+    if (cached_ > std::max(kCacheFloor, live / 2))
+      trim_to(std::max(kCacheFloor, live / 4));
   }
 
-  // Frees cached (unused) blocks.  Callers must have synchronised the
-  // device work that last used them.
-  void trim() {
-    for (auto &kv : free_) {
-      (void)hipFree(kv.second);
-      held_ -= kv.first;
-      size_of_.erase(kv.second);
+  // Frees cached blocks, largest first, until at most `limit` bytes stay
+  // cached.  hipFree waits for the device first.
+  void trim_to(std::size_t limit) {
+    while (cached_ > limit && !free_.empty()) {
+      auto last = std::prev(free_.end());
+      (void)hipFree(last->second);
+      held_ -= last->first;
+      cached_ -= last->first;
+      if (group_) group_->held -= last->first;
+      size_of_.erase(last->second);
+      free_.erase(last);
+      ++trims_;
     }
-    free_.clear();
   }
+  void trim() { trim_to(0); }
 
   void release_all() {
     for (auto &kv : size_of_) (void)hipFree(kv.first);
     size_of_.clear();
     free_.clear();
+    if (group_) group_->held -= held_;
     held_ = 0;
+    cached_ = 0;
   }
 
   std::size_t held_bytes() const { return held_; }
+  std::size_t cached_bytes() const { return cached_; }
+  std::size_t live_bytes() const { return held_ - cached_; }
+  std::size_t peak_bytes() const { return peak_; }
+  std::size_t trimmed_blocks() const { return trims_; }
 
  private:
   std::multimap<std::size_t, void *> free_;
   std::unordered_map<void *, std::size_t> size_of_;
-  std::size_t held_ = 0;
+  PoolUsage *group_ = nullptr;
+  std::size_t held_ = 0, cached_ = 0, peak_ = 0, trims_ = 0;
 };
 
 // RAII lease of a pool buffer.

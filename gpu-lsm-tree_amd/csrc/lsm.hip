@@ -214,6 +214,13 @@ gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes) {
   GD_LSM_CALL(h, *out_bytes = h->resident_bytes());
 }
 
+gd_status gd_lsm_memory(gd_lsm *h, gd_memory_stats *out) {
+  if (!out) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, h->memory(out));
+}
+
+gd_status gd_lsm_trim(gd_lsm *h) { GD_LSM_CALL(h, h->trim()); }
+
 gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning) {
   if (!tuning) return GD_INVALID_ARGUMENT;
   GD_LSM_CALL(h, h->set_tuning(*tuning));
@@ -346,6 +353,13 @@ gd_status gd_lsm64_resident_bytes(gd_lsm64 *h, uint64_t *out_bytes) {
   if (!out_bytes) return GD_INVALID_ARGUMENT;
   GD_LSM_CALL(h, *out_bytes = h->resident_bytes());
 }
+
+gd_status gd_lsm64_memory(gd_lsm64 *h, gd_memory_stats *out) {
+  if (!out) return GD_INVALID_ARGUMENT;
+  GD_LSM_CALL(h, h->memory(out));
+}
+
+gd_status gd_lsm64_trim(gd_lsm64 *h) { GD_LSM_CALL(h, h->trim()); }
 
 gd_status gd_lsm64_set_tuning(gd_lsm64 *h, const gd_lsm_tuning *tuning) {
   if (!tuning) return GD_INVALID_ARGUMENT;

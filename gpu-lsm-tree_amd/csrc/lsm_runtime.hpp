@@ -155,8 +155,11 @@ class LsmT : public HandleBase {
     while (consumed < n) {
       const u32 tile =
           static_cast<u32>(std::min<u64>(n - consumed, batch_capacity_));
+      ProfRec rec{};
+      if (profiling_) rec = prof_begin(2, s, tile);
       Run r = sorted_run(keys + consumed, values ? values + consumed : nullptr,
                          flags ? flags + consumed : nullptr, tile, mode, s);
+      if (profiling_) prof_end(rec, s);
       runs_.insert(runs_.begin(), r);
       consumed += tile;
       hipStream_t ms = s;
@@ -812,10 +815,14 @@ class LsmT : public HandleBase {
         out->merge_ms += ms;
         for (u64 i = 0; i < r.n; ++i) out->merge_rows_in += m[i].count;
         out->merge_rows_out += m[kMaxWay].count;
-      } else {
+      } else if (r.kind == 1) {
         out->lookup_launches += 1;
         out->lookup_ms += ms;
         out->lookup_queries += r.n;
+      } else {
+        out->admission_batches += 1;
+        out->admission_ms += ms;
+        out->admission_keys += r.n;
       }
     }
     if (reset) clear_profile();
@@ -824,6 +831,23 @@ class LsmT : public HandleBase {
   u32 carry_status() const { return status_; }
   u64 resident_bytes() const {
     return pool_.held_bytes() + merge_pool_.held_bytes();
+  }
+  void memory(gd_memory_stats *out) const {
+    out->held_bytes = resident_bytes();
+    out->cached_bytes = pool_.cached_bytes() + merge_pool_.cached_bytes();
+    out->live_bytes = out->held_bytes - out->cached_bytes;
+    out->peak_held_bytes = usage_.peak;
+    out->trimmed_blocks = pool_.trimmed_blocks() + merge_pool_.trimmed_blocks();
+    u64 rows = 0;
+    for (const auto &r : runs_) rows += r.rows_ub;
+    out->row_bytes_ub = rows * sizeof(Row);
+  }
+  // Returns every cached block to HIP (waits for the device).
+  void trim() {
+    order_.wait_host();
+    reclaim(/*wait=*/true);
+    pool_.trim();
+    merge_pool_.trim();
   }
   // Synchronous: exact rows per run (refreshes the host upper bounds).
   void shape(u32 *runs, u64 *rows) {
@@ -1336,7 +1360,7 @@ class LsmT : public HandleBase {
                         1, 512, kMaxWay, 2};
 
   struct ProfRec {
-    int kind;  // 0 merge, 1 lookup
+    int kind;  // 0 merge, 1 lookup, 2 admission of one tile
     hipEvent_t start, end;
     u64 slot, n;  // n: merge inputs or lookup queries
   };
@@ -1370,8 +1394,9 @@ class LsmT : public HandleBase {
   std::deque<Deferred> deferred_;
   hipStream_t ms_ = nullptr, qs_ = nullptr;  // serving mode streams
   std::unordered_map<const void *, PendingMerge> pending_;
-  DevicePool pool_;
-  DevicePool merge_pool_;
+  PoolUsage usage_;
+  DevicePool pool_{&usage_};
+  DevicePool merge_pool_{&usage_};
   OperationOrder order_;
   std::vector<Run> runs_;  // newest first
   u32 pending_batches_ = 0;

@@ -155,6 +155,25 @@ gd_status gd_lsm_snapshot_restore(gd_lsm *h, const char *path,
 gd_status gd_lsm_carry_status(gd_lsm *h, uint32_t *out_status);
 /* gpu_resident_bytes()  GPULSMOpt.cuh:6341 */
 gd_status gd_lsm_resident_bytes(gd_lsm *h, uint64_t *out_bytes);
+/* No reference counterpart: the handle's device memory.  held = bytes its
+ * pools hold from HIP (what gd_lsm_resident_bytes reports); cached = blocks
+ * released by the structure and kept for reuse; live = held - cached; peak =
+ * the highest held so far.  A pool keeps at most max(4 GiB, live / 2)
+ * cached and returns the largest blocks above max(4 GiB, live / 4) to HIP
+ * (hipFree waits for the device).  row_bytes_ub = the rows the runs hold
+ * (an upper bound; tombstones included) times the row width. */
+typedef struct gd_memory_stats {
+  uint64_t held_bytes;
+  uint64_t live_bytes;
+  uint64_t cached_bytes;
+  uint64_t peak_held_bytes;
+  uint64_t trimmed_blocks;
+  uint64_t row_bytes_ub;
+} gd_memory_stats;
+gd_status gd_lsm_memory(gd_lsm *h, gd_memory_stats *out);
+/* No reference counterpart: returns every cached block to HIP (waits for the
+ * handle's outstanding work). */
+gd_status gd_lsm_trim(gd_lsm *h);
 /* No reference counterpart: launch-shape knobs picked by the autotuner
  * (gpu-lsm-tree_amd/autotune.py).  None changes any result: they select
  * directory density, merge tile, sort block shape and when to group
@@ -186,7 +205,9 @@ gd_status gd_lsm_get_tuning(gd_lsm *h, gd_lsm_tuning *out);
 
 /* No reference counterpart: opt-in per-kernel timing (HIP events on the
  * launch stream) of the merge and lookup kernels, with the merges' row
- * counts for their algorithmic bytes (rows_in + rows_out) * 8. */
+ * counts for their algorithmic bytes (rows_in + rows_out) * 8, and of the
+ * admission of every update tile (all of its kernels, from the first to
+ * the last). */
 typedef struct gd_lsm_profile {
   uint64_t merge_launches;
   double merge_ms;
@@ -195,6 +216,9 @@ typedef struct gd_lsm_profile {
   uint64_t lookup_launches;
   double lookup_ms;
   uint64_t lookup_queries;
+  uint64_t admission_batches;  /* admitted tiles          */
+  double admission_ms;
+  uint64_t admission_keys;     /* keys of those tiles     */
 } gd_lsm_profile;
 
 gd_status gd_lsm_set_profiling(gd_lsm *h, int enable);
@@ -251,6 +275,8 @@ gd_status gd_lsm64_dump_sorted(gd_lsm64 *h, uint64_t *out_keys,
 gd_status gd_lsm64_compact(gd_lsm64 *h, hipStream_t stream);
 gd_status gd_lsm64_carry_status(gd_lsm64 *h, uint32_t *out_status);
 gd_status gd_lsm64_resident_bytes(gd_lsm64 *h, uint64_t *out_bytes);
+gd_status gd_lsm64_memory(gd_lsm64 *h, gd_memory_stats *out);
+gd_status gd_lsm64_trim(gd_lsm64 *h);
 gd_status gd_lsm64_set_tuning(gd_lsm64 *h, const gd_lsm_tuning *tuning);
 gd_status gd_lsm64_get_tuning(gd_lsm64 *h, gd_lsm_tuning *out);
 gd_status gd_lsm64_shape(gd_lsm64 *h, uint32_t *out_runs,
@@ -397,6 +423,11 @@ gd_status gd_flix_total_keys(gd_flix *h, uint64_t *out_keys,
                              hipStream_t stream);
 /* gpu_resident_bytes()  impl:1600 */
 gd_status gd_flix_resident_bytes(gd_flix *h, uint64_t *out_bytes);
+/* No reference counterpart: as gd_lsm_memory; row_bytes_ub = the bucket
+ * layer's node bytes in use. */
+gd_status gd_flix_memory(gd_flix *h, gd_memory_stats *out);
+/* No reference counterpart: as gd_lsm_trim. */
+gd_status gd_flix_trim(gd_flix *h);
 
 #ifdef __cplusplus
 }

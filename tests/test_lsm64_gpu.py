@@ -304,3 +304,56 @@ def test_compact_64(gd, orc, io):
     lo, hi = _ranges(rng, 800, 1 << 40, 1 << 34)
     p.check_ranges(lo, hi)
     p.check_successor(q)
+
+
+def test_pool_cache_bounded_and_trim(gd, dev):
+    """The device pools keep at most max(4 GiB, live / 2) cached per pool
+    while a 2^28-key (4 GiB of rows) dictionary is built, merged with new
+    batches and compacted; held = live + cached at every point; trim()
+    returns the cache, and the answers do not change."""
+    import torch
+    n = 1 << 28
+    mul = -7046029254386353131  # 0x9E3779B97F4A7C15 as int64 (odd)
+    keys = torch.arange(n, dtype=torch.int64, device=dev) * mul
+    vals = keys ^ 0x5DEECE66D
+    d = gd.GPULSMOpt(key_bytes=8, max_elements=n + (1 << 22),
+                     batch_capacity=1 << 20)
+    floor = 4 << 30
+
+    def check(m):
+        assert m["held_bytes"] == m["live_bytes"] + m["cached_bytes"]
+        assert m["peak_held_bytes"] >= m["held_bytes"]
+        assert m["cached_bytes"] <= 2 * floor + m["live_bytes"] // 2, m
+
+    d.bulk_build(keys, vals)
+    check(d.memory())
+    for b in range(4):
+        k = keys[b << 20:(b + 1) << 20] + 1  # odd offsets: new keys
+        d.insert(k, k ^ 0x5DEECE66D)
+        check(d.memory())
+    d.compact()
+    torch.cuda.synchronize()
+    m = d.memory()
+    check(m)
+    assert m["row_bytes_ub"] == (n + (4 << 20)) * 16
+    assert m["live_bytes"] >= m["row_bytes_ub"]
+    h = 1 << 16
+    # present (bulk), present (inserted), present (bulk), absent
+    q = torch.cat([keys[:h], keys[:h] + 1, keys[-h:], keys[:h] + 2])
+    want = q ^ 0x5DEECE66D
+    found_want = torch.ones(q.numel(), dtype=torch.uint8, device=dev)
+    found_want[3 * h:] = 0
+    out = torch.empty_like(q)
+    found = torch.empty(q.numel(), dtype=torch.uint8, device=dev)
+    d.lookup(q, out, found)
+    torch.cuda.synchronize()
+    assert torch.equal(found, found_want)
+    assert torch.equal(out[found.bool()], want[found.bool()])
+    d.trim()
+    m = d.memory()
+    assert m["cached_bytes"] == 0 and m["held_bytes"] == m["live_bytes"]
+    out2 = torch.empty_like(q)
+    d.lookup(q, out2, found)
+    torch.cuda.synchronize()
+    assert torch.equal(out2[found.bool()], want[found.bool()])
+    d.close()

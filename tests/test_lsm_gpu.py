@@ -489,6 +489,9 @@ def test_kernel_profile_counts(gd, orc, io, ways):
         assert prof["merge_rows_in"] == 4096 * (2 + 4 + 2 + 8)
     assert prof["merge_rows_out"] == prof["merge_rows_in"]  # distinct keys
     assert prof["merge_ms"] > 0
+    assert prof["admission_batches"] == 8
+    assert prof["admission_keys"] == 8 * 4096
+    assert prof["admission_ms"] > 0
     assert prof["lookup_launches"] == 2
     assert prof["lookup_queries"] == 8000
     assert prof["lookup_ms"] > 0
