@@ -349,10 +349,20 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   static_assert(sizeof(Row) == 2 * sizeof(K) && sizeof(K) == sizeof(V),
                 "row = key word + value");
   static_assert(kTile <= kTombSlot, "slots are 15-bit");
+  // LDS slots are swizzled (lds_swz): a thread's merge window and its
+  // staged rows start kItems slots after the previous lane's
+  constexpr u32 kKeySlots0 = lds_lines<sizeof(K)>(kTile);
+  constexpr u32 kStageSlots =
+      lds_lines<sizeof(Row)>(kStageHalf) * sizeof(Row) / sizeof(K);
+  constexpr u32 kKeySlots = kKeySlots0 > kStageSlots ? kKeySlots0
+                                                     : kStageSlots;
+  auto key_slot = [](u32 i) { return lds_swz<sizeof(K)>(i); };
+  auto slot_slot = [](u32 i) { return lds_swz<2>(i); };
+  auto row_slot = [](u32 i) { return lds_swz<sizeof(Row)>(i); };
   __shared__ u32 s_tile;
-  __shared__ __align__(16) K s_key[kTile];
-  __shared__ __align__(16) V s_val[kTile];
-  __shared__ unsigned short s_slot[kTile];
+  __shared__ __align__(16) K s_key[kKeySlots];
+  __shared__ __align__(16) V s_val[kKeySlots];
+  __shared__ unsigned short s_slot[lds_lines<2>(kTile)];
   __shared__ u32 s_scan[kWaves + 1];
   __shared__ K s_wave_last[kWaves];
   __shared__ u64 s_base;
@@ -470,9 +480,9 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
       const u32 i = (segs >> (3 * r)) & 7u;
       const u32 split = s_seg_split[i], pos = s + s_seg_delta[i];
       const K w = T::kw(rr[r]);
-      s_key[s] = decode_key<T>(w, pos, split);
-      s_val[s] = T::val(rr[r]);
-      s_slot[s] = static_cast<unsigned short>(
+      s_key[key_slot(s)] = decode_key<T>(w, pos, split);
+      s_val[key_slot(s)] = T::val(rr[r]);
+      s_slot[slot_slot(s)] = static_cast<unsigned short>(
           s | ((w >> (T::kBits - 1)) ? kTombSlot : 0u));
     }
   }
@@ -516,12 +526,14 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
       u32 hi = diag < na ? diag : na;
       while (lo < hi) {
         const u32 mid = (lo + hi) >> 1;
-        if (s_key[glo + mid] <= s_key[gmid + (diag - mid - 1)]) lo = mid + 1;
+        if (s_key[key_slot(glo + mid)] <=
+            s_key[key_slot(gmid + (diag - mid - 1))])
+          lo = mid + 1;
         else hi = mid;
       }
       u32 ia = glo + lo, ib = gmid + (diag - lo);  // absolute positions
-      K ka = ia < gmid ? s_key[ia] : K(0);
-      K kb = ib < ghi ? s_key[ib] : K(0);
+      K ka = ia < gmid ? s_key[key_slot(ia)] : K(0);
+      K kb = ib < ghi ? s_key[key_slot(ib)] : K(0);
 #pragma unroll
       for (int s = 0; s < kItems; ++s) {
         const u32 q = p0 + s;
@@ -530,18 +542,18 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
             group_of(q, glo, gmid, ghi);
             ia = glo;
             ib = gmid;
-            ka = ia < gmid ? s_key[ia] : K(0);
-            kb = ib < ghi ? s_key[ib] : K(0);
+            ka = ia < gmid ? s_key[key_slot(ia)] : K(0);
+            kb = ib < ghi ? s_key[key_slot(ib)] : K(0);
           }
           const bool a_left = ia < gmid;
           const bool b_left = ib < ghi;
           const bool take_a = a_left && (!b_left || ka <= kb);
           const u32 from = take_a ? ia : ib;
           ok[s] = take_a ? ka : kb;
-          os[s] = s_slot[from];
+          os[s] = s_slot[slot_slot(from)];
           const u32 nxt = from + 1;
           const bool more = nxt < (take_a ? gmid : ghi);
-          const K kn = more ? s_key[nxt] : K(0);
+          const K kn = more ? s_key[key_slot(nxt)] : K(0);
           if (take_a) {
             ia = nxt;
             ka = kn;
@@ -557,8 +569,8 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
 #pragma unroll
       for (int s = 0; s < kItems; ++s)
         if (p0 + s < n) {
-          s_key[p0 + s] = ok[s];
-          s_slot[p0 + s] = static_cast<unsigned short>(os[s]);
+          s_key[key_slot(p0 + s)] = ok[s];
+          s_slot[slot_slot(p0 + s)] = static_cast<unsigned short>(os[s]);
         }
       __syncthreads();
     }
@@ -591,7 +603,7 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
     ov[s] = 0;
     okeep[s] = false;
     if (static_cast<u32>(s) < steps) {
-      ov[s] = s_val[os[s] & (kTombSlot - 1)];
+      ov[s] = s_val[key_slot(os[s] & (kTombSlot - 1))];
       // every version of a key is in this tile: the first (newest) stays
       const bool dup = s > 0 ? ok[s - 1] == ok[s]
                              : (threadIdx.x > 0 && prev == ok[s]);
@@ -617,8 +629,8 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
       const bool tomb = (os[s] & kTombSlot) != 0;
       const Row row =
           T::make((ok[s] & T::kLow) | (tomb ? T::kTomb : K(0)), ov[s]);
-      if (at < kStageHalf) stage_lo[at] = row;
-      else stage_hi[at - kStageHalf] = row;
+      if (at < kStageHalf) stage_lo[row_slot(at)] = row;
+      else stage_hi[row_slot(at - kStageHalf)] = row;
       ++at;
       low += ok[s] < T::kTomb;
       tombs += tomb;
@@ -628,7 +640,8 @@ __global__ __launch_bounds__(kThreads) void kway_merge_kernel(
   const u64 obase = s_base;
   for (u32 t = threadIdx.x; t < tile_total; t += kThreads)
     if (obase + t < out_rows)
-      out[obase + t] = t < kStageHalf ? stage_lo[t] : stage_hi[t - kStageHalf];
+      out[obase + t] = t < kStageHalf ? stage_lo[row_slot(t)]
+                                      : stage_hi[row_slot(t - kStageHalf)];
   // directory entries inside the bucket window (first to last input key):
   // every input element owns the buckets in (bucket(previous input key),
   // bucket(its key)] and writes them the survivors before it

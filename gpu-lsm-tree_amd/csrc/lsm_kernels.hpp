@@ -626,6 +626,24 @@ __global__ void build_directory_kernel(const typename T::Row *__restrict__ rows,
 // --------------------------------------------------------------- merging
 // merge workgroups: 256 or 512 threads (tile = threads x items per thread)
 
+// LDS slot of element i of an array of E-byte elements, XOR-swizzled: the
+// bits of i that select its bank inside a 128-byte line of the 32 banks are
+// XORed with the line number, a bijection inside every line.  Lanes whose
+// accesses start 4, 8 or 16 elements apart (a thread's merge window, its
+// staged output rows) then spread over the banks as consecutive lanes do,
+// instead of meeting in a few.  lds_lines rounds an array up to whole lines.
+template <int E>
+__host__ __device__ constexpr u32 lds_swz(u32 i) {
+  constexpr u32 per = 128 / E;
+  constexpr u32 gran = E >= 4 ? 1 : 4 / E;
+  return i ^ ((((i / per) & 31u) * gran) & (per - 1));
+}
+template <int E>
+__host__ __device__ constexpr u32 lds_lines(u32 n) {
+  constexpr u32 per = 128 / E;
+  return (n + per - 1) / per * per;
+}
+
 template <class T>
 struct MergeSide {
   const typename T::Row *rows;
@@ -731,16 +749,25 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   using V = typename T::V;
   using Row = typename T::Row;
   constexpr int kMergeTile = kMergeThreads * kMergeItems;
-  // survivors are staged as rows over the key array, then the value array:
-  // each holds (tile + 2) * sizeof(K) / sizeof(Row) rows
+  // survivors are staged as rows over the key array, then the value array
   constexpr u32 kStageHalf = (kMergeTile + 2) / 2;
   static_assert(sizeof(Row) == 2 * sizeof(K) && sizeof(K) == sizeof(V),
                 "row = key word + value");
+  // LDS slots are swizzled (lds_swz) so that the lanes of a wave, whose
+  // merge windows and staged outputs start kMergeItems slots apart, spread
+  // over the banks
+  constexpr u32 kSlots = lds_lines<sizeof(K)>(kMergeTile + 2);
+  constexpr u32 kStageSlots =
+      lds_lines<sizeof(Row)>(kStageHalf) * sizeof(Row) / sizeof(K);
+  constexpr u32 kKeySlots = kSlots > kStageSlots ? kSlots : kStageSlots;
+  auto key_slot = [](u32 i) { return lds_swz<sizeof(K)>(i); };
+  auto tomb_slot = [](u32 i) { return lds_swz<1>(i); };
+  auto row_slot = [](u32 i) { return lds_swz<sizeof(Row)>(i); };
   __shared__ u32 s_tile;
   // inputs: decoded key, value, tombstone flag
-  __shared__ __align__(16) K s_key[kMergeTile + 2];
-  __shared__ __align__(16) V s_val[kMergeTile + 2];
-  __shared__ u8 s_tomb[kMergeTile + 2];
+  __shared__ __align__(16) K s_key[kKeySlots];
+  __shared__ __align__(16) V s_val[kKeySlots];
+  __shared__ u8 s_tomb[lds_lines<1>(kMergeTile + 2)];
   __shared__ u32 s_scan[kMergeThreads / kWave + 1];
   __shared__ u64 s_base;
   __shared__ u32 s_low, s_tombs;
@@ -792,9 +819,11 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
       } else {
         const bool from_a = t <= na;
         const u32 pos = from_a ? i0 + t - 1 : j0 + (t - na - 1);
-        s_key[t] = decode_key<T>(T::kw(rr[r]), pos, from_a ? sa : sb);
-        s_tomb[t] = static_cast<u8>(T::kw(rr[r]) >> (T::kBits - 1));
-        s_val[t] = T::val(rr[r]);
+        s_key[key_slot(t)] =
+            decode_key<T>(T::kw(rr[r]), pos, from_a ? sa : sb);
+        s_tomb[tomb_slot(t)] =
+            static_cast<u8>(T::kw(rr[r]) >> (T::kBits - 1));
+        s_val[key_slot(t)] = T::val(rr[r]);
       }
     }
   }
@@ -811,8 +840,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   {
     const bool before_low = (i0 == 0 || s_key[0] < T::kTomb) &&
                             (j0 == 0 || s_prev_b < T::kTomb);
-    const bool tile_high = (na && s_key[na] >= T::kTomb) ||
-                           (nb && s_key[na + nb] >= T::kTomb);
+    const bool tile_high = (na && s_key[key_slot(na)] >= T::kTomb) ||
+                           (nb && s_key[key_slot(na + nb)] >= T::kTomb);
     transition = before_low && (tile_high || last_tile);
   }
 
@@ -825,7 +854,9 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     u32 hi = dk < na ? dk : na;
     while (lo < hi) {
       const u32 mid = (lo + hi) >> 1;
-      if (s_key[1 + mid] <= s_key[na + 1 + (dk - mid - 1)]) lo = mid + 1;
+      if (s_key[key_slot(1 + mid)] <=
+          s_key[key_slot(na + 1 + (dk - mid - 1))])
+        lo = mid + 1;
       else hi = mid;
     }
     ia = lo;
@@ -837,10 +868,10 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   if (dk < tile_len) {
     if (ia > 0 || i0 > 0) {
       has_prev = true;
-      prev = s_key[ia];  // LDS index ia holds A[ia - 1] (A[i0 - 1] at 0)
+      prev = s_key[key_slot(ia)];  // slot ia: A[ia - 1] (A[i0 - 1] at 0)
     }
     if (ib > 0 || j0 > 0) {
-      const K kb = ib > 0 ? s_key[na + ib] : s_prev_b;
+      const K kb = ib > 0 ? s_key[key_slot(na + ib)] : s_prev_b;
       prev = has_prev && prev > kb ? prev : kb;
       has_prev = true;
     }
@@ -861,11 +892,11 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   // advanced; values and tombstone flags are read afterwards, all at once.
   {
     bool a_left = ia < na, b_left = ib < nb;
-    K ka = (steps && a_left) ? s_key[1 + ia] : K(0);
-    K kb = (steps && b_left) ? s_key[na + 1 + ib] : K(0);
+    K ka = (steps && a_left) ? s_key[key_slot(1 + ia)] : K(0);
+    K kb = (steps && b_left) ? s_key[key_slot(na + 1 + ib)] : K(0);
     // LDS slot ia holds A[ia - 1] (A[i0 - 1] at 0, flag 2 when none)
-    bool have_last_a = steps && s_tomb[ia] != 2;
-    K last_a = steps ? s_key[ia] : K(0);
+    bool have_last_a = steps && s_tomb[tomb_slot(ia)] != 2;
+    K last_a = steps ? s_key[key_slot(ia)] : K(0);
 #pragma unroll
     for (int s = 0; s < kMergeItems; ++s) {
       okeep[s] = false;
@@ -886,7 +917,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
         a_left = ia < na;
         b_left = ib < nb;
         const bool more = take_a ? a_left : b_left;
-        const K next = more ? s_key[take_a ? 1 + ia : na + 1 + ib] : K(0);
+        const K next =
+            more ? s_key[key_slot(take_a ? 1 + ia : na + 1 + ib)] : K(0);
         if (take_a) ka = next;
         else kb = next;
       }
@@ -897,8 +929,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     ov[s] = 0;
     ot[s] = 0;
     if (static_cast<u32>(s) < steps) {
-      ot[s] = s_tomb[osrc[s]];
-      ov[s] = s_val[osrc[s]];
+      ot[s] = s_tomb[tomb_slot(osrc[s])];
+      ov[s] = s_val[key_slot(osrc[s])];
     }
   }
 #pragma unroll
@@ -923,8 +955,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
     if (okeep[s]) {
       const Row r =
           T::make((ok[s] & T::kLow) | (ot[s] ? T::kTomb : K(0)), ov[s]);
-      if (at < kStageHalf) stage_lo[at] = r;
-      else stage_hi[at - kStageHalf] = r;
+      if (at < kStageHalf) stage_lo[row_slot(at)] = r;
+      else stage_hi[row_slot(at - kStageHalf)] = r;
       ++at;
       low += ok[s] < T::kTomb;
       tombs += ot[s];
@@ -933,7 +965,8 @@ __global__ __launch_bounds__(kMergeThreads) void merge_kernel(
   __syncthreads();
   const u64 obase = s_base;
   for (u32 t = threadIdx.x; t < tile_total; t += kMergeThreads)
-    out[obase + t] = t < kStageHalf ? stage_lo[t] : stage_hi[t - kStageHalf];
+    out[obase + t] = t < kStageHalf ? stage_lo[row_slot(t)]
+                                    : stage_hi[row_slot(t - kStageHalf)];
   // directory entries inside the bucket window (first to last input key)
   {
     const u32 shift = T::kBits - dir_bits;
