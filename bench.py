@@ -55,10 +55,10 @@ def parse():
     ap.add_argument("--lsm-tuning", default="",
                     help="JSON dict of gd_lsm_tuning knobs for the config-2 "
                          "dictionary (default: the built-in defaults)")
-    ap.add_argument("--c5-scale-log", type=int, default=-1,
+    ap.add_argument("--c5-scale-log", type=int, default=0,
                     help="run the config-5 serving sweep on a 2^(31-s)-key "
-                         "uint64 index (-1 = skip)")
-    ap.add_argument("--c5-requests", type=int, default=2000)
+                         "uint64 index (default 0: 2^31 keys; -1 = skip)")
+    ap.add_argument("--c5-requests", type=int, default=10000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="first run an int32 negation over BYTES bytes (a "
                          "plain copy of known size that calibrates "
@@ -619,7 +619,7 @@ def c5_measurement(args, pkg, wl, torch, dev):
 
     saturated = run(None)
     sweep = [run(f * saturated["requests_per_s"]) for f in (0.25, 0.5, 0.75)]
-    resident = srv.stats()["resident_bytes"]
+    mem = srv.stats()
     srv.close()
     return {
         "workload": "config5: uint64 keys/values, index grown to %d keys, "
@@ -629,7 +629,10 @@ def c5_measurement(args, pkg, wl, torch, dev):
                                                           ops),
         "grow_seconds": round(grow_s, 2),
         "grow_mkeys_per_s": round(universe / grow_s / 1e6, 1),
-        "resident_gb": round(resident / 2 ** 30, 2),
+        "resident_gb": round(mem["resident_bytes"] / 2 ** 30, 2),
+        "live_gb": round(mem["live_bytes"] / 2 ** 30, 2),
+        "peak_resident_gb": round(mem["peak_resident_bytes"] / 2 ** 30, 2),
+        "row_data_gb": round(universe * 16 / 2 ** 30, 2),
         "saturated": saturated, "poisson_sweep": sweep}
 
 

@@ -161,6 +161,8 @@ class _ServeStats(ctypes.Structure):
         ("peak_staged_bytes", ctypes.c_uint64),
         ("resident_bytes", ctypes.c_uint64),
         ("graph_steps", ctypes.c_uint64),
+        ("live_bytes", ctypes.c_uint64),
+        ("peak_resident_bytes", ctypes.c_uint64),
     ]
 
 

@@ -527,6 +527,8 @@ class Server : public ServerBase {
     out->latency_max_us = l.empty() ? 0.0 : l.back();
     out->peak_staged_bytes = peak_staged_;
     out->resident_bytes = resident_bytes_.load();
+    out->live_bytes = live_bytes_.load();
+    out->peak_resident_bytes = peak_resident_.load();
     out->graph_steps = graph_steps_;
   }
 
@@ -963,7 +965,12 @@ class Server : public ServerBase {
     done_cv_.notify_all();
     if (any) {
       lsm_.reclaim();
-      resident_bytes_.store(lsm_.resident_bytes() + pool_.held_bytes());
+      gd_memory_stats m{};
+      lsm_.memory(&m);
+      const u64 held = m.held_bytes + pool_.held_bytes();
+      resident_bytes_.store(held);
+      live_bytes_.store(m.live_bytes + pool_.live_bytes());
+      if (held > peak_resident_.load()) peak_resident_.store(held);
     }
   }
 
@@ -983,6 +990,8 @@ class Server : public ServerBase {
   DevicePool pool_;  // step staging, released when the step completes
   PinnedCache pinned_;
   std::atomic<u64> resident_bytes_{0};
+  std::atomic<u64> live_bytes_{0};
+  std::atomic<u64> peak_resident_{0};  // written by the completion thread only
   hipStream_t us_ = nullptr, ms_ = nullptr, qs_ = nullptr;
   u32 max_inflight_ = 2;
   u64 max_step_updates_ = 0, max_step_queries_ = 0;

@@ -337,8 +337,11 @@ typedef struct gd_serve_stats {
   double latency_p999_us;
   double latency_max_us;
   uint64_t peak_staged_bytes;
-  uint64_t resident_bytes;
+  uint64_t resident_bytes;     /* held by the dictionary and step pools,
+                                  as of the last completed step           */
   uint64_t graph_steps;        /* small steps run fused (step_launch 1, 2) */
+  uint64_t live_bytes;         /* resident_bytes minus cached free blocks */
+  uint64_t peak_resident_bytes;/* highest resident_bytes seen (not reset) */
 } gd_serve_stats;
 
 gd_status gd_serve_create(const gd_serve_config *config, gd_server **out);
