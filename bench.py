@@ -146,24 +146,26 @@ def progress(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(kernel):
+def pmc_traffic(kernel, stat="PER_DISPATCH"):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
-    summary: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each divided by
-    the factor the same counter shows on a plain copy of known size."""
+    summary: FETCH_SIZE and WRITE_SIZE in KiB per dispatch (the mean, or
+    with stat="MEDIAN_PER_DISPATCH" the median over the dispatches), each
+    divided by the factor the same counter shows on a plain copy of known
+    size."""
     path = os.path.join(ROOT, "profiles", kernel + "_pmc.txt")
     if not os.path.exists(path):
         return None
     text = open(path).read()
     vals = {}
-    for key in ("FETCH_KIB_PER_DISPATCH", "WRITE_KIB_PER_DISPATCH",
+    for key in ("FETCH_KIB_" + stat, "WRITE_KIB_" + stat,
                 "FETCH_CALIBRATION", "WRITE_CALIBRATION"):
         m = re.search(key + r"\s*[:=]\s*([0-9.eE+]+)", text)
         if not m:
             return None
         vals[key] = float(m.group(1))
-    return int(1024.0 * (vals["FETCH_KIB_PER_DISPATCH"] /
+    return int(1024.0 * (vals["FETCH_KIB_" + stat] /
                          vals["FETCH_CALIBRATION"] +
-                         vals["WRITE_KIB_PER_DISPATCH"] /
+                         vals["WRITE_KIB_" + stat] /
                          vals["WRITE_CALIBRATION"]))
 
 
@@ -337,6 +339,8 @@ def pmc_report(args):
             continue
         fk = float(np.mean(fetch[k]))
         wk = float(np.mean(write[k]))
+        fm = float(np.median(fetch[k]))
+        wm = float(np.median(write[k]))
         text = (
             "# rocprofv3 --pmc HBM traffic of %s (config-2 bench step)\n\n"
             "Commands (FETCH_SIZE pass, then WRITE_SIZE pass; summary by "
@@ -347,13 +351,17 @@ def pmc_report(args):
             "DISPATCHES = %d\n"
             "FETCH_KIB_PER_DISPATCH = %.3f\n"
             "WRITE_KIB_PER_DISPATCH = %.3f\n"
+            "FETCH_KIB_MEDIAN_PER_DISPATCH = %.3f\n"
+            "WRITE_KIB_MEDIAN_PER_DISPATCH = %.3f\n"
             "FETCH_CALIBRATION = %.5f\n"
             "WRITE_CALIBRATION = %.5f\n"
-            "HBM_BYTES_PER_DISPATCH = %d\n" % (
+            "HBM_BYTES_PER_DISPATCH = %d\n"
+            "HBM_BYTES_MEDIAN_PER_DISPATCH = %d\n" % (
                 k, args.pmc_command or "(not given)", args.copy_probe,
                 max(fetch[COPY_PROBE_KERNEL]), max(write[COPY_PROBE_KERNEL]),
-                len(fetch[k]), fk, wk, f_cal, w_cal,
-                int(1024 * (fk / f_cal + wk / w_cal))))
+                len(fetch[k]), fk, wk, fm, wm, f_cal, w_cal,
+                int(1024 * (fk / f_cal + wk / w_cal)),
+                int(1024 * (fm / f_cal + wm / w_cal))))
         path = os.path.join(ROOT, "profiles", k + "_pmc.txt")
         with open(path, "w") as f:
             f.write(text)
@@ -1046,7 +1054,10 @@ def main():
              "kernels": "seg_tile_hist_kernel, seg_tile_scan_kernel, the "
                         "SegPayload onesweep pass and seg_sort_kernel of each "
                         "2^20-key tile (one event pair around them)"})
-        parts = [pmc_traffic(k) for k in ADMISSION_KERNELS]
+        # medians: the tile histogram and scan kernels also serve the
+        # lookups' query partition (two 2^26-key dispatches per profile)
+        parts = [pmc_traffic(k, "MEDIAN_PER_DISPATCH")
+                 for k in ADMISSION_KERNELS]
         adm["traffic"] = sum(parts) if all(p is not None for p in parts) \
             else None
         kernels.append(adm)
