@@ -500,7 +500,7 @@ def serving_comparison(args, pkg, wl, torch, dev, d, w, inserted):
     plain_curve = [run_plain(f * base_rate) for f in (0.5, 0.7, 0.8, 0.9)]
     progress("plain sweep done")
     serve_curve = [run_serving(x * base_rate)
-                   for x in (1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0)]
+                   for x in (0.5, 0.7, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0)]
     serve_sat = run_serving(None)
     progress("serving sweep done")
 
