@@ -141,14 +141,15 @@ class FlixImpl : public FlixBase {
       GD_HIP(rs::sort_pairs<K>(keys, nn, ka.get(), va.get(), kb.get(),
                                vb.get(), ctrl.get(), s));
       Lease<u64> begin(pool_, u64(P_) + 1);
-      Lease<u32> work(pool_, std::min<u64>(P_, n) + 1);
-      const u32 work_items = slices(ka.get(), n, begin.get(), work.get(), s);
-      if (work_items) {
-        delete_kernel<K, V><<<(work_items + 3) / 4, 256, 0, s>>>(
-            pool_view(), ring_view(), ka.get(), begin.get(), work.get(),
-            work_items);
-        GD_LAUNCH_CHECK();
-      }
+      const u64 most = std::min<u64>(P_, n);  // touched buckets, at most
+      Lease<u32> work(pool_, most + 1);
+      const u32 *count = slices(ka.get(), n, begin.get(), work.get(), s);
+      // a wave per touched bucket, the grid sized by the upper bound (capped;
+      // the waves stride over the list)
+      const u64 blocks = std::min<u64>((most + 3) / 4, u64(1) << 20);
+      delete_kernel<K, V><<<static_cast<u32>(blocks), 256, 0, s>>>(
+          pool_view(), ring_view(), ka.get(), begin.get(), work.get(), count);
+      GD_LAUNCH_CHECK();
     } catch (...) {
       order_.end(s);
       throw;
@@ -433,7 +434,9 @@ class FlixImpl : public FlixBase {
   }
 
   // Bucket slices of a sorted batch + compact list of touched buckets.
-  u32 slices(const K *batch, u64 m, u64 *begin, u32 *work, hipStream_t s) {
+  // Returns the device word holding the list's length (behind the list).
+  const u32 *slices(const K *batch, u64 m, u64 *begin, u32 *work,
+                    hipStream_t s) {
     bucket_slices_kernel<K><<<grid_for(u64(P_) + 1, 256), 256, 0, s>>>(
         maxv_, P_, batch, m, begin);
     GD_LAUNCH_CHECK();
@@ -443,11 +446,7 @@ class FlixImpl : public FlixBase {
                            (u64(P_) + kWorkListBuckets - 1) / kWorkListBuckets),
                        256, 0, s>>>(begin, P_, work, count);
     GD_LAUNCH_CHECK();
-    u32 items = 0;
-    GD_HIP(hipMemcpyAsync(&items, count, sizeof(u32), hipMemcpyDeviceToHost,
-                          s));
-    GD_HIP(hipStreamSynchronize(s));
-    return items;
+    return count;
   }
 
   // Lays sorted unique (key, value) rows into a fresh bucket layer.

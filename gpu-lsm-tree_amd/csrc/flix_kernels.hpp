@@ -530,17 +530,12 @@ static __global__ void plan_reset_kernel(InsertPlan *plan) {
   *plan = InsertPlan{0u, 0u, 0u, 0u, 0u, 0u, 0ull};
 }
 
-// One wave per bucket with a non-empty slice of the sorted delete batch.
+// Removes bucket b's slice of the sorted delete batch from its chain (one
+// wave): a ballot keep-mask compacts each node, emptied non-head nodes are
+// unlinked and returned to the ring, bsize/blen follow.
 template <class K, class V>
-static __global__ __launch_bounds__(256) void delete_kernel(
-    Pool<K, V> p, Ring ring, const K *dk, const u64 *begin, const u32 *work,
-    u32 work_count) {
-  constexpr int kWavesPerBlock = 4;
-  const u32 lane = lane_id();
-  const u32 w = threadIdx.x / kWave;
-  const u32 item = blockIdx.x * kWavesPerBlock + w;
-  if (item >= work_count) return;
-  const u32 b = work[item];
+__device__ void delete_bucket(Pool<K, V> &p, Ring &ring, const K *dk,
+                              const u64 *begin, u32 b, u32 lane) {
   u64 pos = begin[b];
   const u64 end = begin[b + 1];
   const u32 S = p.S;
@@ -603,6 +598,22 @@ static __global__ __launch_bounds__(256) void delete_kernel(
     p.bsize[b] -= removed;
     p.blen[b] -= unlinked;
   }
+}
+
+// One wave per bucket with a non-empty slice of the sorted delete batch;
+// the waves stride over the work list, whose length the device holds (the
+// grid is sized by an upper bound, so the host reads nothing back).
+template <class K, class V>
+static __global__ __launch_bounds__(256) void delete_kernel(
+    Pool<K, V> p, Ring ring, const K *dk, const u64 *begin, const u32 *work,
+    const u32 *work_count) {
+  constexpr u32 kWavesPerBlock = 4;
+  const u32 lane = lane_id();
+  const u32 w = threadIdx.x / kWave;
+  const u32 items = *work_count;
+  for (u32 item = blockIdx.x * kWavesPerBlock + w; item < items;
+       item += gridDim.x * kWavesPerBlock)
+    delete_bucket(p, ring, dk, begin, work[item], lane);
 }
 
 // Point lookups (any order).  miss -> all-ones value.
