@@ -1044,6 +1044,18 @@ def main():
             prof["lookup_launches"],
             {"Q": Q, "R_rows": int(rows), "L_runs": int(runs),
              "formula": "Q*(K+V+1) + min(R*(K+V), Q*L*64)"}))
+    if prof["range_launches"]:
+        hits = int(counts.sum())
+        L = prof["range_runs"] / prof["range_launches"]
+        kernels.append(roof(
+            "range_kernel", prof["range_ms"],
+            R * (2 * 4 + 4) + min(rows * 8, hits * 8 + int(L * 64) * R),
+            prof["range_launches"],
+            {"Q": R, "R_rows": int(rows), "L_runs": L, "hits_H": hits,
+             "formula": "Q*(2K+4) + min(R*(K+V), sum_q H_q*(K+V) + Q*L*64)",
+             "note": "counts by rank difference read two row windows per "
+                     "query and run; the formula's bytes are the contract's "
+                     "algorithmic bytes, not the kernel's traffic"}))
     if prof["admission_batches"]:
         adm = roof(
             "admission", prof["admission_ms"], prof["admission_keys"] * 16,

@@ -178,6 +178,10 @@ class _Profile(ctypes.Structure):
         ("admission_batches", ctypes.c_uint64),
         ("admission_ms", ctypes.c_double),
         ("admission_keys", ctypes.c_uint64),
+        ("range_launches", ctypes.c_uint64),
+        ("range_ms", ctypes.c_double),
+        ("range_queries", ctypes.c_uint64),
+        ("range_runs", ctypes.c_uint64),
     ]
 
 

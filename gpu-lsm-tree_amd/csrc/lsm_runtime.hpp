@@ -335,10 +335,16 @@ class LsmT : public HandleBase {
     const RunSet<T> set = run_set();
     const u64 waves = (n + range_queries_per_wave(set.count) - 1) /
                       range_queries_per_wave(set.count);
+    ProfRec rec{};
+    if (profiling_) rec = prof_begin(3, s, n);
     range_kernel<T><<<grid_for(waves, kRangeWaves, 1u << 16), kRangeThreads, 0,
                       s>>>(PlainRange<T>{lo, hi, out, count_only ? 1 : 0}, n,
                            set);
     GD_LAUNCH_CHECK();
+    if (profiling_) {
+      rec.runs = set.count;
+      prof_end(rec, s);
+    }
     order_.end(s);
   }
 
@@ -819,6 +825,11 @@ class LsmT : public HandleBase {
         out->lookup_launches += 1;
         out->lookup_ms += ms;
         out->lookup_queries += r.n;
+      } else if (r.kind == 3) {
+        out->range_launches += 1;
+        out->range_ms += ms;
+        out->range_queries += r.n;
+        out->range_runs += r.runs;
       } else {
         out->admission_batches += 1;
         out->admission_ms += ms;
@@ -1360,9 +1371,10 @@ class LsmT : public HandleBase {
                         1, 512, kMaxWay, 2};
 
   struct ProfRec {
-    int kind;  // 0 merge, 1 lookup, 2 admission of one tile
+    int kind;  // 0 merge, 1 lookup, 2 admission of one tile, 3 range
     hipEvent_t start, end;
-    u64 slot, n;  // n: merge inputs or lookup queries
+    u64 slot, n;  // n: merge inputs, lookup or range queries, tile keys
+    u64 runs;     // range: runs searched
   };
   static constexpr u64 kProfSlots = 4096;
   static constexpr u64 kProfMetas = kMaxWay + 1;  // inputs, then the output
@@ -1372,7 +1384,7 @@ class LsmT : public HandleBase {
   u64 prof_next_ = 0;
 
   ProfRec prof_begin(int kind, hipStream_t s, u64 n) {
-    ProfRec r{kind, nullptr, nullptr, prof_next_++, n};
+    ProfRec r{kind, nullptr, nullptr, prof_next_++, n, 0};
     GD_HIP(hipEventCreate(&r.start));
     GD_HIP(hipEventCreate(&r.end));
     GD_HIP(hipEventRecord(r.start, s));

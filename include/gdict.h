@@ -204,10 +204,10 @@ gd_status gd_lsm_set_tuning(gd_lsm *h, const gd_lsm_tuning *tuning);
 gd_status gd_lsm_get_tuning(gd_lsm *h, gd_lsm_tuning *out);
 
 /* No reference counterpart: opt-in per-kernel timing (HIP events on the
- * launch stream) of the merge and lookup kernels, with the merges' row
- * counts for their algorithmic bytes (rows_in + rows_out) * 8, and of the
- * admission of every update tile (all of its kernels, from the first to
- * the last). */
+ * launch stream) of the merge, lookup and range kernels, with the merges'
+ * row counts for their algorithmic bytes (rows_in + rows_out) * 8, and of
+ * the admission of every update tile (all of its kernels, from the first
+ * to the last). */
 typedef struct gd_lsm_profile {
   uint64_t merge_launches;
   double merge_ms;
@@ -219,6 +219,10 @@ typedef struct gd_lsm_profile {
   uint64_t admission_batches;  /* admitted tiles          */
   double admission_ms;
   uint64_t admission_keys;     /* keys of those tiles     */
+  uint64_t range_launches;     /* range_kernel launches   */
+  double range_ms;
+  uint64_t range_queries;
+  uint64_t range_runs;         /* runs searched, summed over the launches */
 } gd_lsm_profile;
 
 gd_status gd_lsm_set_profiling(gd_lsm *h, int enable);

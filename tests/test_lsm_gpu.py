@@ -477,7 +477,17 @@ def test_kernel_profile_counts(gd, orc, io, ways):
     q = _queries(rng, p.o.dump()[0], 5000, 1 << 32)
     p.check_lookup(q, False)
     p.check_lookup(q[:3000], True)
+    lo = np.sort(rng.integers(0, 1 << 32, 700, dtype=np.uint64)).astype(
+        np.uint32)
+    hi = (lo.astype(np.uint64) + (1 << 24)).clip(max=(1 << 32) - 1).astype(
+        np.uint32)
+    p.check_ranges(lo, hi)  # a sum launch and a count launch
+    runs, _ = p.d.shape()
     prof = p.d.read_profile(reset=True)
+    assert prof["range_launches"] == 2
+    assert prof["range_queries"] == 1400
+    assert prof["range_runs"] == 2 * runs
+    assert prof["range_ms"] > 0
     if ways == 2:
         # eight equal batches under the binary counter: 4+2+1 merges
         assert prof["merge_launches"] == 7
@@ -497,6 +507,7 @@ def test_kernel_profile_counts(gd, orc, io, ways):
     assert prof["lookup_ms"] > 0
     again = p.d.read_profile(reset=False)
     assert again["merge_launches"] == 0 and again["lookup_launches"] == 0
+    assert again["range_launches"] == 0
     p.d.set_profiling(False)
 
 
