@@ -1068,13 +1068,18 @@ def main():
                         "2^20-key tile (one event pair around them)"})
         # medians: the tile histogram and scan kernels also serve the
         # lookups' query partition (two 2^26-key dispatches per profile)
+        adm["composite"] = True
         parts = [pmc_traffic(k, "MEDIAN_PER_DISPATCH")
                  for k in ADMISSION_KERNELS]
         adm["traffic"] = sum(parts) if all(p is not None for p in parts) \
             else None
         kernels.append(adm)
     kernels.sort(key=lambda k: -k["avg_launch_ms"] * k["launches"])
-    roofline = dict(kernels[0]) if kernels else None
+    # the headline is the kernel with the largest time per step; the
+    # admission entry spans four kernels (each below the merges' time) and
+    # stays listed beside it
+    single = [k for k in kernels if not k.get("composite")]
+    roofline = dict(single[0]) if single else None
 
     result = {
         "metric": METRIC,
