@@ -195,6 +195,37 @@ def test_allocation_region_growth(gd, orc, io):
     p.check_all(rng, 2, 1 << 20)
 
 
+def test_memory_stats_and_trim(gd, orc, io):
+    """gd_flix_memory: held = live + cached, the peak bounds the held bytes,
+    the node layer's bytes grow with a growth rebuild; trim returns the
+    cache and changes no answer."""
+    rng = np.random.default_rng(21)
+    p = FlixPair(gd, orc, io, 8, 8, node_slots=32)
+    keys = np.unique(rng.integers(2, 1 << 40, 50000, dtype=np.uint64))
+    p.build(keys, max_size=60000)
+
+    def check(m):
+        assert m["held_bytes"] == m["live_bytes"] + m["cached_bytes"]
+        assert m["peak_held_bytes"] >= m["held_bytes"]
+        assert m["held_bytes"] == p.f.gpu_resident_bytes()
+        assert m["live_bytes"] >= m["row_bytes_ub"] > 0
+
+    m0 = p.f.memory()
+    check(m0)
+    ins = rng.integers(2, 1 << 40, 400000, dtype=np.uint64)
+    p.insert(ins, ins ^ np.uint64(0x5DEECE66D))  # grows the region
+    m1 = p.f.memory()
+    check(m1)
+    assert m1["row_bytes_ub"] > m0["row_bytes_ub"]
+    p.remove(ins[:100000])
+    check(p.f.memory())
+    p.f.trim()
+    m2 = p.f.memory()
+    assert m2["cached_bytes"] == 0 and m2["held_bytes"] == m2["live_bytes"]
+    check(m2)
+    p.check_all(rng, 2, 1 << 40)
+
+
 @pytest.mark.parametrize("slots", [8, 32])
 def test_long_chains(gd, orc, io, slots):
     """Small batches aimed at a few buckets grow their chains past what one

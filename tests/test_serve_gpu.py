@@ -136,6 +136,9 @@ def test_served_results_equal_step_replay(gd, orc, io, seed, span, max_size,
     st = srv.stats()
     assert st["requests"] == len(reqs)
     assert st["steps"] < len(reqs)  # requests were coalesced
+    # device memory as of the last completed step
+    assert 0 < st["live_bytes"] <= st["resident_bytes"]
+    assert st["peak_resident_bytes"] >= st["resident_bytes"]
     _check_against_replay(gd, orc, io, reqs, served, cfg)
     srv.close()
 
