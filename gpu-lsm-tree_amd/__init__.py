@@ -269,6 +269,7 @@ SIGNATURES = {
     "gd_flix_build": [_P, _P, _U64, _U64, _P],
     "gd_flix_insert": [_P, _P, _P, _U64, _P],
     "gd_flix_remove": [_P, _P, _U64, _P],
+    "gd_flix_remove_sorted": [_P, _P, _U64, _P],
     "gd_flix_lookup_sorted": [_P, _P, _U64, _P, _P],
     "gd_flix_lookup": [_P, _P, _U64, _P, _P],
     "gd_flix_successor_sorted": [_P, _P, _U64, _P, _P],
@@ -708,9 +709,13 @@ class FlixIndex:
         self._call("gd_flix_insert", self._k(keys, "keys", n),
                    self._v(values, "values", n), n, _stream(stream))
 
-    def remove(self, keys, count=None, stream=None):
+    def remove(self, keys, count=None, stream=None, assume_sorted=False):
+        """assume_sorted=True: the keys are ascending already (the
+        reference's remove(sorted_keys, ...) contract) and are not sorted
+        again; unsorted keys then give undefined results."""
         n = _count(keys, count)
-        self._call("gd_flix_remove", self._k(keys, "keys", n), n,
+        self._call("gd_flix_remove_sorted" if assume_sorted
+                   else "gd_flix_remove", self._k(keys, "keys", n), n,
                    _stream(stream))
 
     def lookups_ordered(self, sorted_keys, out_values, count=None,

@@ -43,7 +43,7 @@ class FlixBase {
   virtual void build(const void *keys, u64 n, u64 max_size, hipStream_t s) = 0;
   virtual void insert(const void *keys, const void *vals, u64 n,
                       hipStream_t s) = 0;
-  virtual void remove(const void *keys, u64 n, hipStream_t s) = 0;
+  virtual void remove(const void *keys, u64 n, bool sorted, hipStream_t s) = 0;
   virtual void lookup(const void *q, u64 n, void *out, hipStream_t s) = 0;
   virtual void successor(const void *q, u64 n, void *out, hipStream_t s) = 0;
   virtual void range(const void *lo, const void *hi, u64 n, u64 *offsets,
@@ -125,7 +125,9 @@ class FlixImpl : public FlixBase {
     order_.end(s);
   }
 
-  void remove(const void *keys_v, u64 n, hipStream_t s) override {
+  // sorted: the caller's keys are ascending already (the reference's
+  // remove(sorted_keys, ...) contract), so they are not sorted again.
+  void remove(const void *keys_v, u64 n, bool sorted, hipStream_t s) override {
     require_built();
     if (!n) return;
     const K *keys = static_cast<const K *>(keys_v);
@@ -135,20 +137,28 @@ class FlixImpl : public FlixBase {
     order_.begin(s);
     try {
       const u32 nn = static_cast<u32>(n);
-      Lease<K> ka(pool_, n), kb(pool_, n);
-      Lease<u32> va(pool_, n), vb(pool_, n);
-      Lease<u32> ctrl(pool_, rs::control_words(n, rs::passes_for<K>()));
-      GD_HIP(rs::sort_pairs<K>(keys, nn, ka.get(), va.get(), kb.get(),
-                               vb.get(), ctrl.get(), s));
+      Lease<K> ka, kb;
+      Lease<u32> va, vb, ctrl;
+      const K *batch = keys;
+      if (!sorted) {
+        ka = Lease<K>(pool_, n);
+        kb = Lease<K>(pool_, n);
+        va = Lease<u32>(pool_, n);
+        vb = Lease<u32>(pool_, n);
+        ctrl = Lease<u32>(pool_, rs::control_words(n, rs::passes_for<K>()));
+        GD_HIP(rs::sort_pairs<K>(keys, nn, ka.get(), va.get(), kb.get(),
+                                 vb.get(), ctrl.get(), s));
+        batch = ka.get();
+      }
       Lease<u64> begin(pool_, u64(P_) + 1);
       const u64 most = std::min<u64>(P_, n);  // touched buckets, at most
       Lease<u32> work(pool_, most + 1);
-      const u32 *count = slices(ka.get(), n, begin.get(), work.get(), s);
+      const u32 *count = slices(batch, n, begin.get(), work.get(), s);
       // a wave per touched bucket, the grid sized by the upper bound (capped;
       // the waves stride over the list)
       const u64 blocks = std::min<u64>((most + 3) / 4, u64(1) << 20);
       delete_kernel<K, V><<<static_cast<u32>(blocks), 256, 0, s>>>(
-          pool_view(), ring_view(), ka.get(), begin.get(), work.get(), count);
+          pool_view(), ring_view(), batch, begin.get(), work.get(), count);
       GD_LAUNCH_CHECK();
     } catch (...) {
       order_.end(s);
@@ -588,7 +598,12 @@ gd_status gd_flix_insert(gd_flix *h, const void *keys, const void *values,
 
 gd_status gd_flix_remove(gd_flix *h, const void *keys, uint64_t count,
                          hipStream_t s) {
-  GD_FLIX_CALL(h, f->remove(keys, count, s));
+  GD_FLIX_CALL(h, f->remove(keys, count, false, s));
+}
+
+gd_status gd_flix_remove_sorted(gd_flix *h, const void *sorted_keys,
+                                uint64_t count, hipStream_t s) {
+  GD_FLIX_CALL(h, f->remove(sorted_keys, count, true, s));
 }
 
 gd_status gd_flix_lookup_sorted(gd_flix *h, const void *sorted_queries,

@@ -530,14 +530,15 @@ static __global__ void plan_reset_kernel(InsertPlan *plan) {
   *plan = InsertPlan{0u, 0u, 0u, 0u, 0u, 0u, 0ull};
 }
 
-// Removes bucket b's slice of the sorted delete batch from its chain (one
-// wave): a ballot keep-mask compacts each node, emptied non-head nodes are
-// unlinked and returned to the ring, bsize/blen follow.
+// Removes bucket b's slice dk[pos, end) of the sorted delete batch from its
+// chain (one wave): a ballot keep-mask compacts each node, emptied non-head
+// nodes are unlinked and returned to the ring, bsize/blen follow.  dk is the
+// batch in global memory or the wave's copy of the slice in LDS (then pos =
+// 0); every search of the slice reads dk.
 template <class K, class V>
-__device__ void delete_bucket(Pool<K, V> &p, Ring &ring, const K *dk,
-                              const u64 *begin, u32 b, u32 lane) {
-  u64 pos = begin[b];
-  const u64 end = begin[b + 1];
+__device__ __forceinline__ void delete_bucket(Pool<K, V> &p, Ring &ring,
+                                              const K *dk, u64 pos, u64 end,
+                                              u32 b, u32 lane) {
   const u32 S = p.S;
   u32 node = b;
   u32 prev = ~0u;
@@ -603,17 +604,37 @@ __device__ void delete_bucket(Pool<K, V> &p, Ring &ring, const K *dk,
 // One wave per bucket with a non-empty slice of the sorted delete batch;
 // the waves stride over the work list, whose length the device holds (the
 // grid is sized by an upper bound, so the host reads nothing back).
+// A slice of at most kDeleteStage keys is copied to the wave's LDS first, so
+// the per-node searches of the slice (one per lane and one for the next
+// node's first key) read LDS instead of dependent global loads.
+constexpr u32 kDeleteStage = 512;
+
 template <class K, class V>
 static __global__ __launch_bounds__(256) void delete_kernel(
     Pool<K, V> p, Ring ring, const K *dk, const u64 *begin, const u32 *work,
     const u32 *work_count) {
   constexpr u32 kWavesPerBlock = 4;
+  __shared__ K s_dk[kWavesPerBlock][kDeleteStage];
   const u32 lane = lane_id();
   const u32 w = threadIdx.x / kWave;
   const u32 items = *work_count;
   for (u32 item = blockIdx.x * kWavesPerBlock + w; item < items;
-       item += gridDim.x * kWavesPerBlock)
-    delete_bucket(p, ring, dk, begin, work[item], lane);
+       item += gridDim.x * kWavesPerBlock) {
+    const u32 b = work[item];
+    const u64 pos = begin[b], end = begin[b + 1];
+    const u64 m = end - pos;
+    if (m <= kDeleteStage) {
+      K *sk = s_dk[w];
+      // the previous bucket's searches are done before the slot is reused
+      __builtin_amdgcn_wave_barrier();
+      for (u32 j = lane; j < m; j += kWave) sk[j] = dk[pos + j];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      delete_bucket(p, ring, static_cast<const K *>(sk), 0, m, b, lane);
+    } else {
+      delete_bucket(p, ring, dk, pos, end, b, lane);
+    }
+  }
 }
 
 // Point lookups (any order).  miss -> all-ones value.

@@ -400,9 +400,15 @@ gd_status gd_flix_build(gd_flix *h, const void *keys, uint64_t count,
  * node plan are read back). */
 gd_status gd_flix_insert(gd_flix *h, const void *keys, const void *values,
                          uint64_t count, hipStream_t stream);
-/* remove(sorted_keys, n, stream)  impl:2253 */
+/* remove(sorted_keys, n, stream)  impl:2253, for keys in any order (they
+ * are sorted first); duplicates and absent keys are ignored. */
 gd_status gd_flix_remove(gd_flix *h, const void *keys, uint64_t count,
                          hipStream_t stream);
+/* remove(sorted_keys, n, stream)  impl:2253 with the reference's contract:
+ * the keys are ascending (duplicates allowed) and are not sorted again.
+ * Unsorted keys give undefined results, as in the reference. */
+gd_status gd_flix_remove_sorted(gd_flix *h, const void *sorted_keys,
+                                uint64_t count, hipStream_t stream);
 /* lookups_ordered(sorted_keys, results, n, stream)  impl:2485: queries must
  * be ascending; results are in query order; miss -> all-ones value. */
 gd_status gd_flix_lookup_sorted(gd_flix *h, const void *sorted_queries,

@@ -32,8 +32,9 @@ class FlixPair:
         self.f.insert(self.up_k(keys), self.up_v(vals), count=len(keys))
         self.o.insert(keys, vals)
 
-    def remove(self, keys):
-        self.f.remove(self.up_k(keys), count=len(keys))
+    def remove(self, keys, assume_sorted=False):
+        self.f.remove(self.up_k(keys), count=len(keys),
+                      assume_sorted=assume_sorted)
         self.o.remove(keys)
 
     def lookup(self, q, sorted_api=False):
@@ -193,6 +194,36 @@ def test_allocation_region_growth(gd, orc, io):
     p.insert(rng.integers(2, 1 << 20, 200000, dtype=np.uint64)
              .astype(np.uint32), np.arange(200000, dtype=np.uint32))
     p.check_all(rng, 2, 1 << 20)
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_sorted_remove(gd, orc, io, kb):
+    """remove(assume_sorted=True), the reference's sorted-keys contract:
+    ascending batches with duplicates and absent keys, slices small enough
+    for the wave's LDS copy and a slice of 3000 keys inside one bucket
+    (searched in global memory), all against the oracle."""
+    rng = np.random.default_rng(31 + kb)
+    kt = KT[kb]
+    p = FlixPair(gd, orc, io, kb, kb, node_slots=32)
+    keys = (np.arange(1, 30001, dtype=np.uint64) * 100003).astype(kt)
+    p.build(keys, max_size=60000)
+    ins = rng.integers(2, 30001 * 100003, 20000, dtype=np.uint64).astype(kt)
+    p.insert(ins, ins)
+    live = p.o.dump()[0]
+    for r in range(3):
+        d = np.sort(np.concatenate([
+            live[rng.integers(0, len(live), 5000)],  # with duplicates
+            rng.integers(2, 30001 * 100003, 2000, dtype=np.uint64)
+            .astype(kt)]))
+        p.remove(d, assume_sorted=True)
+        live = p.o.dump()[0]
+        p.check_all(rng, 2, 30001 * 100003)
+    # 3000 absent keys between two stored keys, plus the keys around them
+    base = int(keys[777])
+    d = np.concatenate([[base], np.arange(base + 1, base + 3001),
+                        [int(keys[778])]]).astype(kt)
+    p.remove(d, assume_sorted=True)
+    p.check_all(rng, 2, 30001 * 100003)
 
 
 def test_memory_stats_and_trim(gd, orc, io):
