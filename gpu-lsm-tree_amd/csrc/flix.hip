@@ -276,7 +276,8 @@ class FlixImpl : public FlixBase {
 
   Pool<K, V> pool_view() const {
     return Pool<K, V>{node_keys_, node_vals_, node_count_, node_next_,
-                      maxv_, bsize_, blen_, S_, P_, total_nodes_};
+                      maxv_,      bsize_,     blen_,       S_,
+                      P_,         total_nodes_, bdir_,     bdir_shift_};
   }
 
   Ring ring_view() const {
@@ -290,6 +291,7 @@ class FlixImpl : public FlixBase {
     pool_.put(node_count_);
     pool_.put(node_next_);
     pool_.put(maxv_);
+    pool_.put(bdir_);
     pool_.put(bsize_);
     pool_.put(blen_);
     pool_.put(ring_ids_);
@@ -299,6 +301,7 @@ class FlixImpl : public FlixBase {
     node_count_ = nullptr;
     node_next_ = nullptr;
     maxv_ = nullptr;
+    bdir_ = nullptr;
     bsize_ = nullptr;
     blen_ = nullptr;
     ring_ids_ = nullptr;
@@ -475,6 +478,10 @@ class FlixImpl : public FlixBase {
     u32 *count = pool_.get_n<u32>(total);
     u32 *next = pool_.get_n<u32>(total);
     K *maxv = pool_.get_n<K>(P);
+    // bucket directory: 2^d slices, 2^d <= P (at least 2)
+    u32 d = 1;
+    while (d < 24 && (u64(1) << (d + 1)) <= P) ++d;
+    u32 *bdir = pool_.get_n<u32>((u64(1) << d) + 1);
     u32 *bsize = pool_.get_n<u32>(P);
     u32 *blen = pool_.get_n<u32>(P);
     u32 *ring = pool_.get_n<u32>(region);
@@ -485,6 +492,9 @@ class FlixImpl : public FlixBase {
     node_count_ = count;
     node_next_ = next;
     maxv_ = maxv;
+    bdir_ = bdir;
+    bdir_d_ = d;
+    bdir_shift_ = static_cast<u32>(8 * sizeof(K)) - d;
     bsize_ = bsize;
     blen_ = blen;
     ring_ids_ = ring;
@@ -500,6 +510,13 @@ class FlixImpl : public FlixBase {
     Pool<K, V> pv = pool_view();
     layout_kernel<K, V><<<grid_for(P * S_, 256, 1u << 16), 256, 0, s>>>(
         pv, maxv_, uk, uv, m, fill);
+    const u64 slices = u64(1) << bdir_d_;
+    bucket_dir_clear_kernel<<<grid_for(slices, 256, 1u << 16), 256, 0, s>>>(
+        bdir_, slices);
+    bucket_dir_kernel<K><<<grid_for(P, 256), 256, 0, s>>>(
+        maxv_, P_, bdir_, bdir_d_, bdir_shift_);
+    bucket_dir_fill_kernel<K><<<grid_for(slices, 256, 1u << 16), 256, 0, s>>>(
+        maxv_, P_, bdir_, bdir_d_, bdir_shift_);
     ring_init_kernel<<<grid_for(region, 256, 1u << 16), 256, 0, s>>>(
         ring_ids_, P, region);
     ring_counters_init_kernel<<<1, 1, 0, s>>>(counters_, region);
@@ -520,6 +537,8 @@ class FlixImpl : public FlixBase {
   u32 *node_count_ = nullptr;
   u32 *node_next_ = nullptr;
   K *maxv_ = nullptr;
+  u32 *bdir_ = nullptr;  // bucket directory (2^bdir_d_ + 1 entries)
+  u32 bdir_d_ = 1, bdir_shift_ = 0;
   u32 *bsize_ = nullptr;  // live keys of each bucket's chain
   u32 *blen_ = nullptr;   // nodes of each bucket's chain
   u32 *ring_ids_ = nullptr;

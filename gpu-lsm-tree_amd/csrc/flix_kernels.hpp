@@ -10,6 +10,10 @@
 //                     208-240, find_maxvalues_kernel)
 //   bsize[b], blen[b] live keys and nodes of bucket b's chain, kept by every
 //                     update so an insert can size its new nodes up front
+//   bdir[j]           bucket directory: the first bucket whose max is >= the
+//                     j-th of 2^d equal key slices (j << (key bits - d)),
+//                     2^d about the bucket count, so finding a key's
+//                     bucket is a short search in [bdir[j], bdir[j + 1]]
 // Nodes [0, P) are the bucket heads; nodes [P, total) form the allocation
 // region, handed out through a ring of free node ids (bump order at build,
 // released nodes of deletes appended).  Inside a chain every key of node i
@@ -40,6 +44,8 @@ struct Pool {
   u32 S;
   u32 buckets;
   u64 total_nodes;
+  const u32 *bdir;  // 2^d + 1 entries
+  u32 bdir_shift;   // key bits - d
 };
 
 // Ring of free node ids.  alloc_cursor / free_count grow monotonically;
@@ -51,16 +57,66 @@ struct Ring {
   unsigned long long *free_count;
 };
 
-template <class K>
-__device__ __forceinline__ u32 bucket_of(const K *maxv, u32 buckets, K key) {
-  // first bucket whose max >= key
-  u32 lo = 0, hi = buckets - 1;
+// First bucket whose max >= key: the directory slice of the key bounds it
+// (the last bucket's max is all-ones, so every key has one).
+template <class K, class V>
+__device__ __forceinline__ u32 bucket_of(const Pool<K, V> &p, K key) {
+  const u64 j = static_cast<u64>(key) >> p.bdir_shift;
+  u32 lo = p.bdir[j], hi = p.bdir[j + 1];
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
-    if (maxv[mid] < key) lo = mid + 1;
+    if (p.maxv[mid] < key) lo = mid + 1;
     else hi = mid;
   }
   return lo;
+}
+
+// bdir[j] = first bucket whose max >= j << shift (j < 2^d), bdir[2^d] = the
+// last bucket.  Bucket b is that bucket for the slices j with
+// maxv[b-1] >> shift < j <= maxv[b] >> shift; bucket_dir_kernel writes them
+// for a thread per bucket when they are at most kDirSpan, and
+// bucket_dir_fill_kernel finds the rest (slices inside wide gaps between
+// keys, left at ~0 by the clear) by a lower bound over maxv.
+constexpr u32 kDirSpan = 64;
+
+static __global__ void bucket_dir_clear_kernel(u32 *bdir, u64 n) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    bdir[i] = ~0u;
+}
+
+template <class K>
+static __global__ void bucket_dir_kernel(const K *maxv, u32 buckets, u32 *bdir,
+                                         u32 d, u32 shift) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 bb = u64(blockIdx.x) * blockDim.x + threadIdx.x; bb < buckets;
+       bb += stride) {
+    const u32 b = static_cast<u32>(bb);
+    const u64 j0 = b == 0 ? 0 : (static_cast<u64>(maxv[b - 1]) >> shift) + 1;
+    const u64 j1 = static_cast<u64>(maxv[b]) >> shift;  // < 2^d
+    if (j0 <= j1 && j1 - j0 < kDirSpan)
+      for (u64 j = j0; j <= j1; ++j) bdir[j] = b;
+    if (b + 1 == buckets) bdir[u64(1) << d] = b;
+  }
+}
+
+template <class K>
+static __global__ void bucket_dir_fill_kernel(const K *maxv, u32 buckets,
+                                              u32 *bdir, u32 d, u32 shift) {
+  const u64 slices = u64(1) << d;
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 j = u64(blockIdx.x) * blockDim.x + threadIdx.x; j < slices;
+       j += stride) {
+    if (bdir[j] != ~0u) continue;
+    const K v = static_cast<K>(j << shift);
+    u32 lo = 0, hi = buckets - 1;
+    while (lo < hi) {
+      const u32 mid = (lo + hi) >> 1;
+      if (maxv[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    bdir[j] = lo;
+  }
 }
 
 // lower_bound over a sorted global array segment [lo, hi).
@@ -158,7 +214,7 @@ static __global__ void mark_new_kernel(Pool<K, V> p, const K *batch, u64 n,
     const K key = batch[i];
     bool fresh = i == 0 || batch[i - 1] != key;
     if (fresh) {
-      const u32 b = bucket_of(p.maxv, p.buckets, key);
+      const u32 b = bucket_of(p, key);
       const u32 node = owner_node(p, b, key);
       const u32 c = p.count[node];
       const K *nk = p.keys + u64(node) * p.S;
@@ -644,7 +700,7 @@ static __global__ void lookup_kernel(Pool<K, V> p, const K *q, u64 n, V *out) {
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride) {
     const K key = q[i];
-    const u32 b = bucket_of(p.maxv, p.buckets, key);
+    const u32 b = bucket_of(p, key);
     const u32 node = owner_node(p, b, key);
     const u32 c = p.count[node];
     const K *nk = p.keys + u64(node) * p.S;
@@ -664,7 +720,7 @@ template <class K, class V>
 __device__ __forceinline__ void ceiling_position(const Pool<K, V> &p, K key,
                                                  u32 &node_out,
                                                  u32 &slot_out) {
-  u32 b = bucket_of(p.maxv, p.buckets, key);
+  u32 b = bucket_of(p, key);
   u32 node = owner_node(p, b, key);
   while (true) {
     const u32 c = p.count[node];
@@ -740,11 +796,10 @@ static __global__ void range_kernel(Pool<K, V> p, const K *lo, const K *hi, u64 
     if (a <= z) {
       u32 node, slot;
       ceiling_position(p, a, node, slot);
-      u32 bucket = node == ~0u ? 0 : bucket_of(p.maxv, p.buckets, a);
       // the bucket index only matters for crossing to the next bucket:
-      // recover the bucket that owns `node` by its first key
-      if (node != ~0u) bucket = bucket_of(p.maxv, p.buckets,
-                                          p.keys[u64(node) * p.S + slot]);
+      // the bucket that owns `node`, found by the key at the position
+      u32 bucket =
+          node == ~0u ? 0u : bucket_of(p, p.keys[u64(node) * p.S + slot]);
       u64 w = out_keys ? offsets[i] : 0;
       while (node != ~0u) {
         const K k = p.keys[u64(node) * p.S + slot];
