@@ -199,15 +199,17 @@ class FlixImpl : public FlixBase {
     order_.begin(s);
     try {
       const Pool<K, V> pv = pool_view();
+      // a wave per query (4 per block)
+      const u32 blocks = grid_for(n, 4, 1u << 16);
       if (!ok) {
         Lease<u64> counts(pool_, n + 1);
-        range_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+        range_wave_kernel<K, V><<<blocks, 256, 0, s>>>(
             pv, static_cast<const K *>(lo), static_cast<const K *>(hi), n,
             counts.get(), nullptr, nullptr);
         GD_LAUNCH_CHECK();
         exclusive_scan(counts.get(), n, offsets, s);
       } else {
-        range_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+        range_wave_kernel<K, V><<<blocks, 256, 0, s>>>(
             pv, static_cast<const K *>(lo), static_cast<const K *>(hi), n,
             offsets, static_cast<K *>(ok), static_cast<V *>(ov));
         GD_LAUNCH_CHECK();

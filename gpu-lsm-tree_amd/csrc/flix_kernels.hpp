@@ -784,36 +784,57 @@ static __global__ void successor_kernel(Pool<K, V> p, const K *q, u64 n, K *out)
   }
 }
 
-// Range [lo, hi]: count pass (out_keys == nullptr) or fill pass.
+// Range [lo, hi] with a wave per query: the lanes hold one node's slots, so
+// a node's keys in range leave with one ballot and coalesced stores, and a
+// query walks its range a node per step rather than a key per step.  Count
+// pass (out_keys == nullptr) or fill pass; S <= 64.
 template <class K, class V>
-static __global__ void range_kernel(Pool<K, V> p, const K *lo, const K *hi, u64 n,
-                             u64 *offsets, K *out_keys, V *out_vals) {
-  const u64 stride = u64(gridDim.x) * blockDim.x;
-  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += stride) {
+static __global__ __launch_bounds__(256) void range_wave_kernel(
+    Pool<K, V> p, const K *lo, const K *hi, u64 n, u64 *offsets, K *out_keys,
+    V *out_vals) {
+  const u32 lane = lane_id();
+  const u64 waves = u64(gridDim.x) * (blockDim.x / kWave);
+  for (u64 i = u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+       i < n; i += waves) {
     const K a = lo[i], z = hi[i];
     u64 cnt = 0;
     if (a <= z) {
       u32 node, slot;
       ceiling_position(p, a, node, slot);
-      // the bucket index only matters for crossing to the next bucket:
-      // the bucket that owns `node`, found by the key at the position
       u32 bucket =
           node == ~0u ? 0u : bucket_of(p, p.keys[u64(node) * p.S + slot]);
       u64 w = out_keys ? offsets[i] : 0;
       while (node != ~0u) {
-        const K k = p.keys[u64(node) * p.S + slot];
-        if (k > z) break;
-        if (out_keys) {
-          out_keys[w] = k;
-          out_vals[w] = p.vals[u64(node) * p.S + slot];
-          ++w;
+        const u32 c = p.count[node];
+        const u64 base = u64(node) * p.S;
+        const bool mine = lane >= slot && lane < c;
+        K k = K(0);
+        if (mine) k = p.keys[base + lane];
+        const bool in = mine && k <= z;
+        // keys ascend inside a node: the lanes in range are a prefix
+        const u32 m = popc64(ballot(in));
+        if (out_keys && in) {
+          out_keys[w + (lane - slot)] = k;
+          out_vals[w + (lane - slot)] = p.vals[base + lane];
         }
-        ++cnt;
-        advance(p, node, slot, bucket);
+        cnt += m;
+        w += m;
+        if (slot + m < c) break;  // a key above the range
+        // next non-empty node in key order
+        slot = 0;
+        while (true) {
+          const u32 nxt = p.next[node];
+          if (nxt) node = nxt - 1;
+          else if (++bucket < p.buckets) node = bucket;
+          else {
+            node = ~0u;
+            break;
+          }
+          if (p.count[node]) break;
+        }
       }
     }
-    if (!out_keys) offsets[i] = cnt;
+    if (!out_keys && lane == 0) offsets[i] = cnt;
   }
 }
 
