@@ -131,6 +131,33 @@ __device__ __forceinline__ u64 lower_bound_g(const K *a, u64 lo, u64 hi,
   return lo;
 }
 
+// First slot of a node's c ascending keys whose key >= key, in two rounds
+// of independent loads instead of log2(c) dependent ones: the last key of
+// every 8-slot block (at most 8 of them, S <= 64) picks the block, then the
+// block's keys give the slot.
+template <class K>
+__device__ __forceinline__ u32 node_lower_bound(const K *nk, u32 c, K key) {
+  u32 blk = 0;
+  if (c > 8) {
+    K last[8];
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) {
+      const u32 at = 8 * j + 7;
+      last[j] = at < c ? nk[at] : K(0);
+    }
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) blk += (8 * j + 7 < c && last[j] < key);
+  }
+  const u32 base = 8 * blk;
+  K k[8];
+#pragma unroll
+  for (u32 j = 0; j < 8; ++j) k[j] = base + j < c ? nk[base + j] : K(0);
+  u32 below = 0;
+#pragma unroll
+  for (u32 j = 0; j < 8; ++j) below += (base + j < c && k[j] < key);
+  return base + below;
+}
+
 // Node of bucket b's chain that owns `key`.
 template <class K, class V>
 __device__ __forceinline__ u32 owner_node(const Pool<K, V> &p, u32 b, K key) {
@@ -218,12 +245,7 @@ static __global__ void mark_new_kernel(Pool<K, V> p, const K *batch, u64 n,
       const u32 node = owner_node(p, b, key);
       const u32 c = p.count[node];
       const K *nk = p.keys + u64(node) * p.S;
-      u32 lo = 0, hi = c;
-      while (lo < hi) {
-        const u32 mid = (lo + hi) >> 1;
-        if (nk[mid] < key) lo = mid + 1;
-        else hi = mid;
-      }
+      const u32 lo = node_lower_bound(nk, c, key);
       if (lo < c && nk[lo] == key) fresh = false;
     }
     keep[i] = fresh ? 1u : 0u;
@@ -704,12 +726,7 @@ static __global__ void lookup_kernel(Pool<K, V> p, const K *q, u64 n, V *out) {
     const u32 node = owner_node(p, b, key);
     const u32 c = p.count[node];
     const K *nk = p.keys + u64(node) * p.S;
-    u32 lo = 0, hi = c;
-    while (lo < hi) {
-      const u32 mid = (lo + hi) >> 1;
-      if (nk[mid] < key) lo = mid + 1;
-      else hi = mid;
-    }
+    const u32 lo = node_lower_bound(nk, c, key);
     out[i] = (lo < c && nk[lo] == key) ? p.vals[u64(node) * p.S + lo]
                                        : ~V(0);
   }
@@ -725,12 +742,7 @@ __device__ __forceinline__ void ceiling_position(const Pool<K, V> &p, K key,
   while (true) {
     const u32 c = p.count[node];
     const K *nk = p.keys + u64(node) * p.S;
-    u32 lo = 0, hi = c;
-    while (lo < hi) {
-      const u32 mid = (lo + hi) >> 1;
-      if (nk[mid] < key) lo = mid + 1;
-      else hi = mid;
-    }
+    const u32 lo = node_lower_bound(nk, c, key);
     if (lo < c) {
       node_out = node;
       slot_out = lo;
