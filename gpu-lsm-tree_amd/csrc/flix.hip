@@ -14,6 +14,7 @@
 #include "flix_kernels.hpp"
 #include "host_common.hpp"
 #include "radix_sort.hpp"
+#include "seg_admission.hpp"
 
 namespace gdict {
 namespace flix {
@@ -44,7 +45,9 @@ class FlixBase {
   virtual void insert(const void *keys, const void *vals, u64 n,
                       hipStream_t s) = 0;
   virtual void remove(const void *keys, u64 n, bool sorted, hipStream_t s) = 0;
-  virtual void lookup(const void *q, u64 n, void *out, hipStream_t s) = 0;
+  // sorted: the queries ascend (lookups_ordered), so they are not grouped
+  virtual void lookup(const void *q, u64 n, void *out, bool sorted,
+                      hipStream_t s) = 0;
   virtual void successor(const void *q, u64 n, void *out, hipStream_t s) = 0;
   virtual void range(const void *lo, const void *hi, u64 n, u64 *offsets,
                      void *ok, void *ov, hipStream_t s) = 0;
@@ -54,6 +57,9 @@ class FlixBase {
   virtual void memory(gd_memory_stats *out) const = 0;
   virtual void trim() = 0;
 };
+
+// Any-order lookups of this many queries or more are grouped first.
+constexpr u64 kGroupedLookups = u64(1) << 18;
 
 template <class K, class V>
 class FlixImpl : public FlixBase {
@@ -168,14 +174,39 @@ class FlixImpl : public FlixBase {
   }
 
   // ------------------------------------------------------------ reads
-  void lookup(const void *q, u64 n, void *out, hipStream_t s) override {
+  void lookup(const void *q, u64 n, void *out, bool sorted,
+              hipStream_t s) override {
     require_built();
     if (!n) return;
     if (!q || !out) throw Error(GD_INVALID_ARGUMENT, "null lookup buffers");
     order_.begin(s);
-    lookup_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
-        pool_view(), static_cast<const K *>(q), n, static_cast<V *>(out));
-    GD_LAUNCH_CHECK();
+    try {
+      if (!sorted && n >= kGroupedLookups && n <= 0xffffffffull) {
+        // group the queries by their top varying key byte, look them up in
+        // grouped order (each XCD on its own share of the buckets), then
+        // bring the results back to query order
+        const u32 nn = static_cast<u32>(n);
+        Lease<K> qk(pool_, n);
+        Lease<u32> dest(pool_, n);
+        Lease<V> res(pool_, n);
+        Lease<u32> ctrl(pool_, lsm::SpreadLayout<K>(n).words);
+        lsm::partition_by_spread<K>(static_cast<const K *>(q), nn, qk.get(),
+                                    nullptr, dest.get(), rs::PayloadPos{},
+                                    ctrl.get(), s, 0);
+        lookup_grouped_kernel<K, V><<<lsm::grouped_lookup_blocks(n), 256, 0,
+                                      s>>>(pool_view(), qk.get(), n,
+                                           res.get());
+        lookup_gather_kernel<V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            dest.get(), res.get(), n, static_cast<V *>(out));
+      } else {
+        lookup_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            pool_view(), static_cast<const K *>(q), n, static_cast<V *>(out));
+      }
+      GD_LAUNCH_CHECK();
+    } catch (...) {
+      order_.end(s);
+      throw;
+    }
     order_.end(s);
   }
 
@@ -630,12 +661,12 @@ gd_status gd_flix_remove_sorted(gd_flix *h, const void *sorted_keys,
 gd_status gd_flix_lookup_sorted(gd_flix *h, const void *sorted_queries,
                                 uint64_t count, void *out_values,
                                 hipStream_t s) {
-  GD_FLIX_CALL(h, f->lookup(sorted_queries, count, out_values, s));
+  GD_FLIX_CALL(h, f->lookup(sorted_queries, count, out_values, true, s));
 }
 
 gd_status gd_flix_lookup(gd_flix *h, const void *queries, uint64_t count,
                          void *out_values, hipStream_t s) {
-  GD_FLIX_CALL(h, f->lookup(queries, count, out_values, s));
+  GD_FLIX_CALL(h, f->lookup(queries, count, out_values, false, s));
 }
 
 gd_status gd_flix_successor_sorted(gd_flix *h, const void *sorted_queries,

@@ -732,6 +732,41 @@ static __global__ void lookup_kernel(Pool<K, V> p, const K *q, u64 n, V *out) {
   }
 }
 
+// Lookups of queries grouped by their top varying key byte (qk), results
+// in grouped order.  Block b serves the contiguous chunk
+// (b % 8) * (G / 8) + b / 8 of the grouped queries, so each XCD (blocks are
+// dealt to the 8 XCDs in turn) walks its own eighth of the key space and
+// the buckets and nodes it reads stay in its L2.  G is a multiple of 8.
+template <class K, class V>
+static __global__ void lookup_grouped_kernel(Pool<K, V> p, const K *qk, u64 n,
+                                             V *res) {
+  const u32 G = gridDim.x;
+  const u32 logical = (blockIdx.x % 8u) * (G / 8u) + blockIdx.x / 8u;
+  const u64 chunk = (n + G - 1) / G;
+  const u64 begin = u64(logical) * chunk;
+  const u64 end = begin + chunk < n ? begin + chunk : n;
+  for (u64 i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    const K key = qk[i];
+    const u32 b = bucket_of(p, key);
+    const u32 node = owner_node(p, b, key);
+    const u32 c = p.count[node];
+    const K *nk = p.keys + u64(node) * p.S;
+    const u32 lo = node_lower_bound(nk, c, key);
+    res[i] = (lo < c && nk[lo] == key) ? p.vals[u64(node) * p.S + lo]
+                                       : ~V(0);
+  }
+}
+
+// out[i] = res[dest[i]]: the grouped results back in query order.
+template <class V>
+static __global__ void lookup_gather_kernel(const u32 *dest, const V *res,
+                                            u64 n, V *out) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    out[i] = res[dest[i]];
+}
+
 // First stored key >= key, starting at bucket b; (node, slot) or node = ~0.
 template <class K, class V>
 __device__ __forceinline__ void ceiling_position(const Pool<K, V> &p, K key,

@@ -226,6 +226,32 @@ def test_sorted_remove(gd, orc, io, kb):
     p.check_all(rng, 2, 30001 * 100003)
 
 
+@pytest.mark.parametrize("kb,vb", [(4, 4), (8, 8)])
+def test_grouped_any_order_lookups(gd, orc, io, kb, vb):
+    """Any-order lookups of 2^18 queries or more are grouped by their top
+    varying key byte, looked up and gathered back: every value (hits,
+    misses, duplicate queries) equals the oracle's, and equals the sorted
+    API on the sorted queries."""
+    rng = np.random.default_rng(41 + kb)
+    kt = KT[kb]
+    p = FlixPair(gd, orc, io, kb, vb, node_slots=32)
+    top = (1 << 31) if kb == 4 else (1 << 60)
+    keys = np.unique(rng.integers(2, top, 1 << 20, dtype=np.uint64)).astype(kt)
+    p.build(keys)
+    p.insert(rng.integers(2, top, 1 << 16, dtype=np.uint64).astype(kt),
+             rng.integers(0, 1 << 31, 1 << 16, dtype=np.uint64).astype(KT[vb]))
+    p.remove(keys[rng.integers(0, len(keys), 1 << 15)])
+    nq = (1 << 18) + 12345
+    q = rng.integers(2, top, nq, dtype=np.uint64).astype(kt)
+    live = p.o.dump()[0]
+    q[: nq // 2] = live[rng.integers(0, len(live), nq // 2)]
+    q[nq // 2: nq // 2 + 1000] = q[:1000]  # repeated queries
+    got = p.lookup(q)
+    assert np.array_equal(got, p.o.lookup(q))
+    qs = np.sort(q)
+    assert np.array_equal(p.lookup(qs, True), p.o.lookup(qs))
+
+
 def test_memory_stats_and_trim(gd, orc, io):
     """gd_flix_memory: held = live + cached, the peak bounds the held bytes,
     the node layer's bytes grow with a growth rebuild; trim returns the
