@@ -49,6 +49,9 @@ def parse():
                     help="requests in the serving comparison (0 = skip)")
     ap.add_argument("--flix-batches", type=int, default=32,
                     help="config-3 FliX batches to run (0 = skip)")
+    ap.add_argument("--flix-rebuild-chain", type=int, default=-1,
+                    help="config 3: rebuild_chain_nodes of the FliX index "
+                         "(-1 = the tuned value, 0 = never rebuild)")
     ap.add_argument("--flix-ref", type=int, default=1,
                     help="FliX on the shape of the reference's recorded run "
                     "(1 = run, 0 = skip)")
@@ -774,7 +777,9 @@ def flix_measurement(args, pkg, wl, torch, dev):
         return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)) \
             .to(dev)
 
-    f = pkg.FlixIndex(key_bytes=8, value_bytes=8, tuning="auto")
+    chain = args.flix_rebuild_chain
+    f = pkg.FlixIndex(key_bytes=8, value_bytes=8, tuning="auto",
+                      rebuild_chain_nodes=None if chain < 0 else chain)
     dk = up(built)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()

@@ -662,14 +662,19 @@ class FlixIndex:
     """Mirror of the FliX updatable index over gd_flix_*."""
 
     def __init__(self, key_bytes=4, value_bytes=4, node_slots=0,
-                 fill_percent=0, tuning=None, rebuild_chain_nodes=0):
-        """tuning="auto": node_slots / fill_percent from tuned.json.
-        rebuild_chain_nodes > 0: an insert leaving a chain longer than that
-        rebuilds the bucket layer afterwards."""
+                 fill_percent=0, tuning=None, rebuild_chain_nodes=None):
+        """tuning="auto": node_slots, fill_percent and (unless given)
+        rebuild_chain_nodes from tuned.json.  rebuild_chain_nodes > 0: an
+        insert leaving a chain longer than that rebuilds the bucket layer
+        afterwards; 0 never rebuilds."""
         if tuning == "auto":
             t = tuned_config("flix")
             node_slots = node_slots or t.get("node_slots", 0)
             fill_percent = fill_percent or t.get("fill_percent", 0)
+            if rebuild_chain_nodes is None:
+                rebuild_chain_nodes = t.get("rebuild_chain_nodes", 0)
+        if rebuild_chain_nodes is None:
+            rebuild_chain_nodes = 0
         cfg = _FlixConfig(key_bytes, value_bytes, node_slots, fill_percent,
                           rebuild_chain_nodes)
         h = _P()

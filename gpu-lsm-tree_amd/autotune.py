@@ -8,7 +8,8 @@ result:
   limit, the run cap, the admission path (segmented or radix) and the most
   runs one merge pass takes;
 * serving: keys coalesced per step and steps in flight;
-* FliX: slots per node and initial fill.
+* FliX: slots per node, initial fill and the chain length that triggers a
+  rebuild.
 
 It writes gpu-lsm-tree_amd/tuned.json, which GPULSMOpt(tuning="auto") and
 Server(tuning="auto") load.  Search is coordinate descent from the defaults
@@ -45,7 +46,8 @@ LSM_DEFAULT = {"dir_rows_log": 3, "merge_tile_log": 12, "sort_block": 1024,
                "merge_ways": 8}
 SERVE_SPACE = {"max_step_keys": [1 << 20, 1 << 22, 1 << 24],
                "max_inflight_steps": [1, 2, 3]}
-FLIX_SPACE = {"node_slots": [16, 32, 64], "fill_percent": [50, 75]}
+FLIX_SPACE = {"node_slots": [16, 32, 64], "fill_percent": [50, 75],
+              "rebuild_chain_nodes": [0, 16, 32, 64, 128]}
 
 
 def _pkg():
@@ -178,21 +180,34 @@ def tune_flix(pkg, torch, dev, scale_log, log):
     out = torch.empty(len(q), dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
 
+    rounds = 8
+    step = len(ins) // rounds
+    qstep = len(q) // rounds
+
     def measure(knobs):
+        # eight rounds of inserts (uniform keys: most land in the gaps
+        # between the clustered runs, so those buckets' chains grow from
+        # round to round, which the rebuild threshold answers), lookups and
+        # deletes
         f = pkg.FlixIndex(key_bytes=8, value_bytes=8, **knobs)
         f.build_bucket_layer_only(dk, max_size=2 * n)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        f.insert(di, di)
-        f.lookup(dq, out)
-        f.remove(di[: len(ins) // 2])
+        for r in range(rounds):
+            part = di[r * step:(r + 1) * step]
+            f.insert(part, part)
+            f.lookup(dq[r * qstep:(r + 1) * qstep],
+                     out[r * qstep:(r + 1) * qstep])
+            f.remove(part[: step // 2])
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3
         f.close()
         return ms
 
     return coordinate_descent(FLIX_SPACE, {"node_slots": 32,
-                                           "fill_percent": 50}, measure, log)
+                                           "fill_percent": 50,
+                                           "rebuild_chain_nodes": 0},
+                              measure, log)
 
 
 def main(argv=None):
