@@ -162,9 +162,15 @@ class FlixImpl : public FlixBase {
       const u32 *count = slices(batch, n, begin.get(), work.get(), s);
       // a wave per touched bucket, the grid sized by the upper bound (capped;
       // the waves stride over the list)
-      const u64 blocks = std::min<u64>((most + 3) / 4, u64(1) << 20);
-      delete_kernel<K, V><<<static_cast<u32>(blocks), 256, 0, s>>>(
-          pool_view(), ring_view(), batch, begin.get(), work.get(), count);
+      if (S_ <= 32) {  // a half-wave per bucket
+        const u64 blocks = std::min<u64>((most + 7) / 8, u64(1) << 20);
+        delete_kernel<K, V, 32><<<static_cast<u32>(blocks), 256, 0, s>>>(
+            pool_view(), ring_view(), batch, begin.get(), work.get(), count);
+      } else {
+        const u64 blocks = std::min<u64>((most + 3) / 4, u64(1) << 20);
+        delete_kernel<K, V, 64><<<static_cast<u32>(blocks), 256, 0, s>>>(
+            pool_view(), ring_view(), batch, begin.get(), work.get(), count);
+      }
       GD_LAUNCH_CHECK();
     } catch (...) {
       order_.end(s);
@@ -460,8 +466,14 @@ class FlixImpl : public FlixBase {
     const Pool<K, V> pv = pool_view();
     const unsigned long long *cursor = &counters_->alloc_cursor;
     if (plan.n_fast) {
-      insert_fast_kernel<K, V><<<(plan.n_fast + 3) / 4, 256, 0, s>>>(
-          pv, ring_ids_, ring_cap_, cursor, bk, bv, begin, fast, plan.n_fast);
+      if (S_ <= 32)  // a half-wave per chain, eight chains per block
+        insert_fast_kernel<K, V, 32><<<(plan.n_fast + 7) / 8, 256, 0, s>>>(
+            pv, ring_ids_, ring_cap_, cursor, bk, bv, begin, fast,
+            plan.n_fast);
+      else
+        insert_fast_kernel<K, V, 64><<<(plan.n_fast + 3) / 4, 256, 0, s>>>(
+            pv, ring_ids_, ring_cap_, cursor, bk, bv, begin, fast,
+            plan.n_fast);
       GD_LAUNCH_CHECK();
     }
     if (plan.n_big) {

@@ -400,20 +400,23 @@ __device__ __forceinline__ void spread_slot(u32 rank, u32 q, u32 r, u32 &j,
   }
 }
 
-// One wave per fast item: rewrites the bucket's chain with its slice merged
-// in (see the plan above).  The batch holds only keys absent from the
+// One wave per fast item (a half-wave when nodes have at most 32 slots, so
+// a wave rewrites two chains): rewrites the bucket's chain with its slice
+// merged in (see the plan above).  The batch holds only keys absent from the
 // index, sorted and unique, so every rank is distinct.
-template <class K, class V>
+template <class K, class V, u32 kLanes>
 static __global__ __launch_bounds__(256) void insert_fast_kernel(
     Pool<K, V> p, const u32 *ring_ids, u64 ring_cap,
     const unsigned long long *alloc_cursor, const K *bk, const V *bv,
     const u64 *begin, const uint2 *items, u32 n_items) {
-  constexpr int kWavesPerBlock = 4;
-  __shared__ K s_k[kWavesPerBlock][kFastKeys];
-  __shared__ u32 s_seq[kWavesPerBlock][kFastOut];
-  const u32 lane = lane_id();
-  const u32 w = threadIdx.x / kWave;
-  const u32 item = blockIdx.x * kWavesPerBlock + w;
+  static_assert(kLanes == 32 || kLanes == 64, "a node per wave or half-wave");
+  constexpr u32 kParts = kWave / kLanes;  // items per wave
+  constexpr u32 kItemsPerBlock = 4 * kParts;
+  __shared__ K s_k[kItemsPerBlock][kFastKeys];
+  __shared__ u32 s_seq[kItemsPerBlock][kFastOut];
+  const u32 lane = lane_id() % kLanes;  // the slot this lane holds
+  const u32 w = threadIdx.x / kLanes;   // the item of the block
+  const u32 item = blockIdx.x * kItemsPerBlock + w;
   if (item >= n_items) return;
   const uint2 it = items[item];
   const u32 b = it.x;
@@ -427,7 +430,7 @@ static __global__ __launch_bounds__(256) void insert_fast_kernel(
   K *sk = s_k[w];
   u32 *seq = s_seq[w];
   // the slice's keys behind the chain's keys: sk = [chain keys | slice]
-  for (u32 j = lane; j < m; j += kWave) sk[t0 + j] = bk[pos + j];
+  for (u32 j = lane; j < m; j += kLanes) sk[t0 + j] = bk[pos + j];
   // the chain, a node per round trip: its slots stay in registers
   K ak[kFastNodes];
   V av[kFastNodes];
@@ -455,7 +458,7 @@ static __global__ __launch_bounds__(256) void insert_fast_kernel(
   }
   // new nodes reserved by the plan
   const unsigned long long first_new = *alloc_cursor + it.y;
-  for (u32 j = lane; j + L < Lp; j += kWave)
+  for (u32 j = lane; j + L < Lp; j += kLanes)
     seq[L + j] = ring_ids[(first_new + j) % ring_cap];
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -482,7 +485,7 @@ static __global__ __launch_bounds__(256) void insert_fast_kernel(
     }
   }
   // slice rows: rank = own index + chain keys below
-  for (u32 x = lane; x < m; x += kWave) {
+  for (u32 x = lane; x < m; x += kLanes) {
     const K key = sk[t0 + x];
     u32 lo = 0, hi = t0;
     while (lo < hi) {
@@ -497,7 +500,7 @@ static __global__ __launch_bounds__(256) void insert_fast_kernel(
     p.vals[dst] = bv[pos + x];
   }
   // counts and links: chain nodes in order, then the new ones
-  for (u32 j = lane; j < Lp; j += kWave) {
+  for (u32 j = lane; j < Lp; j += kLanes) {
     const u32 id = seq[j];
     p.count[id] = q + (j < r ? 1u : 0u);
     p.next[id] = j + 1 < Lp ? seq[j + 1] + 1 : 0u;
@@ -613,11 +616,18 @@ static __global__ void plan_reset_kernel(InsertPlan *plan) {
 // nodes are unlinked and returned to the ring, bsize/blen follow.  dk is the
 // batch in global memory or the wave's copy of the slice in LDS (then pos =
 // 0); every search of the slice reads dk.
-template <class K, class V>
+template <class K, class V, u32 kLanes>
 __device__ __forceinline__ void delete_bucket(Pool<K, V> &p, Ring &ring,
                                               const K *dk, u64 pos, u64 end,
                                               u32 b, u32 lane) {
   const u32 S = p.S;
+  // the ballot bits of this lane's group (the wave, or its half holding
+  // this bucket), lane = the slot index inside the group
+  const u32 shift = kLanes == 64 ? 0u : (lane_id() & 32u);
+  auto group_ballot = [&](bool v) {
+    const u64 all = ballot(v);
+    return kLanes == 64 ? all : (all >> shift) & 0xffffffffull;
+  };
   u32 node = b;
   u32 prev = ~0u;
   u32 removed = 0, unlinked = 0;  // wave-uniform
@@ -643,12 +653,12 @@ __device__ __forceinline__ void delete_bucket(Pool<K, V> &p, Ring &ring,
         const u64 f = lower_bound_g(dk, pos, q, key);
         hit = f < q && dk[f] == key;
       }
-      const u64 keep = ballot(lane < c && !hit);
+      const u64 keep = group_ballot(lane < c && !hit);
       const u32 kept = popc64(keep);
       if (kept != c) {
         __builtin_amdgcn_wave_barrier();
         if ((keep >> lane) & 1ull) {
-          const u32 o = popc64(keep & lanemask_lt());
+          const u32 o = popc64(keep & ((u64(1) << lane) - 1));
           p.keys[base + o] = key;
           p.vals[base + o] = val;
         }
@@ -679,7 +689,8 @@ __device__ __forceinline__ void delete_bucket(Pool<K, V> &p, Ring &ring,
   }
 }
 
-// One wave per bucket with a non-empty slice of the sorted delete batch;
+// One wave per bucket (a half-wave when nodes have at most 32 slots) with a
+// non-empty slice of the sorted delete batch;
 // the waves stride over the work list, whose length the device holds (the
 // grid is sized by an upper bound, so the host reads nothing back).
 // A slice of at most kDeleteStage keys is copied to the wave's LDS first, so
@@ -687,17 +698,18 @@ __device__ __forceinline__ void delete_bucket(Pool<K, V> &p, Ring &ring,
 // node's first key) read LDS instead of dependent global loads.
 constexpr u32 kDeleteStage = 512;
 
-template <class K, class V>
+template <class K, class V, u32 kLanes>
 static __global__ __launch_bounds__(256) void delete_kernel(
     Pool<K, V> p, Ring ring, const K *dk, const u64 *begin, const u32 *work,
     const u32 *work_count) {
-  constexpr u32 kWavesPerBlock = 4;
-  __shared__ K s_dk[kWavesPerBlock][kDeleteStage];
-  const u32 lane = lane_id();
-  const u32 w = threadIdx.x / kWave;
+  static_assert(kLanes == 32 || kLanes == 64, "a node per wave or half-wave");
+  constexpr u32 kPerBlock = 256 / kLanes;  // buckets in flight per block
+  __shared__ K s_dk[kPerBlock][kDeleteStage];
+  const u32 lane = lane_id() % kLanes;
+  const u32 w = threadIdx.x / kLanes;
   const u32 items = *work_count;
-  for (u32 item = blockIdx.x * kWavesPerBlock + w; item < items;
-       item += gridDim.x * kWavesPerBlock) {
+  for (u32 item = blockIdx.x * kPerBlock + w; item < items;
+       item += gridDim.x * kPerBlock) {
     const u32 b = work[item];
     const u64 pos = begin[b], end = begin[b + 1];
     const u64 m = end - pos;
@@ -705,12 +717,13 @@ static __global__ __launch_bounds__(256) void delete_kernel(
       K *sk = s_dk[w];
       // the previous bucket's searches are done before the slot is reused
       __builtin_amdgcn_wave_barrier();
-      for (u32 j = lane; j < m; j += kWave) sk[j] = dk[pos + j];
+      for (u32 j = lane; j < m; j += kLanes) sk[j] = dk[pos + j];
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      delete_bucket(p, ring, static_cast<const K *>(sk), 0, m, b, lane);
+      delete_bucket<K, V, kLanes>(p, ring, static_cast<const K *>(sk), 0, m,
+                                  b, lane);
     } else {
-      delete_bucket(p, ring, dk, pos, end, b, lane);
+      delete_bucket<K, V, kLanes>(p, ring, dk, pos, end, b, lane);
     }
   }
 }
