@@ -281,6 +281,28 @@ struct SegPayload {
   }
 };
 
+// Output offset of segment `seg` of the 256: the sum of the survivor counts
+// the segments before it publish in status[] (each publishes once, as soon
+// as it has counted its survivors, and waits for nothing later).  One wave
+// reads all of them in one round, four per lane, instead of walking back a
+// window at a time.
+__device__ __forceinline__ u64 segments_before(u64 *status, u32 seg) {
+  const u32 lane = lane_id();
+  u64 sum = 0;
+#pragma unroll
+  for (u32 k = 0; k < 4; ++k) {
+    const u32 t = lane * 4 + k;
+    if (t < seg) {
+      u64 word;
+      do {
+        word = load_relaxed(&status[t]);
+      } while ((word >> kFlagShift) == 0);
+      sum += word & kValueMask;
+    }
+  }
+  return wave_sum_u64(sum);
+}
+
 // Shared state of seg_sort_kernel.
 struct SegShared {
   u32 k[2][kSegTile];
@@ -803,12 +825,14 @@ __global__ __launch_bounds__(kSegThreads) void seg_sort_kernel(
     if (tid == 0) {
       sh.misc[1] = 0;  // survivors below the top key bit
       sh.misc[2] = 0;  // tombstones
+      store_relaxed(&status[seg], kFlagAggregate | seg_total);
     }
-    // the chained scan over segments (wave 0) runs beside the first chunk's
-    // survivor offsets (wave 1); an empty segment scans on its own
+    // the offset over the segments before (wave 0) is gathered beside the
+    // first chunk's survivor offsets (wave 1); an empty segment gathers it
+    // on its own
     auto lookback = [&]() {
       if (wave == 0) {
-        const u64 ex = wave_lookback(status, seg, seg_total);
+        const u64 ex = segments_before(status, seg);
         if (tid == 0) sh.base = ex;
       }
     };
