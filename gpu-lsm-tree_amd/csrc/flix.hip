@@ -239,10 +239,16 @@ class FlixImpl : public FlixBase {
       // a wave per query (4 per block)
       const u32 blocks = grid_for(n, 4, 1u << 16);
       if (!ok) {
+        // counts from ranks: the keys before each bucket (a scan of the
+        // chains' live keys), then two point searches per query
+        Lease<u64> sizes(pool_, u64(P_) + 1);
+        Lease<u64> bprefix(pool_, u64(P_) + 1);
+        widen(bsize_, P_, sizes.get(), s);
+        exclusive_scan(sizes.get(), P_, bprefix.get(), s);
         Lease<u64> counts(pool_, n + 1);
-        range_wave_kernel<K, V><<<blocks, 256, 0, s>>>(
-            pv, static_cast<const K *>(lo), static_cast<const K *>(hi), n,
-            counts.get(), nullptr, nullptr);
+        range_count_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            pv, bprefix.get(), static_cast<const K *>(lo),
+            static_cast<const K *>(hi), n, counts.get());
         GD_LAUNCH_CHECK();
         exclusive_scan(counts.get(), n, offsets, s);
       } else {

@@ -158,6 +158,31 @@ __device__ __forceinline__ u32 node_lower_bound(const K *nk, u32 c, K key) {
   return base + below;
 }
 
+// Slots of a node's c ascending keys that hold a key <= key (the first slot
+// above it), by the same two rounds as node_lower_bound.
+template <class K>
+__device__ __forceinline__ u32 node_upper_bound(const K *nk, u32 c, K key) {
+  u32 blk = 0;
+  if (c > 8) {
+    K last[8];
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) {
+      const u32 at = 8 * j + 7;
+      last[j] = at < c ? nk[at] : K(0);
+    }
+#pragma unroll
+    for (u32 j = 0; j < 8; ++j) blk += (8 * j + 7 < c && last[j] <= key);
+  }
+  const u32 base = 8 * blk;
+  K k[8];
+#pragma unroll
+  for (u32 j = 0; j < 8; ++j) k[j] = base + j < c ? nk[base + j] : K(0);
+  u32 upto = 0;
+#pragma unroll
+  for (u32 j = 0; j < 8; ++j) upto += (base + j < c && k[j] <= key);
+  return base + upto;
+}
+
 // Node of bucket b's chain that owns `key`.
 template <class K, class V>
 __device__ __forceinline__ u32 owner_node(const Pool<K, V> &p, u32 b, K key) {
@@ -844,10 +869,52 @@ static __global__ void successor_kernel(Pool<K, V> p, const K *q, u64 n, K *out)
   }
 }
 
+// Keys of the index below `key` (kInclusive: up to and including it): the
+// keys of the buckets before its bucket (bprefix, an exclusive scan of
+// bsize), of its chain's nodes before the one that holds the rank's end,
+// and of that node's slots.  Keys of node i are below the first key of
+// node i + 1, so the walk stops at the first node whose successor starts
+// at or above the key.
+template <class K, class V, bool kInclusive>
+__device__ __forceinline__ u64 rank_of(const Pool<K, V> &p, const u64 *bprefix,
+                                       K key) {
+  const u32 b = bucket_of(p, key);
+  u64 r = bprefix[b];
+  u32 node = b;
+  while (true) {
+    const u32 nxt = p.next[node];
+    if (!nxt) break;
+    const K first = p.keys[u64(nxt - 1) * p.S];
+    if (kInclusive ? key < first : key <= first) break;
+    r += p.count[node];
+    node = nxt - 1;
+  }
+  const u32 c = p.count[node];
+  const K *nk = p.keys + u64(node) * p.S;
+  return r + (kInclusive ? node_upper_bound(nk, c, key)
+                         : node_lower_bound(nk, c, key));
+}
+
+// Count pass of the range enumeration: |[lo, hi]| = rank(<= hi) - rank(< lo),
+// two point searches per query instead of a walk over the range.
+template <class K, class V>
+static __global__ void range_count_kernel(Pool<K, V> p, const u64 *bprefix,
+                                          const K *lo, const K *hi, u64 n,
+                                          u64 *counts) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const K a = lo[i], z = hi[i];
+    counts[i] = a <= z ? rank_of<K, V, true>(p, bprefix, z) -
+                             rank_of<K, V, false>(p, bprefix, a)
+                       : 0;
+  }
+}
+
 // Range [lo, hi] with a wave per query: the lanes hold one node's slots, so
 // a node's keys in range leave with one ballot and coalesced stores, and a
-// query walks its range a node per step rather than a key per step.  Count
-// pass (out_keys == nullptr) or fill pass; S <= 64.
+// query walks its range a node per step rather than a key per step.  Fill
+// pass (or, with out_keys == nullptr, a count pass by walking); S <= 64.
 template <class K, class V>
 static __global__ __launch_bounds__(256) void range_wave_kernel(
     Pool<K, V> p, const K *lo, const K *hi, u64 n, u64 *offsets, K *out_keys,
