@@ -58,6 +58,12 @@ class FlixBase {
   virtual void trim() = 0;
 };
 
+inline u64 pow2_at_least(u64 n) {
+  u64 p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
 // Any-order lookups of this many queries or more are grouped first.
 constexpr u64 kGroupedLookups = u64(1) << 18;
 
@@ -483,10 +489,15 @@ class FlixImpl : public FlixBase {
       GD_LAUNCH_CHECK();
     }
     if (plan.n_big) {
-      Lease<u32> ids(pool_, u64(plan.big_nodes) + 1);
-      Lease<u32> at(pool_, u64(plan.big_nodes) + 1);
-      Lease<K> sk(pool_, plan.big_keys + 1);
-      Lease<V> sv(pool_, plan.big_keys + 1);
+      // scratch sizes follow the data from batch to batch; rounded up to a
+      // power of two they recur, so the pool hands back cached blocks
+      // instead of allocating anew
+      const u64 nodes = pow2_at_least(u64(plan.big_nodes) + 1);
+      const u64 keys = pow2_at_least(plan.big_keys + 1);
+      Lease<u32> ids(pool_, nodes);
+      Lease<u32> at(pool_, nodes);
+      Lease<K> sk(pool_, keys);
+      Lease<V> sv(pool_, keys);
       insert_big_kernel<K, V><<<plan.n_big, 256, 0, s>>>(
           pv, ring_ids_, ring_cap_, cursor, bk, bv, begin, big, ids.get(),
           at.get(), sk.get(), sv.get());
