@@ -911,57 +911,53 @@ static __global__ void range_count_kernel(Pool<K, V> p, const u64 *bprefix,
   }
 }
 
-// Range [lo, hi] with a wave per query: the lanes hold one node's slots, so
-// a node's keys in range leave with one ballot and coalesced stores, and a
-// query walks its range a node per step rather than a key per step.  Fill
-// pass (or, with out_keys == nullptr, a count pass by walking); S <= 64.
+// Fill pass of the range enumeration: a wave per query whose lanes hold one
+// node's slots, so a node's keys in range leave with one ballot and
+// coalesced stores, and a query walks its range a node per step rather than
+// a key per step.  offsets[i] is where query i's pairs start; S <= 64.
 template <class K, class V>
 static __global__ __launch_bounds__(256) void range_wave_kernel(
-    Pool<K, V> p, const K *lo, const K *hi, u64 n, u64 *offsets, K *out_keys,
-    V *out_vals) {
+    Pool<K, V> p, const K *lo, const K *hi, u64 n, const u64 *offsets,
+    K *out_keys, V *out_vals) {
   const u32 lane = lane_id();
   const u64 waves = u64(gridDim.x) * (blockDim.x / kWave);
   for (u64 i = u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
        i < n; i += waves) {
     const K a = lo[i], z = hi[i];
-    u64 cnt = 0;
-    if (a <= z) {
-      u32 node, slot;
-      ceiling_position(p, a, node, slot);
-      u32 bucket =
-          node == ~0u ? 0u : bucket_of(p, p.keys[u64(node) * p.S + slot]);
-      u64 w = out_keys ? offsets[i] : 0;
-      while (node != ~0u) {
-        const u32 c = p.count[node];
-        const u64 base = u64(node) * p.S;
-        const bool mine = lane >= slot && lane < c;
-        K k = K(0);
-        if (mine) k = p.keys[base + lane];
-        const bool in = mine && k <= z;
-        // keys ascend inside a node: the lanes in range are a prefix
-        const u32 m = popc64(ballot(in));
-        if (out_keys && in) {
-          out_keys[w + (lane - slot)] = k;
-          out_vals[w + (lane - slot)] = p.vals[base + lane];
+    if (a > z) continue;
+    u32 node, slot;
+    ceiling_position(p, a, node, slot);
+    u32 bucket =
+        node == ~0u ? 0u : bucket_of(p, p.keys[u64(node) * p.S + slot]);
+    u64 w = offsets[i];
+    while (node != ~0u) {
+      const u32 c = p.count[node];
+      const u64 base = u64(node) * p.S;
+      const bool mine = lane >= slot && lane < c;
+      K k = K(0);
+      if (mine) k = p.keys[base + lane];
+      const bool in = mine && k <= z;
+      // keys ascend inside a node: the lanes in range are a prefix
+      const u32 m = popc64(ballot(in));
+      if (in) {
+        out_keys[w + (lane - slot)] = k;
+        out_vals[w + (lane - slot)] = p.vals[base + lane];
+      }
+      w += m;
+      if (slot + m < c) break;  // a key above the range
+      // next non-empty node in key order
+      slot = 0;
+      while (true) {
+        const u32 nxt = p.next[node];
+        if (nxt) node = nxt - 1;
+        else if (++bucket < p.buckets) node = bucket;
+        else {
+          node = ~0u;
+          break;
         }
-        cnt += m;
-        w += m;
-        if (slot + m < c) break;  // a key above the range
-        // next non-empty node in key order
-        slot = 0;
-        while (true) {
-          const u32 nxt = p.next[node];
-          if (nxt) node = nxt - 1;
-          else if (++bucket < p.buckets) node = bucket;
-          else {
-            node = ~0u;
-            break;
-          }
-          if (p.count[node]) break;
-        }
+        if (p.count[node]) break;
       }
     }
-    if (!out_keys && lane == 0) offsets[i] = cnt;
   }
 }
 

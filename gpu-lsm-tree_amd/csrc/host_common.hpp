@@ -208,12 +208,22 @@ class OperationOrder {
       (void)hipEventDestroy(done_);
     }
   }
-  void begin(hipStream_t s) { GD_HIP(hipStreamWaitEvent(s, done_, 0)); }
-  void end(hipStream_t s) { GD_HIP(hipEventRecord(done_, s)); }
+  // An operation on the stream the previous one ended on is ordered after
+  // it by the stream itself; one on another stream waits for its event.
+  void begin(hipStream_t s) {
+    if (!ended_ || s != last_) GD_HIP(hipStreamWaitEvent(s, done_, 0));
+  }
+  void end(hipStream_t s) {
+    GD_HIP(hipEventRecord(done_, s));
+    last_ = s;
+    ended_ = true;
+  }
   void wait_host() { GD_HIP(hipEventSynchronize(done_)); }
 
  private:
   hipEvent_t done_ = nullptr;
+  hipStream_t last_ = nullptr;
+  bool ended_ = false;
 };
 
 // Base of every handle: last error message.
