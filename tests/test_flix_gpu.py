@@ -409,3 +409,37 @@ def test_reference_shape_scaled(gd, orc, io):
         live = p.o.dump()[0]
         p.remove(np.sort(live[rng.choice(len(live), n_step, replace=False)]))
         check()
+
+
+def test_operations_on_alternating_streams(gd, orc, io):
+    """Inserts and removes of one FliX index issued on alternating streams,
+    two calls per stream, with no host wait between them: each call sees
+    the state every earlier call left, and the contents and lookups match
+    the oracle."""
+    import torch
+    rng = np.random.default_rng(71)
+    p = FlixPair(gd, orc, io, 8, 8, node_slots=32)
+    keys = np.unique(rng.integers(2, 1 << 40, 1 << 16, dtype=np.uint64))
+    p.build(keys, max_size=1 << 19)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream(), None]
+    keep = []  # device inputs stay alive until the streams finish
+    for b in range(16):
+        s = streams[(b // 2) % 3]
+        if b % 4 == 3:
+            live = p.o.dump()[0]
+            k = live[rng.integers(0, len(live), 4000)]
+            dk = p.up_k(k)
+            torch.cuda.synchronize()
+            keep.append(dk)
+            p.f.remove(dk, count=len(k), stream=s)
+            p.o.remove(k)
+        else:
+            k = rng.integers(2, 1 << 40, 6000, dtype=np.uint64)
+            v = rng.integers(0, 1 << 62, 6000, dtype=np.uint64)
+            dk, dv = p.up_k(k), p.up_v(v)
+            torch.cuda.synchronize()
+            keep += [dk, dv]
+            p.f.insert(dk, dv, count=len(k), stream=s)
+            p.o.insert(k, v)
+    torch.cuda.synchronize()
+    p.check_all(rng, 2, 1 << 40)

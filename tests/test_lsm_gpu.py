@@ -780,3 +780,35 @@ def test_segmented_admission(gd, orc, io, span, dup, base):
                                               dtype=np.uint64), 0xFFFFFFFF)
             p.check_ranges(lo.astype(np.uint32), hi.astype(np.uint32))
             p.check_successor(q[:4096])
+
+
+def test_operations_on_alternating_streams(gd, orc, io):
+    """Consecutive calls of one handle on different streams (and runs of
+    calls on one stream) stay ordered: every batch is enqueued without a
+    host wait, queries read the state after all earlier updates, and the
+    results match the oracle."""
+    import torch
+    rng = np.random.default_rng(8)
+    d = gd.GPULSMOpt(max_elements=1 << 22, batch_capacity=1 << 16)
+    o = orc.LsmOracle(max_elements=1 << 22, batch_capacity=1 << 16)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream(), None]
+    keep = []  # device inputs stay alive until the streams finish
+    for b in range(24):
+        n = int(rng.integers(1000, 1 << 16))
+        k = rng.integers(0, 1 << 24, n, dtype=np.uint64).astype(np.uint32)
+        v = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        e = (rng.random(n) < 0.2).astype(np.uint8)
+        dk, dv, de = io.up(k), io.up(v), io.up(e, np.uint8)
+        torch.cuda.synchronize()  # the uploads are done before any stream
+        keep += [dk, dv, de]
+        s = streams[(b // 2) % 3]  # two calls per stream, then the next
+        d.apply_mixed(dk, dv, de, count=n, stream=s)
+        assert o.apply_mixed(k, v, e) == 0
+    q = _queries(rng, o.dump()[0], 20000, 1 << 24)
+    dq = io.up(q)
+    out = io.empty(len(q))
+    torch.cuda.synchronize()
+    d.lookup(dq, out, None, count=len(q), stream=streams[1])
+    streams[1].synchronize()
+    assert np.array_equal(io.down(out, len(q)), o.lookup(q))
+    d.close()
