@@ -834,27 +834,6 @@ __device__ __forceinline__ void ceiling_position(const Pool<K, V> &p, K key,
   }
 }
 
-// Next (node, slot) in key order, node = ~0 at the end.
-template <class K, class V>
-__device__ __forceinline__ void advance(const Pool<K, V> &p, u32 &node,
-                                        u32 &slot, u32 &bucket) {
-  if (slot + 1 < p.count[node]) {
-    ++slot;
-    return;
-  }
-  slot = 0;
-  while (true) {
-    const u32 nxt = p.next[node];
-    if (nxt) node = nxt - 1;
-    else if (++bucket < p.buckets) node = bucket;
-    else {
-      node = ~0u;
-      return;
-    }
-    if (p.count[node]) return;
-  }
-}
-
 template <class K, class V>
 static __global__ void successor_kernel(Pool<K, V> p, const K *q, u64 n, K *out) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
@@ -915,48 +894,52 @@ static __global__ void range_count_kernel(Pool<K, V> p, const u64 *bprefix,
 // node's slots, so a node's keys in range leave with one ballot and
 // coalesced stores, and a query walks its range a node per step rather than
 // a key per step.  offsets[i] is where query i's pairs start; S <= 64.
+// The walk starts at the node of lo's bucket chain that would own lo (the
+// lanes below lo are masked, so no separate ceiling search), and each step
+// issues the node's count, successor link, keys and values together: the
+// slots past the count are in bounds (every node has S of them) and are
+// masked, so a node costs one round of loads instead of three dependent
+// ones (count, then keys and values, then the link).
 template <class K, class V>
 static __global__ __launch_bounds__(256) void range_wave_kernel(
     Pool<K, V> p, const K *lo, const K *hi, u64 n, const u64 *offsets,
     K *out_keys, V *out_vals) {
   const u32 lane = lane_id();
+  const bool slot_lane = lane < p.S;
   const u64 waves = u64(gridDim.x) * (blockDim.x / kWave);
   for (u64 i = u64(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
        i < n; i += waves) {
     const K a = lo[i], z = hi[i];
     if (a > z) continue;
-    u32 node, slot;
-    ceiling_position(p, a, node, slot);
-    u32 bucket =
-        node == ~0u ? 0u : bucket_of(p, p.keys[u64(node) * p.S + slot]);
+    u32 bucket = bucket_of(p, a);
+    u32 node = owner_node(p, bucket, a);
     u64 w = offsets[i];
-    while (node != ~0u) {
-      const u32 c = p.count[node];
+    while (true) {
       const u64 base = u64(node) * p.S;
-      const bool mine = lane >= slot && lane < c;
+      const u32 c = p.count[node];
+      const u32 nxt = p.next[node];
       K k = K(0);
-      if (mine) k = p.keys[base + lane];
-      const bool in = mine && k <= z;
-      // keys ascend inside a node: the lanes in range are a prefix
+      V v = V(0);
+      if (slot_lane) {
+        k = p.keys[base + lane];
+        v = p.vals[base + lane];
+      }
+      const bool live = lane < c;
+      // keys ascend inside a node and across nodes: the lanes below lo
+      // are a prefix, the lanes in range the run after it
+      const u32 below = popc64(ballot(live && k < a));
+      const bool in = live && k >= a && k <= z;
       const u32 m = popc64(ballot(in));
       if (in) {
-        out_keys[w + (lane - slot)] = k;
-        out_vals[w + (lane - slot)] = p.vals[base + lane];
+        out_keys[w + (lane - below)] = k;
+        out_vals[w + (lane - below)] = v;
       }
       w += m;
-      if (slot + m < c) break;  // a key above the range
-      // next non-empty node in key order
-      slot = 0;
-      while (true) {
-        const u32 nxt = p.next[node];
-        if (nxt) node = nxt - 1;
-        else if (++bucket < p.buckets) node = bucket;
-        else {
-          node = ~0u;
-          break;
-        }
-        if (p.count[node]) break;
-      }
+      if (below + m < c) break;  // a key above the range
+      // next node in key order (an empty head has count 0 and passes)
+      if (nxt) node = nxt - 1;
+      else if (++bucket < p.buckets) node = bucket;
+      else break;
     }
   }
 }
