@@ -443,3 +443,48 @@ def test_operations_on_alternating_streams(gd, orc, io):
             p.o.insert(k, v)
     torch.cuda.synchronize()
     p.check_all(rng, 2, 1 << 40)
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_range_edges(gd, orc, io, kb):
+    """Range enumeration where its walk changes course: bounds on, just
+    below and just above stored keys, single-key and empty ranges, ranges
+    that start or end inside emptied buckets or span several of them,
+    ranges that start in the middle of a bucket chain (a node after the
+    head), below the smallest and above the largest key, and lo > hi;
+    offsets, keys and values equal the oracle's."""
+    rng = np.random.default_rng(51 + kb)
+    kt = KT[kb]
+    p = FlixPair(gd, orc, io, kb, kb, node_slots=16, fill_percent=50)
+    keys = (np.arange(1, 4001, dtype=np.uint64) * 1000).astype(kt)
+    p.build(keys, max_size=1 << 16)
+    # 600 keys between two stored keys: one bucket's chain of many nodes
+    chain = (np.uint64(20001) + np.arange(600, dtype=np.uint64)).astype(kt)
+    p.insert(chain, chain)
+    # whole buckets emptied (8 keys per head at 50% of 16 slots)
+    gone = keys[((keys >= 1000000) & (keys <= 1200000)) |
+                ((keys >= 1500000) & (keys <= 1520000))]
+    p.remove(gone)
+    live = p.o.dump()[0].astype(np.uint64)
+    top = int(np.iinfo(kt).max)
+    lo, hi = [], []
+    for k in live[rng.integers(0, len(live), 60)].tolist():
+        for a, z in ((k, k), (k - 1, k), (k, k + 1), (k + 1, k + 2500),
+                     (k - 5, k + 5), (k, k + 20000)):
+            lo.append(a)
+            hi.append(z)
+    edges = [(0, top), (0, 5), (999990, 1000010), (1000005, 1199990),
+             (1199995, 1200500), (1400000, 1600000), (20000, 21000),
+             (20500, 20700), (20999, 21001), (20300, 1000000),
+             (3999000, top), (4000001, top), (5, 9), (40010, 39990)]
+    lo += [a for a, _ in edges]
+    hi += [z for _, z in edges]
+    lo = np.array(lo, np.uint64).astype(kt)
+    hi = np.minimum(np.array(hi, np.uint64), np.uint64(top)).astype(kt)
+    go, gk, gv = p.range(lo, hi)
+    eo, ek, ev = p.o.range(lo, hi)
+    assert np.array_equal(go, eo)
+    assert np.array_equal(gk, ek)
+    assert np.array_equal(gv, ev)
+    # the mid-chain ranges hold what they should
+    assert int(eo[-7] - eo[-8]) == 101  # 20500..20600 of the chain
