@@ -412,8 +412,8 @@ class FlixImpl : public FlixBase {
   }
 
   // rebuild_bucket_layer_only (impl_cg_rtx_index_updates.cuh:1821-1881):
-  // the chains, flattened in key order, are laid out as a fresh bucket
-  // layer.  grow_for > 0 (an insert of that many keys that the ring cannot
+  // the chains' keys, in key order, are laid out as a fresh bucket layer
+  // (each chain writes its rows there directly).  grow_for > 0 (an insert of that many keys that the ring cannot
   // hold) sizes the new allocation region for twice the keys after it.
   void rebuild_into(u64 grow_for, hipStream_t s) {
     Lease<u64> sizes(pool_, u64(P_) + 1);
@@ -426,14 +426,9 @@ class FlixImpl : public FlixBase {
     GD_HIP(hipMemcpyAsync(&total, offs.get() + P_, sizeof(u64),
                           hipMemcpyDeviceToHost, s));
     GD_HIP(hipStreamSynchronize(s));
-    Lease<K> uk(pool_, std::max<u64>(total, 1));
-    Lease<V> uv(pool_, std::max<u64>(total, 1));
-    flatten_kernel<K, V><<<(P_ + 3) / 4, 256, 0, s>>>(pool_view(), offs.get(),
-                                                      uk.get(), uv.get());
-    GD_LAUNCH_CHECK();
     max_size_ = std::max(max_size_, total);
     if (grow_for) max_size_ = std::max(max_size_, 2 * (total + grow_for));
-    layout(uk.get(), uv.get(), total, s);
+    relayout(offs.get(), total, s);
   }
 
   // Applies a sorted batch of keys absent from the index (see insert()).
@@ -526,62 +521,139 @@ class FlixImpl : public FlixBase {
 
   // Lays sorted unique (key, value) rows into a fresh bucket layer.
   void layout(const K *uk, const V *uv, u64 m, hipStream_t s) {
-    const u32 fill = std::max<u32>(1, std::min<u32>(
+    const u32 fill = layer_fill();
+    const Layer L = new_layer(m);
+    install(L);
+    clear_headers(s);
+    layout_kernel<K, V><<<grid_for(u64(P_) * S_, 256, 1u << 16), 256, 0, s>>>(
+        pool_view(), maxv_, uk, uv, m, fill);
+    GD_LAUNCH_CHECK();
+    finish_layer(s);
+  }
+
+  // Rebuild of the current layer's m live keys into a fresh layer, straight
+  // from the chains (offsets = the live keys before each chain) with no
+  // flattened copy in between.
+  void relayout(const u64 *offsets, u64 m, hipStream_t s) {
+    const u32 fill = layer_fill();
+    const Pool<K, V> old = pool_view();
+    const Layer L = new_layer(m);
+    const Pool<K, V> nv{L.keys, L.vals, L.count, L.next, L.maxv,
+                        L.bsize, L.blen, S_, static_cast<u32>(L.P), L.total,
+                        L.bdir, static_cast<u32>(8 * sizeof(K)) - L.d};
+    rs::zero_u32_kernel<<<grid_for(L.total, 256, 1u << 16), 256, 0, s>>>(
+        L.count, L.total);
+    rs::zero_u32_kernel<<<grid_for(L.total, 256, 1u << 16), 256, 0, s>>>(
+        L.next, L.total);
+    relayout_kernel<K, V><<<(old.buckets + 3) / 4, 256, 0, s>>>(old, offsets,
+                                                                nv, fill);
+    GD_LAUNCH_CHECK();
+    install(L);  // the old arrays go back to the pool behind the kernel
+    layout_heads_kernel<K, V><<<grid_for(u64(P_), 256, 1u << 16), 256, 0, s>>>(
+        pool_view(), maxv_, m, fill);
+    GD_LAUNCH_CHECK();
+    finish_layer(s);
+  }
+
+  u32 layer_fill() const {
+    return std::max<u32>(1, std::min<u32>(
         S_, static_cast<u32>((u64(S_) * fill_percent_ + 99) / 100)));
-    const u64 P = std::max<u64>(1, (m + fill - 1) / fill);
+  }
+
+  // The arrays of a bucket layer for m keys, allocated but not installed.
+  struct Layer {
+    K *keys;
+    V *vals;
+    u32 *count, *next;
+    K *maxv;
+    u32 *bdir, *bsize, *blen, *ring;
+    RingCounters *ctr;
+    u64 P, total, region;
+    u32 d;
+  };
+
+  Layer new_layer(u64 m) {
+    const u32 fill = layer_fill();
+    Layer L{};
+    L.P = std::max<u64>(1, (m + fill - 1) / fill);
     const u64 need = (2 * std::max(max_size_, m) + S_ - 1) / S_;
-    const u64 region = std::max<u64>(need > P ? need - P : 0, P / 4) + 1024;
-    const u64 total = P + region;
-    if (total >= 0xffffffffull)
+    L.region = std::max<u64>(need > L.P ? need - L.P : 0, L.P / 4) + 1024;
+    L.total = L.P + L.region;
+    if (L.total >= 0xffffffffull)
       throw Error(GD_INVALID_ARGUMENT, "FliX node pool exceeds 2^32-1 nodes");
-    // new structure first, then release the old one (rebuild reads it)
-    K *keys = pool_.get_n<K>(total * S_);
-    V *vals = pool_.get_n<V>(total * S_);
-    u32 *count = pool_.get_n<u32>(total);
-    u32 *next = pool_.get_n<u32>(total);
-    K *maxv = pool_.get_n<K>(P);
     // bucket directory: 2^d slices, 2^d <= P (at least 2)
-    u32 d = 1;
-    while (d < 24 && (u64(1) << (d + 1)) <= P) ++d;
-    u32 *bdir = pool_.get_n<u32>((u64(1) << d) + 1);
-    u32 *bsize = pool_.get_n<u32>(P);
-    u32 *blen = pool_.get_n<u32>(P);
-    u32 *ring = pool_.get_n<u32>(region);
-    RingCounters *ctr = pool_.get_n<RingCounters>(1);
+    L.d = 1;
+    while (L.d < 24 && (u64(1) << (L.d + 1)) <= L.P) ++L.d;
+    // a failed allocation returns what was taken so far
+    std::vector<void *> got;
+    try {
+      L.keys = pool_.get_n<K>(L.total * S_);
+      got.push_back(L.keys);
+      L.vals = pool_.get_n<V>(L.total * S_);
+      got.push_back(L.vals);
+      L.count = pool_.get_n<u32>(L.total);
+      got.push_back(L.count);
+      L.next = pool_.get_n<u32>(L.total);
+      got.push_back(L.next);
+      L.maxv = pool_.get_n<K>(L.P);
+      got.push_back(L.maxv);
+      L.bdir = pool_.get_n<u32>((u64(1) << L.d) + 1);
+      got.push_back(L.bdir);
+      L.bsize = pool_.get_n<u32>(L.P);
+      got.push_back(L.bsize);
+      L.blen = pool_.get_n<u32>(L.P);
+      got.push_back(L.blen);
+      L.ring = pool_.get_n<u32>(L.region);
+      got.push_back(L.ring);
+      L.ctr = pool_.get_n<RingCounters>(1);
+    } catch (...) {
+      for (void *b : got) pool_.put(b);
+      throw;
+    }
+    return L;
+  }
+
+  // Makes L the current layer; the previous layer's arrays return to the
+  // pool (work enqueued later on the stream runs after every use of them).
+  void install(const Layer &L) {
     release_structure();
-    node_keys_ = keys;
-    node_vals_ = vals;
-    node_count_ = count;
-    node_next_ = next;
-    maxv_ = maxv;
-    bdir_ = bdir;
-    bdir_d_ = d;
-    bdir_shift_ = static_cast<u32>(8 * sizeof(K)) - d;
-    bsize_ = bsize;
-    blen_ = blen;
-    ring_ids_ = ring;
-    counters_ = ctr;
-    P_ = static_cast<u32>(P);
-    total_nodes_ = total;
-    ring_cap_ = region;
-    // allocation-region headers start empty
-    rs::zero_u32_kernel<<<grid_for(total, 256, 1u << 16), 256, 0, s>>>(
-        node_count_, total);
-    rs::zero_u32_kernel<<<grid_for(total, 256, 1u << 16), 256, 0, s>>>(
-        node_next_, total);
-    Pool<K, V> pv = pool_view();
-    layout_kernel<K, V><<<grid_for(P * S_, 256, 1u << 16), 256, 0, s>>>(
-        pv, maxv_, uk, uv, m, fill);
+    node_keys_ = L.keys;
+    node_vals_ = L.vals;
+    node_count_ = L.count;
+    node_next_ = L.next;
+    maxv_ = L.maxv;
+    bdir_ = L.bdir;
+    bdir_d_ = L.d;
+    bdir_shift_ = static_cast<u32>(8 * sizeof(K)) - L.d;
+    bsize_ = L.bsize;
+    blen_ = L.blen;
+    ring_ids_ = L.ring;
+    counters_ = L.ctr;
+    P_ = static_cast<u32>(L.P);
+    total_nodes_ = L.total;
+    ring_cap_ = L.region;
+  }
+
+  // Allocation-region headers start empty.
+  void clear_headers(hipStream_t s) {
+    rs::zero_u32_kernel<<<grid_for(total_nodes_, 256, 1u << 16), 256, 0, s>>>(
+        node_count_, total_nodes_);
+    rs::zero_u32_kernel<<<grid_for(total_nodes_, 256, 1u << 16), 256, 0, s>>>(
+        node_next_, total_nodes_);
+  }
+
+  // Bucket directory and ring of a layer whose heads are written.
+  void finish_layer(hipStream_t s) {
     const u64 slices = u64(1) << bdir_d_;
     bucket_dir_clear_kernel<<<grid_for(slices, 256, 1u << 16), 256, 0, s>>>(
         bdir_, slices);
-    bucket_dir_kernel<K><<<grid_for(P, 256), 256, 0, s>>>(
+    bucket_dir_kernel<K><<<grid_for(P_, 256), 256, 0, s>>>(
         maxv_, P_, bdir_, bdir_d_, bdir_shift_);
     bucket_dir_fill_kernel<K><<<grid_for(slices, 256, 1u << 16), 256, 0, s>>>(
         maxv_, P_, bdir_, bdir_d_, bdir_shift_);
-    ring_init_kernel<<<grid_for(region, 256, 1u << 16), 256, 0, s>>>(
-        ring_ids_, P, region);
-    ring_counters_init_kernel<<<1, 1, 0, s>>>(counters_, region);
+    ring_init_kernel<<<grid_for(ring_cap_, 256, 1u << 16), 256, 0, s>>>(
+        ring_ids_, P_, ring_cap_);
+    ring_counters_init_kernel<<<1, 1, 0, s>>>(counters_, ring_cap_);
     GD_LAUNCH_CHECK();
   }
 

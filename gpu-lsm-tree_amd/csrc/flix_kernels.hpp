@@ -249,6 +249,61 @@ static __global__ void layout_kernel(Pool<K, V> p, K *maxv_out, const K *keys,
   }
 }
 
+// Rebuild without a flattened copy: a wave per bucket chain of the old
+// layer p0 writes the chain's keys and values straight to their rows of the
+// new layer p (offsets[b] = the live keys of the chains before chain b, so
+// row r of the sorted contents goes to bucket r / fill, slot r % fill);
+// layout_heads_kernel then writes the new heads.  S <= 64.
+template <class K, class V>
+static __global__ __launch_bounds__(256) void relayout_kernel(
+    Pool<K, V> p0, const u64 *offsets, Pool<K, V> p, u32 fill) {
+  const u32 lane = lane_id();
+  const u32 b = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (b >= p0.buckets) return;
+  u64 at = offsets[b];
+  u32 node = b;
+  while (true) {
+    const u32 c = p0.count[node];
+    const u32 nxt = p0.next[node];
+    if (lane < c) {
+      // row at + lane: the node's first row by one division, the lane's
+      // step from it (fewer than fill + 64 slots) by a 32-bit one
+      const u64 q0 = at / fill;
+      const u32 t = static_cast<u32>(at - q0 * fill) + lane;
+      const u64 nb = q0 + t / fill;
+      const u64 to = nb * p.S + t % fill;
+      p.keys[to] = p0.keys[u64(node) * p0.S + lane];
+      p.vals[to] = p0.vals[u64(node) * p0.S + lane];
+    }
+    at += c;
+    if (!nxt) break;
+    node = nxt - 1;
+  }
+}
+
+// Heads of a layer whose n rows relayout_kernel placed: counts, empty
+// links, chain sizes and bucket bounds (a bucket's last key, all-ones for
+// the last bucket), as layout_kernel writes them.
+template <class K, class V>
+static __global__ void layout_heads_kernel(Pool<K, V> p, K *maxv_out, u64 n,
+                                           u32 fill) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 bb = u64(blockIdx.x) * blockDim.x + threadIdx.x; bb < p.buckets;
+       bb += stride) {
+    const u32 b = static_cast<u32>(bb);
+    const u64 first = u64(b) * fill;
+    const u64 end = first + fill < n ? first + fill : n;
+    const u32 c = first < n ? static_cast<u32>(end - first) : 0u;
+    p.count[b] = c;
+    p.next[b] = 0;
+    p.bsize[b] = c;
+    p.blen[b] = 1;
+    maxv_out[b] = (b + 1 == p.buckets) ? ~K(0)
+                  : c                  ? p.keys[u64(b) * p.S + c - 1]
+                                       : K(0);
+  }
+}
+
 static __global__ void ring_init_kernel(u32 *ids, u64 first_node, u64 n) {
   const u64 stride = u64(gridDim.x) * blockDim.x;
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -983,29 +1038,6 @@ static __global__ void chain_sizes_kernel(Pool<K, V> p, u64 *sizes) {
     node = nxt - 1;
   }
   sizes[b] = s;
-}
-
-// Writes every chain into a flat sorted array at its offset (wave/bucket).
-template <class K, class V>
-static __global__ __launch_bounds__(256) void flatten_kernel(Pool<K, V> p,
-                                                      const u64 *offsets,
-                                                      K *ok, V *ov) {
-  const u32 lane = lane_id();
-  const u32 b = blockIdx.x * 4 + threadIdx.x / kWave;
-  if (b >= p.buckets) return;
-  u64 at = offsets[b];
-  u32 node = b;
-  while (true) {
-    const u32 c = p.count[node];
-    if (lane < c) {
-      ok[at + lane] = p.keys[u64(node) * p.S + lane];
-      ov[at + lane] = p.vals[u64(node) * p.S + lane];
-    }
-    at += c;
-    const u32 nxt = p.next[node];
-    if (!nxt) break;
-    node = nxt - 1;
-  }
 }
 
 // Exclusive scan of u64 values with a single-pass chained scan over tiles
