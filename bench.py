@@ -939,9 +939,12 @@ def main():
         a = t * w.batch
         n = min(w.batch, n_upd - a)
         views.append((d_keys[a:a + n], d_vals[a:a + n], d_erase[a:a + n], n))
-    evs = [(torch.cuda.Event(enable_timing=True),
-            torch.cuda.Event(enable_timing=True))
-           for _ in range(w.n_batches + 2)]
+    # one completion event per call (the previous call's event is the
+    # next call's start mark: a second record per call would sit in every
+    # call's critical path), plus a start mark after the clear and one
+    # before the lookups (after the first timed step's shape() read)
+    nb = w.n_batches
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(nb + 4)]
 
     def wait(e):
         # the host polls each call's completion event: a blocking wait
@@ -949,39 +952,38 @@ def main():
         while not e.query():
             pass
 
+    def timed_call(fn, e):
+        h0 = time.perf_counter()
+        fn()
+        e.record(stream)
+        wait(e)
+        return (time.perf_counter() - h0) * 1e3
+
     def step(record):
         lat = []
         phases = {"update": 0.0, "lookup": 0.0, "range": 0.0}
         d.clear(stream=s_ptr)
-        for (k, v, f, n), (e0, e1) in zip(views, evs):
+        evs[0].record(stream)
+        for t, (k, v, f, n) in enumerate(views):
+            e = evs[t + 1]
             h0 = time.perf_counter()
-            e0.record(stream)
             d.apply_mixed(k, v, f, count=n, stream=s_ptr)
-            e1.record(stream)
-            wait(e1)
+            e.record(stream)
+            wait(e)
             lat.append((time.perf_counter() - h0) * 1e3)
         shape = d.shape() if record else None
-        e0, e1 = evs[w.n_batches]
-        h0 = time.perf_counter()
-        e0.record(stream)
-        d.lookup(d_q, d_out, d_found, stream=s_ptr)
-        e1.record(stream)
-        wait(e1)
-        lat.append((time.perf_counter() - h0) * 1e3)
-        e0, e1 = evs[w.n_batches + 1]
-        h0 = time.perf_counter()
-        e0.record(stream)
-        d.range_count(d_lo, d_hi, d_cnt, stream=s_ptr)
-        e1.record(stream)
-        wait(e1)
-        lat.append((time.perf_counter() - h0) * 1e3)
+        evs[nb + 1].record(stream)
+        lat.append(timed_call(
+            lambda: d.lookup(d_q, d_out, d_found, stream=s_ptr), evs[nb + 2]))
+        lat.append(timed_call(
+            lambda: d.range_count(d_lo, d_hi, d_cnt, stream=s_ptr),
+            evs[nb + 3]))
         # every event has completed; read the phase times after the calls
-        phases["update"] = sum(e0.elapsed_time(e1)
-                               for e0, e1 in evs[:w.n_batches])
-        phases["lookup"] = evs[w.n_batches][0].elapsed_time(
-            evs[w.n_batches][1])
-        phases["range"] = evs[w.n_batches + 1][0].elapsed_time(
-            evs[w.n_batches + 1][1])
+        # (the update phase spans the 64 calls and the host time between
+        # them)
+        phases["update"] = evs[0].elapsed_time(evs[nb])
+        phases["lookup"] = evs[nb + 1].elapsed_time(evs[nb + 2])
+        phases["range"] = evs[nb + 2].elapsed_time(evs[nb + 3])
         return lat, phases, shape
 
     for _ in range(args.warmup):
