@@ -11,9 +11,11 @@ result:
 * FliX: slots per node, initial fill and the chain length that triggers a
   rebuild.
 
-It writes gpu-lsm-tree_amd/tuned.json, which GPULSMOpt(tuning="auto") and
-Server(tuning="auto") load.  Search is coordinate descent from the defaults
-over a config-2-shaped workload scaled by 2^-scale_log (a few minutes).
+It writes gpu-lsm-tree_amd/tuned.json, which GPULSMOpt(tuning="auto"),
+Server(tuning="auto") and FlixIndex(tuning="auto") load.  The LSM and
+serving knobs are searched by coordinate descent from the defaults over a
+config-2-shaped workload, the FliX knobs by a full grid over config 3's
+timed work, both scaled by 2^-scale_log (a few minutes).
 
     python gpu-lsm-tree_amd/autotune.py [--scale-log 2] [--out PATH]
 """
@@ -166,48 +168,61 @@ def tune_serve(pkg, torch, dev, lsm_knobs, scale_log, log):
                               measure, log)
 
 
-def tune_flix(pkg, torch, dev, scale_log, log):
-    rng = np.random.default_rng(3)
-    n = 1 << (24 - scale_log)
-    runs = n >> 12
-    starts = rng.integers(2, 1 << 62, runs, dtype=np.uint64)
-    keys = (starts[:, None] + np.arange(4096, dtype=np.uint64)).ravel()
-    ins = rng.integers(2, 1 << 62, n // 4, dtype=np.uint64)
-    q = keys[rng.integers(0, len(keys), n)]
+def tune_flix(pkg, torch, dev, scale_log, log, batches=8):
+    """Every combination of the FliX knobs (a grid: a fuller node wants
+    another rebuild threshold, which a descent along one knob at a time
+    from a single start misses) on config 3's timed work scaled by
+    2^-scale_log: a build of clustered uint64 keys (untimed), then
+    `batches` batches of mixed updates (75% inserts of uniform keys, most
+    of which land in the gaps between the clustered runs, so those chains
+    grow from batch to batch; 25% deletes), each followed by its lookups
+    and range enumerations (count pass, then fill pass).  Best of two runs
+    per combination."""
+    import itertools
+    import gdict_amd.workload as wl
+    w = wl.config3(scale_log)
+    built = w.build_keys()
     up = lambda a: torch.from_numpy(  # noqa: E731
         np.ascontiguousarray(a).view(np.int64)).to(dev)
-    dk, di, dq = up(keys), up(ins), up(q)
-    out = torch.empty(len(q), dtype=torch.int64, device=dev)
+    dk = up(built)
+    work = []
+    for t in range(batches):
+        ins, vals, dels = w.batch_ops(t, built)
+        q, lo, hi = w.queries(t, built)
+        work.append(tuple(up(a) for a in (ins, vals, dels, q, lo, hi)))
+    out = torch.empty(w.lookups, dtype=torch.int64, device=dev)
+    offs = torch.empty(w.ranges + 1, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
 
-    rounds = 8
-    step = len(ins) // rounds
-    qstep = len(q) // rounds
-
     def measure(knobs):
-        # eight rounds of inserts (uniform keys: most land in the gaps
-        # between the clustered runs, so those buckets' chains grow from
-        # round to round, which the rebuild threshold answers), lookups and
-        # deletes
         f = pkg.FlixIndex(key_bytes=8, value_bytes=8, **knobs)
-        f.build_bucket_layer_only(dk, max_size=2 * n)
+        f.build_bucket_layer_only(dk, max_size=len(built) + 2 * batches *
+                                  w.batch)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for r in range(rounds):
-            part = di[r * step:(r + 1) * step]
-            f.insert(part, part)
-            f.lookup(dq[r * qstep:(r + 1) * qstep],
-                     out[r * qstep:(r + 1) * qstep])
-            f.remove(part[: step // 2])
+        for di, dv, dd, dq, dlo, dhi in work:
+            f.insert(di, dv)
+            f.remove(dd)
+            f.lookup(dq, out[:len(dq)])
+            f.range(dlo, dhi, offs)
+            total = max(int(offs[-1].item()), 1)
+            ok = torch.empty(total, dtype=torch.int64, device=dev)
+            ov = torch.empty(total, dtype=torch.int64, device=dev)
+            f.range(dlo, dhi, offs, ok, ov)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3
         f.close()
         return ms
 
-    return coordinate_descent(FLIX_SPACE, {"node_slots": 32,
-                                           "fill_percent": 50,
-                                           "rebuild_chain_nodes": 0},
-                              measure, log)
+    best, best_ms = None, None
+    names = list(FLIX_SPACE)
+    for combo in itertools.product(*(FLIX_SPACE[k] for k in names)):
+        knobs = dict(zip(names, combo))
+        ms = min(measure(knobs), measure(knobs))
+        log.append({"knobs": knobs, "ms": ms})
+        if best_ms is None or ms < best_ms:
+            best, best_ms = knobs, ms
+    return best, best_ms
 
 
 def main(argv=None):
