@@ -115,6 +115,7 @@ class LsmBench:
 
 def coordinate_descent(space, start, measure, log):
     best = dict(start)
+    measure(best)  # warm-up: the first run pays one-time costs
     best_ms = measure(best)
     log.append({"knobs": dict(best), "ms": best_ms})
     for knob, values in space.items():
