@@ -488,3 +488,31 @@ def test_range_edges(gd, orc, io, kb):
     assert np.array_equal(gv, ev)
     # the mid-chain ranges hold what they should
     assert int(eo[-7] - eo[-8]) == 101  # 20500..20600 of the chain
+
+
+@pytest.mark.parametrize("kb,vb", [(4, 8), (8, 4)])
+def test_rebuild_after_emptying(gd, orc, io, kb, vb):
+    """A rebuild writes each chain's rows straight into the new layer: an
+    index emptied by deletes rebuilds into one empty bucket, answers every
+    query as empty, and takes inserts again; a half-emptied one rebuilds
+    into fewer buckets with the same contents."""
+    rng = np.random.default_rng(61 + kb)
+    kt, vt = KT[kb], KT[vb]
+    p = FlixPair(gd, orc, io, kb, vb, node_slots=16, fill_percent=50)
+    keys = np.unique(rng.integers(2, 1 << 30, 30000, dtype=np.uint64)
+                     ).astype(kt)
+    p.build(keys, max_size=1 << 16)
+    p.insert(rng.integers(2, 1 << 30, 5000, dtype=np.uint64).astype(kt),
+             rng.integers(0, 1 << 31, 5000, dtype=np.uint64).astype(vt))
+    live = p.o.dump()[0]
+    p.remove(live[::2])
+    p.f.rebuild_bucket_layer_only()
+    p.check_all(rng, 2, 1 << 30)
+    p.remove(p.o.dump()[0])
+    assert p.f.compute_total_size() == 0
+    p.f.rebuild_bucket_layer_only()
+    assert p.f.compute_total_size() == 0
+    p.check_all(rng, 2, 1 << 30)
+    p.insert(rng.integers(2, 1 << 30, 3000, dtype=np.uint64).astype(kt),
+             np.arange(3000, dtype=vt))
+    p.check_all(rng, 2, 1 << 30)
