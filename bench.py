@@ -55,12 +55,14 @@ def parse():
     ap.add_argument("--flix-ref", type=int, default=1,
                     help="FliX on the shape of the reference's recorded run "
                     "(1 = run, 0 = skip)")
-    ap.add_argument("--lsm-tuning", default='{"merge_fanout": 4}',
+    ap.add_argument("--lsm-tuning",
+                    default='{"merge_fanout": 4, "dir_rows_log": 2}',
                     help="JSON dict of gd_lsm_tuning knobs for the config-2 "
                          "dictionary, over the built-in defaults (default: "
                          "merge_fanout 4, the newest runs of one size merge "
                          "in fours, which suits a bulk stream of equal "
-                         "batches; '' = the defaults alone)")
+                         "batches, and dir_rows_log 2, a directory entry per "
+                         "4 rows; '' = the defaults alone)")
     ap.add_argument("--c5-scale-log", type=int, default=0,
                     help="run the config-5 serving sweep on a 2^(31-s)-key "
                          "uint64 index (default 0: 2^31 keys; -1 = skip)")
