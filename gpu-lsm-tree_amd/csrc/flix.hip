@@ -95,8 +95,14 @@ class FlixImpl : public FlixBase {
       Lease<K> uk(pool_, std::max<u64>(n, 1));
       Lease<V> uv(pool_, std::max<u64>(n, 1));
       u64 m = 0;
-      if (n) m = sort_unique(keys, nullptr, n, /*positions=*/true, nullptr,
-                             uk.get(), uv.get(), s);
+      if (n) {
+        Lease<u64> offs;
+        sort_unique(keys, nullptr, n, /*positions=*/true, nullptr, uk.get(),
+                    uv.get(), offs, s);
+        GD_HIP(hipMemcpyAsync(&m, offs.get() + n, sizeof(u64),
+                              hipMemcpyDeviceToHost, s));
+        GD_HIP(hipStreamSynchronize(s));
+      }
       layout(uk.get(), uv.get(), m, s);
     } catch (...) {
       order_.end(s);
@@ -127,9 +133,12 @@ class FlixImpl : public FlixBase {
       Lease<K> bk(pool_, n);
       Lease<V> bv(pool_, n);
       const Pool<K, V> existing = pool_view();
-      const u64 m = sort_unique(keys, vals, n, false, &existing, bk.get(),
-                                bv.get(), s);
-      if (m) apply_insert(bk.get(), bv.get(), m, s);
+      // the new keys' count stays on the device (offs[n]): the plan reads
+      // it there, and the plan's own read-back is the call's only wait
+      Lease<u64> offs;
+      sort_unique(keys, vals, n, false, &existing, bk.get(), bv.get(), offs,
+                  s);
+      apply_insert(bk.get(), bv.get(), offs.get() + n, n, s);
     } catch (...) {
       order_.end(s);
       throw;
@@ -372,10 +381,12 @@ class FlixImpl : public FlixBase {
 
   // Sorts (key, position), keeps the first occurrence of each key (and, when
   // `existing` is given, only keys not yet stored), compacts into (ok, ov)
-  // with values gathered from `vals` (or = input positions).  Returns the
-  // number of rows (synchronises).
-  u64 sort_unique(const K *keys, const V *vals, u64 n, bool positions,
-                  const Pool<K, V> *existing, K *ok, V *ov, hipStream_t s) {
+  // with values gathered from `vals` (or = input positions).  offs receives
+  // n + 1 words, the rows' positions; offs[n] = the number of rows (on the
+  // device: nothing here waits for it).
+  void sort_unique(const K *keys, const V *vals, u64 n, bool positions,
+                   const Pool<K, V> *existing, K *ok, V *ov, Lease<u64> &offs,
+                   hipStream_t s) {
     const u32 nn = static_cast<u32>(n);
     Lease<K> ka(pool_, n), kb(pool_, n);
     Lease<u32> va(pool_, n), vb(pool_, n);
@@ -393,17 +404,12 @@ class FlixImpl : public FlixBase {
     GD_LAUNCH_CHECK();
     Lease<u64> keep64(pool_, n + 1);
     widen(keep.get(), n, keep64.get(), s);
-    Lease<u64> offs(pool_, n + 1);
+    offs = Lease<u64>(pool_, n + 1);
     exclusive_scan(keep64.get(), n, offs.get(), s);
     compact_batch_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
         ka.get(), va.get(), n, keep.get(), offs.get(), vals,
         positions ? 1 : 0, ok, ov);
     GD_LAUNCH_CHECK();
-    u64 m = 0;
-    GD_HIP(hipMemcpyAsync(&m, offs.get() + n, sizeof(u64),
-                          hipMemcpyDeviceToHost, s));
-    GD_HIP(hipStreamSynchronize(s));
-    return m;
   }
 
   void widen(const u32 *in, u64 n, u64 *out, hipStream_t s) {
@@ -431,15 +437,17 @@ class FlixImpl : public FlixBase {
     relayout(offs.get(), total, s);
   }
 
-  // Applies a sorted batch of keys absent from the index (see insert()).
-  void apply_insert(const K *bk, const V *bv, u64 m, hipStream_t s) {
+  // Applies a sorted batch of keys absent from the index (see insert()):
+  // *m_dev of them (device memory), at most n_ub.
+  void apply_insert(const K *bk, const V *bv, const u64 *m_dev, u64 n_ub,
+                    hipStream_t s) {
     RingCounters rc{};
     for (int attempt = 0;; ++attempt) {
       Lease<u64> begin(pool_, u64(P_) + 1);
-      Lease<uint2> fast(pool_, std::min<u64>(m, u64(P_)) + 1);
-      Lease<BigItem> big(pool_, std::min<u64>(m, u64(P_)) + 1);
+      Lease<uint2> fast(pool_, std::min<u64>(n_ub, u64(P_)) + 1);
+      Lease<BigItem> big(pool_, std::min<u64>(n_ub, u64(P_)) + 1);
       bucket_slices_kernel<K><<<grid_for(u64(P_) + 1, 256), 256, 0, s>>>(
-          maxv_, P_, bk, m, begin.get());
+          maxv_, P_, bk, 0, m_dev, begin.get());
       plan_reset_kernel<<<1, 1, 0, s>>>(&counters_->plan);
       insert_plan_kernel<<<static_cast<u32>((u64(P_) + kPlanBuckets - 1) /
                                             kPlanBuckets),
@@ -460,8 +468,9 @@ class FlixImpl : public FlixBase {
                     "FliX allocation region of " + std::to_string(ring_cap_) +
                         " nodes cannot hold the batch even after a rebuild; "
                         "no key was inserted");
-      // grow: rebuild with room for twice the keys after this batch
-      rebuild_into(m, s);
+      // grow: rebuild with room for twice the keys after this batch (sized
+      // by the batch's length, an upper bound of its new keys)
+      rebuild_into(n_ub, s);
     }
     if (rebuild_chain_nodes_ && rc.plan.max_nodes > rebuild_chain_nodes_)
       rebuild_into(0, s);
@@ -508,7 +517,7 @@ class FlixImpl : public FlixBase {
   const u32 *slices(const K *batch, u64 m, u64 *begin, u32 *work,
                     hipStream_t s) {
     bucket_slices_kernel<K><<<grid_for(u64(P_) + 1, 256), 256, 0, s>>>(
-        maxv_, P_, batch, m, begin);
+        maxv_, P_, batch, m, nullptr, begin);
     GD_LAUNCH_CHECK();
     u32 *count = work + std::min<u64>(P_, m);
     rs::zero_u32_kernel<<<1, 64, 0, s>>>(count, 1);

@@ -313,9 +313,12 @@ static __global__ void ring_init_kernel(u32 *ids, u64 first_node, u64 n) {
 // Batch slice [begin[b], begin[b+1]) of every bucket for a sorted batch.
 template <class K>
 static __global__ void bucket_slices_kernel(const K *maxv, u32 buckets,
-                                     const K *batch, u64 n, u64 *begin) {
+                                     const K *batch, u64 n_arg,
+                                     const u64 *n_dev, u64 *begin) {
   const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > buckets) return;
+  // the batch length: n_arg, or the count a previous kernel left at n_dev
+  const u64 n = n_dev ? *n_dev : n_arg;
   if (b == 0) {
     begin[0] = 0;
     return;

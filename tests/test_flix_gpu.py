@@ -516,3 +516,24 @@ def test_rebuild_after_emptying(gd, orc, io, kb, vb):
     p.insert(rng.integers(2, 1 << 30, 3000, dtype=np.uint64).astype(kt),
              np.arange(3000, dtype=vt))
     p.check_all(rng, 2, 1 << 30)
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_insert_of_stored_keys_only(gd, orc, io, kb):
+    """A batch whose keys are all stored already (with duplicates inside
+    it) inserts nothing and changes no value; a batch mixing such keys with
+    new ones takes only the new ones."""
+    rng = np.random.default_rng(81 + kb)
+    kt = KT[kb]
+    p = FlixPair(gd, orc, io, kb, kb, node_slots=32)
+    keys = np.unique(rng.integers(2, 1 << 30, 20000, dtype=np.uint64)
+                     ).astype(kt)
+    p.build(keys, max_size=1 << 16)
+    again = keys[rng.integers(0, len(keys), 5000)]
+    p.insert(again, np.full(len(again), 7, kt))
+    assert p.f.compute_total_size() == len(keys)
+    p.check_all(rng, 2, 1 << 30)
+    mixed = np.concatenate([again[:1000], rng.integers(
+        2, 1 << 30, 1000, dtype=np.uint64).astype(kt)])
+    p.insert(mixed, np.arange(len(mixed), dtype=kt))
+    p.check_all(rng, 2, 1 << 30)
