@@ -203,22 +203,43 @@ class FlixImpl : public FlixBase {
     order_.begin(s);
     try {
       if (!sorted && n >= kGroupedLookups && n <= 0xffffffffull) {
-        // group the queries by their top varying key byte, look them up in
-        // grouped order (each XCD on its own share of the buckets), then
-        // bring the results back to query order
+        // group the queries by the 16 bits below their top varying one
+        // (two stable passes: the lower digit, then the top digit), look
+        // them up in grouped order (each XCD on its own share of the
+        // buckets, each group's nodes in L2), then bring the results back
+        // to query order through both passes' destinations
         const u32 nn = static_cast<u32>(n);
-        Lease<K> qk(pool_, n);
-        Lease<u32> dest(pool_, n);
+        const K *qk = static_cast<const K *>(q);
+        const lsm::SpreadLayout<K> L(n);
+        const u64 lo_words = rs::control_words(n, 1);
+        Lease<K> qa(pool_, n), qb(pool_, n);
+        Lease<u32> dest_lo(pool_, n), dest_hi(pool_, n);
         Lease<V> res(pool_, n);
-        Lease<u32> ctrl(pool_, lsm::SpreadLayout<K>(n).words);
-        lsm::partition_by_spread<K>(static_cast<const K *>(q), nn, qk.get(),
-                                    nullptr, dest.get(), rs::PayloadPos{},
+        Lease<u32> ctrl(pool_, L.words);
+        Lease<u32> lo_ctrl(pool_, lo_words + 1);
+        u32 *shift_lo = lo_ctrl.get() + lo_words;
+        const rs::Control lc = rs::control_layout(lo_ctrl.get(), 1);
+        // the top digit's shift, then the lower digit's counts and pass
+        lsm::spread_counts<K>(qk, nn, ctrl.get(), s);
+        rs::clear_control(lo_ctrl.get(), lo_words, s);
+        shift_below_kernel<<<1, 1, 0, s>>>(ctrl.get() + L.shift_at, shift_lo);
+        digit_hist_kernel<K><<<rs::hist_blocks_for(n), rs::kThreads, 0, s>>>(
+            qk, nn, shift_lo, lc.rows);
+        rs::scan_kernel<<<1, rs::kRadix, 0, s>>>(lc.rows, 1, lc.offsets);
+        rs::launch_pass<K>(rs::block_for(n), rs::tiles_for(n), qk,
+                           rs::PayloadPos{}, qa.get(), nullptr, nn, 0,
+                           lc.offsets, lc.status, lc.tickets, s,
+                           dest_lo.get(), shift_lo, nullptr);
+        // the top digit, stable over the lower one
+        lsm::partition_by_spread<K>(qa.get(), nn, qb.get(), nullptr,
+                                    dest_hi.get(), rs::PayloadPos{},
                                     ctrl.get(), s, 0);
         lookup_grouped_kernel<K, V><<<lsm::grouped_lookup_blocks(n), 256, 0,
-                                      s>>>(pool_view(), qk.get(), n,
+                                      s>>>(pool_view(), qb.get(), n,
                                            res.get());
-        lookup_gather_kernel<V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
-            dest.get(), res.get(), n, static_cast<V *>(out));
+        lookup_gather2_kernel<V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
+            dest_lo.get(), dest_hi.get(), res.get(), n,
+            static_cast<V *>(out));
       } else {
         lookup_kernel<K, V><<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(
             pool_view(), static_cast<const K *>(q), n, static_cast<V *>(out));

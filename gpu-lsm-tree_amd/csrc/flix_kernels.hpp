@@ -26,6 +26,7 @@
 #pragma once
 
 #include "device_utils.hpp"
+#include "radix_sort.hpp"
 
 namespace gdict {
 namespace flix {
@@ -886,6 +887,64 @@ static __global__ void lookup_gather_kernel(const u32 *dest, const V *res,
   for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += stride)
     out[i] = res[dest[i]];
+}
+
+// Two-level grouping of any-order lookups (stable LSD passes: the digit
+// below the queries' top varying one, then the top one), so that a group
+// of queries covers 1/2^16 of the key spread instead of 1/2^8 and the
+// nodes it reads stay in L2 while the group is looked up.
+//
+// shift_lo = the spread partition's shift - 8 (0 when the spread is
+// narrower than 16 bits; the two digits then overlap, which changes the
+// order only).
+static __global__ void shift_below_kernel(const u32 *shift_top,
+                                          u32 *shift_lo) {
+  const u32 t = *shift_top;
+  *shift_lo = t >= 8 ? t - 8 : 0u;
+}
+
+// rows[b % kHistRows][d] += the block's count of digit (key >> shift) & 0xff
+// (the layout rs::scan_kernel sums, for one pass).
+template <class K>
+static __global__ __launch_bounds__(rs::kThreads) void digit_hist_kernel(
+    const K *__restrict__ keys, u32 n, const u32 *shift_dev, u32 *rows) {
+  static_assert(rs::kThreads == rs::kRadix, "a thread per digit");
+  __shared__ u32 s_hist[rs::kRadix];
+  s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const u32 shift = *shift_dev;
+  const u64 round = u64(rs::kThreads) * rs::kHistItems;
+  for (u64 base = u64(blockIdx.x) * round; base < n;
+       base += u64(gridDim.x) * round) {
+    K k[rs::kHistItems];
+#pragma unroll
+    for (int j = 0; j < rs::kHistItems; ++j) {
+      const u64 i = base + u64(j) * rs::kThreads + threadIdx.x;
+      k[j] = i < n ? keys[i] : K(0);
+    }
+#pragma unroll
+    for (int j = 0; j < rs::kHistItems; ++j)
+      if (base + u64(j) * rs::kThreads + threadIdx.x < n)
+        atomicAdd(&s_hist[static_cast<u32>(k[j] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  const u32 c = s_hist[threadIdx.x];
+  if (c)
+    atomicAdd(&rows[u64(blockIdx.x % rs::kHistRows) * rs::kRadix +
+                    threadIdx.x],
+              c);
+}
+
+// out[i] = res[dest_hi[dest_lo[i]]]: the results of the two-level grouping
+// back in query order.
+template <class V>
+static __global__ void lookup_gather2_kernel(const u32 *dest_lo,
+                                             const u32 *dest_hi,
+                                             const V *res, u64 n, V *out) {
+  const u64 stride = u64(gridDim.x) * blockDim.x;
+  for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    out[i] = res[dest_hi[dest_lo[i]]];
 }
 
 // First stored key >= key, starting at bucket b; (node, slot) or node = ~0.

@@ -217,25 +217,18 @@ struct SpreadLayout {
   }
 };
 
-// Stable partition of n keys by their top 8 varying bits (the 256 groups
-// split the keys' actual spread, not the key space): keys_out receives the
-// grouped keys, pos_out (nullable) each one's input position, dest
-// (nullable) each input key's grouped position, in input order; *shift_out
-// (inside `control`) the digit shift.  `control` holds
-// SpreadLayout<K>(n).words words; payload selects what pos_out carries.
-template <class K, class Payload>
-inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
-                                u32 *pos_out, u32 *dest, Payload payload,
-                                u32 *control, hipStream_t s, int block,
-                                u32 *extra = nullptr, u64 extra_words = 0) {
+// The counting launches of partition_by_spread: per-tile digit counts and
+// prefixes at the keys' spread shift (in the room of the look-back state,
+// which the pass does not use) and the shift itself, at control +
+// SpreadLayout<K>(n).shift_at.  Nothing of the partition's control needs
+// clearing, only the shift and the caller's extra region (the spans are
+// written whole).
+template <class K>
+inline void spread_counts(const K *keys, u32 n, u32 *control, hipStream_t s,
+                          u32 *extra = nullptr, u64 extra_words = 0) {
   const SpreadLayout<K> L(n);
   const rs::Control pc = rs::control_layout(control, 1);
   K *span = reinterpret_cast<K *>(control + L.span_at);
-  // Per-tile digit counts and prefixes (tiles x 256 words each, in the
-  // room of the look-back state, which this pass does not use) give every
-  // tile its offsets; nothing of the partition's control needs clearing,
-  // only the shift and the caller's extra region (the spans are written
-  // whole).
   const u32 tiles = rs::tiles_for(n);
   u32 *tile_hist = reinterpret_cast<u32 *>(pc.status);
   u32 *tile_prefix = tile_hist + u64(tiles) * rs::kRadix;
@@ -245,6 +238,27 @@ inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
   seg_tile_scan_kernel<K><<<rs::kRadix, rs::kThreads, 0, s>>>(
       keys, n, tile_hist, span, tiles, tile_prefix, pc.offsets,
       control + L.shift_at);
+}
+
+// Stable partition of n keys by their top 8 varying bits (the 256 groups
+// split the keys' actual spread, not the key space): keys_out receives the
+// grouped keys, pos_out (nullable) each one's input position, dest
+// (nullable) each input key's grouped position, in input order; *shift_out
+// (inside `control`) the digit shift.  `control` holds
+// SpreadLayout<K>(n).words words; payload selects what pos_out carries.
+// The per-tile digit prefixes give every tile its offsets, so the pass
+// needs no look-back.
+template <class K, class Payload>
+inline void partition_by_spread(const K *keys, u32 n, K *keys_out,
+                                u32 *pos_out, u32 *dest, Payload payload,
+                                u32 *control, hipStream_t s, int block,
+                                u32 *extra = nullptr, u64 extra_words = 0) {
+  spread_counts<K>(keys, n, control, s, extra, extra_words);
+  const SpreadLayout<K> L(n);
+  const rs::Control pc = rs::control_layout(control, 1);
+  const u32 tiles = rs::tiles_for(n);
+  const u32 *tile_prefix =
+      reinterpret_cast<u32 *>(pc.status) + u64(tiles) * rs::kRadix;
   rs::launch_pass<K>(block ? block : rs::block_for(n), tiles, keys, payload,
                      keys_out, pos_out, n, 0, pc.offsets, pc.status,
                      pc.tickets, s, dest, control + L.shift_at,
