@@ -13,9 +13,11 @@ result:
 
 It writes gpu-lsm-tree_amd/tuned.json, which GPULSMOpt(tuning="auto"),
 Server(tuning="auto") and FlixIndex(tuning="auto") load.  The LSM and
-serving knobs are searched by coordinate descent from the defaults over a
-config-2-shaped workload, the FliX knobs by a full grid over config 3's
-timed work, both scaled by 2^-scale_log (a few minutes).
+serving knobs are searched by coordinate descent from the defaults over the
+bulk LSM step (2^26 uint32 keys in 2^20-key mixed batches, lookups, range
+counts), the FliX knobs by a full grid over the FliX update workload
+(clustered uint64 build, mixed batches, lookups, range enumerations), both
+scaled by 2^-scale_log (a few minutes).
 
     python gpu-lsm-tree_amd/autotune.py [--scale-log 2] [--out PATH]
 """
@@ -66,7 +68,9 @@ def _pkg():
 
 
 class LsmBench:
-    """A config-2-shaped step: mixed batches, grouped lookups, ranges."""
+    """The bulk LSM step (2^26 uint32 keys in 2^20-key mixed batches with
+    10% erases, 2^26 lookups, 2^20 range counts), scaled: mixed batches,
+    grouped lookups, ranges."""
 
     def __init__(self, pkg, torch, dev, scale_log):
         import gdict_amd.workload as wl
@@ -172,8 +176,10 @@ def tune_serve(pkg, torch, dev, lsm_knobs, scale_log, log):
 def tune_flix(pkg, torch, dev, scale_log, log, batches=8):
     """Every combination of the FliX knobs (a grid: a fuller node wants
     another rebuild threshold, which a descent along one knob at a time
-    from a single start misses) on config 3's timed work scaled by
-    2^-scale_log: a build of clustered uint64 keys (untimed), then
+    from a single start misses) on the FliX update workload's timed work
+    (2^27 keys, 2^22-op batches, 2^24 lookups and 2^18 range enumerations
+    per batch) scaled by 2^-scale_log: a build of clustered uint64 keys
+    (untimed), then
     `batches` batches of mixed updates (75% inserts of uniform keys, most
     of which land in the gaps between the clustered runs, so those chains
     grow from batch to batch; 25% deletes), each followed by its lookups
